@@ -1,0 +1,689 @@
+// api.hip -- C ABI of libfasticehip.so: model handles, device fields, the PT
+// iteration driver, residual norms, and the level-set set-up calls.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+#include "fi_internal.h"
+
+using namespace fi;
+
+namespace {
+
+thread_local char g_err[512] = "";
+
+int fail(int code, const char* fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIPCHK(expr)                                                                             \
+    do {                                                                                         \
+        hipError_t e_ = (expr);                                                                  \
+        if (e_ != hipSuccess) return fail(FI_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_));   \
+    } while (0)
+
+int field_loc(int nd, int id)
+{
+    switch (id) {
+    case FI_F_P: case FI_F_TXX: case FI_F_TYY: case FI_F_TZZ: case FI_F_ETA: case FI_F_ETA_NEXT: case FI_F_RP:
+    case FI_F_RHEO: return L_CCC;
+    case FI_F_TXY: return L_VVC;
+    case FI_F_TXZ: return L_VCV;
+    case FI_F_TYZ: return L_CVV;
+    case FI_F_VX: case FI_F_RVX: case FI_F_GX: return L_VCC;
+    case FI_F_VY: case FI_F_RVY: case FI_F_GY: return L_CVC;
+    case FI_F_VZ: case FI_F_RVZ: case FI_F_GZ: return L_CCV;
+    case FI_F_PSI_NA: case FI_F_PSI_NS: return (nd == 3) ? L_VVV : L_VVC;
+    default: break;
+    }
+    if (id >= FI_F_OM_NA0 && id < FI_F_COUNT) {
+        int q = (id - FI_F_OM_NA0) % 8;
+        if (nd == 3) {
+            const int L3[8] = {L_CCC, L_VVV, L_VCC, L_CVC, L_CCV, L_VVC, L_VCV, L_CVV};
+            return L3[q];
+        }
+        const int L2[4] = {L_CCC, L_VVC, L_VCC, L_CVC};
+        return q < 4 ? L2[q] : -1;
+    }
+    return -1;
+}
+
+} // namespace
+
+struct fi_model_s {
+    fi_model_desc desc;
+    Geo g;
+    hipStream_t stream = nullptr;
+    double* base[FI_F_COUNT] = {};
+    double* f[FI_F_COUNT] = {};
+    int eta_cur = 0; // 0: logical eta lives in buffer FI_F_ETA, 1: in FI_F_ETA_NEXT
+    unsigned long long* d_norm = nullptr;
+    unsigned long long* h_norm = nullptr;
+    double* d_dem = nullptr;
+    size_t dem_elems = 0;
+
+    double* eta() { return f[eta_cur ? FI_F_ETA_NEXT : FI_F_ETA]; }
+    double* eta_next() { return f[eta_cur ? FI_F_ETA : FI_F_ETA_NEXT]; }
+    // physical buffer behind a logical field id
+    double* buf(int id)
+    {
+        if (id == FI_F_ETA) return eta();
+        if (id == FI_F_ETA_NEXT) return eta_next();
+        return f[id];
+    }
+};
+
+namespace {
+
+bool field_used(const fi_model_desc& d, int id)
+{
+    if (id == FI_F_TZZ || id == FI_F_TXZ || id == FI_F_TYZ || id == FI_F_VZ || id == FI_F_RVZ || id == FI_F_GZ)
+        return d.ndims == 3;
+    if (id == FI_F_GX || id == FI_F_GY) return d.gravity_is_field != 0;
+    if (id == FI_F_RHEO)
+        return d.rheology.kind == FI_RHEO_LINEAR_FIELD || (d.rheology.kind == FI_RHEO_POWERLAW && d.rheology.A_is_field);
+    if (id == FI_F_PSI_NA || id == FI_F_PSI_NS) return d.immersed_boundary != 0;
+    if (id >= FI_F_OM_NA0) {
+        if (!d.immersed_boundary) return false;
+        int q = (id - FI_F_OM_NA0) % 8;
+        return d.ndims == 3 || q < 4;
+    }
+    return true;
+}
+
+StokesArgs<double> make_args(fi_model_s* m)
+{
+    StokesArgs<double> a;
+    std::memset(&a, 0, sizeof(a));
+    const fi_model_desc& d = m->desc;
+    a.P = m->f[FI_F_P];
+    a.txx = m->f[FI_F_TXX];
+    a.tyy = m->f[FI_F_TYY];
+    a.tzz = m->f[FI_F_TZZ];
+    a.txy = m->f[FI_F_TXY];
+    a.txz = m->f[FI_F_TXZ];
+    a.tyz = m->f[FI_F_TYZ];
+    a.Vx = m->f[FI_F_VX];
+    a.Vy = m->f[FI_F_VY];
+    a.Vz = m->f[FI_F_VZ];
+    a.eta = m->eta();
+    a.eta_next = m->eta_next();
+    a.gx = m->f[FI_F_GX];
+    a.gy = m->f[FI_F_GY];
+    a.gz = m->f[FI_F_GZ];
+    a.rh = m->f[FI_F_RHEO];
+    a.rP = m->f[FI_F_RP];
+    a.rVx = m->f[FI_F_RVX];
+    a.rVy = m->f[FI_F_RVY];
+    a.rVz = m->f[FI_F_RVZ];
+    for (int q = 0; q < 8; ++q) {
+        a.na[q] = m->f[FI_F_OM_NA0 + q];
+        a.ns[q] = m->f[FI_F_OM_NS0 + q];
+    }
+    a.nx = m->g.n[0];
+    a.ny = m->g.n[1];
+    a.nz = m->g.n[2];
+    a.sy = m->g.sy;
+    a.sz = m->g.sz;
+    a.ix = m->g.idx[0];
+    a.iy = m->g.idx[1];
+    a.iz = m->g.idx[2];
+    a.dPr = d.dtau_P;
+    for (int q = 0; q < 6; ++q) a.dtau[q] = d.dtau_tau[q];
+    for (int q = 0; q < 3; ++q) {
+        a.dV[q] = d.dtau_V[q];
+        a.grav[q] = d.gravity[q];
+    }
+    a.grav_field = d.gravity_is_field;
+    a.rh_kind = d.rheology.kind;
+    a.rh_has_reg = d.rheology.has_reg;
+    a.rh_relax = d.rheology.relax;
+    a.rh_eta = d.rheology.eta;
+    a.rh_A = d.rheology.A;
+    a.rh_n = d.rheology.n;
+    a.rh_inv_eta_reg = d.rheology.has_reg ? 1.0 / d.rheology.eta_reg : 0.0;
+    a.rh_rate = d.rheology.rate;
+    if (!(d.rheology.kind == FI_RHEO_POWERLAW && d.rheology.A_is_field) && d.rheology.kind != FI_RHEO_LINEAR_FIELD)
+        a.rh = nullptr;
+    return a;
+}
+
+void add_op(BcBatch& b, double* p, int L, int kind, double v)
+{
+    BcOp& o = b.op[b.nop++];
+    o.ptr = p;
+    o.L = L;
+    o.kind = kind;
+    o.v = v;
+}
+
+// Both sides of one dimension share a launch; a side that does not touch a
+// field gets a no-op by pointing that op at the other side's identical value.
+// To keep the face kernel simple, batches are built per (dim, side) and a
+// side mask is carried in the op's kind: kind 0/1 = both sides, 2/3 = low
+// only, 4/5 = high only.
+void add_side_op(BcBatch& b, double* p, int L, int kind, double v, int side)
+{
+    add_op(b, p, L, kind + 2 * (side + 1), v);
+}
+
+// stress batch: boundary_conditions.jl:26-100
+BcBatch stress_batch(fi_model_s* m, int d)
+{
+    BcBatch b;
+    b.nop = 0;
+    b.d = d;
+    const fi_model_desc& ds = m->desc;
+    for (int s = 0; s < 2; ++s) {
+        const int kind = ds.bc_kind[d][s];
+        const double* c = ds.bc_value[d][s];
+        if (ds.ndims == 3) {
+            if (kind == FI_BC_TRACTION) {
+                double* diag = m->f[d == 0 ? FI_F_TXX : d == 1 ? FI_F_TYY : FI_F_TZZ];
+                add_side_op(b, m->f[FI_F_P], L_CCC, 0, c[d], s);
+                add_side_op(b, diag, L_CCC, 0, c[d], s);
+            }
+            if (kind == FI_BC_TRACTION || kind == FI_BC_SLIP) {
+                if (d == 0) {
+                    add_side_op(b, m->f[FI_F_TXY], L_VVC, 0, c[1], s);
+                    add_side_op(b, m->f[FI_F_TXZ], L_VCV, 0, c[2], s);
+                } else if (d == 1) {
+                    add_side_op(b, m->f[FI_F_TXY], L_VVC, 0, c[0], s);
+                    add_side_op(b, m->f[FI_F_TYZ], L_CVV, 0, c[2], s);
+                } else {
+                    add_side_op(b, m->f[FI_F_TXZ], L_VCV, 0, c[0], s);
+                    add_side_op(b, m->f[FI_F_TYZ], L_CVV, 0, c[1], s);
+                }
+            }
+        } else {
+            if (kind == FI_BC_TRACTION) {
+                double* diag = m->f[d == 0 ? FI_F_TXX : FI_F_TYY];
+                add_side_op(b, m->f[FI_F_P], L_CCC, 0, c[d], s);
+                add_side_op(b, diag, L_CCC, 0, c[d], s);
+            }
+            if (kind == FI_BC_TRACTION || kind == FI_BC_SLIP) add_side_op(b, m->f[FI_F_TXY], L_VVC, 0, c[1 - d], s);
+        }
+    }
+    return b;
+}
+
+// velocity batch + eta_next Neumann: boundary_conditions.jl:104-115, Isothermal.jl:103-104
+BcBatch velocity_batch(fi_model_s* m, int d, bool with_eta)
+{
+    BcBatch b;
+    b.nop = 0;
+    b.d = d;
+    const fi_model_desc& ds = m->desc;
+    const int idV[3] = {FI_F_VX, FI_F_VY, FI_F_VZ};
+    const int LV[3] = {L_VCC, L_CVC, L_CCV};
+    for (int s = 0; s < 2; ++s) {
+        const int kind = ds.bc_kind[d][s];
+        const double* c = ds.bc_value[d][s];
+        if (kind == FI_BC_VELOCITY) {
+            for (int q = 0; q < ds.ndims; ++q) add_side_op(b, m->f[idV[q]], LV[q], 0, c[q], s);
+        } else if (kind == FI_BC_SLIP) {
+            add_side_op(b, m->f[idV[d]], LV[d], 0, c[d], s);
+        }
+    }
+    if (with_eta) add_op(b, m->eta_next(), L_CCC, 1, 0.0);
+    return b;
+}
+
+// residual batch: boundary_conditions.jl:120-128
+BcBatch residual_batch(fi_model_s* m, int d)
+{
+    BcBatch b;
+    b.nop = 0;
+    b.d = d;
+    const int idR[3] = {FI_F_RVX, FI_F_RVY, FI_F_RVZ};
+    const int LV[3] = {L_VCC, L_CVC, L_CCV};
+    for (int s = 0; s < 2; ++s)
+        if (m->desc.bc_kind[d][s] != FI_BC_TRACTION) add_side_op(b, m->f[idR[d]], LV[d], 0, 0.0, s);
+    return b;
+}
+
+BcBatch neumann_batch(double* p, int L, int d)
+{
+    BcBatch b;
+    b.nop = 0;
+    b.d = d;
+    add_op(b, p, L, 1, 0.0);
+    return b;
+}
+
+void run_bc(fi_model_s* m, int group)
+{
+    for (int d = 0; d < m->g.nd; ++d) {
+        BcBatch b;
+        if (group == FI_BCGROUP_STRESS) b = stress_batch(m, d);
+        else if (group == FI_BCGROUP_VELOCITY) b = velocity_batch(m, d, true);
+        else if (group == FI_BCGROUP_ETA) b = neumann_batch(m->eta(), L_CCC, d);
+        else b = residual_batch(m, d);
+        launch_bc_dim(m->g, b, m->stream);
+    }
+}
+
+void advance(fi_model_s* m, int niter)
+{
+    const bool ib = m->desc.immersed_boundary != 0;
+    for (int it = 0; it < niter; ++it) {
+        StokesArgs<double> a = make_args(m);
+        launch_sigma(m->g, a, ib, m->stream);
+        run_bc(m, FI_BCGROUP_STRESS);
+        launch_velocity(m->g, a, ib, m->stream);
+        run_bc(m, FI_BCGROUP_VELOCITY);
+        m->eta_cur ^= 1; // Isothermal.jl:110
+    }
+}
+
+double bits_to_double(unsigned long long b)
+{
+    double v;
+    std::memcpy(&v, &b, sizeof(v));
+    return v;
+}
+
+int residual_norms(fi_model_s* m, double out[4])
+{
+    const bool ib = m->desc.immersed_boundary != 0;
+    StokesArgs<double> a = make_args(m);
+    launch_residuals(m->g, a, ib, m->stream);
+    run_bc(m, FI_BCGROUP_RESIDUAL);
+    const double* f[4] = {m->f[FI_F_RP], m->f[FI_F_RVX], m->f[FI_F_RVY], m->f[FI_F_RVZ]};
+    const int L[4] = {L_CCC, L_VCC, L_CVC, L_CCV};
+    const int nf = m->g.nd == 3 ? 4 : 3;
+    launch_norms(m->g, f, L, nf, m->d_norm, m->stream);
+    HIPCHK(hipMemcpyAsync(m->h_norm, m->d_norm, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));
+    for (int q = 0; q < 4; ++q) out[q] = (q < nf) ? bits_to_double(m->h_norm[q]) : 0.0;
+    return FI_OK;
+}
+
+int check_desc(const fi_model_desc* d)
+{
+    if (!d) return fail(FI_ERR_ARG, "null descriptor");
+    if (d->ndims != 2 && d->ndims != 3) return fail(FI_ERR_ARG, "ndims must be 2 or 3 (got %d)", d->ndims);
+    for (int q = 0; q < d->ndims; ++q) {
+        if (d->n[q] < 1) return fail(FI_ERR_ARG, "n[%d] must be >= 1 (got %d)", q, d->n[q]);
+        if (!(d->extent[q] > 0.0)) return fail(FI_ERR_ARG, "extent[%d] must be > 0", q);
+        for (int s = 0; s < 2; ++s)
+            if (d->bc_kind[q][s] < FI_BC_TRACTION || d->bc_kind[q][s] > FI_BC_SLIP)
+                return fail(FI_ERR_ARG, "bad boundary-condition kind on dim %d side %d", q, s);
+    }
+    if (d->rheology.kind < FI_RHEO_LINEAR_CONST || d->rheology.kind > FI_RHEO_POWERLAW)
+        return fail(FI_ERR_ARG, "bad rheology kind %d", d->rheology.kind);
+    if (d->rheology.relax < FI_RELAX_NONE || d->rheology.relax > FI_RELAX_LOG10)
+        return fail(FI_ERR_ARG, "bad relaxation scale %d", d->rheology.relax);
+    if (d->rheology.has_reg && !(d->rheology.eta_reg > 0.0)) return fail(FI_ERR_ARG, "eta_reg must be > 0");
+    if (d->precision != FI_PREC_FP64 && d->precision != FI_PREC_FP32_REFINED)
+        return fail(FI_ERR_ARG, "bad precision %d", d->precision);
+    return FI_OK;
+}
+
+void free_model(fi_model_s* m)
+{
+    for (int q = 0; q < FI_F_COUNT; ++q)
+        if (m->base[q]) (void)hipFree(m->base[q]);
+    if (m->d_norm) (void)hipFree(m->d_norm);
+    if (m->h_norm) (void)hipHostFree(m->h_norm);
+    if (m->d_dem) (void)hipFree(m->d_dem);
+    if (m->stream) (void)hipStreamDestroy(m->stream);
+    delete m;
+}
+
+size_t interior_count(const Geo& g, int L)
+{
+    size_t c = 1;
+    for (int d = 0; d < g.nd; ++d) c *= (size_t)(g.n[d] + ((L >> d) & 1));
+    return c;
+}
+
+size_t parent_count(const Geo& g)
+{
+    size_t c = 1;
+    for (int d = 0; d < g.nd; ++d) c *= (size_t)(g.n[d] + 3);
+    return c;
+}
+
+} // namespace
+
+extern "C" {
+
+int fi_abi_version(void) { return FI_ABI_VERSION; }
+
+const char* fi_last_error(void) { return g_err; }
+
+int fi_model_create(const fi_model_desc* desc, fi_model_t* out)
+{
+    if (!out) return fail(FI_ERR_ARG, "null output handle");
+    *out = nullptr;
+    int rc = check_desc(desc);
+    if (rc) return rc;
+    if (desc->precision == FI_PREC_FP32_REFINED)
+        return fail(FI_ERR_UNSUPPORTED, "FP32-refined models are created through fi_refined_* entry points");
+    fi_model_s* m = new (std::nothrow) fi_model_s();
+    if (!m) return fail(FI_ERR_OOM, "host allocation failed");
+    m->desc = *desc;
+    Geo& g = m->g;
+    g.nd = desc->ndims;
+    for (int d = 0; d < 3; ++d) {
+        if (d < g.nd) {
+            g.n[d] = desc->n[d];
+            g.dx[d] = desc->extent[d] / (double)desc->n[d];
+            g.idx[d] = 1.0 / g.dx[d];
+        } else {
+            g.n[d] = 0;
+            g.dx[d] = 1.0;
+            g.idx[d] = 1.0;
+        }
+    }
+    const long align = 16; // doubles per 128-byte line
+    g.sy = ((long)(g.n[0] + 3) + align - 1) / align * align;
+    g.sz = (g.nd == 3) ? g.sy * (g.n[1] + 3) : 0;
+    g.off = align - 1;
+    const long rows = (g.nd == 3) ? (long)(g.n[1] + 3) * (g.n[2] + 3) : (long)(g.n[1] + 3);
+    g.elems = (size_t)(g.off + g.sy * rows + align);
+    if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess) {
+        free_model(m);
+        return fail(FI_ERR_HIP, "hipStreamCreate failed");
+    }
+    for (int id = 0; id < FI_F_COUNT; ++id) {
+        if (!field_used(*desc, id)) continue;
+        if (hipMalloc(&m->base[id], g.elems * sizeof(double)) != hipSuccess) {
+            free_model(m);
+            return fail(FI_ERR_OOM, "device allocation of field %d (%zu bytes) failed", id, g.elems * sizeof(double));
+        }
+        m->f[id] = m->base[id] + g.off;
+        if (hipMemsetAsync(m->base[id], 0, g.elems * sizeof(double), m->stream) != hipSuccess) {
+            free_model(m);
+            return fail(FI_ERR_HIP, "hipMemsetAsync failed");
+        }
+    }
+    if (hipMalloc(&m->d_norm, 4 * sizeof(unsigned long long)) != hipSuccess ||
+        hipHostMalloc(&m->h_norm, 4 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess) {
+        free_model(m);
+        return fail(FI_ERR_OOM, "norm buffers");
+    }
+    if (hipStreamSynchronize(m->stream) != hipSuccess) {
+        free_model(m);
+        return fail(FI_ERR_HIP, "stream sync after allocation failed");
+    }
+    *out = m;
+    return FI_OK;
+}
+
+int fi_model_destroy(fi_model_t m)
+{
+    if (!m) return FI_OK;
+    (void)hipStreamSynchronize(m->stream);
+    free_model(m);
+    return FI_OK;
+}
+
+size_t fi_field_size(fi_model_t m, int id, int layout)
+{
+    if (!m || id < 0 || id >= FI_F_COUNT || !m->f[id]) return 0;
+    if (layout == FI_LAYOUT_PARENT) return parent_count(m->g);
+    int L = field_loc(m->g.nd, id);
+    return L < 0 ? 0 : interior_count(m->g, L);
+}
+
+static int field_copy(fi_model_t m, int id, double* host, size_t count, int layout, bool up)
+{
+    if (!m) return fail(FI_ERR_ARG, "null model");
+    if (id < 0 || id >= FI_F_COUNT || !m->f[id]) return fail(FI_ERR_ARG, "field %d not present in this model", id);
+    if (!host) return fail(FI_ERR_ARG, "null host pointer");
+    size_t need = fi_field_size(m, id, layout);
+    if (layout != FI_LAYOUT_PARENT && layout != FI_LAYOUT_INTERIOR) return fail(FI_ERR_ARG, "bad layout %d", layout);
+    if (count != need) return fail(FI_ERR_ARG, "field %d needs %zu doubles in this layout, got %zu", id, need, count);
+    const Geo& g = m->g;
+    double* dev = m->buf(id);
+    HIPCHK(hipStreamSynchronize(m->stream));
+    if (layout == FI_LAYOUT_PARENT) {
+        const size_t w = (size_t)(g.n[0] + 3) * sizeof(double);
+        const size_t rows = (g.nd == 3) ? (size_t)(g.n[1] + 3) * (g.n[2] + 3) : (size_t)(g.n[1] + 3);
+        if (up) HIPCHK(hipMemcpy2D(dev, g.sy * sizeof(double), host, w, w, rows, hipMemcpyHostToDevice));
+        else HIPCHK(hipMemcpy2D(host, w, dev, g.sy * sizeof(double), w, rows, hipMemcpyDeviceToHost));
+    } else {
+        const int L = field_loc(g.nd, id);
+        const int mx = g.n[0] + (L & 1), my = g.n[1] + ((L >> 1) & 1);
+        const int mz = (g.nd == 3) ? g.n[2] + ((L >> 2) & 1) : 1;
+        const size_t w = (size_t)mx * sizeof(double);
+        for (int k = 0; k < mz; ++k) {
+            double* d0 = dev + 1 + g.sy + ((g.nd == 3) ? g.sz * (k + 1) : 0);
+            double* h0 = host + (size_t)k * mx * my;
+            if (up) HIPCHK(hipMemcpy2D(d0, g.sy * sizeof(double), h0, w, w, my, hipMemcpyHostToDevice));
+            else HIPCHK(hipMemcpy2D(h0, w, d0, g.sy * sizeof(double), w, my, hipMemcpyDeviceToHost));
+        }
+    }
+    return FI_OK;
+}
+
+int fi_field_upload(fi_model_t m, int id, const double* host, size_t count, int layout)
+{
+    return field_copy(m, id, const_cast<double*>(host), count, layout, true);
+}
+
+int fi_field_download(fi_model_t m, int id, double* host, size_t count, int layout)
+{
+    return field_copy(m, id, host, count, layout, false);
+}
+
+int fi_field_fill(fi_model_t m, int id, double value)
+{
+    if (!m) return fail(FI_ERR_ARG, "null model");
+    if (id < 0 || id >= FI_F_COUNT || !m->f[id]) return fail(FI_ERR_ARG, "field %d not present in this model", id);
+    double* p = m->buf(id);
+    // the whole allocation, i.e. the parent box including halos and row padding
+    launch_fill(p - m->g.off, m->g.elems, value, m->stream);
+    HIPCHK(hipGetLastError());
+    return FI_OK;
+}
+
+int fi_apply_bc(fi_model_t m, int group)
+{
+    if (!m) return fail(FI_ERR_ARG, "null model");
+    if (group < FI_BCGROUP_STRESS || group > FI_BCGROUP_RESIDUAL) return fail(FI_ERR_ARG, "bad bc group %d", group);
+    if (group == FI_BCGROUP_ETA) {
+        // bc!(eta => Neumann()) and bc!(eta_next => Neumann()) (tm_stokes_chmy.jl:93-94)
+        for (int d = 0; d < m->g.nd; ++d) {
+            launch_bc_dim(m->g, neumann_batch(m->eta(), L_CCC, d), m->stream);
+        }
+        for (int d = 0; d < m->g.nd; ++d) {
+            launch_bc_dim(m->g, neumann_batch(m->eta_next(), L_CCC, d), m->stream);
+        }
+    } else if (group == FI_BCGROUP_VELOCITY) {
+        for (int d = 0; d < m->g.nd; ++d) launch_bc_dim(m->g, velocity_batch(m, d, false), m->stream);
+    } else {
+        run_bc(m, group);
+    }
+    HIPCHK(hipGetLastError());
+    return FI_OK;
+}
+
+int fi_advance_iterations(fi_model_t m, int n_iter)
+{
+    if (!m) return fail(FI_ERR_ARG, "null model");
+    if (n_iter < 0) return fail(FI_ERR_ARG, "negative iteration count");
+    advance(m, n_iter);
+    HIPCHK(hipGetLastError());
+    return FI_OK;
+}
+
+int fi_compute_residuals(fi_model_t m, double norms_inf[4])
+{
+    if (!m || !norms_inf) return fail(FI_ERR_ARG, "null argument");
+    int rc = residual_norms(m, norms_inf);
+    if (rc) return rc;
+    HIPCHK(hipGetLastError());
+    return FI_OK;
+}
+
+int fi_compute_tau(fi_model_t m)
+{
+    if (!m) return fail(FI_ERR_ARG, "null model");
+    StokesArgs<double> a = make_args(m);
+    launch_tau(m->g, a, m->desc.immersed_boundary != 0, m->stream);
+    HIPCHK(hipGetLastError());
+    return FI_OK;
+}
+
+int fi_solve(fi_model_t m, int max_iter, int ncheck, double tol, int* iters_out, double norms_inf[4])
+{
+    if (!m || !norms_inf) return fail(FI_ERR_ARG, "null argument");
+    if (max_iter < 0 || ncheck < 1) return fail(FI_ERR_ARG, "bad max_iter/ncheck");
+    int done = 0;
+    for (int q = 0; q < 4; ++q) norms_inf[q] = 0.0;
+    while (done < max_iter) {
+        int chunk = ncheck - (done % ncheck);
+        if (done + chunk > max_iter) chunk = max_iter - done;
+        advance(m, chunk);
+        done += chunk;
+        if (done % ncheck == 0) {
+            int rc = residual_norms(m, norms_inf);
+            if (rc) return rc;
+            bool conv = true;
+            for (int q = 0; q < 4; ++q) {
+                if (!std::isfinite(norms_inf[q])) {
+                    if (iters_out) *iters_out = done;
+                    return fail(FI_ERR_NONFINITE, "simulation failed at iteration %d: non-finite residual", done);
+                }
+                if (!(norms_inf[q] < tol)) conv = false;
+            }
+            if (conv) break;
+        }
+    }
+    if (iters_out) *iters_out = done;
+    HIPCHK(hipStreamSynchronize(m->stream));
+    return FI_OK;
+}
+
+int fi_time_iterations(fi_model_t m, int n_iter, double out_ms[4])
+{
+    if (!m || !out_ms || n_iter < 1) return fail(FI_ERR_ARG, "bad arguments");
+    const bool ib = m->desc.immersed_boundary != 0;
+    hipEvent_t ev[5];
+    for (int q = 0; q < 5; ++q) HIPCHK(hipEventCreate(&ev[q]));
+    double acc[4] = {0, 0, 0, 0};
+    for (int it = 0; it < n_iter; ++it) {
+        StokesArgs<double> a = make_args(m);
+        HIPCHK(hipEventRecord(ev[0], m->stream));
+        launch_sigma(m->g, a, ib, m->stream);
+        HIPCHK(hipEventRecord(ev[1], m->stream));
+        run_bc(m, FI_BCGROUP_STRESS);
+        HIPCHK(hipEventRecord(ev[2], m->stream));
+        launch_velocity(m->g, a, ib, m->stream);
+        HIPCHK(hipEventRecord(ev[3], m->stream));
+        run_bc(m, FI_BCGROUP_VELOCITY);
+        HIPCHK(hipEventRecord(ev[4], m->stream));
+        m->eta_cur ^= 1;
+        HIPCHK(hipEventSynchronize(ev[4]));
+        float t;
+        HIPCHK(hipEventElapsedTime(&t, ev[0], ev[4]));
+        acc[0] += t;
+        HIPCHK(hipEventElapsedTime(&t, ev[0], ev[1]));
+        acc[1] += t;
+        HIPCHK(hipEventElapsedTime(&t, ev[2], ev[3]));
+        acc[2] += t;
+        float t1, t2;
+        HIPCHK(hipEventElapsedTime(&t1, ev[1], ev[2]));
+        HIPCHK(hipEventElapsedTime(&t2, ev[3], ev[4]));
+        acc[3] += t1 + t2;
+    }
+    for (int q = 0; q < 5; ++q) (void)hipEventDestroy(ev[q]);
+    for (int q = 0; q < 4; ++q) out_ms[q] = acc[q] / n_iter;
+    return FI_OK;
+}
+
+int fi_synchronize(fi_model_t m)
+{
+    if (!m) return fail(FI_ERR_ARG, "null model");
+    HIPCHK(hipStreamSynchronize(m->stream));
+    return FI_OK;
+}
+
+int fi_levelset_from_dem(fi_model_t m, int phase, const double* dem_parent, int dem_nx, int dem_ny,
+                         const double dem_origin[2], const double dem_extent[2], const double R[9])
+{
+    if (!m) return fail(FI_ERR_ARG, "null model");
+    if (!m->desc.immersed_boundary) return fail(FI_ERR_STATE, "model has no immersed boundary");
+    if (m->g.nd != 3) return fail(FI_ERR_UNSUPPORTED, "level sets from a DEM are defined for 3D grids");
+    if (phase != FI_PHASE_NA && phase != FI_PHASE_NS) return fail(FI_ERR_ARG, "bad phase %d", phase);
+    if (!dem_parent || dem_nx < 1 || dem_ny < 1 || !dem_origin || !dem_extent) return fail(FI_ERR_ARG, "bad DEM");
+    if (!(dem_extent[0] > 0.0) || !(dem_extent[1] > 0.0)) return fail(FI_ERR_ARG, "bad DEM extent");
+    const size_t elems = (size_t)(dem_nx + 3) * (dem_ny + 3);
+    if (elems > m->dem_elems) {
+        if (m->d_dem) HIPCHK(hipFree(m->d_dem));
+        m->d_dem = nullptr;
+        m->dem_elems = 0;
+        HIPCHK(hipMalloc(&m->d_dem, elems * sizeof(double)));
+        m->dem_elems = elems;
+    }
+    HIPCHK(hipStreamSynchronize(m->stream));
+    HIPCHK(hipMemcpy(m->d_dem, dem_parent, elems * sizeof(double), hipMemcpyHostToDevice));
+    DemArgs a;
+    a.dem = m->d_dem;
+    a.nv[0] = dem_nx + 1;
+    a.nv[1] = dem_ny + 1;
+    a.sy = dem_nx + 3;
+    for (int d = 0; d < 2; ++d) {
+        a.o[d] = dem_origin[d];
+        a.ext[d] = dem_extent[d];
+        a.d[d] = dem_extent[d] / (double)(d == 0 ? dem_nx : dem_ny);
+    }
+    a.has_R = R ? 1 : 0;
+    for (int q = 0; q < 9; ++q) a.R[q] = R ? R[q] : (q % 4 == 0 ? 1.0 : 0.0);
+    double* psi = m->f[phase == FI_PHASE_NA ? FI_F_PSI_NA : FI_F_PSI_NS];
+    launch_levelset_from_dem(m->g, m->desc.origin, psi, a, m->stream);
+    for (int d = 0; d < 3; ++d) launch_bc_dim(m->g, neumann_batch(psi, L_VVV, d), m->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(m->stream));
+    return FI_OK;
+}
+
+int fi_invert_levelset(fi_model_t m, int phase)
+{
+    if (!m) return fail(FI_ERR_ARG, "null model");
+    if (!m->desc.immersed_boundary) return fail(FI_ERR_STATE, "model has no immersed boundary");
+    if (phase != FI_PHASE_NA && phase != FI_PHASE_NS) return fail(FI_ERR_ARG, "bad phase %d", phase);
+    double* psi = m->f[phase == FI_PHASE_NA ? FI_F_PSI_NA : FI_F_PSI_NS];
+    launch_negate_vertex(m->g, psi, m->stream);
+    const int L = m->g.nd == 3 ? L_VVV : L_VVC;
+    for (int d = 0; d < m->g.nd; ++d) launch_bc_dim(m->g, neumann_batch(psi, L, d), m->stream);
+    HIPCHK(hipGetLastError());
+    return FI_OK;
+}
+
+int fi_omega_from_psi(fi_model_t m, int phase)
+{
+    if (!m) return fail(FI_ERR_ARG, "null model");
+    if (!m->desc.immersed_boundary) return fail(FI_ERR_STATE, "model has no immersed boundary");
+    if (phase != FI_PHASE_NA && phase != FI_PHASE_NS) return fail(FI_ERR_ARG, "bad phase %d", phase);
+    const double* psi = m->f[phase == FI_PHASE_NA ? FI_F_PSI_NA : FI_F_PSI_NS];
+    double* om[8];
+    const int base = phase == FI_PHASE_NA ? FI_F_OM_NA0 : FI_F_OM_NS0;
+    for (int q = 0; q < 8; ++q) om[q] = m->f[base + q];
+    launch_omega_from_psi(m->g, psi, om, m->stream);
+    HIPCHK(hipGetLastError());
+    return FI_OK;
+}
+
+int fi_mem_info(size_t* free_bytes, size_t* total_bytes)
+{
+    size_t f = 0, t = 0;
+    HIPCHK(hipMemGetInfo(&f, &t));
+    if (free_bytes) *free_bytes = f;
+    if (total_bytes) *total_bytes = t;
+    return FI_OK;
+}
+
+} // extern "C"

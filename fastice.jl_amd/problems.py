@@ -1,0 +1,158 @@
+"""Problem set-ups of the reference drivers (the callers of the PT path).
+
+pt_params:        PT parameters, stokes_manufactured_solution_free_slip_3D.jl:92-101
+                  and test_vavilov.jl:266-274.
+manufactured_3d:  confined free-slip flow, stokes_manufactured_solution_free_slip_3D.jl:25-48,70-89.
+manufactured_2d:  stokes_manufactured_solution_free_slip_2D.jl:24-32,264-281.
+manufactured_errors: relative L-inf/L1/L2 errors of |V| and tau_II against the
+                  closed form, ..._3D.jl:140-167 / ..._2D.jl:331-352.
+"""
+import math
+
+import numpy as np
+
+from .full_stokes import SBC
+from .grids import UniformGrid
+from .physics import LinearViscousRheology
+
+PI = math.pi
+
+
+def pt_params(grid, r=0.7, re_mech=8 * PI, vdtau_div=1.1):
+    """solver_params = (Δτ = (Pr, τ, V),) from the reference's PT parameter formulas."""
+    nd = grid.ndims
+    lt_re_m = min(grid.extent_) / re_mech
+    vdtau = min(grid.spacing_) / math.sqrt(nd * vdtau_div)
+    theta_dtau = lt_re_m * (r + 4 / 3) / vdtau
+    dtau_r = 1.0 / (theta_dtau + 1.0)
+    nudtau = vdtau * lt_re_m
+    names = ("xx", "yy", "zz", "xy", "xz", "yz") if nd == 3 else ("xx", "yy", "xy")
+    return {"Δτ": {"Pr": r / theta_dtau, "τ": {k: dtau_r for k in names},
+                   "V": {k: nudtau for k in ("x", "y", "z")[:nd]}}}
+
+
+# closed forms, 3D (..._3D.jl:27-48)
+def _f(a, b):
+    return np.cos(PI * a) * (b ** 2 - 1) ** 2 - np.cos(PI * b) * (a ** 2 - 1) ** 2
+
+
+def _g(a, b):
+    return np.sin(PI * b) * (a ** 2 - 1) * a - np.sin(PI * a) * (b ** 2 - 1) * b
+
+
+def _p(a, b):
+    return np.cos(PI * b) * (1 - 3 * a ** 2) * 2 - np.cos(PI * a) * (1 - 3 * b ** 2) * 2
+
+
+EXACT3 = {
+    "vx": lambda x, y, z: np.sin(PI * x) * _f(y, z),
+    "vy": lambda x, y, z: np.sin(PI * y) * _f(z, x),
+    "vz": lambda x, y, z: np.sin(PI * z) * _f(x, y),
+    "txx": lambda x, y, z, eta=1.0: 2 * eta * PI * np.cos(PI * x) * _f(y, z),
+    "tyy": lambda x, y, z, eta=1.0: 2 * eta * PI * np.cos(PI * y) * _f(z, x),
+    "tzz": lambda x, y, z, eta=1.0: 2 * eta * PI * np.cos(PI * z) * _f(x, y),
+    "txy": lambda x, y, z, eta=1.0: 4 * eta * np.cos(PI * z) * _g(x, y),
+    "txz": lambda x, y, z, eta=1.0: 4 * eta * np.cos(PI * y) * _g(z, x),
+    "tyz": lambda x, y, z, eta=1.0: 4 * eta * np.cos(PI * x) * _g(y, z),
+    "gx": lambda x, y, z, eta=1.0: -2 * eta * np.sin(PI * x) * (_f(y, z) * PI ** 2 - _p(y, z)),
+    "gy": lambda x, y, z, eta=1.0: -2 * eta * np.sin(PI * y) * (_f(z, x) * PI ** 2 - _p(z, x)),
+    "gz": lambda x, y, z, eta=1.0: -2 * eta * np.sin(PI * z) * (_f(x, y) * PI ** 2 - _p(x, y)),
+}
+
+# closed forms, 2D (..._2D.jl:24-32)
+EXACT2 = {
+    "vx": lambda x, y: np.sin(PI * x) * np.cos(PI * y),
+    "vy": lambda x, y: -np.cos(PI * x) * np.sin(PI * y),
+    "txx": lambda x, y, eta=1.0: 2 * eta * PI * np.cos(PI * x) * np.cos(PI * y),
+    "tyy": lambda x, y, eta=1.0: -2 * eta * PI * np.cos(PI * x) * np.cos(PI * y),
+    "txy": lambda x, y, eta=1.0: np.zeros_like(x),
+    "gx": lambda x, y, eta=1.0: -2 * eta * PI ** 2 * np.sin(PI * x) * np.cos(PI * y),
+    "gy": lambda x, y, eta=1.0: 2 * eta * PI ** 2 * np.cos(PI * x) * np.sin(PI * y),
+}
+
+
+def manufactured_3d(n, origin=(-1.0, -1.0, -1.0), extent=(2.0, 2.0, 2.0)):
+    """Model arguments and driver cadence of the 3D free-slip manufactured solution."""
+    dims = (n, n, n) if np.isscalar(n) else tuple(n)
+    grid = UniformGrid(origin=origin, extent=extent, dims=dims)
+    fs = SBC(0.0, 0.0, 0.0)
+    return dict(
+        grid=grid,
+        boundary_conditions=dict(x=(fs, fs), y=(fs, fs), z=(fs, fs)),
+        gravity=dict(x=EXACT3["gx"], y=EXACT3["gy"], z=EXACT3["gz"]),
+        rheology=LinearViscousRheology(1.0),
+        solver_params=pt_params(grid, r=0.7, re_mech=8 * PI, vdtau_div=1.1),
+    ), dict(niter=10 * max(dims), ncheck=max(dims))
+
+
+def manufactured_2d(n, origin=(-1.0, -1.0), extent=(2.0, 2.0)):
+    dims = (n, n) if np.isscalar(n) else tuple(n)
+    grid = UniformGrid(origin=origin, extent=extent, dims=dims)
+    fs = SBC(0.0, 0.0)
+    return dict(
+        grid=grid,
+        boundary_conditions=dict(x=(fs, fs), y=(fs, fs)),
+        gravity=dict(x=EXACT2["gx"], y=EXACT2["gy"]),
+        rheology=LinearViscousRheology(1.0),
+        solver_params=pt_params(grid, r=0.7, re_mech=8 * PI, vdtau_div=1.1),
+    ), dict(niter=10 * max(dims), ncheck=max(dims))
+
+
+def _avg(a, axis):
+    s0 = [slice(None)] * a.ndim
+    s1 = [slice(None)] * a.ndim
+    s0[axis] = slice(0, -1)
+    s1[axis] = slice(1, None)
+    return 0.5 * (a[tuple(s0)] + a[tuple(s1)])
+
+
+def _av4(a, ax0, ax1):
+    s = lambda i0, i1: tuple(slice(i0, a.shape[d] - 1 + i0) if d == ax0 else
+                             (slice(i1, a.shape[d] - 1 + i1) if d == ax1 else slice(None)) for d in range(a.ndim))
+    return 0.25 * (a[s(0, 0)] + a[s(1, 0)] + a[s(1, 1)] + a[s(0, 1)])
+
+
+def magnitudes(grid, V, tau):
+    """|V| (face -> centre averages) and tau_II at cell centres (..._3D.jl:154-158)."""
+    nd = grid.ndims
+    if nd == 3:
+        vm = np.sqrt(_avg(V["x"], 0) ** 2 + _avg(V["y"], 1) ** 2 + _avg(V["z"], 2) ** 2)
+        tm = np.sqrt(0.5 * (tau["xx"] ** 2 + tau["yy"] ** 2 + tau["zz"] ** 2) + _av4(tau["xy"], 0, 1) ** 2 +
+                     _av4(tau["xz"], 0, 2) ** 2 + _av4(tau["yz"], 1, 2) ** 2)
+    else:
+        vm = np.sqrt(_avg(V["x"], 0) ** 2 + _avg(V["y"], 1) ** 2)
+        tm = np.sqrt(0.5 * (tau["xx"] ** 2 + tau["yy"] ** 2) + _av4(tau["xy"], 0, 1) ** 2)
+    return vm, tm
+
+
+def exact_fields(grid):
+    """Closed-form V and tau sampled at the interior staggered locations."""
+    nd = grid.ndims
+    E = EXACT3 if nd == 3 else EXACT2
+    locs3 = {"x": 0b001, "y": 0b010, "z": 0b100, "xx": 0, "yy": 0, "zz": 0, "xy": 0b011, "xz": 0b101, "yz": 0b110}
+    locs2 = {"x": 0b01, "y": 0b10, "xx": 0, "yy": 0, "xy": 0b11}
+    locs = locs3 if nd == 3 else locs2
+
+    def axis(d, vertex):
+        if vertex:
+            return grid.origin[d] + (np.arange(1, grid.dims[d] + 2) - 1.0) * grid.spacing_[d]
+        return grid.origin[d] + (np.arange(1, grid.dims[d] + 1) - 0.5) * grid.spacing_[d]
+
+    def sample(fn, bits):
+        axes = [axis(d, (bits >> d) & 1) for d in range(nd)]
+        return fn(*np.meshgrid(*axes, indexing="ij"))
+
+    V = {k: sample(E["v" + k], locs[k]) for k in ("x", "y", "z")[:nd]}
+    tau = {k: sample(E["t" + k], locs[k]) for k in (("xx", "yy", "zz", "xy", "xz", "yz") if nd == 3 else ("xx", "yy", "xy"))}
+    return V, tau
+
+
+def manufactured_errors(grid, V, tau):
+    """(Linf_v, Linf_tau, L1_v, L1_tau, L2_v, L2_tau), each relative to the closed form."""
+    Ve, te = exact_fields(grid)
+    vm, tm = magnitudes(grid, Ve, te)
+    v, t = magnitudes(grid, V, tau)
+    n = lambda a, p: np.linalg.norm(a.ravel(), p)
+    return (n(vm - v, np.inf) / n(vm, np.inf), n(tm - t, np.inf) / n(tm, np.inf),
+            n(vm - v, 1) / n(vm, 1), n(tm - t, 1) / n(tm, 1),
+            n(vm - v, 2) / n(vm, 2), n(tm - t, 2) / n(tm, 2))

@@ -1,0 +1,171 @@
+/*
+ * fastice_hip.h -- C ABI of libfasticehip.so, the MI355X (gfx950) implementation
+ * of FastIce.jl's pseudo-transient (PT) Stokes hot path.
+ *
+ * Reference interface replaced (FastIce.jl v0.3.3; paths relative to the
+ * FastIce.jl source tree).  The reference has no FFI: its boundary is Julia
+ * multiple dispatch on the Models / LevelSets API, which a Julia package would
+ * re-expose by `ccall`ing the functions below.
+ *
+ *   fi_model_create      <- IsothermalFullStokesModel(; arch, grid, boundary_conditions,
+ *                           gravity, rheology, solver_params, outer_width)
+ *                           src/Models/FullStokes/Isothermal/Isothermal.jl:60-89 and
+ *                           IsothermalImmersedBoundaryFullStokesModel(...)
+ *                           src/Models/ImmersedBoundaryFullStokes/Isothermal/Isothermal.jl:63-98
+ *   fi_model_destroy     <- garbage collection of the model's device Fields
+ *   fi_field_upload      <- set!(field, array) on model.stress/velocity/viscosity fields
+ *                           (scripts_future_API/tm_stokes_chmy.jl:86-91)
+ *   fi_field_download    <- interior(field) / Array(field) reads
+ *                           (scripts_future_API/tm_stokes_chmy.jl:110-113)
+ *   fi_field_fill        <- set!(field, value) (tm_stokes_chmy.jl:86-88)
+ *   fi_apply_bc          <- bc!(arch, grid, batch) for stress / velocity / eta groups
+ *                           (tm_stokes_chmy.jl:93-94; Isothermal.jl:102-104)
+ *   fi_advance_iterations<- advance_iteration!(model, t, dt), n times
+ *                           (FullStokes Isothermal.jl:91-112; IB Isothermal.jl:100-122)
+ *   fi_compute_residuals <- compute_residuals!(model) + norm(r, Inf) per residual
+ *                           (Isothermal.jl:114-125; tm_stokes_chmy.jl:99-106)
+ *   fi_solve             <- the driver loop of tm_stokes_chmy.jl:96-108 with a tolerance exit
+ *   fi_compute_tau       <- compute_tau! helper kernel (kernels_3d.jl:171-191)
+ *   fi_levelset_from_dem <- compute_levelset_from_dem!(arch, launch, psi, dem, grid2D, grid, R)
+ *                           (src/LevelSets/compute_levelsets.jl:16-25)
+ *   fi_invert_levelset   <- invert_levelset!(arch, launch, psi, grid) (src/LevelSets/compute_levelsets.jl:27-35)
+ *   fi_omega_from_psi    <- omega_from_psi!(arch, launch, mask, psi, grid)
+ *                           (src/LevelSets/volume_fractions_kernels.jl:101-110)
+ *
+ * Conventions
+ *   - Every call returns FI_OK (0) or a negative fi_status; fi_last_error()
+ *     returns a thread-local message for the last failure.
+ *   - The library owns all device memory; host pointers are borrowed for the
+ *     duration of a call.  One handle must not be used from two threads at once;
+ *     distinct handles are independent.  Each handle owns one HIP stream.
+ *   - Calls that return host data (downloads, norms, fi_solve) synchronise the
+ *     handle's stream; the others only enqueue work.
+ *   - Field arrays on the host are x-fastest (Julia column-major order).  Layout
+ *     FI_LAYOUT_PARENT is the field with its 1-cell halo in a uniform
+ *     (nx+3)*(ny+3)*(nz+3) box (2D: (nx+3)*(ny+3)) for every staggered
+ *     location; element (i,j,k) of the 1-based Chmy index is at
+ *     i + (nx+3)*j + (nx+3)*(ny+3)*k.  FI_LAYOUT_INTERIOR is the interior only,
+ *     (mx,my,mz) with m = n for Center and n+1 for Vertex dimensions.
+ */
+#ifndef FASTICE_HIP_H
+#define FASTICE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FI_ABI_VERSION 1
+
+typedef enum fi_status {
+    FI_OK = 0,
+    FI_ERR_ARG = -1,
+    FI_ERR_OOM = -2,
+    FI_ERR_HIP = -3,
+    FI_ERR_NONFINITE = -4, /* tm_stokes_chmy.jl:104-106 "simulation failed" */
+    FI_ERR_UNSUPPORTED = -5,
+    FI_ERR_STATE = -6
+} fi_status;
+
+/* BoundaryCondition{Traction|Velocity|Slip}: boundary_conditions.jl:1-13 */
+enum { FI_BC_TRACTION = 0, FI_BC_VELOCITY = 1, FI_BC_SLIP = 2 };
+
+/* rheologies: src/Physics.jl:7-47 */
+enum { FI_RHEO_LINEAR_CONST = 0, FI_RHEO_LINEAR_FIELD = 1, FI_RHEO_POWERLAW = 2 };
+enum { FI_RELAX_NONE = 0, FI_RELAX_LIN = 1, FI_RELAX_LOG = 2, FI_RELAX_LOG10 = 3 };
+
+enum { FI_PREC_FP64 = 0, FI_PREC_FP32_REFINED = 1 };
+
+enum { FI_LAYOUT_PARENT = 0, FI_LAYOUT_INTERIOR = 1 };
+
+/* fi_apply_bc groups */
+enum { FI_BCGROUP_STRESS = 0, FI_BCGROUP_VELOCITY = 1, FI_BCGROUP_ETA = 2, FI_BCGROUP_RESIDUAL = 3 };
+
+/* field identifiers (2D models use the x/y members only) */
+enum {
+    FI_F_P = 0,
+    FI_F_TXX, FI_F_TYY, FI_F_TZZ, FI_F_TXY, FI_F_TXZ, FI_F_TYZ,
+    FI_F_VX, FI_F_VY, FI_F_VZ,
+    FI_F_ETA, FI_F_ETA_NEXT,
+    FI_F_RP, FI_F_RVX, FI_F_RVY, FI_F_RVZ,
+    FI_F_GX, FI_F_GY, FI_F_GZ,  /* gravity fields when gravity_is_field */
+    FI_F_RHEO,                  /* LinearViscousRheology eta field / PowerLawRheology A field */
+    FI_F_PSI_NA, FI_F_PSI_NS,   /* level sets at vertices */
+    FI_F_OM_NA0,                /* + component: ccc vvv vcc cvc ccv vvc vcv cvv (2D: cc vv vc cv) */
+    FI_F_OM_NS0 = FI_F_OM_NA0 + 8,
+    FI_F_COUNT = FI_F_OM_NS0 + 8
+};
+
+/* mask phases */
+enum { FI_PHASE_NA = 0, FI_PHASE_NS = 1 };
+
+typedef struct fi_rheology {
+    int32_t kind;       /* FI_RHEO_* */
+    int32_t has_reg;    /* wrap in ViscoplasticRegularisation(inner, eta_reg) */
+    int32_t relax;      /* wrap in Relaxation(eta, inner, rate; scale) */
+    int32_t A_is_field; /* POWERLAW: A read from FI_F_RHEO */
+    double eta;         /* LINEAR_CONST viscosity */
+    double A;           /* POWERLAW prefactor (constant) */
+    double n;           /* POWERLAW exponent */
+    double eta_reg;     /* regularisation viscosity */
+    double rate;        /* relaxation rate */
+} fi_rheology;
+
+typedef struct fi_model_desc {
+    int32_t ndims;              /* 2 or 3 */
+    int32_t n[3];               /* cells per dimension */
+    double origin[3];
+    double extent[3];
+    int32_t bc_kind[3][2];      /* [dim][low/high] FI_BC_* */
+    double bc_value[3][2][3];   /* components of BoundaryCondition{Kind}(c1, c2, c3) */
+    fi_rheology rheology;
+    int32_t gravity_is_field;   /* 0: constant vector (ValueField/ZeroField), 1: FI_F_G* fields */
+    double gravity[3];
+    double dtau_P;              /* solver_params.dtau.Pr */
+    double dtau_tau[6];         /* xx yy zz xy xz yz (2D: xx yy xy) */
+    double dtau_V[3];
+    int32_t immersed_boundary;  /* 1: IsothermalImmersedBoundaryFullStokesModel */
+    int32_t precision;          /* FI_PREC_* */
+} fi_model_desc;
+
+typedef struct fi_model_s* fi_model_t;
+
+int fi_abi_version(void);
+const char* fi_last_error(void);
+
+int fi_model_create(const fi_model_desc* desc, fi_model_t* out);
+int fi_model_destroy(fi_model_t m);
+
+int fi_field_upload(fi_model_t m, int field_id, const double* host, size_t count, int layout);
+int fi_field_download(fi_model_t m, int field_id, double* host, size_t count, int layout);
+int fi_field_fill(fi_model_t m, int field_id, double value);
+/* number of doubles of a field in the given layout (0 on a bad id) */
+size_t fi_field_size(fi_model_t m, int field_id, int layout);
+
+int fi_apply_bc(fi_model_t m, int group);
+int fi_advance_iterations(fi_model_t m, int n_iter);
+int fi_compute_residuals(fi_model_t m, double norms_inf[4]);
+int fi_compute_tau(fi_model_t m);
+int fi_solve(fi_model_t m, int max_iter, int ncheck, double tol, int* iters_out, double norms_inf[4]);
+int fi_synchronize(fi_model_t m);
+/* Measurement hook (no counterpart in the reference, whose timing is host
+ * wall-clock around the loop, scripts_future_API/tm_stokes_mpi_wip.jl:151-183):
+ * runs n_iter iterations with HIP events around every launch on the model's
+ * stream, synchronising after each iteration.  out_ms = {iteration, stress
+ * kernel, velocity kernel, boundary-condition kernels} averages in ms. */
+int fi_time_iterations(fi_model_t m, int n_iter, double out_ms[4]);
+
+int fi_levelset_from_dem(fi_model_t m, int phase, const double* dem_parent, int dem_nx, int dem_ny,
+                         const double dem_origin[2], const double dem_extent[2], const double R[9]);
+int fi_invert_levelset(fi_model_t m, int phase);
+int fi_omega_from_psi(fi_model_t m, int phase);
+
+/* free / total device memory in bytes (hipMemGetInfo) */
+int fi_mem_info(size_t* free_bytes, size_t* total_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,105 @@
+/*
+ * fastice_oracle.h -- CPU FP64 restatement of FastIce.jl's PT-Stokes path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP
+ * library is compared against; it is never linked into, loaded by, or
+ * called from the product path (libfasticehip.so and the Python host
+ * package).  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg use it.
+ *
+ * Storage convention (shared with the HIP library's "PARENT" layout):
+ * every field, whatever its staggered location, is a dense x-fastest
+ * array of (nx+3)*(ny+3)*(nz+3) doubles (2D: (nx+3)*(ny+3)).  The storage
+ * index of the 1-based Chmy index (i,j,k) is i + sy*j + sz*k with
+ * sy = nx+3, sz = (nx+3)*(ny+3); index 0 is the low halo, 1..m the
+ * interior (m = n for Center, n+1 for Vertex), m+1 the high halo.
+ */
+#ifndef FASTICE_ORACLE_H
+#define FASTICE_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { OR_BC_TRACTION = 0, OR_BC_VELOCITY = 1, OR_BC_SLIP = 2 };
+enum { OR_RHEO_LINEAR_CONST = 0, OR_RHEO_LINEAR_FIELD = 1, OR_RHEO_POWERLAW = 2 };
+enum { OR_RELAX_NONE = 0, OR_RELAX_LIN = 1, OR_RELAX_LOG = 2, OR_RELAX_LOG10 = 3 };
+
+/* Mask component order (3D FieldMask): ccc vvv vcc cvc ccv vvc vcv cvv.
+ * 2D: cc vv vc cv. */
+enum { OM_CCC = 0, OM_VVV, OM_VCC, OM_CVC, OM_CCV, OM_VVC, OM_VCV, OM_CVV };
+enum { OM2_CC = 0, OM2_VV, OM2_VC, OM2_CV };
+
+typedef struct {
+    int ndims;
+    int n[3];
+    double origin[3];
+    double extent[3];
+    double dx[3];   /* spacing  = extent/n  */
+    double idx[3];  /* inverse spacing = 1/dx */
+    long sy, sz;
+    int bc_kind[3][2];
+    double bc_val[3][2][3];
+    double dPr;
+    double dtau[6];  /* xx yy zz xy xz yz  (2D: xx yy xy) */
+    double dV[3];
+    /* rheology: inner law, optional viscoplastic cap, optional relaxation */
+    int rh_kind;
+    double rh_eta;       /* LINEAR_CONST viscosity */
+    double rh_A;         /* POWERLAW constant A (used when rh_field is NULL) */
+    double rh_n;         /* POWERLAW exponent */
+    int rh_has_reg;
+    double rh_inv_eta_reg;
+    int rh_relax;
+    double rh_rate;
+    int grav_field;      /* 0: constant vector grav, 1: fields gx,gy,gz */
+    double grav[3];
+} or_model;
+
+typedef struct {
+    double *P, *txx, *tyy, *tzz, *txy, *txz, *tyz;
+    double *Vx, *Vy, *Vz;
+    double *eta, *eta_next;
+    double *rP, *rVx, *rVy, *rVz;
+    double *gx, *gy, *gz;
+    double *rh_field;         /* LINEAR_FIELD eta, or POWERLAW A field; NULL = constant */
+    double *na[8], *ns[8];    /* immersed-boundary masks (NULL when unused) */
+} or_fields;
+
+/* grid set-up: fills dx, idx, sy, sz from ndims, n, origin, extent */
+void or_init_grid(or_model* m);
+
+/* one PT iteration (update_sigma, stress BCs, update_V, velocity + eta BCs,
+ * eta swap), repeated niter times; ib != 0 selects the masked kernels */
+void or_advance(const or_model* m, or_fields* f, int niter, int ib);
+void or_update_sigma(const or_model* m, or_fields* f, int ib);
+void or_update_V(const or_model* m, or_fields* f, int ib);
+void or_bc_stress(const or_model* m, or_fields* f);
+void or_bc_velocity(const or_model* m, or_fields* f);
+void or_bc_neumann_center(const or_model* m, double* a);
+void or_bc_neumann_vertex(const or_model* m, double* a);
+void or_compute_residuals(const or_model* m, or_fields* f, int ib);
+void or_compute_tau(const or_model* m, or_fields* f, int ib);
+/* Inf-norms over the interior of rP, rVx, rVy(, rVz); NaN propagates */
+void or_residual_norms(const or_model* m, const or_fields* f, double out[4]);
+/* fixed-cadence solve: every ncheck iterations compute residuals and stop
+ * when all Inf-norms < tol; returns iterations done, -1 on non-finite */
+int or_solve(const or_model* m, or_fields* f, int max_iter, int ncheck, double tol,
+             int ib, double norms[4]);
+
+/* rheology closure at one cell (exposed for closed-form tests) */
+double or_rheology(const or_model* m, const or_fields* f, long c, double tII);
+
+/* level sets and volume fractions */
+void or_omega_from_psi(const or_model* m, const double* psi, double* om[8]);
+double or_psi2omega3(const double psi8[8], const double d[3]);
+double or_psi2omega2(const double psi4[4], const double d[2]);
+void or_levelset_from_dem(const or_model* m, double* psi, const double* dem,
+                          const int dem_n[2], const double dem_origin[2],
+                          const double dem_extent[2], const double R[9]);
+void or_invert_levelset(const or_model* m, double* psi);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,163 @@
+"""GPU parity of the FP64 PT-Stokes path against the CPU oracle (through the C ABI).
+
+Tolerance: relative 1e-12 per field (max |gpu - cpu| / max |cpu|) after a
+fixed iteration count, equal iteration counts to a residual tolerance, as
+the FP64 target of the path states.  With FP contraction off on both sides
+the FP64 kernels are expected to agree bit for bit.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from helpers import compare_state, make_pair, rel_err, state_names
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def test_manufactured_3d_fixed_iterations(pkg, orc):
+    _say("case: 3D manufactured free slip 32^3, 2000 iterations")
+    kw, cad = pkg.manufactured_3d(32)
+    model, ref = make_pair(pkg, orc, kw)
+    model.apply_bc("eta")
+    ref.apply_bc("eta")
+    model.advance(2000)
+    ref.advance(2000)
+    worst, bad = compare_state(model, ref, state_names(3) + ["ETA", "ETA_NEXT"], TOL)
+    assert not bad, worst
+    g = model.compute_residuals()
+    c = ref.compute_residuals()
+    for name in ("RP", "RVX", "RVY", "RVZ"):
+        assert rel_err(model.field(name).parent(), ref.get(name)) <= TOL
+    np.testing.assert_allclose(g, c, rtol=TOL, atol=0)
+    model.close()
+
+
+def test_manufactured_2d_converged_iteration_count(pkg, orc):
+    _say("case: 2D manufactured free slip 48^2, solve to 1e-8")
+    kw, cad = pkg.manufactured_2d(48)
+    model, ref = make_pair(pkg, orc, kw)
+    it_g, n_g = model.solve(200 * 48, cad["ncheck"], 1e-8)
+    it_c, n_c = ref.solve(200 * 48, cad["ncheck"], 1e-8)
+    assert it_g == it_c, (it_g, it_c)
+    assert max(n_g) < 1e-8
+    np.testing.assert_allclose(n_g, n_c, rtol=TOL, atol=0)
+    worst, bad = compare_state(model, ref, state_names(2), TOL)
+    assert not bad, worst
+    model.close()
+
+
+@pytest.mark.parametrize("dims", [(20, 17, 12), (9, 13, 6)])
+def test_mixed_bcs_powerlaw_3d(pkg, orc, dims):
+    _say("case: 3D mixed BCs + regularised relaxed power law, dims %s" % (dims,))
+    grid = pkg.UniformGrid(origin=(-0.5, -0.3, 0.0), extent=(1.0, 0.8, 0.6), dims=dims)
+    bcs = dict(x=(pkg.SBC(0.0, 0.1, -0.2), pkg.TBC(0.3, -0.1, 0.05)),
+               y=(pkg.VBC(0.02, 0.0, -0.01), pkg.SBC(0.0, 0.2, 0.1)),
+               z=(pkg.VBC(0.0, 0.0, 0.0), pkg.TBC(0.0, 0.0, 0.0)))
+    rheo = pkg.Relaxation("eta", pkg.ViscoplasticRegularisation(pkg.PowerLawRheology(0.7, 3), 50.0), 0.3)
+    kw = dict(grid=grid, boundary_conditions=bcs, gravity=dict(x=0.1, y=-0.2, z=1.0), rheology=rheo,
+              solver_params=pkg.pt_params(grid, r=0.9, re_mech=5 * math.pi))
+    model, ref = make_pair(pkg, orc, kw, seed=7)
+    for m in (model, ref):
+        m.apply_bc("eta")
+    model.advance(300)
+    ref.advance(300)
+    worst, bad = compare_state(model, ref, state_names(3) + ["ETA", "ETA_NEXT"], TOL)
+    assert not bad, worst
+    np.testing.assert_allclose(model.compute_residuals(), ref.compute_residuals(), rtol=TOL, atol=0)
+    model.close()
+
+
+def test_mixed_bcs_linear_field_2d(pkg, orc):
+    _say("case: 2D traction/no-slip with a viscosity field")
+    grid = pkg.UniformGrid(origin=(0.0, 0.0), extent=(1.0, 0.5), dims=(40, 23))
+    etaf = lambda x, y: 1.0 + 0.5 * np.sin(3 * x) * np.cos(5 * y)
+    kw = dict(grid=grid, boundary_conditions=dict(x=(pkg.SBC(0, 0), pkg.SBC(0, 0)),
+                                                   y=(pkg.VBC(0.0, 0.0), pkg.TBC(0.0, 0.0))),
+              gravity=dict(x=0.0, y=1.0), rheology=pkg.LinearViscousRheology(etaf),
+              solver_params=pkg.pt_params(grid, r=0.7, re_mech=2 * math.pi))
+    model, ref = make_pair(pkg, orc, kw, eta=etaf, seed=3)
+    for m in (model, ref):
+        m.apply_bc("eta")
+    model.advance(500)
+    ref.advance(500)
+    worst, bad = compare_state(model, ref, state_names(2) + ["ETA", "ETA_NEXT"], TOL)
+    assert not bad, worst
+    model.close()
+
+
+def test_nonfinite_is_reported(pkg, orc):
+    _say("case: NaN injected into one pressure cell")
+    kw, cad = pkg.manufactured_3d(12)
+    model, ref = make_pair(pkg, orc, kw)
+    P = model.stress.P.parent()
+    P[5, 6, 7] = np.nan
+    model.stress.P.set(P)
+    with pytest.raises(pkg.NonFiniteError):
+        model.solve(100, 12, 1e-8)
+    model.close()
+
+
+def _glacier_ib_pair(pkg, orc, dims, seed=5):
+    gl = pkg.GaussianGlacier(seed=seed)
+    grid = pkg.UniformGrid(origin=(-1.0, -1.0, 0.0), extent=(2.0, 2.0, 1.0), dims=dims)
+    dgrid = pkg.UniformGrid(origin=(-1.0, -1.0), extent=(2.0, 2.0), dims=(dims[0], dims[1]))
+    tr = pkg.TBC(0.0, 0.0, 0.0)
+    kw = dict(grid=grid, boundary_conditions=dict(x=(tr, tr), y=(tr, tr), z=(tr, tr)),
+              gravity=dict(x=0.0, y=0.0, z=1.0), rheology=pkg.LinearViscousRheology(1.0),
+              solver_params=pkg.pt_params(grid, r=0.9, re_mech=5 * math.pi))
+    model, ref = make_pair(pkg, orc, kw, ib=True)
+    return gl, grid, dgrid, model, ref
+
+
+def test_levelset_and_volume_fractions_3d(pkg, orc):
+    _say("case: level sets from DEM + volume fractions, 24x20x12 glacier")
+    gl, grid, dgrid, model, ref = _glacier_ib_pair(pkg, orc, (24, 20, 12))
+    from fastice_jl_amd.level_sets import dem_parent
+    for phase, fn, psi, om in ((0, gl.surface, model.level_sets.na, model.field_masks.na),
+                               (1, gl.bed, model.level_sets.ns, model.field_masks.ns)):
+        pkg.compute_levelset_from_dem(psi, fn, dgrid, grid)
+        ref.levelset_from_dem(phase, dem_parent(fn, dgrid), dgrid.dims, dgrid.origin, dgrid.extent_)
+    pkg.invert_levelset(model.level_sets.ns)
+    ref.invert_levelset(1)
+    for name in ("PSI_NA", "PSI_NS"):
+        assert rel_err(model.field(name).parent(), ref.get(name)) <= TOL, name
+    pkg.omega_from_psi(model.field_masks.ns, model.level_sets.ns)
+    pkg.omega_from_psi(model.field_masks.na, model.level_sets.na)
+    ref.omega_from_psi(1)
+    ref.omega_from_psi(0)
+    for ph, base in (("NA", pkg._abi.FI_F_OM_NA0), ("NS", pkg._abi.FI_F_OM_NS0)):
+        for q in range(8):
+            g = pkg.full_stokes.DeviceField(model, base + q).parent()
+            c = ref.get("OM_%s%d" % (ph, q))
+            assert np.max(np.abs(g - c)) <= 1e-13, (ph, q)
+    model.close()
+
+
+def test_immersed_boundary_iterations_3d(pkg, orc):
+    _say("case: immersed-boundary PT iterations, 24x20x12 glacier, 400 iterations")
+    gl, grid, dgrid, model, ref = _glacier_ib_pair(pkg, orc, (24, 20, 12))
+    from fastice_jl_amd.level_sets import dem_parent
+    pkg.compute_levelset_from_dem(model.level_sets.na, gl.surface, dgrid, grid)
+    pkg.compute_levelset_from_dem(model.level_sets.ns, gl.bed, dgrid, grid)
+    pkg.invert_levelset(model.level_sets.ns)
+    pkg.omega_from_psi(model.field_masks.ns, model.level_sets.ns)
+    pkg.omega_from_psi(model.field_masks.na, model.level_sets.na)
+    # hand the GPU masks to the oracle so the sweep is compared on identical inputs
+    for ph, base in (("NA", pkg._abi.FI_F_OM_NA0), ("NS", pkg._abi.FI_F_OM_NS0)):
+        for q in range(8):
+            ref.set("OM_%s%d" % (ph, q), pkg.full_stokes.DeviceField(model, base + q).parent())
+    for m in (model, ref):
+        m.apply_bc("eta")
+    model.advance(400)
+    ref.advance(400)
+    worst, bad = compare_state(model, ref, state_names(3) + ["ETA"], TOL)
+    assert not bad, worst
+    np.testing.assert_allclose(model.compute_residuals(), ref.compute_residuals(), rtol=TOL, atol=0)
+    model.close()
