@@ -39,6 +39,8 @@ def parse():
     p.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS))
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU-oracle baseline")
     p.add_argument("--cpu-seconds", type=float, default=15.0)
+    p.add_argument("--copy-calib", action="store_true",
+                   help="run a 1 GiB device copy first (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     return p.parse_args()
 
 
@@ -82,6 +84,14 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path (each solve is one device-resident "
               "domain); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calib:
+        import torch
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     pkg = ge.load_package()
     dims, desc = WORKLOADS[args.workload]
     cells = dims[0] * dims[1] * dims[2]
