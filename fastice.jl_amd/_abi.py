@@ -26,6 +26,7 @@ FI_PREC_FP64, FI_PREC_FP32_REFINED = 0, 1
 FI_LAYOUT_PARENT, FI_LAYOUT_INTERIOR = 0, 1
 FI_BCGROUP_STRESS, FI_BCGROUP_VELOCITY, FI_BCGROUP_ETA, FI_BCGROUP_RESIDUAL = 0, 1, 2, 3
 FI_PHASE_NA, FI_PHASE_NS = 0, 1
+FI_OPT_FUSED, FI_OPT_ZCHUNK = 1, 2
 
 FIELD_NAMES = [
     "P", "TXX", "TYY", "TZZ", "TXY", "TXZ", "TYZ", "VX", "VY", "VZ", "ETA", "ETA_NEXT",
@@ -135,6 +136,7 @@ def _declare(lib):
         "fi_solve": (C.c_int, [H, C.c_int, C.c_int, C.c_double, C.POINTER(C.c_int), _DP]),
         "fi_synchronize": (C.c_int, [H]),
         "fi_time_iterations": (C.c_int, [H, C.c_int, _DP]),
+        "fi_set_option": (C.c_int, [H, C.c_int, C.c_int]),
         "fi_levelset_from_dem": (C.c_int, [H, C.c_int, _DP, C.c_int, C.c_int, _DP, _DP, _DP]),
         "fi_invert_levelset": (C.c_int, [H, C.c_int]),
         "fi_omega_from_psi": (C.c_int, [H, C.c_int]),

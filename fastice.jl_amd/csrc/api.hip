@@ -65,19 +65,44 @@ struct fi_model_s {
     hipStream_t stream = nullptr;
     double* base[FI_F_COUNT] = {};
     double* f[FI_F_COUNT] = {};
+    // second buffer of the stress (P..TYZ) and velocity (VX..VZ) state
+    double* sbase[10] = {};
+    double* sf[10] = {};
+    int sig_cur = 0; // 1: current stress lives in sf
+    int v_cur = 0;   // 1: current velocity lives in sf
     int eta_cur = 0; // 0: logical eta lives in buffer FI_F_ETA, 1: in FI_F_ETA_NEXT
+    bool shadow_dirty = true;
+    uint8_t* flags_base = nullptr;
+    uint8_t* flags = nullptr;
+    bool flags_dirty = true;
+    int use_fused = 1;
+    int kch = 32;
+    FusedGeom fg;
+    BoxList shell;
+    bool has_interior = false;
     unsigned long long* d_norm = nullptr;
     unsigned long long* h_norm = nullptr;
     double* d_dem = nullptr;
     size_t dem_elems = 0;
 
+    static bool is_sig(int id) { return id >= FI_F_P && id <= FI_F_TYZ; }
+    static bool is_vel(int id) { return id >= FI_F_VX && id <= FI_F_VZ; }
     double* eta() { return f[eta_cur ? FI_F_ETA_NEXT : FI_F_ETA]; }
     double* eta_next() { return f[eta_cur ? FI_F_ETA : FI_F_ETA_NEXT]; }
-    // physical buffer behind a logical field id
+    // physical buffer holding the current value of a logical field id
     double* buf(int id)
     {
         if (id == FI_F_ETA) return eta();
         if (id == FI_F_ETA_NEXT) return eta_next();
+        if (is_sig(id)) return sig_cur ? sf[id] : f[id];
+        if (is_vel(id)) return v_cur ? sf[id] : f[id];
+        return f[id];
+    }
+    // buffer the next iteration writes
+    double* alt(int id)
+    {
+        if (is_sig(id)) return sig_cur ? f[id] : sf[id];
+        if (is_vel(id)) return v_cur ? f[id] : sf[id];
         return f[id];
     }
 };
@@ -100,21 +125,34 @@ bool field_used(const fi_model_desc& d, int id)
     return true;
 }
 
-StokesArgs<double> make_args(fi_model_s* m)
+// in_place: kernels read and write the current buffers (residuals, compute_tau);
+// otherwise they read the current and write the other buffer.
+StokesArgs<double> make_args(fi_model_s* m, bool in_place)
 {
     StokesArgs<double> a;
     std::memset(&a, 0, sizeof(a));
     const fi_model_desc& d = m->desc;
-    a.P = m->f[FI_F_P];
-    a.txx = m->f[FI_F_TXX];
-    a.tyy = m->f[FI_F_TYY];
-    a.tzz = m->f[FI_F_TZZ];
-    a.txy = m->f[FI_F_TXY];
-    a.txz = m->f[FI_F_TXZ];
-    a.tyz = m->f[FI_F_TYZ];
-    a.Vx = m->f[FI_F_VX];
-    a.Vy = m->f[FI_F_VY];
-    a.Vz = m->f[FI_F_VZ];
+    a.P = m->buf(FI_F_P);
+    a.txx = m->buf(FI_F_TXX);
+    a.tyy = m->buf(FI_F_TYY);
+    a.tzz = m->buf(FI_F_TZZ);
+    a.txy = m->buf(FI_F_TXY);
+    a.txz = m->buf(FI_F_TXZ);
+    a.tyz = m->buf(FI_F_TYZ);
+    a.Vx = m->buf(FI_F_VX);
+    a.Vy = m->buf(FI_F_VY);
+    a.Vz = m->buf(FI_F_VZ);
+    auto out = [&](int id) { return in_place ? m->buf(id) : m->alt(id); };
+    a.Pn = out(FI_F_P);
+    a.txxn = out(FI_F_TXX);
+    a.tyyn = out(FI_F_TYY);
+    a.tzzn = out(FI_F_TZZ);
+    a.txyn = out(FI_F_TXY);
+    a.txzn = out(FI_F_TXZ);
+    a.tyzn = out(FI_F_TYZ);
+    a.Vxn = out(FI_F_VX);
+    a.Vyn = out(FI_F_VY);
+    a.Vzn = out(FI_F_VZ);
     a.eta = m->eta();
     a.eta_next = m->eta_next();
     a.gx = m->f[FI_F_GX];
@@ -129,6 +167,7 @@ StokesArgs<double> make_args(fi_model_s* m)
         a.na[q] = m->f[FI_F_OM_NA0 + q];
         a.ns[q] = m->f[FI_F_OM_NS0 + q];
     }
+    a.flags = m->flags;
     a.nx = m->g.n[0];
     a.ny = m->g.n[1];
     a.nz = m->g.n[2];
@@ -163,14 +202,12 @@ void add_op(BcBatch& b, double* p, int L, int kind, double v)
     o.ptr = p;
     o.L = L;
     o.kind = kind;
+    o.f32 = 0;
     o.v = v;
 }
 
-// Both sides of one dimension share a launch; a side that does not touch a
-// field gets a no-op by pointing that op at the other side's identical value.
-// To keep the face kernel simple, batches are built per (dim, side) and a
-// side mask is carried in the op's kind: kind 0/1 = both sides, 2/3 = low
-// only, 4/5 = high only.
+// Both sides of one dimension share a launch; an op that belongs to one side
+// only carries the side in its kind: 0/1 = both sides, 2/3 = low, 4/5 = high.
 void add_side_op(BcBatch& b, double* p, int L, int kind, double v, int side)
 {
     add_op(b, p, L, kind + 2 * (side + 1), v);
@@ -188,29 +225,29 @@ BcBatch stress_batch(fi_model_s* m, int d)
         const double* c = ds.bc_value[d][s];
         if (ds.ndims == 3) {
             if (kind == FI_BC_TRACTION) {
-                double* diag = m->f[d == 0 ? FI_F_TXX : d == 1 ? FI_F_TYY : FI_F_TZZ];
-                add_side_op(b, m->f[FI_F_P], L_CCC, 0, c[d], s);
+                double* diag = m->buf(d == 0 ? FI_F_TXX : d == 1 ? FI_F_TYY : FI_F_TZZ);
+                add_side_op(b, m->buf(FI_F_P), L_CCC, 0, c[d], s);
                 add_side_op(b, diag, L_CCC, 0, c[d], s);
             }
             if (kind == FI_BC_TRACTION || kind == FI_BC_SLIP) {
                 if (d == 0) {
-                    add_side_op(b, m->f[FI_F_TXY], L_VVC, 0, c[1], s);
-                    add_side_op(b, m->f[FI_F_TXZ], L_VCV, 0, c[2], s);
+                    add_side_op(b, m->buf(FI_F_TXY), L_VVC, 0, c[1], s);
+                    add_side_op(b, m->buf(FI_F_TXZ), L_VCV, 0, c[2], s);
                 } else if (d == 1) {
-                    add_side_op(b, m->f[FI_F_TXY], L_VVC, 0, c[0], s);
-                    add_side_op(b, m->f[FI_F_TYZ], L_CVV, 0, c[2], s);
+                    add_side_op(b, m->buf(FI_F_TXY), L_VVC, 0, c[0], s);
+                    add_side_op(b, m->buf(FI_F_TYZ), L_CVV, 0, c[2], s);
                 } else {
-                    add_side_op(b, m->f[FI_F_TXZ], L_VCV, 0, c[0], s);
-                    add_side_op(b, m->f[FI_F_TYZ], L_CVV, 0, c[1], s);
+                    add_side_op(b, m->buf(FI_F_TXZ), L_VCV, 0, c[0], s);
+                    add_side_op(b, m->buf(FI_F_TYZ), L_CVV, 0, c[1], s);
                 }
             }
         } else {
             if (kind == FI_BC_TRACTION) {
-                double* diag = m->f[d == 0 ? FI_F_TXX : FI_F_TYY];
-                add_side_op(b, m->f[FI_F_P], L_CCC, 0, c[d], s);
+                double* diag = m->buf(d == 0 ? FI_F_TXX : FI_F_TYY);
+                add_side_op(b, m->buf(FI_F_P), L_CCC, 0, c[d], s);
                 add_side_op(b, diag, L_CCC, 0, c[d], s);
             }
-            if (kind == FI_BC_TRACTION || kind == FI_BC_SLIP) add_side_op(b, m->f[FI_F_TXY], L_VVC, 0, c[1 - d], s);
+            if (kind == FI_BC_TRACTION || kind == FI_BC_SLIP) add_side_op(b, m->buf(FI_F_TXY), L_VVC, 0, c[1 - d], s);
         }
     }
     return b;
@@ -229,9 +266,9 @@ BcBatch velocity_batch(fi_model_s* m, int d, bool with_eta)
         const int kind = ds.bc_kind[d][s];
         const double* c = ds.bc_value[d][s];
         if (kind == FI_BC_VELOCITY) {
-            for (int q = 0; q < ds.ndims; ++q) add_side_op(b, m->f[idV[q]], LV[q], 0, c[q], s);
+            for (int q = 0; q < ds.ndims; ++q) add_side_op(b, m->buf(idV[q]), LV[q], 0, c[q], s);
         } else if (kind == FI_BC_SLIP) {
-            add_side_op(b, m->f[idV[d]], LV[d], 0, c[d], s);
+            add_side_op(b, m->buf(idV[d]), LV[d], 0, c[d], s);
         }
     }
     if (with_eta) add_op(b, m->eta_next(), L_CCC, 1, 0.0);
@@ -272,17 +309,60 @@ void run_bc(fi_model_s* m, int group)
     }
 }
 
-void advance(fi_model_s* m, int niter)
+int prepare(fi_model_s* m)
+{
+    if (m->shadow_dirty) {
+        // positions no kernel or boundary condition writes must hold the same
+        // value in both buffers
+        const size_t bytes = m->g.elems * sizeof(double);
+        for (int id = FI_F_P; id <= FI_F_VZ; ++id)
+            if (m->f[id]) HIPCHK(hipMemcpyAsync(m->alt(id) - m->g.off, m->buf(id) - m->g.off, bytes,
+                                                hipMemcpyDeviceToDevice, m->stream));
+        m->shadow_dirty = false;
+    }
+    if (m->desc.immersed_boundary && m->flags_dirty) {
+        const double* na[8];
+        const double* ns[8];
+        for (int q = 0; q < 8; ++q) {
+            na[q] = m->f[FI_F_OM_NA0 + q];
+            ns[q] = m->f[FI_F_OM_NS0 + q];
+        }
+        launch_ib_flags(m->g, na, ns, m->flags, m->stream);
+        m->flags_dirty = false;
+    }
+    return FI_OK;
+}
+
+// One PT iteration: stress update, stress BCs, velocity + viscosity update,
+// velocity and viscosity BCs, eta swap (Isothermal.jl:91-112).  On 3D grids
+// the interior goes through the fused sweep and the boundary shell through
+// the pointwise kernels; both write the other state buffer.
+void iterate(fi_model_s* m, hipEvent_t* ev)
 {
     const bool ib = m->desc.immersed_boundary != 0;
-    for (int it = 0; it < niter; ++it) {
-        StokesArgs<double> a = make_args(m);
-        launch_sigma(m->g, a, ib, m->stream);
-        run_bc(m, FI_BCGROUP_STRESS);
-        launch_velocity(m->g, a, ib, m->stream);
-        run_bc(m, FI_BCGROUP_VELOCITY);
-        m->eta_cur ^= 1; // Isothermal.jl:110
-    }
+    StokesArgs<double> a = make_args(m, false);
+    const bool fused = m->use_fused && m->has_interior;
+    if (ev) (void)hipEventRecord(ev[0], m->stream);
+    if (fused) launch_fused(m->g, a, m->fg, ib, m->stream);
+    launch_sigma_boxes(m->g, a, fused ? m->shell : full_box(m->g), ib, m->stream);
+    m->sig_cur ^= 1;
+    if (ev) (void)hipEventRecord(ev[1], m->stream);
+    run_bc(m, FI_BCGROUP_STRESS);
+    if (ev) (void)hipEventRecord(ev[2], m->stream);
+    launch_velocity_boxes(m->g, a, fused ? m->shell : full_box(m->g), ib, m->stream);
+    m->v_cur ^= 1;
+    if (ev) (void)hipEventRecord(ev[3], m->stream);
+    run_bc(m, FI_BCGROUP_VELOCITY);
+    if (ev) (void)hipEventRecord(ev[4], m->stream);
+    m->eta_cur ^= 1;
+}
+
+int advance(fi_model_s* m, int niter)
+{
+    int rc = prepare(m);
+    if (rc) return rc;
+    for (int it = 0; it < niter; ++it) iterate(m, nullptr);
+    return FI_OK;
 }
 
 double bits_to_double(unsigned long long b)
@@ -295,7 +375,9 @@ double bits_to_double(unsigned long long b)
 int residual_norms(fi_model_s* m, double out[4])
 {
     const bool ib = m->desc.immersed_boundary != 0;
-    StokesArgs<double> a = make_args(m);
+    int rc = prepare(m);
+    if (rc) return rc;
+    StokesArgs<double> a = make_args(m, true);
     launch_residuals(m->g, a, ib, m->stream);
     run_bc(m, FI_BCGROUP_RESIDUAL);
     const double* f[4] = {m->f[FI_F_RP], m->f[FI_F_RVX], m->f[FI_F_RVY], m->f[FI_F_RVZ]};
@@ -333,6 +415,9 @@ void free_model(fi_model_s* m)
 {
     for (int q = 0; q < FI_F_COUNT; ++q)
         if (m->base[q]) (void)hipFree(m->base[q]);
+    for (int q = 0; q < 10; ++q)
+        if (m->sbase[q]) (void)hipFree(m->sbase[q]);
+    if (m->flags_base) (void)hipFree(m->flags_base);
     if (m->d_norm) (void)hipFree(m->d_norm);
     if (m->h_norm) (void)hipHostFree(m->h_norm);
     if (m->d_dem) (void)hipFree(m->d_dem);
@@ -408,6 +493,28 @@ int fi_model_create(const fi_model_desc* desc, fi_model_t* out)
             return fail(FI_ERR_HIP, "hipMemsetAsync failed");
         }
     }
+    for (int id = FI_F_P; id <= FI_F_VZ; ++id) {
+        if (!m->f[id]) continue;
+        if (hipMalloc(&m->sbase[id], g.elems * sizeof(double)) != hipSuccess) {
+            free_model(m);
+            return fail(FI_ERR_OOM, "device allocation of the second buffer of field %d failed", id);
+        }
+        m->sf[id] = m->sbase[id] + g.off;
+    }
+    if (desc->immersed_boundary) {
+        if (hipMalloc(&m->flags_base, g.elems) != hipSuccess) {
+            free_model(m);
+            return fail(FI_ERR_OOM, "device allocation of the validity flags failed");
+        }
+        m->flags = m->flags_base + g.off;
+        if (hipMemsetAsync(m->flags_base, 0, g.elems, m->stream) != hipSuccess) {
+            free_model(m);
+            return fail(FI_ERR_HIP, "hipMemsetAsync failed");
+        }
+    }
+    // the fused sweep addresses fields with 32-bit byte offsets
+    if (g.nd == 3 && g.elems * sizeof(double) < (1ull << 32))
+        m->has_interior = fused_split(g, 8, m->kch, m->fg, m->shell);
     if (hipMalloc(&m->d_norm, 4 * sizeof(unsigned long long)) != hipSuccess ||
         hipHostMalloc(&m->h_norm, 4 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess) {
         free_model(m);
@@ -437,6 +544,12 @@ size_t fi_field_size(fi_model_t m, int id, int layout)
     return L < 0 ? 0 : interior_count(m->g, L);
 }
 
+static void mark_dirty(fi_model_t m, int id)
+{
+    if (id >= FI_F_P && id <= FI_F_VZ) m->shadow_dirty = true;
+    if (id >= FI_F_OM_NA0 && id < FI_F_COUNT) m->flags_dirty = true;
+}
+
 static int field_copy(fi_model_t m, int id, double* host, size_t count, int layout, bool up)
 {
     if (!m) return fail(FI_ERR_ARG, "null model");
@@ -447,6 +560,7 @@ static int field_copy(fi_model_t m, int id, double* host, size_t count, int layo
     if (count != need) return fail(FI_ERR_ARG, "field %d needs %zu doubles in this layout, got %zu", id, need, count);
     const Geo& g = m->g;
     double* dev = m->buf(id);
+    if (up) mark_dirty(m, id);
     HIPCHK(hipStreamSynchronize(m->stream));
     if (layout == FI_LAYOUT_PARENT) {
         const size_t w = (size_t)(g.n[0] + 3) * sizeof(double);
@@ -483,6 +597,7 @@ int fi_field_fill(fi_model_t m, int id, double value)
     if (!m) return fail(FI_ERR_ARG, "null model");
     if (id < 0 || id >= FI_F_COUNT || !m->f[id]) return fail(FI_ERR_ARG, "field %d not present in this model", id);
     double* p = m->buf(id);
+    mark_dirty(m, id);
     // the whole allocation, i.e. the parent box including halos and row padding
     launch_fill(p - m->g.off, m->g.elems, value, m->stream);
     HIPCHK(hipGetLastError());
@@ -503,8 +618,10 @@ int fi_apply_bc(fi_model_t m, int group)
         }
     } else if (group == FI_BCGROUP_VELOCITY) {
         for (int d = 0; d < m->g.nd; ++d) launch_bc_dim(m->g, velocity_batch(m, d, false), m->stream);
+        m->shadow_dirty = true;
     } else {
         run_bc(m, group);
+        if (group == FI_BCGROUP_STRESS) m->shadow_dirty = true;
     }
     HIPCHK(hipGetLastError());
     return FI_OK;
@@ -514,7 +631,8 @@ int fi_advance_iterations(fi_model_t m, int n_iter)
 {
     if (!m) return fail(FI_ERR_ARG, "null model");
     if (n_iter < 0) return fail(FI_ERR_ARG, "negative iteration count");
-    advance(m, n_iter);
+    int rc = advance(m, n_iter);
+    if (rc) return rc;
     HIPCHK(hipGetLastError());
     return FI_OK;
 }
@@ -531,8 +649,11 @@ int fi_compute_residuals(fi_model_t m, double norms_inf[4])
 int fi_compute_tau(fi_model_t m)
 {
     if (!m) return fail(FI_ERR_ARG, "null model");
-    StokesArgs<double> a = make_args(m);
+    int rc = prepare(m);
+    if (rc) return rc;
+    StokesArgs<double> a = make_args(m, true);
     launch_tau(m->g, a, m->desc.immersed_boundary != 0, m->stream);
+    m->shadow_dirty = true;
     HIPCHK(hipGetLastError());
     return FI_OK;
 }
@@ -546,7 +667,8 @@ int fi_solve(fi_model_t m, int max_iter, int ncheck, double tol, int* iters_out,
     while (done < max_iter) {
         int chunk = ncheck - (done % ncheck);
         if (done + chunk > max_iter) chunk = max_iter - done;
-        advance(m, chunk);
+        int rc = advance(m, chunk);
+        if (rc) return rc;
         done += chunk;
         if (done % ncheck == 0) {
             int rc = residual_norms(m, norms_inf);
@@ -570,37 +692,43 @@ int fi_solve(fi_model_t m, int max_iter, int ncheck, double tol, int* iters_out,
 int fi_time_iterations(fi_model_t m, int n_iter, double out_ms[4])
 {
     if (!m || !out_ms || n_iter < 1) return fail(FI_ERR_ARG, "bad arguments");
-    const bool ib = m->desc.immersed_boundary != 0;
+    int rc = prepare(m);
+    if (rc) return rc;
     hipEvent_t ev[5];
     for (int q = 0; q < 5; ++q) HIPCHK(hipEventCreate(&ev[q]));
     double acc[4] = {0, 0, 0, 0};
     for (int it = 0; it < n_iter; ++it) {
-        StokesArgs<double> a = make_args(m);
-        HIPCHK(hipEventRecord(ev[0], m->stream));
-        launch_sigma(m->g, a, ib, m->stream);
-        HIPCHK(hipEventRecord(ev[1], m->stream));
-        run_bc(m, FI_BCGROUP_STRESS);
-        HIPCHK(hipEventRecord(ev[2], m->stream));
-        launch_velocity(m->g, a, ib, m->stream);
-        HIPCHK(hipEventRecord(ev[3], m->stream));
-        run_bc(m, FI_BCGROUP_VELOCITY);
-        HIPCHK(hipEventRecord(ev[4], m->stream));
-        m->eta_cur ^= 1;
+        iterate(m, ev);
         HIPCHK(hipEventSynchronize(ev[4]));
-        float t;
+        float t, t1, t2;
         HIPCHK(hipEventElapsedTime(&t, ev[0], ev[4]));
         acc[0] += t;
         HIPCHK(hipEventElapsedTime(&t, ev[0], ev[1]));
         acc[1] += t;
         HIPCHK(hipEventElapsedTime(&t, ev[2], ev[3]));
         acc[2] += t;
-        float t1, t2;
         HIPCHK(hipEventElapsedTime(&t1, ev[1], ev[2]));
         HIPCHK(hipEventElapsedTime(&t2, ev[3], ev[4]));
         acc[3] += t1 + t2;
     }
     for (int q = 0; q < 5; ++q) (void)hipEventDestroy(ev[q]);
     for (int q = 0; q < 4; ++q) out_ms[q] = acc[q] / n_iter;
+    return FI_OK;
+}
+
+int fi_set_option(fi_model_t m, int key, int value)
+{
+    if (!m) return fail(FI_ERR_ARG, "null model");
+    if (key == FI_OPT_FUSED) {
+        m->use_fused = value ? 1 : 0;
+    } else if (key == FI_OPT_ZCHUNK) {
+        if (value < 1) return fail(FI_ERR_ARG, "z chunk must be >= 1");
+        m->kch = value;
+        if (m->g.nd == 3 && m->g.elems * sizeof(double) < (1ull << 32))
+            m->has_interior = fused_split(m->g, 8, m->kch, m->fg, m->shell);
+    } else {
+        return fail(FI_ERR_ARG, "unknown option %d", key);
+    }
     return FI_OK;
 }
 
@@ -673,6 +801,7 @@ int fi_omega_from_psi(fi_model_t m, int phase)
     const int base = phase == FI_PHASE_NA ? FI_F_OM_NA0 : FI_F_OM_NS0;
     for (int q = 0; q < 8; ++q) om[q] = m->f[base + q];
     launch_omega_from_psi(m->g, psi, om, m->stream);
+    m->flags_dirty = true;
     HIPCHK(hipGetLastError());
     return FI_OK;
 }

@@ -13,6 +13,23 @@
 
 namespace fi {
 
+template <typename T>
+__device__ __forceinline__ void apply_face(T* f, int kind, int vert, int s, int nn, long base, long st, double v)
+{
+    if (kind == 0) {
+        if (vert) f[base + (long)((s == 0) ? 1 : nn + 1) * st] = T(v);
+        else {
+            long t = (s == 0) ? 0 : nn + 1, src = (s == 0) ? 1 : nn;
+            f[base + t * st] = T(2) * T(v) - f[base + src * st];
+        }
+    } else {
+        long t, src;
+        if (vert) { t = (s == 0) ? 0 : nn + 2; src = (s == 0) ? 2 : nn; }
+        else { t = (s == 0) ? 0 : nn + 1; src = (s == 0) ? 1 : nn; }
+        f[base + t * st] = f[base + src * st];
+    }
+}
+
 __global__ __launch_bounds__(256) void k_bc_dim(BcBatch b, int nd, int n0, int n1, int n2, long sy, long sz)
 {
     const int s = blockIdx.z; // side
@@ -38,19 +55,8 @@ __global__ __launch_bounds__(256) void k_bc_dim(BcBatch b, int nd, int n0, int n
         const long base = q1 * st[e1] + ((e2 >= 0) ? q2 * st[e2] : 0);
         const int vert = (op.L >> d) & 1;
         const int nn = n[d];
-        double* f = op.ptr;
-        if (kind == 0) {
-            if (vert) f[base + (long)((s == 0) ? 1 : nn + 1) * st[d]] = op.v;
-            else {
-                long t = (s == 0) ? 0 : nn + 1, src = (s == 0) ? 1 : nn;
-                f[base + t * st[d]] = 2.0 * op.v - f[base + src * st[d]];
-            }
-        } else {
-            long t, src;
-            if (vert) { t = (s == 0) ? 0 : nn + 2; src = (s == 0) ? 2 : nn; }
-            else { t = (s == 0) ? 0 : nn + 1; src = (s == 0) ? 1 : nn; }
-            f[base + t * st[d]] = f[base + src * st[d]];
-        }
+        if (op.f32) apply_face(static_cast<float*>(op.ptr), kind, vert, s, nn, base, st[d], op.v);
+        else apply_face(static_cast<double*>(op.ptr), kind, vert, s, nn, base, st[d], op.v);
     }
 }
 

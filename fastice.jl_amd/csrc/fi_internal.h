@@ -31,11 +31,16 @@ struct Geo {
     size_t elems; // elements per allocation
 };
 
-// Arguments of the PT sweep / residual kernels (passed by value).
+// Arguments of the PT sweep / residual kernels (passed by value).  The
+// stress and velocity state is double buffered: kernels read the previous
+// iteration (P, txx, ..., Vx, ...) and write this iteration (Pn, ..., Vxn,
+// ...).  Equal pointers give the in-place update.
 template <typename T>
 struct StokesArgs {
-    T *P, *txx, *tyy, *tzz, *txy, *txz, *tyz;
-    T *Vx, *Vy, *Vz;
+    const T *P, *txx, *tyy, *tzz, *txy, *txz, *tyz;
+    T *Pn, *txxn, *tyyn, *tzzn, *txyn, *txzn, *tyzn;
+    const T *Vx, *Vy, *Vz;
+    T *Vxn, *Vyn, *Vzn;
     const T* eta;
     T* eta_next;
     const T *gx, *gy, *gz;
@@ -43,6 +48,7 @@ struct StokesArgs {
     T *rP, *rVx, *rVy, *rVz;
     const T* na[8];
     const T* ns[8];
+    const uint8_t* flags; // immersed-boundary validity bits (F_* below)
     int nx, ny, nz;
     long sy, sz;
     T ix, iy, iz;
@@ -53,10 +59,31 @@ struct StokesArgs {
     T rh_eta, rh_A, rh_n, rh_inv_eta_reg, rh_rate;
 };
 
+// validity bits: sigma locations use isvalid(na) && !isempty(ns), velocity
+// locations isvalid(ns) && !isempty(na) (IB kernels_3d.jl:144,164,170,176,188,194,200)
+enum { F_CCC = 1, F_VVC = 2, F_VCV = 4, F_CVV = 8, F_VCC = 16, F_CVC = 32, F_CCV = 64 };
+
+// inclusive index boxes over Chmy indices
+struct Box {
+    int i0, i1, j0, j1, k0, k1;
+};
+struct BoxList {
+    Box b[6];
+    int nb;
+    int start[7]; // block prefix sums
+};
+
+// interior region of the fused sweep
+struct FusedGeom {
+    int lo[3], hi[3];
+    int kch; // planes per workgroup
+};
+
 struct BcOp {
-    double* ptr;
+    void* ptr;
     int L;    // location bits
-    int kind; // 0 Dirichlet, 1 Neumann
+    int kind; // bit 0: Neumann; bits 1-2: side restriction (0 both, 1 low, 2 high)
+    int f32;  // field stored as float
     double v;
 };
 struct BcBatch {
@@ -66,10 +93,19 @@ struct BcBatch {
 };
 
 // kernel launchers (stokes_kernels.hip)
+// pointwise sweeps over a box list (3D) or the full work range (2D)
 template <typename T>
-void launch_sigma(const Geo& g, const StokesArgs<T>& a, bool ib, hipStream_t s);
+void launch_sigma_boxes(const Geo& g, const StokesArgs<T>& a, const BoxList& bl, bool ib, hipStream_t s);
 template <typename T>
-void launch_velocity(const Geo& g, const StokesArgs<T>& a, bool ib, hipStream_t s);
+void launch_velocity_boxes(const Geo& g, const StokesArgs<T>& a, const BoxList& bl, bool ib, hipStream_t s);
+// fused stress + velocity sweep over the interior region (3D)
+template <typename T>
+void launch_fused(const Geo& g, const StokesArgs<T>& a, const FusedGeom& fg, bool ib, hipStream_t s);
+// interior region and the shell boxes around it; false when the interior is empty
+bool fused_split(const Geo& g, int ty, int kch, FusedGeom& fg, BoxList& shell);
+BoxList full_box(const Geo& g);
+void launch_ib_flags(const Geo& g, const double* const na[8], const double* const ns[8], uint8_t* flags,
+                     hipStream_t s);
 template <typename T>
 void launch_residuals(const Geo& g, const StokesArgs<T>& a, bool ib, hipStream_t s);
 template <typename T>

@@ -1,10 +1,11 @@
 // stokes_kernels.hip -- PT-Stokes sweep kernels for gfx950.
 //
-// One thread per staggered index I in [1, n+1]^d, x fastest across the
-// wavefront (coalesced 64-wide rows).  Every expression keeps the operation
-// order of the reference source line it cites (Julia evaluates left to right)
-// and the library is compiled with -ffp-contract=off, so the FP64 kernels
-// round exactly like the CPU restatement of the same formulas.
+// Every expression keeps the operation order of the reference source line it
+// cites (Julia evaluates left to right) and the library is compiled with
+// -ffp-contract=off, so the FP64 kernels round exactly like the CPU
+// restatement of the same formulas.  The arithmetic of one staggered point
+// lives in the inline helpers below and is shared by the pointwise box
+// kernels and the fused 2.5D sweep.
 //
 // Discrete operators (Chmy semantics, fixed by this library):
 //   d/dx of a Vertex-in-x field at a Center index: (f[i+1]-f[i]) * (1/dx)
@@ -16,7 +17,55 @@
 
 namespace fi {
 
-#define TOL_EMPTY 1e-6 // null_spaces.jl:16,41
+// ---------------------------------------------------------------------------
+// point arithmetic
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T dif(T r, T l, T inv)
+{
+    return (r - l) * inv;
+}
+template <typename T>
+__device__ __forceinline__ T lerp4(T a, T b, T c, T d)
+{
+    return T(0.5) * (T(0.5) * (a + b) + T(0.5) * (c + d));
+}
+// P -= divV * eta * dtau.Pr                             (kernels_3d.jl:143)
+template <typename T>
+__device__ __forceinline__ T upd_P(T p, T divV, T eta, T dPr)
+{
+    return p - divV * eta * dPr;
+}
+// tau_ii -= (tau_ii - 2 eta (e_ii - divV/3)) * dtau      (kernels_3d.jl:145-147)
+template <typename T>
+__device__ __forceinline__ T upd_td(T t, T eta, T e, T divV, T dt)
+{
+    return t - (t - T(2) * eta * (e - divV / T(3))) * dt;
+}
+// tau_ij -= (tau_ij - 2 eta_ij e_ij) * dtau               (kernels_3d.jl:149-151)
+template <typename T>
+__device__ __forceinline__ T upd_to(T t, T eta, T e, T dt)
+{
+    return t - (t - T(2) * eta * e) * dt;
+}
+// V += (-dP + dtau_a + dtau_b + dtau_c - g) / eta_V * dtau.V   (kernels_3d.jl:158-160)
+template <typename T>
+__device__ __forceinline__ T upd_V(T v, T dP, T da, T db, T dc, T g, T ev, T dv)
+{
+    return v + (-dP + da + db + dc - g) / ev * dv;
+}
+// IB variant: ... / eta_V / omega_ns * dtau.V             (IB kernels_3d.jl:189-190)
+template <typename T>
+__device__ __forceinline__ T upd_V_ib(T v, T dP, T da, T db, T dc, T g, T ev, T w, T dv)
+{
+    return v + (-dP + da + db + dc - g) / ev / w * dv;
+}
+// tau_II (kernels_3d.jl:162-165)
+template <typename T>
+__device__ __forceinline__ T tauII(T xx, T yy, T zz, T lxy, T lxz, T lyz)
+{
+    return sqrt(T(0.5) * (xx * xx + yy * yy + zz * zz) + lxy * lxy + lxz * lxz + lyz * lyz);
+}
 
 template <typename T>
 __device__ __forceinline__ T ipow_(T x, T p)
@@ -28,7 +77,7 @@ __device__ __forceinline__ T ipow_(T x, T p)
     return pow(x, p);
 }
 
-// Physics.jl:11,18,26,46-47
+// Physics.jl:11,18,26,46-47 (the relaxed field is the lagged viscosity eta)
 template <typename T>
 __device__ __forceinline__ T rheology(const StokesArgs<T>& a, long c, T tII)
 {
@@ -50,85 +99,412 @@ __device__ __forceinline__ T rheology(const StokesArgs<T>& a, long c, T tII)
     return eta;
 }
 
-// null_spaces.jl:1-43 in three dimensions
 template <typename T>
-__device__ __forceinline__ bool lt(const T* w, long c)
+__device__ __forceinline__ T rheology_o(const StokesArgs<T>& a, uint32_t bo, T tII)
 {
-    return w[c] < T(TOL_EMPTY);
-}
-
-template <typename T>
-__device__ __forceinline__ bool null3(const T* const* w, int loc, long c, long sy, long sz)
-{
-    switch (loc) {
-    case L_CCC: return lt(w[OM_VCC], c) || lt(w[OM_VCC], c + 1) || lt(w[OM_CVC], c) || lt(w[OM_CVC], c + sy) ||
-                       lt(w[OM_CCV], c) || lt(w[OM_CCV], c + sz);
-    case L_VCC: return lt(w[OM_CCC], c - 1) || lt(w[OM_CCC], c) || lt(w[OM_VVC], c) || lt(w[OM_VVC], c + sy) ||
-                       lt(w[OM_VCV], c) || lt(w[OM_VCV], c + sz);
-    case L_CVC: return lt(w[OM_VVC], c) || lt(w[OM_VVC], c + 1) || lt(w[OM_CCC], c - sy) || lt(w[OM_CCC], c) ||
-                       lt(w[OM_CVV], c) || lt(w[OM_CVV], c + sz);
-    case L_CCV: return lt(w[OM_VCV], c) || lt(w[OM_VCV], c + 1) || lt(w[OM_CVV], c) || lt(w[OM_CVV], c + sy) ||
-                       lt(w[OM_CCC], c - sz) || lt(w[OM_CCC], c);
-    case L_VVC: return lt(w[OM_CVC], c - 1) || lt(w[OM_CVC], c) || lt(w[OM_VCC], c - sy) || lt(w[OM_VCC], c);
-    case L_VCV: return lt(w[OM_CCV], c - 1) || lt(w[OM_CCV], c) || lt(w[OM_VCC], c - sz) || lt(w[OM_VCC], c);
-    case L_CVV: return lt(w[OM_CCV], c - sy) || lt(w[OM_CCV], c) || lt(w[OM_CVC], c - sz) || lt(w[OM_CVC], c);
+    T eta;
+    if (a.rh_kind == FI_RHEO_LINEAR_CONST) eta = a.rh_eta;
+    else if (a.rh_kind == FI_RHEO_LINEAR_FIELD) eta = *reinterpret_cast<const T*>(reinterpret_cast<const char*>(a.rh) + bo);
+    else {
+        T A = a.rh ? *reinterpret_cast<const T*>(reinterpret_cast<const char*>(a.rh) + bo) : a.rh_A;
+        eta = T(0.5) / (A * ipow_(tII, a.rh_n - T(1)));
     }
-    return true;
-}
-
-__device__ __forceinline__ int comp3(int loc)
-{
-    switch (loc) {
-    case L_CCC: return OM_CCC;
-    case L_VCC: return OM_VCC;
-    case L_CVC: return OM_CVC;
-    case L_CCV: return OM_CCV;
-    case L_VVC: return OM_VVC;
-    case L_VCV: return OM_VCV;
-    case L_CVV: return OM_CVV;
-    default: return OM_VVV;
+    if (a.rh_has_reg) eta = T(1) / (T(1) / eta + a.rh_inv_eta_reg);
+    if (a.rh_relax != FI_RELAX_NONE) {
+        T old = *reinterpret_cast<const T*>(reinterpret_cast<const char*>(a.eta) + bo);
+        T w = T(1) - a.rh_rate;
+        if (a.rh_relax == FI_RELAX_LIN) eta = old * w + eta * a.rh_rate;
+        else if (a.rh_relax == FI_RELAX_LOG) eta = exp(log(old) * w + log(eta) * a.rh_rate);
+        else eta = pow(T(10), log10(old) * w + log10(eta) * a.rh_rate);
     }
-}
-
-// isvalid(w, loc, I) && !isempty(w2, loc, I)
-template <typename T>
-__device__ __forceinline__ bool valid3(const T* const* w, const T* const* w2, int loc, long c, long sy, long sz)
-{
-    int k = comp3(loc);
-    return !(null3(w, loc, c, sy, sz) || lt(w[k], c)) && !lt(w2[k], c);
-}
-
-template <typename T>
-__device__ __forceinline__ bool null2(const T* const* w, int loc, long c, long sy)
-{
-    switch (loc) {
-    case L_CCC: return lt(w[OM2_VC], c) || lt(w[OM2_VC], c + 1) || lt(w[OM2_CV], c) || lt(w[OM2_CV], c + sy);
-    case L_VCC: return lt(w[OM2_CC], c - 1) || lt(w[OM2_CC], c) || lt(w[OM2_VV], c) || lt(w[OM2_VV], c + sy);
-    case L_CVC: return lt(w[OM2_VV], c) || lt(w[OM2_VV], c + 1) || lt(w[OM2_CC], c - sy) || lt(w[OM2_CC], c);
-    case L_VVC: return lt(w[OM2_CV], c - 1) || lt(w[OM2_CV], c) || lt(w[OM2_VC], c - sy) || lt(w[OM2_VC], c);
-    }
-    return true;
-}
-
-__device__ __forceinline__ int comp2(int loc)
-{
-    switch (loc) {
-    case L_CCC: return OM2_CC;
-    case L_VCC: return OM2_VC;
-    case L_CVC: return OM2_CV;
-    default: return OM2_VV;
-    }
-}
-
-template <typename T>
-__device__ __forceinline__ bool valid2(const T* const* w, const T* const* w2, int loc, long c, long sy)
-{
-    int k = comp2(loc);
-    return !(null2(w, loc, c, sy) || lt(w[k], c)) && !lt(w2[k], c);
+    return eta;
 }
 
 // ---------------------------------------------------------------------------
-// 3D
+// 3D pointwise sweeps (used on the boundary shell and for small grids)
+// ---------------------------------------------------------------------------
+// update_sigma!: FullStokes kernels_3d.jl:130-152, IB kernels_3d.jl:141-182
+template <typename T, bool IB>
+__device__ __forceinline__ void sigma_point3(const StokesArgs<T>& a, long c, long sy, long sz)
+{
+    const T ix = a.ix, iy = a.iy, iz = a.iz;
+    const T *Vx = a.Vx, *Vy = a.Vy, *Vz = a.Vz, *eta = a.eta;
+    const T ec = eta[c];
+    const T exy_eta = lerp4(eta[c - 1 - sy], eta[c - sy], eta[c - 1], eta[c]);
+    const T exz_eta = lerp4(eta[c - 1 - sz], eta[c - sz], eta[c - 1], eta[c]);
+    const T eyz_eta = lerp4(eta[c - sy - sz], eta[c - sz], eta[c - sy], eta[c]);
+    if (!IB) {
+        T exx = dif(Vx[c + 1], Vx[c], ix);
+        T eyy = dif(Vy[c + sy], Vy[c], iy);
+        T ezz = dif(Vz[c + sz], Vz[c], iz);
+        T exy = T(0.5) * (dif(Vy[c], Vy[c - 1], ix) + dif(Vx[c], Vx[c - sy], iy));
+        T exz = T(0.5) * (dif(Vz[c], Vz[c - 1], ix) + dif(Vx[c], Vx[c - sz], iz));
+        T eyz = T(0.5) * (dif(Vz[c], Vz[c - sy], iy) + dif(Vy[c], Vy[c - sz], iz));
+        T divV = exx + eyy + ezz;
+        a.Pn[c] = upd_P(a.P[c], divV, ec, a.dPr);
+        a.txxn[c] = upd_td(a.txx[c], ec, exx, divV, a.dtau[0]);
+        a.tyyn[c] = upd_td(a.tyy[c], ec, eyy, divV, a.dtau[1]);
+        a.tzzn[c] = upd_td(a.tzz[c], ec, ezz, divV, a.dtau[2]);
+        a.txyn[c] = upd_to(a.txy[c], exy_eta, exy, a.dtau[3]);
+        a.txzn[c] = upd_to(a.txz[c], exz_eta, exz, a.dtau[4]);
+        a.tyzn[c] = upd_to(a.tyz[c], eyz_eta, eyz, a.dtau[5]);
+    } else {
+        const T *wx = a.ns[OM_VCC], *wy = a.ns[OM_CVC], *wz = a.ns[OM_CCV];
+        const unsigned fl = a.flags[c];
+        if (fl & F_CCC) {
+            T exx = dif(wx[c + 1] * Vx[c + 1], wx[c] * Vx[c], ix);
+            T eyy = dif(wy[c + sy] * Vy[c + sy], wy[c] * Vy[c], iy);
+            T ezz = dif(wz[c + sz] * Vz[c + sz], wz[c] * Vz[c], iz);
+            T divV = exx + eyy + ezz;
+            a.Pn[c] = upd_P(a.P[c], divV, ec, a.dPr);
+            a.txxn[c] = upd_td(a.txx[c], ec, exx, divV, a.dtau[0]);
+            a.tyyn[c] = upd_td(a.tyy[c], ec, eyy, divV, a.dtau[1]);
+            a.tzzn[c] = upd_td(a.tzz[c], ec, ezz, divV, a.dtau[2]);
+        } else {
+            a.Pn[c] = T(0); a.txxn[c] = T(0); a.tyyn[c] = T(0); a.tzzn[c] = T(0);
+        }
+        if (fl & F_VVC) {
+            T exy = T(0.5) * (dif(wy[c] * Vy[c], wy[c - 1] * Vy[c - 1], ix) + dif(wx[c] * Vx[c], wx[c - sy] * Vx[c - sy], iy));
+            a.txyn[c] = upd_to(a.txy[c], exy_eta, exy, a.dtau[3]);
+        } else a.txyn[c] = T(0);
+        if (fl & F_VCV) {
+            T exz = T(0.5) * (dif(wz[c] * Vz[c], wz[c - 1] * Vz[c - 1], ix) + dif(wx[c] * Vx[c], wx[c - sz] * Vx[c - sz], iz));
+            a.txzn[c] = upd_to(a.txz[c], exz_eta, exz, a.dtau[4]);
+        } else a.txzn[c] = T(0);
+        if (fl & F_CVV) {
+            T eyz = T(0.5) * (dif(wz[c] * Vz[c], wz[c - sy] * Vz[c - sy], iy) + dif(wy[c] * Vy[c], wy[c - sz] * Vy[c - sz], iz));
+            a.tyzn[c] = upd_to(a.tyz[c], eyz_eta, eyz, a.dtau[5]);
+        } else a.tyzn[c] = T(0);
+    }
+}
+
+// update_V!: FullStokes kernels_3d.jl:154-167, IB kernels_3d.jl:184-212 (reads the new stress)
+template <typename T, bool IB>
+__device__ __forceinline__ void velocity_point3(const StokesArgs<T>& a, long c, long sy, long sz)
+{
+    const T ix = a.ix, iy = a.iy, iz = a.iz;
+    const T *P = a.Pn, *txx = a.txxn, *tyy = a.tyyn, *tzz = a.tzzn, *txy = a.txyn, *txz = a.txzn, *tyz = a.tyzn;
+    const T* eta = a.eta;
+    const T gx = a.grav_field ? a.gx[c] : a.grav[0];
+    const T gy = a.grav_field ? a.gy[c] : a.grav[1];
+    const T gz = a.grav_field ? a.gz[c] : a.grav[2];
+    const T ex = T(0.5) * (eta[c - 1] + eta[c]);
+    const T ey = T(0.5) * (eta[c - sy] + eta[c]);
+    const T ez = T(0.5) * (eta[c - sz] + eta[c]);
+    if (!IB) {
+        a.Vxn[c] = upd_V(a.Vx[c], dif(P[c], P[c - 1], ix), dif(txx[c], txx[c - 1], ix), dif(txy[c + sy], txy[c], iy),
+                         dif(txz[c + sz], txz[c], iz), gx, ex, a.dV[0]);
+        a.Vyn[c] = upd_V(a.Vy[c], dif(P[c], P[c - sy], iy), dif(tyy[c], tyy[c - sy], iy), dif(txy[c + 1], txy[c], ix),
+                         dif(tyz[c + sz], tyz[c], iz), gy, ey, a.dV[1]);
+        a.Vzn[c] = upd_V(a.Vz[c], dif(P[c], P[c - sz], iz), dif(tzz[c], tzz[c - sz], iz), dif(txz[c + 1], txz[c], ix),
+                         dif(tyz[c + sy], tyz[c], iy), gz, ez, a.dV[2]);
+    } else {
+        const T *wc = a.na[OM_CCC], *wxy = a.na[OM_VVC], *wxz = a.na[OM_VCV], *wyz = a.na[OM_CVV];
+        const unsigned fl = a.flags[c];
+        a.Vxn[c] = (fl & F_VCC)
+                       ? upd_V_ib(a.Vx[c], dif(wc[c] * P[c], wc[c - 1] * P[c - 1], ix), dif(wc[c] * txx[c], wc[c - 1] * txx[c - 1], ix),
+                                  dif(wxy[c + sy] * txy[c + sy], wxy[c] * txy[c], iy), dif(wxz[c + sz] * txz[c + sz], wxz[c] * txz[c], iz),
+                                  gx, ex, a.ns[OM_VCC][c], a.dV[0])
+                       : T(0);
+        a.Vyn[c] = (fl & F_CVC)
+                       ? upd_V_ib(a.Vy[c], dif(wc[c] * P[c], wc[c - sy] * P[c - sy], iy), dif(wc[c] * tyy[c], wc[c - sy] * tyy[c - sy], iy),
+                                  dif(wxy[c + 1] * txy[c + 1], wxy[c] * txy[c], ix), dif(wyz[c + sz] * tyz[c + sz], wyz[c] * tyz[c], iz),
+                                  gy, ey, a.ns[OM_CVC][c], a.dV[1])
+                       : T(0);
+        a.Vzn[c] = (fl & F_CCV)
+                       ? upd_V_ib(a.Vz[c], dif(wc[c] * P[c], wc[c - sz] * P[c - sz], iz), dif(wc[c] * tzz[c], wc[c - sz] * tzz[c - sz], iz),
+                                  dif(wxz[c + 1] * txz[c + 1], wxz[c] * txz[c], ix), dif(wyz[c + sy] * tyz[c + sy], wyz[c] * tyz[c], iy),
+                                  gz, ez, a.ns[OM_CCV][c], a.dV[2])
+                       : T(0);
+    }
+    const T t = tauII(txx[c], tyy[c], tzz[c], lerp4(txy[c], txy[c + 1], txy[c + sy], txy[c + 1 + sy]),
+                      lerp4(txz[c], txz[c + 1], txz[c + sz], txz[c + 1 + sz]),
+                      lerp4(tyz[c], tyz[c + sy], tyz[c + sz], tyz[c + sy + sz]));
+    a.eta_next[c] = rheology(a, c, t);
+}
+
+__device__ __forceinline__ bool box_index(const BoxList& bl, int& i, int& j, int& k)
+{
+    const int bid = blockIdx.x;
+    int q = 0;
+    while (q + 1 < bl.nb && bid >= bl.start[q + 1]) ++q;
+    const Box b = bl.b[q];
+    const int nbx = (b.i1 - b.i0 + 64) / 64, nby = (b.j1 - b.j0 + 4) / 4;
+    const int loc = bid - bl.start[q];
+    const int bx = loc % nbx, r = loc / nbx;
+    const int by = r % nby, bz = r / nby;
+    i = b.i0 + bx * 64 + (int)threadIdx.x;
+    j = b.j0 + by * 4 + (int)threadIdx.y;
+    k = b.k0 + bz;
+    return i <= b.i1 && j <= b.j1;
+}
+
+template <typename T, bool IB>
+__global__ __launch_bounds__(256) void k_sigma_box(StokesArgs<T> a, BoxList bl)
+{
+    int i, j, k;
+    if (!box_index(bl, i, j, k)) return;
+    sigma_point3<T, IB>(a, (long)i + a.sy * j + a.sz * k, a.sy, a.sz);
+}
+
+template <typename T, bool IB>
+__global__ __launch_bounds__(256) void k_velocity_box(StokesArgs<T> a, BoxList bl)
+{
+    int i, j, k;
+    if (!box_index(bl, i, j, k)) return;
+    velocity_point3<T, IB>(a, (long)i + a.sy * j + a.sz * k, a.sy, a.sz);
+}
+
+// ---------------------------------------------------------------------------
+// fused 2.5D sweep (3D interior)
+// ---------------------------------------------------------------------------
+// A workgroup owns a (64-2) x (TY-2) column of the interior and marches up a
+// chunk of z planes.  For plane kk it publishes the previous-iteration
+// velocity and viscosity of its 64 x TY footprint to LDS, computes the new
+// stress there (each thread only the components whose stencil lies inside
+// the footprint: the footprint's outer ring supplies exactly the stress the
+// inner velocity update needs), and updates velocity and viscosity at plane
+// kk-1 of the inner threads.  Stress and velocity are double buffered, so
+// every global read is of the previous iteration and the workgroups are
+// independent.  Each field is read and written about once per iteration:
+// z neighbours come from registers, x/y neighbours from LDS.
+// Global accesses of the fused sweep use 32-bit byte offsets from the
+// uniform field base (SGPR), which maps to gfx950's saddr + voffset
+// addressing and keeps one VGPR per offset instead of a 64-bit address per
+// field.  The launcher only takes this path when every field is < 4 GiB.
+template <typename T>
+__device__ __forceinline__ T ldo(const T* base, uint32_t bo)
+{
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + bo);
+}
+template <typename T>
+__device__ __forceinline__ void sto(T* base, uint32_t bo, T v)
+{
+    *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + bo) = v;
+}
+
+template <typename T, bool IB, int TY>
+__global__ __launch_bounds__(64 * TY) void k_fused3(StokesArgs<T> a, FusedGeom fg)
+{
+    constexpr int TX = 64;
+    __shared__ T sVx[TY][TX], sVy[TY][TX], sVz[TY][TX];
+    __shared__ T sE[3][TY][TX];
+    __shared__ T sS[2][6][TY][TX]; // P txx tyy txy txz tyz
+
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int i = fg.lo[0] - 1 + (int)blockIdx.x * (TX - 2) + tx;
+    const int j = fg.lo[1] - 1 + (int)blockIdx.y * (TY - 2) + ty;
+    const int ks = fg.lo[2] + (int)blockIdx.z * fg.kch;
+    const int ke = min(ks + fg.kch - 1, fg.hi[2]);
+    const bool inner = tx >= 1 && tx <= TX - 2 && ty >= 1 && ty <= TY - 2 && i <= fg.hi[0] && j <= fg.hi[1];
+    const bool can_c = tx < TX - 1 && ty < TY - 1;
+    const bool can_xy = tx > 0 && ty > 0;
+    const bool can_xz = tx > 0;
+    const bool can_yz = ty > 0;
+    constexpr uint32_t E = sizeof(T);
+    const uint32_t bsy = (uint32_t)a.sy * E, bsz = (uint32_t)a.sz * E;
+    // footprint positions past hi+1 are never read by inner threads: clamp their addresses
+    const int ic = min(i, fg.hi[0] + 1), jc = min(j, fg.hi[1] + 1);
+    const uint32_t col = ((uint32_t)ic + (uint32_t)a.sy * (uint32_t)jc) * E;
+    const T ix = a.ix, iy = a.iy, iz = a.iz;
+
+    // registers of the own column
+    const uint32_t c2 = col + bsz * (uint32_t)(ks - 2), c1 = c2 + bsz, c3 = c2 - bsz;
+    T vxm = ldo(a.Vx, c2), vym = ldo(a.Vy, c2), vzm = ldo(a.Vz, c2);
+    T vx0 = ldo(a.Vx, c1), vy0 = ldo(a.Vy, c1), vz0 = ldo(a.Vz, c1);
+    T vzp = ldo(a.Vz, c1 + bsz);
+    T emm = ldo(a.eta, c3), em = ldo(a.eta, c2), e0 = ldo(a.eta, c1);
+    T oP = ldo(a.P, c1), oxx = ldo(a.txx, c1), oyy = ldo(a.tyy, c1), ozz = ldo(a.tzz, c1), oxy = ldo(a.txy, c1),
+      oxz = ldo(a.txz, c1), oyz = ldo(a.tyz, c1);
+    T gx = T(0), gy = T(0), gz = T(0);
+    T Pk = T(0), xxk = T(0), yyk = T(0), zzk = T(0), Pkm = T(0), zzkm = T(0);
+    sE[(ks - 2) % 3][ty][tx] = em;
+
+    for (int kk = ks - 1; kk <= ke + 1; ++kk) {
+        const uint32_t c = col + bsz * (uint32_t)kk;
+        // A: publish the plane-kk inputs
+        sVx[ty][tx] = vx0;
+        sVy[ty][tx] = vy0;
+        sVz[ty][tx] = vz0;
+        sE[kk % 3][ty][tx] = e0;
+        // prefetch plane kk+1 (velocity z at kk+2, gravity at kk)
+        T nvx = T(0), nvy = T(0), nvz2 = T(0), ne = T(0), nP = T(0), nxx = T(0), nyy = T(0), nzz = T(0), nxy = T(0),
+          nxz = T(0), nyz = T(0), ngx = T(0), ngy = T(0), ngz = T(0);
+        if (kk <= ke) {
+            const uint32_t cn = c + bsz;
+            nvx = ldo(a.Vx, cn);
+            nvy = ldo(a.Vy, cn);
+            nvz2 = ldo(a.Vz, cn + bsz);
+            ne = ldo(a.eta, cn);
+            nP = ldo(a.P, cn);
+            nxx = ldo(a.txx, cn);
+            nyy = ldo(a.tyy, cn);
+            nzz = ldo(a.tzz, cn);
+            nxy = ldo(a.txy, cn);
+            nxz = ldo(a.txz, cn);
+            nyz = ldo(a.tyz, cn);
+        }
+        if (a.grav_field) {
+            ngx = ldo(a.gx, c);
+            ngy = ldo(a.gy, c);
+            ngz = ldo(a.gz, c);
+        }
+        __syncthreads();
+
+        // B: new stress at plane kk
+        const int e_k = kk % 3, e_km = (kk + 2) % 3;
+        T Pc = T(0), xxc = T(0), yyc = T(0), zzc = T(0), xyc = T(0), xzc = T(0), yzc = T(0);
+        if (!IB) {
+            if (can_c) {
+                T exx = dif(sVx[ty][tx + 1], vx0, ix);
+                T eyy = dif(sVy[ty + 1][tx], vy0, iy);
+                T ezz = dif(vzp, vz0, iz);
+                T divV = exx + eyy + ezz;
+                Pc = upd_P(oP, divV, e0, a.dPr);
+                xxc = upd_td(oxx, e0, exx, divV, a.dtau[0]);
+                yyc = upd_td(oyy, e0, eyy, divV, a.dtau[1]);
+                zzc = upd_td(ozz, e0, ezz, divV, a.dtau[2]);
+            }
+            if (can_xy) {
+                T exy = T(0.5) * (dif(vy0, sVy[ty][tx - 1], ix) + dif(vx0, sVx[ty - 1][tx], iy));
+                T el = lerp4(sE[e_k][ty - 1][tx - 1], sE[e_k][ty - 1][tx], sE[e_k][ty][tx - 1], e0);
+                xyc = upd_to(oxy, el, exy, a.dtau[3]);
+            }
+            if (can_xz) {
+                T exz = T(0.5) * (dif(vz0, sVz[ty][tx - 1], ix) + dif(vx0, vxm, iz));
+                T el = lerp4(sE[e_km][ty][tx - 1], em, sE[e_k][ty][tx - 1], e0);
+                xzc = upd_to(oxz, el, exz, a.dtau[4]);
+            }
+            if (can_yz) {
+                T eyz = T(0.5) * (dif(vz0, sVz[ty - 1][tx], iy) + dif(vy0, vym, iz));
+                T el = lerp4(sE[e_km][ty - 1][tx], em, sE[e_k][ty - 1][tx], e0);
+                yzc = upd_to(oyz, el, eyz, a.dtau[5]);
+            }
+        } else {
+            const T *wx = a.ns[OM_VCC], *wy = a.ns[OM_CVC], *wz = a.ns[OM_CCV];
+            const unsigned fl = ldo(a.flags, c / E);
+            const T wx0 = ldo(wx, c), wy0 = ldo(wy, c), wz0 = ldo(wz, c);
+            if (can_c && (fl & F_CCC)) {
+                T exx = dif(ldo(wx, c + E) * sVx[ty][tx + 1], wx0 * vx0, ix);
+                T eyy = dif(ldo(wy, c + bsy) * sVy[ty + 1][tx], wy0 * vy0, iy);
+                T ezz = dif(ldo(wz, c + bsz) * vzp, wz0 * vz0, iz);
+                T divV = exx + eyy + ezz;
+                Pc = upd_P(oP, divV, e0, a.dPr);
+                xxc = upd_td(oxx, e0, exx, divV, a.dtau[0]);
+                yyc = upd_td(oyy, e0, eyy, divV, a.dtau[1]);
+                zzc = upd_td(ozz, e0, ezz, divV, a.dtau[2]);
+            }
+            if (can_xy && (fl & F_VVC)) {
+                T exy = T(0.5) * (dif(wy0 * vy0, ldo(wy, c - E) * sVy[ty][tx - 1], ix) + dif(wx0 * vx0, ldo(wx, c - bsy) * sVx[ty - 1][tx], iy));
+                T el = lerp4(sE[e_k][ty - 1][tx - 1], sE[e_k][ty - 1][tx], sE[e_k][ty][tx - 1], e0);
+                xyc = upd_to(oxy, el, exy, a.dtau[3]);
+            }
+            if (can_xz && (fl & F_VCV)) {
+                T exz = T(0.5) * (dif(wz0 * vz0, ldo(wz, c - E) * sVz[ty][tx - 1], ix) + dif(wx0 * vx0, ldo(wx, c - bsz) * vxm, iz));
+                T el = lerp4(sE[e_km][ty][tx - 1], em, sE[e_k][ty][tx - 1], e0);
+                xzc = upd_to(oxz, el, exz, a.dtau[4]);
+            }
+            if (can_yz && (fl & F_CVV)) {
+                T eyz = T(0.5) * (dif(wz0 * vz0, ldo(wz, c - bsy) * sVz[ty - 1][tx], iy) + dif(wy0 * vy0, ldo(wy, c - bsz) * vym, iz));
+                T el = lerp4(sE[e_km][ty - 1][tx], em, sE[e_k][ty - 1][tx], e0);
+                yzc = upd_to(oyz, el, eyz, a.dtau[5]);
+            }
+        }
+        const int b = kk & 1;
+        sS[b][0][ty][tx] = Pc;
+        sS[b][1][ty][tx] = xxc;
+        sS[b][2][ty][tx] = yyc;
+        sS[b][3][ty][tx] = xyc;
+        sS[b][4][ty][tx] = xzc;
+        sS[b][5][ty][tx] = yzc;
+        if (inner && kk >= ks && kk <= ke) {
+            sto(a.Pn, c, Pc);
+            sto(a.txxn, c, xxc);
+            sto(a.tyyn, c, yyc);
+            sto(a.tzzn, c, zzc);
+            sto(a.txyn, c, xyc);
+            sto(a.txzn, c, xzc);
+            sto(a.tyzn, c, yzc);
+        }
+        __syncthreads();
+
+        // C: velocity and viscosity at plane k = kk-1
+        const int k = kk - 1;
+        if (inner && k >= ks) {
+            const uint32_t cv = c - bsz;
+            const int bo = k & 1, bn = kk & 1, ek = k % 3;
+            const T P_ = Pk, P_im = sS[bo][0][ty][tx - 1], P_jm = sS[bo][0][ty - 1][tx];
+            const T xx_im = sS[bo][1][ty][tx - 1], yy_jm = sS[bo][2][ty - 1][tx];
+            const T xy = sS[bo][3][ty][tx], xy_ip = sS[bo][3][ty][tx + 1], xy_jp = sS[bo][3][ty + 1][tx],
+                    xy_ijp = sS[bo][3][ty + 1][tx + 1];
+            const T xz = sS[bo][4][ty][tx], xz_ip = sS[bo][4][ty][tx + 1], xz_kp = sS[bn][4][ty][tx],
+                    xz_ikp = sS[bn][4][ty][tx + 1];
+            const T yz = sS[bo][5][ty][tx], yz_jp = sS[bo][5][ty + 1][tx], yz_kp = sS[bn][5][ty][tx],
+                    yz_jkp = sS[bn][5][ty + 1][tx];
+            const T ex = T(0.5) * (sE[ek][ty][tx - 1] + em);
+            const T ey = T(0.5) * (sE[ek][ty - 1][tx] + em);
+            const T ez = T(0.5) * (emm + em);
+            const T g0 = a.grav_field ? gx : a.grav[0];
+            const T g1 = a.grav_field ? gy : a.grav[1];
+            const T g2 = a.grav_field ? gz : a.grav[2];
+            if (!IB) {
+                sto(a.Vxn, cv, upd_V(vxm, dif(P_, P_im, ix), dif(xxk, xx_im, ix), dif(xy_jp, xy, iy), dif(xz_kp, xz, iz), g0, ex, a.dV[0]));
+                sto(a.Vyn, cv, upd_V(vym, dif(P_, P_jm, iy), dif(yyk, yy_jm, iy), dif(xy_ip, xy, ix), dif(yz_kp, yz, iz), g1, ey, a.dV[1]));
+                sto(a.Vzn, cv, upd_V(vzm, dif(P_, Pkm, iz), dif(zzk, zzkm, iz), dif(xz_ip, xz, ix), dif(yz_jp, yz, iy), g2, ez, a.dV[2]));
+            } else {
+                const T *wc = a.na[OM_CCC], *wxy = a.na[OM_VVC], *wxz = a.na[OM_VCV], *wyz = a.na[OM_CVV];
+                const unsigned fl = ldo(a.flags, cv / E);
+                const T wc0 = ldo(wc, cv), wxy0 = ldo(wxy, cv), wxz0 = ldo(wxz, cv), wyz0 = ldo(wyz, cv);
+                sto(a.Vxn, cv, (fl & F_VCC) ? upd_V_ib(vxm, dif(wc0 * P_, ldo(wc, cv - E) * P_im, ix), dif(wc0 * xxk, ldo(wc, cv - E) * xx_im, ix),
+                                                      dif(ldo(wxy, cv + bsy) * xy_jp, wxy0 * xy, iy), dif(ldo(wxz, cv + bsz) * xz_kp, wxz0 * xz, iz),
+                                                      g0, ex, ldo(a.ns[OM_VCC], cv), a.dV[0])
+                                           : T(0));
+                sto(a.Vyn, cv, (fl & F_CVC) ? upd_V_ib(vym, dif(wc0 * P_, ldo(wc, cv - bsy) * P_jm, iy), dif(wc0 * yyk, ldo(wc, cv - bsy) * yy_jm, iy),
+                                                      dif(ldo(wxy, cv + E) * xy_ip, wxy0 * xy, ix), dif(ldo(wyz, cv + bsz) * yz_kp, wyz0 * yz, iz),
+                                                      g1, ey, ldo(a.ns[OM_CVC], cv), a.dV[1])
+                                           : T(0));
+                sto(a.Vzn, cv, (fl & F_CCV) ? upd_V_ib(vzm, dif(wc0 * P_, ldo(wc, cv - bsz) * Pkm, iz), dif(wc0 * zzk, ldo(wc, cv - bsz) * zzkm, iz),
+                                                      dif(ldo(wxz, cv + E) * xz_ip, wxz0 * xz, ix), dif(ldo(wyz, cv + bsy) * yz_jp, wyz0 * yz, iy),
+                                                      g2, ez, ldo(a.ns[OM_CCV], cv), a.dV[2])
+                                           : T(0));
+            }
+            const T t = tauII(xxk, yyk, zzk, lerp4(xy, xy_ip, xy_jp, xy_ijp), lerp4(xz, xz_ip, xz_kp, xz_ikp),
+                              lerp4(yz, yz_jp, yz_kp, yz_jkp));
+            sto(a.eta_next, cv, rheology_o(a, cv, t));
+        }
+
+        // rotate the column registers
+        Pkm = Pk;
+        zzkm = zzk;
+        Pk = Pc;
+        xxk = xxc;
+        yyk = yyc;
+        zzk = zzc;
+        vxm = vx0;
+        vym = vy0;
+        vzm = vz0;
+        vx0 = nvx;
+        vy0 = nvy;
+        vz0 = vzp;
+        vzp = nvz2;
+        emm = em;
+        em = e0;
+        e0 = ne;
+        oP = nP;
+        oxx = nxx;
+        oyy = nyy;
+        ozz = nzz;
+        oxy = nxy;
+        oxz = nxz;
+        oyz = nyz;
+        gx = ngx;
+        gy = ngy;
+        gz = ngz;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// residuals and compute_tau (3D pointwise, full work range)
 // ---------------------------------------------------------------------------
 #define IDX3                                                                      \
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;                      \
@@ -137,125 +513,6 @@ __device__ __forceinline__ bool valid2(const T* const* w, const T* const* w2, in
     if (i > a.nx + 1 || j > a.ny + 1) return;                                     \
     const long sy = a.sy, sz = a.sz;                                              \
     const long c = (long)i + sy * j + sz * k;
-
-// update_sigma!: FullStokes kernels_3d.jl:3-25, IB kernels_3d.jl:141-182
-template <typename T, bool IB>
-__global__ __launch_bounds__(256) void k_sigma3(StokesArgs<T> a)
-{
-    IDX3
-    const T ix = a.ix, iy = a.iy, iz = a.iz;
-    T *P = a.P, *txx = a.txx, *tyy = a.tyy, *tzz = a.tzz, *txy = a.txy, *txz = a.txz, *tyz = a.tyz;
-    const T *Vx = a.Vx, *Vy = a.Vy, *Vz = a.Vz, *eta = a.eta;
-    if (!IB) {
-        T exx = (Vx[c + 1] - Vx[c]) * ix;
-        T eyy = (Vy[c + sy] - Vy[c]) * iy;
-        T ezz = (Vz[c + sz] - Vz[c]) * iz;
-        T exy = T(0.5) * ((Vy[c] - Vy[c - 1]) * ix + (Vx[c] - Vx[c - sy]) * iy);
-        T exz = T(0.5) * ((Vz[c] - Vz[c - 1]) * ix + (Vx[c] - Vx[c - sz]) * iz);
-        T eyz = T(0.5) * ((Vz[c] - Vz[c - sy]) * iy + (Vy[c] - Vy[c - sz]) * iz);
-        T divV = exx + eyy + ezz;
-        T ec = eta[c];
-        T exy_eta = T(0.5) * (T(0.5) * (eta[c - 1 - sy] + eta[c - sy]) + T(0.5) * (eta[c - 1] + eta[c]));
-        T exz_eta = T(0.5) * (T(0.5) * (eta[c - 1 - sz] + eta[c - sz]) + T(0.5) * (eta[c - 1] + eta[c]));
-        T eyz_eta = T(0.5) * (T(0.5) * (eta[c - sy - sz] + eta[c - sz]) + T(0.5) * (eta[c - sy] + eta[c]));
-        P[c] = P[c] - divV * ec * a.dPr;
-        txx[c] = txx[c] - (txx[c] - T(2) * ec * (exx - divV / T(3))) * a.dtau[0];
-        tyy[c] = tyy[c] - (tyy[c] - T(2) * ec * (eyy - divV / T(3))) * a.dtau[1];
-        tzz[c] = tzz[c] - (tzz[c] - T(2) * ec * (ezz - divV / T(3))) * a.dtau[2];
-        txy[c] = txy[c] - (txy[c] - T(2) * exy_eta * exy) * a.dtau[3];
-        txz[c] = txz[c] - (txz[c] - T(2) * exz_eta * exz) * a.dtau[4];
-        tyz[c] = tyz[c] - (tyz[c] - T(2) * eyz_eta * eyz) * a.dtau[5];
-    } else {
-        const T* const* na = a.na;
-        const T* const* ns = a.ns;
-        const T *wx = ns[OM_VCC], *wy = ns[OM_CVC], *wz = ns[OM_CCV];
-        if (valid3(na, ns, L_CCC, c, sy, sz)) {
-            T exx = (wx[c + 1] * Vx[c + 1] - wx[c] * Vx[c]) * ix;
-            T eyy = (wy[c + sy] * Vy[c + sy] - wy[c] * Vy[c]) * iy;
-            T ezz = (wz[c + sz] * Vz[c + sz] - wz[c] * Vz[c]) * iz;
-            T divV = exx + eyy + ezz;
-            T ec = eta[c];
-            P[c] = P[c] - divV * ec * a.dPr;
-            txx[c] = txx[c] - (txx[c] - T(2) * ec * (exx - divV / T(3))) * a.dtau[0];
-            tyy[c] = tyy[c] - (tyy[c] - T(2) * ec * (eyy - divV / T(3))) * a.dtau[1];
-            tzz[c] = tzz[c] - (tzz[c] - T(2) * ec * (ezz - divV / T(3))) * a.dtau[2];
-        } else {
-            P[c] = T(0); txx[c] = T(0); tyy[c] = T(0); tzz[c] = T(0);
-        }
-        if (valid3(na, ns, L_VVC, c, sy, sz)) {
-            T exy = T(0.5) * ((wy[c] * Vy[c] - wy[c - 1] * Vy[c - 1]) * ix + (wx[c] * Vx[c] - wx[c - sy] * Vx[c - sy]) * iy);
-            T e = T(0.5) * (T(0.5) * (eta[c - 1 - sy] + eta[c - sy]) + T(0.5) * (eta[c - 1] + eta[c]));
-            txy[c] = txy[c] - (txy[c] - T(2) * e * exy) * a.dtau[3];
-        } else txy[c] = T(0);
-        if (valid3(na, ns, L_VCV, c, sy, sz)) {
-            T exz = T(0.5) * ((wz[c] * Vz[c] - wz[c - 1] * Vz[c - 1]) * ix + (wx[c] * Vx[c] - wx[c - sz] * Vx[c - sz]) * iz);
-            T e = T(0.5) * (T(0.5) * (eta[c - 1 - sz] + eta[c - sz]) + T(0.5) * (eta[c - 1] + eta[c]));
-            txz[c] = txz[c] - (txz[c] - T(2) * e * exz) * a.dtau[4];
-        } else txz[c] = T(0);
-        if (valid3(na, ns, L_CVV, c, sy, sz)) {
-            T eyz = T(0.5) * ((wz[c] * Vz[c] - wz[c - sy] * Vz[c - sy]) * iy + (wy[c] * Vy[c] - wy[c - sz] * Vy[c - sz]) * iz);
-            T e = T(0.5) * (T(0.5) * (eta[c - sy - sz] + eta[c - sz]) + T(0.5) * (eta[c - sy] + eta[c]));
-            tyz[c] = tyz[c] - (tyz[c] - T(2) * e * eyz) * a.dtau[5];
-        } else tyz[c] = T(0);
-    }
-}
-
-// tau_II at a cell centre: kernels_3d.jl:162-165
-template <typename T>
-__device__ __forceinline__ T tauII3(const StokesArgs<T>& a, long c, long sy, long sz)
-{
-    const T *txy = a.txy, *txz = a.txz, *tyz = a.tyz;
-    T p = a.txx[c], q = a.tyy[c], r = a.tzz[c];
-    T lxy = T(0.5) * (T(0.5) * (txy[c] + txy[c + 1]) + T(0.5) * (txy[c + sy] + txy[c + 1 + sy]));
-    T lxz = T(0.5) * (T(0.5) * (txz[c] + txz[c + 1]) + T(0.5) * (txz[c + sz] + txz[c + 1 + sz]));
-    T lyz = T(0.5) * (T(0.5) * (tyz[c] + tyz[c + sy]) + T(0.5) * (tyz[c + sz] + tyz[c + sy + sz]));
-    return sqrt(T(0.5) * (p * p + q * q + r * r) + lxy * lxy + lxz * lxz + lyz * lyz);
-}
-
-// update_V!: FullStokes kernels_3d.jl:154-167, IB kernels_3d.jl:184-212
-template <typename T, bool IB>
-__global__ __launch_bounds__(256) void k_velocity3(StokesArgs<T> a)
-{
-    IDX3
-    const T ix = a.ix, iy = a.iy, iz = a.iz;
-    const T *P = a.P, *txx = a.txx, *tyy = a.tyy, *tzz = a.tzz, *txy = a.txy, *txz = a.txz, *tyz = a.tyz;
-    T *Vx = a.Vx, *Vy = a.Vy, *Vz = a.Vz;
-    const T* eta = a.eta;
-    T gx = a.grav_field ? a.gx[c] : a.grav[0];
-    T gy = a.grav_field ? a.gy[c] : a.grav[1];
-    T gz = a.grav_field ? a.gz[c] : a.grav[2];
-    T ex = T(0.5) * (eta[c - 1] + eta[c]);
-    T ey = T(0.5) * (eta[c - sy] + eta[c]);
-    T ez = T(0.5) * (eta[c - sz] + eta[c]);
-    if (!IB) {
-        Vx[c] = Vx[c] + (-((P[c] - P[c - 1]) * ix) + (txx[c] - txx[c - 1]) * ix + (txy[c + sy] - txy[c]) * iy +
-                         (txz[c + sz] - txz[c]) * iz - gx) / ex * a.dV[0];
-        Vy[c] = Vy[c] + (-((P[c] - P[c - sy]) * iy) + (tyy[c] - tyy[c - sy]) * iy + (txy[c + 1] - txy[c]) * ix +
-                         (tyz[c + sz] - tyz[c]) * iz - gy) / ey * a.dV[1];
-        Vz[c] = Vz[c] + (-((P[c] - P[c - sz]) * iz) + (tzz[c] - tzz[c - sz]) * iz + (txz[c + 1] - txz[c]) * ix +
-                         (tyz[c + sy] - tyz[c]) * iy - gz) / ez * a.dV[2];
-    } else {
-        const T* const* na = a.na;
-        const T* const* ns = a.ns;
-        const T *wc = na[OM_CCC], *wxy = na[OM_VVC], *wxz = na[OM_VCV], *wyz = na[OM_CVV];
-        if (valid3(ns, na, L_VCC, c, sy, sz))
-            Vx[c] = Vx[c] + (-((wc[c] * P[c] - wc[c - 1] * P[c - 1]) * ix) + (wc[c] * txx[c] - wc[c - 1] * txx[c - 1]) * ix +
-                             (wxy[c + sy] * txy[c + sy] - wxy[c] * txy[c]) * iy +
-                             (wxz[c + sz] * txz[c + sz] - wxz[c] * txz[c]) * iz - gx) / ex / ns[OM_VCC][c] * a.dV[0];
-        else Vx[c] = T(0);
-        if (valid3(ns, na, L_CVC, c, sy, sz))
-            Vy[c] = Vy[c] + (-((wc[c] * P[c] - wc[c - sy] * P[c - sy]) * iy) + (wc[c] * tyy[c] - wc[c - sy] * tyy[c - sy]) * iy +
-                             (wxy[c + 1] * txy[c + 1] - wxy[c] * txy[c]) * ix +
-                             (wyz[c + sz] * tyz[c + sz] - wyz[c] * tyz[c]) * iz - gy) / ey / ns[OM_CVC][c] * a.dV[1];
-        else Vy[c] = T(0);
-        if (valid3(ns, na, L_CCV, c, sy, sz))
-            Vz[c] = Vz[c] + (-((wc[c] * P[c] - wc[c - sz] * P[c - sz]) * iz) + (wc[c] * tzz[c] - wc[c - sz] * tzz[c - sz]) * iz +
-                             (wxz[c + 1] * txz[c + 1] - wxz[c] * txz[c]) * ix +
-                             (wyz[c + sy] * tyz[c + sy] - wyz[c] * tyz[c]) * iy - gz) / ez / ns[OM_CCV][c] * a.dV[2];
-        else Vz[c] = T(0);
-    }
-    a.eta_next[c] = rheology(a, c, tauII3(a, c, sy, sz));
-}
 
 // compute_residuals!: FullStokes kernels_3d.jl:193-202, IB kernels_3d.jl:256-281
 template <typename T, bool IB>
@@ -281,19 +538,20 @@ __global__ __launch_bounds__(256) void k_residuals3(StokesArgs<T> a)
         const T* const* ns = a.ns;
         const T *wx = ns[OM_VCC], *wy = ns[OM_CVC], *wz = ns[OM_CCV];
         const T *wc = na[OM_CCC], *wxy = na[OM_VVC], *wxz = na[OM_VCV], *wyz = na[OM_CVV];
-        if (valid3(na, ns, L_CCC, c, sy, sz))
+        const unsigned fl = a.flags[c];
+        if (fl & F_CCC)
             a.rP[c] = (wx[c + 1] * Vx[c + 1] - wx[c] * Vx[c]) * ix + (wy[c + sy] * Vy[c + sy] - wy[c] * Vy[c]) * iy +
                       (wz[c + sz] * Vz[c + sz] - wz[c] * Vz[c]) * iz;
         else a.rP[c] = T(0);
-        if (valid3(ns, na, L_VCC, c, sy, sz))
+        if (fl & F_VCC)
             a.rVx[c] = -((wc[c] * P[c] - wc[c - 1] * P[c - 1]) * ix) + (wc[c] * txx[c] - wc[c - 1] * txx[c - 1]) * ix +
                        (wxy[c + sy] * txy[c + sy] - wxy[c] * txy[c]) * iy + (wxz[c + sz] * txz[c + sz] - wxz[c] * txz[c]) * iz - gx;
         else a.rVx[c] = T(0);
-        if (valid3(ns, na, L_CVC, c, sy, sz))
+        if (fl & F_CVC)
             a.rVy[c] = -((wc[c] * P[c] - wc[c - sy] * P[c - sy]) * iy) + (wc[c] * tyy[c] - wc[c - sy] * tyy[c - sy]) * iy +
                        (wxy[c + 1] * txy[c + 1] - wxy[c] * txy[c]) * ix + (wyz[c + sz] * tyz[c + sz] - wyz[c] * tyz[c]) * iz - gy;
         else a.rVy[c] = T(0);
-        if (valid3(ns, na, L_CCV, c, sy, sz))
+        if (fl & F_CCV)
             a.rVz[c] = -((wc[c] * P[c] - wc[c - sz] * P[c - sz]) * iz) + (wc[c] * tzz[c] - wc[c - sz] * tzz[c - sz]) * iz +
                        (wxz[c + 1] * txz[c + 1] - wxz[c] * txz[c]) * ix + (wyz[c + sy] * tyz[c + sy] - wyz[c] * tyz[c]) * iy - gz;
         else a.rVz[c] = T(0);
@@ -301,17 +559,18 @@ __global__ __launch_bounds__(256) void k_residuals3(StokesArgs<T> a)
 }
 
 // compute_tau!: kernels_3d.jl:171-191 (IB variant: kernels_3d.jl:216-254 with
-// its undefined `Pr` location read as the Center location)
+// its undefined `Pr` location read as the Center location); writes the
+// current stress (txxn == txx)
 template <typename T, bool IB>
 __global__ __launch_bounds__(256) void k_tau3(StokesArgs<T> a)
 {
     IDX3
     const T ix = a.ix, iy = a.iy, iz = a.iz;
     const T *Vx = a.Vx, *Vy = a.Vy, *Vz = a.Vz, *eta = a.eta;
-    const T* const* na = a.na;
     const T* const* ns = a.ns;
+    const unsigned fl = IB ? a.flags[c] : 0xffu;
     T ec = eta[c];
-    if (!IB || valid3(na, ns, L_CCC, c, sy, sz)) {
+    if (fl & F_CCC) {
         T exx, eyy, ezz;
         if (IB) {
             exx = (ns[OM_VCC][c + 1] * Vx[c + 1] - ns[OM_VCC][c] * Vx[c]) * ix;
@@ -323,36 +582,113 @@ __global__ __launch_bounds__(256) void k_tau3(StokesArgs<T> a)
             ezz = (Vz[c + sz] - Vz[c]) * iz;
         }
         T divV = exx + eyy + ezz;
-        a.txx[c] = T(2) * ec * (exx - divV / T(3));
-        a.tyy[c] = T(2) * ec * (eyy - divV / T(3));
-        a.tzz[c] = T(2) * ec * (ezz - divV / T(3));
+        a.txxn[c] = T(2) * ec * (exx - divV / T(3));
+        a.tyyn[c] = T(2) * ec * (eyy - divV / T(3));
+        a.tzzn[c] = T(2) * ec * (ezz - divV / T(3));
     } else {
-        a.txx[c] = T(0); a.tyy[c] = T(0); a.tzz[c] = T(0);
+        a.txxn[c] = T(0); a.tyyn[c] = T(0); a.tzzn[c] = T(0);
     }
-    if (!IB || valid3(na, ns, L_VVC, c, sy, sz)) {
+    if (fl & F_VVC) {
         T exy;
         if (IB) exy = T(0.5) * ((ns[OM_CVC][c] * Vy[c] - ns[OM_CVC][c - 1] * Vy[c - 1]) * ix +
                                 (ns[OM_VCC][c] * Vx[c] - ns[OM_VCC][c - sy] * Vx[c - sy]) * iy);
         else exy = T(0.5) * ((Vy[c] - Vy[c - 1]) * ix + (Vx[c] - Vx[c - sy]) * iy);
-        T e = T(0.5) * (T(0.5) * (eta[c - 1 - sy] + eta[c - sy]) + T(0.5) * (eta[c - 1] + eta[c]));
-        a.txy[c] = T(2) * e * exy;
-    } else a.txy[c] = T(0);
-    if (!IB || valid3(na, ns, L_VCV, c, sy, sz)) {
+        a.txyn[c] = T(2) * lerp4(eta[c - 1 - sy], eta[c - sy], eta[c - 1], eta[c]) * exy;
+    } else a.txyn[c] = T(0);
+    if (fl & F_VCV) {
         T exz;
         if (IB) exz = T(0.5) * ((ns[OM_CCV][c] * Vz[c] - ns[OM_CCV][c - 1] * Vz[c - 1]) * ix +
                                 (ns[OM_VCC][c] * Vx[c] - ns[OM_VCC][c - sz] * Vx[c - sz]) * iz);
         else exz = T(0.5) * ((Vz[c] - Vz[c - 1]) * ix + (Vx[c] - Vx[c - sz]) * iz);
-        T e = T(0.5) * (T(0.5) * (eta[c - 1 - sz] + eta[c - sz]) + T(0.5) * (eta[c - 1] + eta[c]));
-        a.txz[c] = T(2) * e * exz;
-    } else a.txz[c] = T(0);
-    if (!IB || valid3(na, ns, L_CVV, c, sy, sz)) {
+        a.txzn[c] = T(2) * lerp4(eta[c - 1 - sz], eta[c - sz], eta[c - 1], eta[c]) * exz;
+    } else a.txzn[c] = T(0);
+    if (fl & F_CVV) {
         T eyz;
         if (IB) eyz = T(0.5) * ((ns[OM_CCV][c] * Vz[c] - ns[OM_CCV][c - sy] * Vz[c - sy]) * iy +
                                 (ns[OM_CVC][c] * Vy[c] - ns[OM_CVC][c - sz] * Vy[c - sz]) * iz);
         else eyz = T(0.5) * ((Vz[c] - Vz[c - sy]) * iy + (Vy[c] - Vy[c - sz]) * iz);
-        T e = T(0.5) * (T(0.5) * (eta[c - sy - sz] + eta[c - sz]) + T(0.5) * (eta[c - sy] + eta[c]));
-        a.tyz[c] = T(2) * e * eyz;
-    } else a.tyz[c] = T(0);
+        a.tyzn[c] = T(2) * lerp4(eta[c - sy - sz], eta[c - sz], eta[c - sy], eta[c]) * eyz;
+    } else a.tyzn[c] = T(0);
+}
+
+// IB validity bits, once per geometry (null_spaces.jl:1-43)
+__device__ __forceinline__ bool lt(const double* w, long c) { return w[c] < 1e-6; }
+
+__device__ bool null3(const double* const* w, int loc, long c, long sy, long sz)
+{
+    switch (loc) {
+    case L_CCC: return lt(w[OM_VCC], c) || lt(w[OM_VCC], c + 1) || lt(w[OM_CVC], c) || lt(w[OM_CVC], c + sy) ||
+                       lt(w[OM_CCV], c) || lt(w[OM_CCV], c + sz);
+    case L_VCC: return lt(w[OM_CCC], c - 1) || lt(w[OM_CCC], c) || lt(w[OM_VVC], c) || lt(w[OM_VVC], c + sy) ||
+                       lt(w[OM_VCV], c) || lt(w[OM_VCV], c + sz);
+    case L_CVC: return lt(w[OM_VVC], c) || lt(w[OM_VVC], c + 1) || lt(w[OM_CCC], c - sy) || lt(w[OM_CCC], c) ||
+                       lt(w[OM_CVV], c) || lt(w[OM_CVV], c + sz);
+    case L_CCV: return lt(w[OM_VCV], c) || lt(w[OM_VCV], c + 1) || lt(w[OM_CVV], c) || lt(w[OM_CVV], c + sy) ||
+                       lt(w[OM_CCC], c - sz) || lt(w[OM_CCC], c);
+    case L_VVC: return lt(w[OM_CVC], c - 1) || lt(w[OM_CVC], c) || lt(w[OM_VCC], c - sy) || lt(w[OM_VCC], c);
+    case L_VCV: return lt(w[OM_CCV], c - 1) || lt(w[OM_CCV], c) || lt(w[OM_VCC], c - sz) || lt(w[OM_VCC], c);
+    case L_CVV: return lt(w[OM_CCV], c - sy) || lt(w[OM_CCV], c) || lt(w[OM_CVC], c - sz) || lt(w[OM_CVC], c);
+    }
+    return true;
+}
+
+__device__ bool null2(const double* const* w, int loc, long c, long sy)
+{
+    switch (loc) {
+    case L_CCC: return lt(w[OM2_VC], c) || lt(w[OM2_VC], c + 1) || lt(w[OM2_CV], c) || lt(w[OM2_CV], c + sy);
+    case L_VCC: return lt(w[OM2_CC], c - 1) || lt(w[OM2_CC], c) || lt(w[OM2_VV], c) || lt(w[OM2_VV], c + sy);
+    case L_CVC: return lt(w[OM2_VV], c) || lt(w[OM2_VV], c + 1) || lt(w[OM2_CC], c - sy) || lt(w[OM2_CC], c);
+    case L_VVC: return lt(w[OM2_CV], c - 1) || lt(w[OM2_CV], c) || lt(w[OM2_VC], c - sy) || lt(w[OM2_VC], c);
+    }
+    return true;
+}
+
+struct MaskPtrs {
+    const double* na[8];
+    const double* ns[8];
+};
+
+__global__ __launch_bounds__(256) void k_flags(MaskPtrs m, uint8_t* flags, int nd, int nx, int ny, long sy, long sz)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = (nd == 3) ? (int)blockIdx.z + 1 : 0;
+    if (i > nx + 1 || j > ny + 1) return;
+    const long c = (long)i + sy * j + sz * k;
+    unsigned f = 0;
+    if (nd == 3) {
+        // (location, component, bit, sigma side?)
+        const int locs[7] = {L_CCC, L_VVC, L_VCV, L_CVV, L_VCC, L_CVC, L_CCV};
+        const int comp[7] = {OM_CCC, OM_VVC, OM_VCV, OM_CVV, OM_VCC, OM_CVC, OM_CCV};
+        for (int q = 0; q < 7; ++q) {
+            const double* const* w = (q < 4) ? m.na : m.ns;  // isvalid(w) ...
+            const double* const* w2 = (q < 4) ? m.ns : m.na; // ... && !isempty(w2)
+            bool v = !(null3(w, locs[q], c, sy, sz) || lt(w[comp[q]], c)) && !lt(w2[comp[q]], c);
+            if (v) f |= 1u << q;
+        }
+    } else {
+        const int locs[4] = {L_CCC, L_VVC, L_VCC, L_CVC};
+        const int comp[4] = {OM2_CC, OM2_VV, OM2_VC, OM2_CV};
+        const unsigned bit[4] = {F_CCC, F_VVC, F_VCC, F_CVC};
+        for (int q = 0; q < 4; ++q) {
+            const double* const* w = (q < 2) ? m.na : m.ns;
+            const double* const* w2 = (q < 2) ? m.ns : m.na;
+            bool v = !(null2(w, locs[q], c, sy) || lt(w[comp[q]], c)) && !lt(w2[comp[q]], c);
+            if (v) f |= bit[q];
+        }
+    }
+    flags[c] = (uint8_t)f;
+}
+
+void launch_ib_flags(const Geo& g, const double* const na[8], const double* const ns[8], uint8_t* flags, hipStream_t s)
+{
+    MaskPtrs m;
+    for (int q = 0; q < 8; ++q) {
+        m.na[q] = na[q];
+        m.ns[q] = ns[q];
+    }
+    dim3 blk(64, 4, 1), grd((g.n[0] + 1 + 63) / 64, (g.n[1] + 1 + 3) / 4, g.nd == 3 ? g.n[2] + 1 : 1);
+    hipLaunchKernelGGL(k_flags, grd, blk, 0, s, m, flags, g.nd, g.n[0], g.n[1], g.sy, g.nd == 3 ? g.sz : 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -370,38 +706,36 @@ __global__ __launch_bounds__(256) void k_sigma2(StokesArgs<T> a)
 {
     IDX2
     const T ix = a.ix, iy = a.iy;
-    T *P = a.P, *txx = a.txx, *tyy = a.tyy, *txy = a.txy;
     const T *Vx = a.Vx, *Vy = a.Vy, *eta = a.eta;
-    T ec = eta[c];
-    T exy_eta = T(0.5) * (T(0.5) * (eta[c - 1 - sy] + eta[c - sy]) + T(0.5) * (eta[c - 1] + eta[c]));
+    const T ec = eta[c];
+    const T exy_eta = lerp4(eta[c - 1 - sy], eta[c - sy], eta[c - 1], eta[c]);
     if (!IB) {
         // kernels_2d.jl:205-220
-        T exx = (Vx[c + 1] - Vx[c]) * ix;
-        T eyy = (Vy[c + sy] - Vy[c]) * iy;
-        T exy = T(0.5) * ((Vy[c] - Vy[c - 1]) * ix + (Vx[c] - Vx[c - sy]) * iy);
+        T exx = dif(Vx[c + 1], Vx[c], ix);
+        T eyy = dif(Vy[c + sy], Vy[c], iy);
+        T exy = T(0.5) * (dif(Vy[c], Vy[c - 1], ix) + dif(Vx[c], Vx[c - sy], iy));
         T divV = exx + eyy;
-        P[c] = P[c] - divV * ec * a.dPr;
-        txx[c] = txx[c] - (txx[c] - T(2) * ec * (exx - divV / T(3))) * a.dtau[0];
-        tyy[c] = tyy[c] - (tyy[c] - T(2) * ec * (eyy - divV / T(3))) * a.dtau[1];
-        txy[c] = txy[c] - (txy[c] - T(2) * exy_eta * exy) * a.dtau[2];
+        a.Pn[c] = upd_P(a.P[c], divV, ec, a.dPr);
+        a.txxn[c] = upd_td(a.txx[c], ec, exx, divV, a.dtau[0]);
+        a.tyyn[c] = upd_td(a.tyy[c], ec, eyy, divV, a.dtau[1]);
+        a.txyn[c] = upd_to(a.txy[c], exy_eta, exy, a.dtau[2]);
     } else {
-        const T* const* na = a.na;
-        const T* const* ns = a.ns;
-        const T *wx = ns[OM2_VC], *wy = ns[OM2_CV];
-        if (valid2(na, ns, L_CCC, c, sy)) {
-            T exx = (wx[c + 1] * Vx[c + 1] - wx[c] * Vx[c]) * ix;
-            T eyy = (wy[c + sy] * Vy[c + sy] - wy[c] * Vy[c]) * iy;
+        const T *wx = a.ns[OM2_VC], *wy = a.ns[OM2_CV];
+        const unsigned fl = a.flags[c];
+        if (fl & F_CCC) {
+            T exx = dif(wx[c + 1] * Vx[c + 1], wx[c] * Vx[c], ix);
+            T eyy = dif(wy[c + sy] * Vy[c + sy], wy[c] * Vy[c], iy);
             T divV = exx + eyy;
-            P[c] = P[c] - divV * ec * a.dPr;
-            txx[c] = txx[c] - (txx[c] - T(2) * ec * (exx - divV / T(3))) * a.dtau[0];
-            tyy[c] = tyy[c] - (tyy[c] - T(2) * ec * (eyy - divV / T(3))) * a.dtau[1];
+            a.Pn[c] = upd_P(a.P[c], divV, ec, a.dPr);
+            a.txxn[c] = upd_td(a.txx[c], ec, exx, divV, a.dtau[0]);
+            a.tyyn[c] = upd_td(a.tyy[c], ec, eyy, divV, a.dtau[1]);
         } else {
-            P[c] = T(0); txx[c] = T(0); tyy[c] = T(0);
+            a.Pn[c] = T(0); a.txxn[c] = T(0); a.tyyn[c] = T(0);
         }
-        if (valid2(na, ns, L_VVC, c, sy)) {
-            T exy = T(0.5) * ((wy[c] * Vy[c] - wy[c - 1] * Vy[c - 1]) * ix + (wx[c] * Vx[c] - wx[c - sy] * Vx[c - sy]) * iy);
-            txy[c] = txy[c] - (txy[c] - T(2) * exy_eta * exy) * a.dtau[2];
-        } else txy[c] = T(0);
+        if (fl & F_VVC) {
+            T exy = T(0.5) * (dif(wy[c] * Vy[c], wy[c - 1] * Vy[c - 1], ix) + dif(wx[c] * Vx[c], wx[c - sy] * Vx[c - sy], iy));
+            a.txyn[c] = upd_to(a.txy[c], exy_eta, exy, a.dtau[2]);
+        } else a.txyn[c] = T(0);
     }
 }
 
@@ -410,33 +744,30 @@ __global__ __launch_bounds__(256) void k_velocity2(StokesArgs<T> a)
 {
     IDX2
     const T ix = a.ix, iy = a.iy;
-    const T *P = a.P, *txx = a.txx, *tyy = a.tyy, *txy = a.txy, *eta = a.eta;
-    T *Vx = a.Vx, *Vy = a.Vy;
+    const T *P = a.Pn, *txx = a.txxn, *tyy = a.tyyn, *txy = a.txyn, *eta = a.eta;
     T gx = a.grav_field ? a.gx[c] : a.grav[0];
     T gy = a.grav_field ? a.gy[c] : a.grav[1];
     // each component interpolates eta to its own location (kernels_2d.jl:226-227
     // reuses location(V.x) for both)
     T ex = T(0.5) * (eta[c - 1] + eta[c]);
     T ey = T(0.5) * (eta[c - sy] + eta[c]);
+    const T zero = T(0);
     if (!IB) {
-        Vx[c] = Vx[c] + (-((P[c] - P[c - 1]) * ix) + (txx[c] - txx[c - 1]) * ix + (txy[c + sy] - txy[c]) * iy - gx) / ex * a.dV[0];
-        Vy[c] = Vy[c] + (-((P[c] - P[c - sy]) * iy) + (tyy[c] - tyy[c - sy]) * iy + (txy[c + 1] - txy[c]) * ix - gy) / ey * a.dV[1];
+        a.Vxn[c] = upd_V(a.Vx[c], dif(P[c], P[c - 1], ix), dif(txx[c], txx[c - 1], ix), dif(txy[c + sy], txy[c], iy), zero, gx, ex, a.dV[0]);
+        a.Vyn[c] = upd_V(a.Vy[c], dif(P[c], P[c - sy], iy), dif(tyy[c], tyy[c - sy], iy), dif(txy[c + 1], txy[c], ix), zero, gy, ey, a.dV[1]);
     } else {
-        const T* const* na = a.na;
-        const T* const* ns = a.ns;
-        const T *wc = na[OM2_CC], *wv = na[OM2_VV];
-        if (valid2(ns, na, L_VCC, c, sy))
-            Vx[c] = Vx[c] + (-((wc[c] * P[c] - wc[c - 1] * P[c - 1]) * ix) + (wc[c] * txx[c] - wc[c - 1] * txx[c - 1]) * ix +
-                             (wv[c + sy] * txy[c + sy] - wv[c] * txy[c]) * iy - gx) / ex / ns[OM2_VC][c] * a.dV[0];
-        else Vx[c] = T(0);
-        if (valid2(ns, na, L_CVC, c, sy))
-            Vy[c] = Vy[c] + (-((wc[c] * P[c] - wc[c - sy] * P[c - sy]) * iy) + (wc[c] * tyy[c] - wc[c - sy] * tyy[c - sy]) * iy +
-                             (wv[c + 1] * txy[c + 1] - wv[c] * txy[c]) * ix - gy) / ey / ns[OM2_CV][c] * a.dV[1];
-        else Vy[c] = T(0);
+        const T *wc = a.na[OM2_CC], *wv = a.na[OM2_VV];
+        const unsigned fl = a.flags[c];
+        a.Vxn[c] = (fl & F_VCC) ? upd_V_ib(a.Vx[c], dif(wc[c] * P[c], wc[c - 1] * P[c - 1], ix), dif(wc[c] * txx[c], wc[c - 1] * txx[c - 1], ix),
+                                           dif(wv[c + sy] * txy[c + sy], wv[c] * txy[c], iy), zero, gx, ex, a.ns[OM2_VC][c], a.dV[0])
+                                : T(0);
+        a.Vyn[c] = (fl & F_CVC) ? upd_V_ib(a.Vy[c], dif(wc[c] * P[c], wc[c - sy] * P[c - sy], iy), dif(wc[c] * tyy[c], wc[c - sy] * tyy[c - sy], iy),
+                                           dif(wv[c + 1] * txy[c + 1], wv[c] * txy[c], ix), zero, gy, ey, a.ns[OM2_CV][c], a.dV[1])
+                                : T(0);
     }
     // kernels_2d.jl:229-230
     T p = txx[c], q = tyy[c];
-    T l = T(0.5) * (T(0.5) * (txy[c] + txy[c + 1]) + T(0.5) * (txy[c + sy] + txy[c + 1 + sy]));
+    T l = lerp4(txy[c], txy[c + 1], txy[c + sy], txy[c + 1 + sy]);
     a.eta_next[c] = rheology(a, c, sqrt(T(0.5) * (p * p + q * q) + l * l));
 }
 
@@ -454,17 +785,16 @@ __global__ __launch_bounds__(256) void k_residuals2(StokesArgs<T> a)
         a.rVx[c] = -((P[c] - P[c - 1]) * ix) + (txx[c] - txx[c - 1]) * ix + (txy[c + sy] - txy[c]) * iy - gx;
         a.rVy[c] = -((P[c] - P[c - sy]) * iy) + (tyy[c] - tyy[c - sy]) * iy + (txy[c + 1] - txy[c]) * ix - gy;
     } else {
-        const T* const* na = a.na;
-        const T* const* ns = a.ns;
-        const T *wx = ns[OM2_VC], *wy = ns[OM2_CV], *wc = na[OM2_CC], *wv = na[OM2_VV];
-        if (valid2(na, ns, L_CCC, c, sy))
+        const T *wx = a.ns[OM2_VC], *wy = a.ns[OM2_CV], *wc = a.na[OM2_CC], *wv = a.na[OM2_VV];
+        const unsigned fl = a.flags[c];
+        if (fl & F_CCC)
             a.rP[c] = (wx[c + 1] * Vx[c + 1] - wx[c] * Vx[c]) * ix + (wy[c + sy] * Vy[c + sy] - wy[c] * Vy[c]) * iy;
         else a.rP[c] = T(0);
-        if (valid2(ns, na, L_VCC, c, sy))
+        if (fl & F_VCC)
             a.rVx[c] = -((wc[c] * P[c] - wc[c - 1] * P[c - 1]) * ix) + (wc[c] * txx[c] - wc[c - 1] * txx[c - 1]) * ix +
                        (wv[c + sy] * txy[c + sy] - wv[c] * txy[c]) * iy - gx;
         else a.rVx[c] = T(0);
-        if (valid2(ns, na, L_CVC, c, sy))
+        if (fl & F_CVC)
             a.rVy[c] = -((wc[c] * P[c] - wc[c - sy] * P[c - sy]) * iy) + (wc[c] * tyy[c] - wc[c - sy] * tyy[c - sy]) * iy +
                        (wv[c + 1] * txy[c + 1] - wv[c] * txy[c]) * ix - gy;
         else a.rVy[c] = T(0);
@@ -477,11 +807,10 @@ __global__ __launch_bounds__(256) void k_tau2(StokesArgs<T> a)
     IDX2
     const T ix = a.ix, iy = a.iy;
     const T *Vx = a.Vx, *Vy = a.Vy, *eta = a.eta;
-    const T* const* na = a.na;
     const T* const* ns = a.ns;
+    const unsigned fl = IB ? a.flags[c] : 0xffu;
     T ec = eta[c];
-    T exy_eta = T(0.5) * (T(0.5) * (eta[c - 1 - sy] + eta[c - sy]) + T(0.5) * (eta[c - 1] + eta[c]));
-    if (!IB || valid2(na, ns, L_CCC, c, sy)) {
+    if (fl & F_CCC) {
         T exx, eyy;
         if (IB) {
             exx = (ns[OM2_VC][c + 1] * Vx[c + 1] - ns[OM2_VC][c] * Vx[c]) * ix;
@@ -491,61 +820,161 @@ __global__ __launch_bounds__(256) void k_tau2(StokesArgs<T> a)
             eyy = (Vy[c + sy] - Vy[c]) * iy;
         }
         T divV = exx + eyy;
-        a.txx[c] = T(2) * ec * (exx - divV / T(3));
-        a.tyy[c] = T(2) * ec * (eyy - divV / T(3));
+        a.txxn[c] = T(2) * ec * (exx - divV / T(3));
+        a.tyyn[c] = T(2) * ec * (eyy - divV / T(3));
     } else {
-        a.txx[c] = T(0); a.tyy[c] = T(0);
+        a.txxn[c] = T(0); a.tyyn[c] = T(0);
     }
-    if (!IB || valid2(na, ns, L_VVC, c, sy)) {
+    if (fl & F_VVC) {
         T exy;
         if (IB) exy = T(0.5) * ((ns[OM2_CV][c] * Vy[c] - ns[OM2_CV][c - 1] * Vy[c - 1]) * ix +
                                 (ns[OM2_VC][c] * Vx[c] - ns[OM2_VC][c - sy] * Vx[c - sy]) * iy);
         else exy = T(0.5) * ((Vy[c] - Vy[c - 1]) * ix + (Vx[c] - Vx[c - sy]) * iy);
-        a.txy[c] = T(2) * exy_eta * exy;
-    } else a.txy[c] = T(0);
+        a.txyn[c] = T(2) * lerp4(eta[c - 1 - sy], eta[c - sy], eta[c - 1], eta[c]) * exy;
+    } else a.txyn[c] = T(0);
 }
 
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-static inline dim3 block3() { return dim3(64, 4, 1); }
-static inline dim3 grid3(const Geo& g) { return dim3((g.n[0] + 1 + 63) / 64, (g.n[1] + 1 + 3) / 4, g.n[2] + 1); }
 static inline dim3 block2() { return dim3(64, 4, 1); }
 static inline dim3 grid2(const Geo& g) { return dim3((g.n[0] + 1 + 63) / 64, (g.n[1] + 1 + 3) / 4, 1); }
+static inline dim3 grid3(const Geo& g) { return dim3((g.n[0] + 1 + 63) / 64, (g.n[1] + 1 + 3) / 4, g.n[2] + 1); }
 
-#define DISPATCH(K3, K2)                                                                          \
-    if (g.nd == 3) {                                                                              \
-        if (ib) hipLaunchKernelGGL((K3<T, true>), grid3(g), block3(), 0, s, a);                   \
-        else hipLaunchKernelGGL((K3<T, false>), grid3(g), block3(), 0, s, a);                     \
-    } else {                                                                                      \
-        if (ib) hipLaunchKernelGGL((K2<T, true>), grid2(g), block2(), 0, s, a);                   \
-        else hipLaunchKernelGGL((K2<T, false>), grid2(g), block2(), 0, s, a);                     \
+static void finish_boxes(BoxList& bl)
+{
+    bl.start[0] = 0;
+    for (int q = 0; q < bl.nb; ++q) {
+        const Box& b = bl.b[q];
+        const int nbx = (b.i1 - b.i0 + 64) / 64, nby = (b.j1 - b.j0 + 4) / 4, nbz = b.k1 - b.k0 + 1;
+        bl.start[q + 1] = bl.start[q] + nbx * nby * nbz;
     }
+}
+
+BoxList full_box(const Geo& g)
+{
+    BoxList bl;
+    bl.nb = 1;
+    bl.b[0] = Box{1, g.n[0] + 1, 1, g.n[1] + 1, 1, g.n[2] + 1};
+    finish_boxes(bl);
+    return bl;
+}
+
+bool fused_split(const Geo& g, int ty, int kch, FusedGeom& fg, BoxList& shell)
+{
+    // interior [3, n-1] in every dimension, trimmed to whole workgroup tiles in x and y
+    const int tile[2] = {64 - 2, ty - 2};
+    int lo[3], hi[3];
+    for (int d = 0; d < 3; ++d) {
+        lo[d] = 3;
+        hi[d] = g.n[d] - 1;
+        if (d < 2) {
+            const int m = (hi[d] - lo[d] + 1) / tile[d];
+            hi[d] = lo[d] + m * tile[d] - 1;
+        }
+        if (hi[d] < lo[d]) {
+            shell = full_box(g);
+            return false;
+        }
+    }
+    for (int d = 0; d < 3; ++d) {
+        fg.lo[d] = lo[d];
+        fg.hi[d] = hi[d];
+    }
+    fg.kch = kch;
+    const int N0 = g.n[0] + 1, N1 = g.n[1] + 1, N2 = g.n[2] + 1;
+    BoxList& bl = shell;
+    bl.nb = 0;
+    auto add = [&](int i0, int i1, int j0, int j1, int k0, int k1) {
+        if (i0 <= i1 && j0 <= j1 && k0 <= k1) bl.b[bl.nb++] = Box{i0, i1, j0, j1, k0, k1};
+    };
+    add(1, N0, 1, N1, 1, lo[2] - 1);                       // z low slab
+    add(1, N0, 1, N1, hi[2] + 1, N2);                      // z high slab
+    add(1, N0, 1, lo[1] - 1, lo[2], hi[2]);                // y low
+    add(1, N0, hi[1] + 1, N1, lo[2], hi[2]);               // y high
+    add(1, lo[0] - 1, lo[1], hi[1], lo[2], hi[2]);         // x low
+    add(hi[0] + 1, N0, lo[1], hi[1], lo[2], hi[2]);        // x high
+    finish_boxes(bl);
+    return true;
+}
 
 template <typename T>
-void launch_sigma(const Geo& g, const StokesArgs<T>& a, bool ib, hipStream_t s)
+void launch_sigma_boxes(const Geo& g, const StokesArgs<T>& a, const BoxList& bl, bool ib, hipStream_t s)
 {
-    DISPATCH(k_sigma3, k_sigma2)
+    if (g.nd == 3) {
+        if (bl.nb == 0 || bl.start[bl.nb] == 0) return;
+        dim3 grd(bl.start[bl.nb]);
+        if (ib) hipLaunchKernelGGL((k_sigma_box<T, true>), grd, dim3(64, 4, 1), 0, s, a, bl);
+        else hipLaunchKernelGGL((k_sigma_box<T, false>), grd, dim3(64, 4, 1), 0, s, a, bl);
+    } else {
+        if (ib) hipLaunchKernelGGL((k_sigma2<T, true>), grid2(g), block2(), 0, s, a);
+        else hipLaunchKernelGGL((k_sigma2<T, false>), grid2(g), block2(), 0, s, a);
+    }
 }
+
 template <typename T>
-void launch_velocity(const Geo& g, const StokesArgs<T>& a, bool ib, hipStream_t s)
+void launch_velocity_boxes(const Geo& g, const StokesArgs<T>& a, const BoxList& bl, bool ib, hipStream_t s)
 {
-    DISPATCH(k_velocity3, k_velocity2)
+    if (g.nd == 3) {
+        if (bl.nb == 0 || bl.start[bl.nb] == 0) return;
+        dim3 grd(bl.start[bl.nb]);
+        if (ib) hipLaunchKernelGGL((k_velocity_box<T, true>), grd, dim3(64, 4, 1), 0, s, a, bl);
+        else hipLaunchKernelGGL((k_velocity_box<T, false>), grd, dim3(64, 4, 1), 0, s, a, bl);
+    } else {
+        if (ib) hipLaunchKernelGGL((k_velocity2<T, true>), grid2(g), block2(), 0, s, a);
+        else hipLaunchKernelGGL((k_velocity2<T, false>), grid2(g), block2(), 0, s, a);
+    }
 }
+
+template <typename T>
+constexpr int fused_ty()
+{
+    return sizeof(T) == 8 ? 8 : 16;
+}
+
+template <typename T>
+void launch_fused(const Geo& g, const StokesArgs<T>& a, const FusedGeom& fg, bool ib, hipStream_t s)
+{
+    constexpr int TY = fused_ty<T>();
+    const int nbx = (fg.hi[0] - fg.lo[0] + 1) / (64 - 2);
+    const int nby = (fg.hi[1] - fg.lo[1] + 1) / (TY - 2);
+    const int nbz = (fg.hi[2] - fg.lo[2] + fg.kch) / fg.kch;
+    dim3 grd(nbx, nby, nbz), blk(64, TY, 1);
+    if (ib) hipLaunchKernelGGL((k_fused3<T, true, TY>), grd, blk, 0, s, a, fg);
+    else hipLaunchKernelGGL((k_fused3<T, false, TY>), grd, blk, 0, s, a, fg);
+}
+
 template <typename T>
 void launch_residuals(const Geo& g, const StokesArgs<T>& a, bool ib, hipStream_t s)
 {
-    DISPATCH(k_residuals3, k_residuals2)
+    if (g.nd == 3) {
+        if (ib) hipLaunchKernelGGL((k_residuals3<T, true>), grid3(g), block2(), 0, s, a);
+        else hipLaunchKernelGGL((k_residuals3<T, false>), grid3(g), block2(), 0, s, a);
+    } else {
+        if (ib) hipLaunchKernelGGL((k_residuals2<T, true>), grid2(g), block2(), 0, s, a);
+        else hipLaunchKernelGGL((k_residuals2<T, false>), grid2(g), block2(), 0, s, a);
+    }
 }
+
 template <typename T>
 void launch_tau(const Geo& g, const StokesArgs<T>& a, bool ib, hipStream_t s)
 {
-    DISPATCH(k_tau3, k_tau2)
+    if (g.nd == 3) {
+        if (ib) hipLaunchKernelGGL((k_tau3<T, true>), grid3(g), block2(), 0, s, a);
+        else hipLaunchKernelGGL((k_tau3<T, false>), grid3(g), block2(), 0, s, a);
+    } else {
+        if (ib) hipLaunchKernelGGL((k_tau2<T, true>), grid2(g), block2(), 0, s, a);
+        else hipLaunchKernelGGL((k_tau2<T, false>), grid2(g), block2(), 0, s, a);
+    }
 }
 
-template void launch_sigma<double>(const Geo&, const StokesArgs<double>&, bool, hipStream_t);
-template void launch_velocity<double>(const Geo&, const StokesArgs<double>&, bool, hipStream_t);
-template void launch_residuals<double>(const Geo&, const StokesArgs<double>&, bool, hipStream_t);
-template void launch_tau<double>(const Geo&, const StokesArgs<double>&, bool, hipStream_t);
+#define INST(T)                                                                                               \
+    template void launch_sigma_boxes<T>(const Geo&, const StokesArgs<T>&, const BoxList&, bool, hipStream_t);    \
+    template void launch_velocity_boxes<T>(const Geo&, const StokesArgs<T>&, const BoxList&, bool, hipStream_t); \
+    template void launch_fused<T>(const Geo&, const StokesArgs<T>&, const FusedGeom&, bool, hipStream_t);        \
+    template void launch_residuals<T>(const Geo&, const StokesArgs<T>&, bool, hipStream_t);                      \
+    template void launch_tau<T>(const Geo&, const StokesArgs<T>&, bool, hipStream_t);
+INST(double)
+INST(float)
 
 } // namespace fi

@@ -312,6 +312,11 @@ class IsothermalFullStokesModel:
     def field(self, name):
         return DeviceField(self, A.F[name])
 
+    def set_option(self, key, value):
+        """Kernel-path option (fused interior sweep on/off, z planes per workgroup)."""
+        k = {"fused": A.FI_OPT_FUSED, "zchunk": A.FI_OPT_ZCHUNK}.get(key, key)
+        A.check(A.lib().fi_set_option(self._h, int(k), int(value)))
+
 
 def advance_iteration(model, t=0.0, dt=1.0):
     """advance_iteration!(model, t, Δt) (Isothermal.jl:91-112); t and Δt are unused there too."""

@@ -157,6 +157,13 @@ int fi_synchronize(fi_model_t m);
  * kernel, velocity kernel, boundary-condition kernels} averages in ms. */
 int fi_time_iterations(fi_model_t m, int n_iter, double out_ms[4]);
 
+/* Kernel-path options (no counterpart in the reference; results do not depend
+ * on them):  FI_OPT_FUSED 1/0 selects the fused interior sweep or the
+ * pointwise sweep everywhere; FI_OPT_ZCHUNK sets the z planes per workgroup of
+ * the fused sweep. */
+enum { FI_OPT_FUSED = 1, FI_OPT_ZCHUNK = 2 };
+int fi_set_option(fi_model_t m, int key, int value);
+
 int fi_levelset_from_dem(fi_model_t m, int phase, const double* dem_parent, int dem_nx, int dem_ny,
                          const double dem_origin[2], const double dem_extent[2], const double R[9]);
 int fi_invert_levelset(fi_model_t m, int phase);

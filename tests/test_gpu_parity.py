@@ -161,3 +161,68 @@ def test_immersed_boundary_iterations_3d(pkg, orc):
     assert not bad, worst
     np.testing.assert_allclose(model.compute_residuals(), ref.compute_residuals(), rtol=TOL, atol=0)
     model.close()
+
+
+# --- fused interior sweep (grids with at least one 62-wide x tile) ---------
+
+@pytest.mark.parametrize("zchunk", [32, 5])
+def test_fused_manufactured_3d(pkg, orc, zchunk):
+    _say("case: fused sweep, 3D manufactured 70x36x24, zchunk %d, 300 iterations" % zchunk)
+    kw, cad = pkg.manufactured_3d((70, 36, 24))
+    model, ref = make_pair(pkg, orc, kw, seed=11)
+    model.set_option("zchunk", zchunk)
+    for m in (model, ref):
+        m.apply_bc("eta")
+    model.advance(300)
+    ref.advance(300)
+    worst, bad = compare_state(model, ref, state_names(3) + ["ETA", "ETA_NEXT"], TOL)
+    assert not bad, worst
+    np.testing.assert_allclose(model.compute_residuals(), ref.compute_residuals(), rtol=TOL, atol=0)
+    model.close()
+
+
+def test_fused_equals_pointwise_bitwise(pkg, orc):
+    _say("case: fused vs pointwise sweep, mixed BCs + power law, 130x20x17")
+    grid = pkg.UniformGrid(origin=(-0.5, -0.3, 0.0), extent=(1.3, 0.5, 0.4), dims=(130, 20, 17))
+    bcs = dict(x=(pkg.SBC(0.0, 0.1, -0.2), pkg.TBC(0.3, -0.1, 0.05)),
+               y=(pkg.VBC(0.02, 0.0, -0.01), pkg.SBC(0.0, 0.2, 0.1)),
+               z=(pkg.VBC(0.0, 0.0, 0.0), pkg.TBC(0.0, 0.0, 0.0)))
+    rheo = pkg.ViscoplasticRegularisation(pkg.PowerLawRheology(0.7, 3), 50.0)
+    kw = dict(grid=grid, boundary_conditions=bcs, gravity=dict(x=0.1, y=-0.2, z=1.0), rheology=rheo,
+              solver_params=pkg.pt_params(grid, r=0.9, re_mech=5 * math.pi))
+    a, ref = make_pair(pkg, orc, kw, seed=4)
+    b, _ = make_pair(pkg, orc, kw, seed=4)
+    b.set_option("fused", 0)
+    for m in (a, b, ref):
+        m.apply_bc("eta")
+    a.advance(200)
+    b.advance(200)
+    ref.advance(200)
+    for name in state_names(3) + ["ETA"]:
+        assert np.array_equal(a.field(name).parent(), b.field(name).parent()), name
+    worst, bad = compare_state(a, ref, state_names(3) + ["ETA"], TOL)
+    assert not bad, worst
+    a.close()
+    b.close()
+
+
+def test_fused_immersed_boundary(pkg, orc):
+    _say("case: fused sweep, immersed boundary glacier 70x40x20, 300 iterations")
+    gl, grid, dgrid, model, ref = _glacier_ib_pair(pkg, orc, (70, 40, 20))
+    model.set_option("zchunk", 7)
+    pkg.compute_levelset_from_dem(model.level_sets.na, gl.surface, dgrid, grid)
+    pkg.compute_levelset_from_dem(model.level_sets.ns, gl.bed, dgrid, grid)
+    pkg.invert_levelset(model.level_sets.ns)
+    pkg.omega_from_psi(model.field_masks.ns, model.level_sets.ns)
+    pkg.omega_from_psi(model.field_masks.na, model.level_sets.na)
+    for ph, base in (("NA", pkg._abi.FI_F_OM_NA0), ("NS", pkg._abi.FI_F_OM_NS0)):
+        for q in range(8):
+            ref.set("OM_%s%d" % (ph, q), pkg.full_stokes.DeviceField(model, base + q).parent())
+    for m in (model, ref):
+        m.apply_bc("eta")
+    model.advance(300)
+    ref.advance(300)
+    worst, bad = compare_state(model, ref, state_names(3) + ["ETA"], TOL)
+    assert not bad, worst
+    np.testing.assert_allclose(model.compute_residuals(), ref.compute_residuals(), rtol=TOL, atol=0)
+    model.close()
