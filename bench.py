@@ -109,17 +109,20 @@ def main():
 
     # per-kernel HIP-event timing on the model's stream
     import ctypes as C
+    A = pkg._abi
     out = np.zeros(4)
     nprof = max(10, min(50, args.steps // 4))
-    pkg._abi.check(pkg._abi.lib().fi_time_iterations(model._h, nprof, pkg._abi.dptr(out)))
-    it_ms, sig_ms, vel_ms, bc_ms = out
-    # algorithmic bytes per launch (distinct arrays touched once, x cells)
-    sig_bytes = 18 * 8 * cells              # R: Vx Vy Vz P tau(6) eta; W: P tau(6)
-    vel_bytes = (15 + 3) * 8 * cells        # R: P tau(6) V(3) eta g(3); W: V(3) eta_next
-    if sig_ms >= vel_ms:
-        kname, kms, kbytes = "k_sigma3", sig_ms, sig_bytes
+    A.check(A.lib().fi_time_iterations(model._h, nprof, A.dptr(out)))
+    it_ms, fused_ms, shell_ms, bc_ms = out
+    lo, hi = (C.c_int * 3)(), (C.c_int * 3)()
+    fused_on = A.lib().fi_fused_region(model._h, lo, hi)
+    nfields = 25 if model.desc.gravity_is_field else 22
+    if fused_on:
+        # fused sweep: R sigma(7) V(3) eta [g(3)], W sigma(7) V(3) eta_next, once per interior cell
+        icells = (hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1)
+        kname, kms, kbytes = "k_fused3", fused_ms, nfields * 8 * icells
     else:
-        kname, kms, kbytes = "k_velocity3", vel_ms, vel_bytes
+        kname, kms, kbytes = "pointwise sweeps", shell_ms, (nfields + 11) * 8 * cells
     achieved = kbytes / (kms * 1e-3) / 1e9
 
     cpu = None
@@ -146,8 +149,8 @@ def main():
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel_ms": round(kms, 4), "algorithmic_bytes_per_launch": kbytes,
-                     "event_ms": {"iteration": round(it_ms, 4), "k_sigma3": round(sig_ms, 4),
-                                  "k_velocity3": round(vel_ms, 4), "bc": round(bc_ms, 4)}},
+                     "event_ms": {"iteration": round(it_ms, 4), "fused": round(fused_ms, 4),
+                                  "shell": round(shell_ms, 4), "bc": round(bc_ms, 4)}},
         "cpu_baseline": cpu,
     }
     model.close()

@@ -137,6 +137,7 @@ def _declare(lib):
         "fi_synchronize": (C.c_int, [H]),
         "fi_time_iterations": (C.c_int, [H, C.c_int, _DP]),
         "fi_set_option": (C.c_int, [H, C.c_int, C.c_int]),
+        "fi_fused_region": (C.c_int, [H, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "fi_levelset_from_dem": (C.c_int, [H, C.c_int, _DP, C.c_int, C.c_int, _DP, _DP, _DP]),
         "fi_invert_levelset": (C.c_int, [H, C.c_int]),
         "fi_omega_from_psi": (C.c_int, [H, C.c_int]),

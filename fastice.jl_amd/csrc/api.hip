@@ -342,18 +342,20 @@ void iterate(fi_model_s* m, hipEvent_t* ev)
     const bool ib = m->desc.immersed_boundary != 0;
     StokesArgs<double> a = make_args(m, false);
     const bool fused = m->use_fused && m->has_interior;
+    const BoxList boxes = fused ? m->shell : full_box(m->g);
     if (ev) (void)hipEventRecord(ev[0], m->stream);
     if (fused) launch_fused(m->g, a, m->fg, ib, m->stream);
-    launch_sigma_boxes(m->g, a, fused ? m->shell : full_box(m->g), ib, m->stream);
-    m->sig_cur ^= 1;
     if (ev) (void)hipEventRecord(ev[1], m->stream);
-    run_bc(m, FI_BCGROUP_STRESS);
+    launch_sigma_boxes(m->g, a, boxes, ib, m->stream);
+    m->sig_cur ^= 1;
     if (ev) (void)hipEventRecord(ev[2], m->stream);
-    launch_velocity_boxes(m->g, a, fused ? m->shell : full_box(m->g), ib, m->stream);
-    m->v_cur ^= 1;
+    run_bc(m, FI_BCGROUP_STRESS);
     if (ev) (void)hipEventRecord(ev[3], m->stream);
-    run_bc(m, FI_BCGROUP_VELOCITY);
+    launch_velocity_boxes(m->g, a, boxes, ib, m->stream);
+    m->v_cur ^= 1;
     if (ev) (void)hipEventRecord(ev[4], m->stream);
+    run_bc(m, FI_BCGROUP_VELOCITY);
+    if (ev) (void)hipEventRecord(ev[5], m->stream);
     m->eta_cur ^= 1;
 }
 
@@ -694,26 +696,33 @@ int fi_time_iterations(fi_model_t m, int n_iter, double out_ms[4])
     if (!m || !out_ms || n_iter < 1) return fail(FI_ERR_ARG, "bad arguments");
     int rc = prepare(m);
     if (rc) return rc;
-    hipEvent_t ev[5];
-    for (int q = 0; q < 5; ++q) HIPCHK(hipEventCreate(&ev[q]));
+    hipEvent_t ev[6];
+    for (int q = 0; q < 6; ++q) HIPCHK(hipEventCreate(&ev[q]));
     double acc[4] = {0, 0, 0, 0};
     for (int it = 0; it < n_iter; ++it) {
         iterate(m, ev);
-        HIPCHK(hipEventSynchronize(ev[4]));
-        float t, t1, t2;
-        HIPCHK(hipEventElapsedTime(&t, ev[0], ev[4]));
-        acc[0] += t;
-        HIPCHK(hipEventElapsedTime(&t, ev[0], ev[1]));
-        acc[1] += t;
-        HIPCHK(hipEventElapsedTime(&t, ev[2], ev[3]));
-        acc[2] += t;
-        HIPCHK(hipEventElapsedTime(&t1, ev[1], ev[2]));
-        HIPCHK(hipEventElapsedTime(&t2, ev[3], ev[4]));
-        acc[3] += t1 + t2;
+        HIPCHK(hipEventSynchronize(ev[5]));
+        float t[5];
+        for (int q = 0; q < 5; ++q) HIPCHK(hipEventElapsedTime(&t[q], ev[q], ev[q + 1]));
+        acc[0] += t[0] + t[1] + t[2] + t[3] + t[4];
+        acc[1] += t[0];
+        acc[2] += t[1] + t[3];
+        acc[3] += t[2] + t[4];
     }
-    for (int q = 0; q < 5; ++q) (void)hipEventDestroy(ev[q]);
+    for (int q = 0; q < 6; ++q) (void)hipEventDestroy(ev[q]);
     for (int q = 0; q < 4; ++q) out_ms[q] = acc[q] / n_iter;
     return FI_OK;
+}
+
+int fi_fused_region(fi_model_t m, int lo[3], int hi[3])
+{
+    if (!m || !lo || !hi) return fail(FI_ERR_ARG, "null argument");
+    const bool on = m->use_fused && m->has_interior;
+    for (int d = 0; d < 3; ++d) {
+        lo[d] = on ? m->fg.lo[d] : 0;
+        hi[d] = on ? m->fg.hi[d] : -1;
+    }
+    return on ? 1 : 0;
 }
 
 int fi_set_option(fi_model_t m, int key, int value)

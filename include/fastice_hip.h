@@ -153,9 +153,13 @@ int fi_synchronize(fi_model_t m);
 /* Measurement hook (no counterpart in the reference, whose timing is host
  * wall-clock around the loop, scripts_future_API/tm_stokes_mpi_wip.jl:151-183):
  * runs n_iter iterations with HIP events around every launch on the model's
- * stream, synchronising after each iteration.  out_ms = {iteration, stress
- * kernel, velocity kernel, boundary-condition kernels} averages in ms. */
+ * stream, synchronising after each iteration.  out_ms = {iteration, fused
+ * interior sweep, pointwise (shell) sweeps, boundary-condition kernels}
+ * averages in ms. */
 int fi_time_iterations(fi_model_t m, int n_iter, double out_ms[4]);
+/* The interior box [lo, hi] (1-based Chmy indices) swept by the fused kernel;
+ * returns 1 when the fused sweep is active, 0 otherwise. */
+int fi_fused_region(fi_model_t m, int lo[3], int hi[3]);
 
 /* Kernel-path options (no counterpart in the reference; results do not depend
  * on them):  FI_OPT_FUSED 1/0 selects the fused interior sweep or the

@@ -51,6 +51,9 @@ def make_pair(pkg, orc, kw, ib=False, eta=1.0, seed=None, amp=1e-2):
 def compare_state(model, ref, names, tol):
     worst = {}
     for name in names:
-        worst[name] = rel_err(model.field(name).parent(), ref.get(name))
+        b = ref.get(name)
+        if not np.all(np.isfinite(b)):
+            raise AssertionError("oracle state %s is not finite: the test case diverged" % name)
+        worst[name] = rel_err(model.field(name).parent(), b)
     bad = {k: v for k, v in worst.items() if not v <= tol}
     return worst, bad
