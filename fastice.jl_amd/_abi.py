@@ -142,6 +142,9 @@ def _declare(lib):
         "fi_invert_levelset": (C.c_int, [H, C.c_int]),
         "fi_omega_from_psi": (C.c_int, [H, C.c_int]),
         "fi_mem_info": (C.c_int, [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
+        "fi_refined_solve": (C.c_int, [H, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.POINTER(C.c_int), _DP]),
+        "fi_refined_advance_inner": (C.c_int, [H, C.c_int]),
+        "fi_refined_time_inner": (C.c_int, [H, C.c_int, _DP]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -8,11 +8,11 @@
 #include <cstring>
 #include <new>
 
-#include "fi_internal.h"
+#include "model.h"
 
 using namespace fi;
 
-namespace {
+namespace fi {
 
 thread_local char g_err[512] = "";
 
@@ -24,12 +24,6 @@ int fail(int code, const char* fmt, ...)
     va_end(ap);
     return code;
 }
-
-#define HIPCHK(expr)                                                                             \
-    do {                                                                                         \
-        hipError_t e_ = (expr);                                                                  \
-        if (e_ != hipSuccess) return fail(FI_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_));   \
-    } while (0)
 
 int field_loc(int nd, int id)
 {
@@ -56,58 +50,6 @@ int field_loc(int nd, int id)
     }
     return -1;
 }
-
-} // namespace
-
-struct fi_model_s {
-    fi_model_desc desc;
-    Geo g;
-    hipStream_t stream = nullptr;
-    double* base[FI_F_COUNT] = {};
-    double* f[FI_F_COUNT] = {};
-    // second buffer of the stress (P..TYZ) and velocity (VX..VZ) state
-    double* sbase[10] = {};
-    double* sf[10] = {};
-    int sig_cur = 0; // 1: current stress lives in sf
-    int v_cur = 0;   // 1: current velocity lives in sf
-    int eta_cur = 0; // 0: logical eta lives in buffer FI_F_ETA, 1: in FI_F_ETA_NEXT
-    bool shadow_dirty = true;
-    uint8_t* flags_base = nullptr;
-    uint8_t* flags = nullptr;
-    bool flags_dirty = true;
-    int use_fused = 1;
-    int kch = 32;
-    FusedGeom fg;
-    BoxList shell;
-    bool has_interior = false;
-    unsigned long long* d_norm = nullptr;
-    unsigned long long* h_norm = nullptr;
-    double* d_dem = nullptr;
-    size_t dem_elems = 0;
-
-    static bool is_sig(int id) { return id >= FI_F_P && id <= FI_F_TYZ; }
-    static bool is_vel(int id) { return id >= FI_F_VX && id <= FI_F_VZ; }
-    double* eta() { return f[eta_cur ? FI_F_ETA_NEXT : FI_F_ETA]; }
-    double* eta_next() { return f[eta_cur ? FI_F_ETA : FI_F_ETA_NEXT]; }
-    // physical buffer holding the current value of a logical field id
-    double* buf(int id)
-    {
-        if (id == FI_F_ETA) return eta();
-        if (id == FI_F_ETA_NEXT) return eta_next();
-        if (is_sig(id)) return sig_cur ? sf[id] : f[id];
-        if (is_vel(id)) return v_cur ? sf[id] : f[id];
-        return f[id];
-    }
-    // buffer the next iteration writes
-    double* alt(int id)
-    {
-        if (is_sig(id)) return sig_cur ? f[id] : sf[id];
-        if (is_vel(id)) return v_cur ? f[id] : sf[id];
-        return f[id];
-    }
-};
-
-namespace {
 
 bool field_used(const fi_model_desc& d, int id)
 {
@@ -196,96 +138,21 @@ StokesArgs<double> make_args(fi_model_s* m, bool in_place)
     return a;
 }
 
-void add_op(BcBatch& b, double* p, int L, int kind, double v)
-{
-    BcOp& o = b.op[b.nop++];
-    o.ptr = p;
-    o.L = L;
-    o.kind = kind;
-    o.f32 = 0;
-    o.v = v;
-}
-
-// Both sides of one dimension share a launch; an op that belongs to one side
-// only carries the side in its kind: 0/1 = both sides, 2/3 = low, 4/5 = high.
-void add_side_op(BcBatch& b, double* p, int L, int kind, double v, int side)
-{
-    add_op(b, p, L, kind + 2 * (side + 1), v);
-}
-
-// stress batch: boundary_conditions.jl:26-100
 BcBatch stress_batch(fi_model_s* m, int d)
 {
-    BcBatch b;
-    b.nop = 0;
-    b.d = d;
-    const fi_model_desc& ds = m->desc;
-    for (int s = 0; s < 2; ++s) {
-        const int kind = ds.bc_kind[d][s];
-        const double* c = ds.bc_value[d][s];
-        if (ds.ndims == 3) {
-            if (kind == FI_BC_TRACTION) {
-                double* diag = m->buf(d == 0 ? FI_F_TXX : d == 1 ? FI_F_TYY : FI_F_TZZ);
-                add_side_op(b, m->buf(FI_F_P), L_CCC, 0, c[d], s);
-                add_side_op(b, diag, L_CCC, 0, c[d], s);
-            }
-            if (kind == FI_BC_TRACTION || kind == FI_BC_SLIP) {
-                if (d == 0) {
-                    add_side_op(b, m->buf(FI_F_TXY), L_VVC, 0, c[1], s);
-                    add_side_op(b, m->buf(FI_F_TXZ), L_VCV, 0, c[2], s);
-                } else if (d == 1) {
-                    add_side_op(b, m->buf(FI_F_TXY), L_VVC, 0, c[0], s);
-                    add_side_op(b, m->buf(FI_F_TYZ), L_CVV, 0, c[2], s);
-                } else {
-                    add_side_op(b, m->buf(FI_F_TXZ), L_VCV, 0, c[0], s);
-                    add_side_op(b, m->buf(FI_F_TYZ), L_CVV, 0, c[1], s);
-                }
-            }
-        } else {
-            if (kind == FI_BC_TRACTION) {
-                double* diag = m->buf(d == 0 ? FI_F_TXX : FI_F_TYY);
-                add_side_op(b, m->buf(FI_F_P), L_CCC, 0, c[d], s);
-                add_side_op(b, diag, L_CCC, 0, c[d], s);
-            }
-            if (kind == FI_BC_TRACTION || kind == FI_BC_SLIP) add_side_op(b, m->buf(FI_F_TXY), L_VVC, 0, c[1 - d], s);
-        }
-    }
-    return b;
+    return stress_batch_g(m->desc, d, [m](int id) { return (void*)m->buf(id); }, false, 0);
 }
 
-// velocity batch + eta_next Neumann: boundary_conditions.jl:104-115, Isothermal.jl:103-104
 BcBatch velocity_batch(fi_model_s* m, int d, bool with_eta)
 {
-    BcBatch b;
-    b.nop = 0;
-    b.d = d;
-    const fi_model_desc& ds = m->desc;
-    const int idV[3] = {FI_F_VX, FI_F_VY, FI_F_VZ};
-    const int LV[3] = {L_VCC, L_CVC, L_CCV};
-    for (int s = 0; s < 2; ++s) {
-        const int kind = ds.bc_kind[d][s];
-        const double* c = ds.bc_value[d][s];
-        if (kind == FI_BC_VELOCITY) {
-            for (int q = 0; q < ds.ndims; ++q) add_side_op(b, m->buf(idV[q]), LV[q], 0, c[q], s);
-        } else if (kind == FI_BC_SLIP) {
-            add_side_op(b, m->buf(idV[d]), LV[d], 0, c[d], s);
-        }
-    }
+    BcBatch b = velocity_batch_g(m->desc, d, [m](int id) { return (void*)m->buf(id); }, false, 0);
     if (with_eta) add_op(b, m->eta_next(), L_CCC, 1, 0.0);
     return b;
 }
 
-// residual batch: boundary_conditions.jl:120-128
 BcBatch residual_batch(fi_model_s* m, int d)
 {
-    BcBatch b;
-    b.nop = 0;
-    b.d = d;
-    const int idR[3] = {FI_F_RVX, FI_F_RVY, FI_F_RVZ};
-    const int LV[3] = {L_VCC, L_CVC, L_CCV};
-    for (int s = 0; s < 2; ++s)
-        if (m->desc.bc_kind[d][s] != FI_BC_TRACTION) add_side_op(b, m->f[idR[d]], LV[d], 0, 0.0, s);
-    return b;
+    return residual_batch_g(m->desc, d, [m](int id) { return (void*)m->f[id]; }, 0);
 }
 
 BcBatch neumann_batch(double* p, int L, int d)
@@ -415,6 +282,7 @@ int check_desc(const fi_model_desc* d)
 
 void free_model(fi_model_s* m)
 {
+    free_refine(m);
     for (int q = 0; q < FI_F_COUNT; ++q)
         if (m->base[q]) (void)hipFree(m->base[q]);
     for (int q = 0; q < 10; ++q)
@@ -441,7 +309,7 @@ size_t parent_count(const Geo& g)
     return c;
 }
 
-} // namespace
+} // namespace fi
 
 extern "C" {
 
@@ -549,7 +417,10 @@ size_t fi_field_size(fi_model_t m, int id, int layout)
 static void mark_dirty(fi_model_t m, int id)
 {
     if (id >= FI_F_P && id <= FI_F_VZ) m->shadow_dirty = true;
-    if (id >= FI_F_OM_NA0 && id < FI_F_COUNT) m->flags_dirty = true;
+    if (id >= FI_F_OM_NA0 && id < FI_F_COUNT) {
+        m->flags_dirty = true;
+        ++m->mask_gen;
+    }
 }
 
 static int field_copy(fi_model_t m, int id, double* host, size_t count, int layout, bool up)
@@ -811,6 +682,7 @@ int fi_omega_from_psi(fi_model_t m, int phase)
     for (int q = 0; q < 8; ++q) om[q] = m->f[base + q];
     launch_omega_from_psi(m->g, psi, om, m->stream);
     m->flags_dirty = true;
+    ++m->mask_gen;
     HIPCHK(hipGetLastError());
     return FI_OK;
 }

@@ -49,6 +49,8 @@ struct StokesArgs {
     const T* na[8];
     const T* ns[8];
     const uint8_t* flags; // immersed-boundary validity bits (F_* below)
+    const T* fP;          // forcing of the continuity equation (defect correction), or null
+    int frozen_eta;       // 1: eta is fixed, eta_next is not written
     int nx, ny, nz;
     long sy, sz;
     T ix, iy, iz;
@@ -110,6 +112,8 @@ template <typename T>
 void launch_residuals(const Geo& g, const StokesArgs<T>& a, bool ib, hipStream_t s);
 template <typename T>
 void launch_tau(const Geo& g, const StokesArgs<T>& a, bool ib, hipStream_t s);
+template <typename T>
+void launch_eta_update(const Geo& g, const StokesArgs<T>& a, hipStream_t s);
 
 // boundary conditions, norms, fills (bc_norm_kernels.hip)
 void launch_bc_dim(const Geo& g, const BcBatch& b, hipStream_t s);

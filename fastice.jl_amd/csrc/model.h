@@ -1,0 +1,178 @@
+// model.h -- the model handle behind fi_model_t and the internal helpers
+// shared by the C ABI (api.hip) and the FP32 defect-correction solver
+// (refine.hip).
+#pragma once
+
+#include "fi_internal.h"
+
+#define HIPCHK(expr)                                                                             \
+    do {                                                                                         \
+        hipError_t e_ = (expr);                                                                  \
+        if (e_ != hipSuccess) return fi::fail(FI_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+namespace fi {
+struct Refine;
+int fail(int code, const char* fmt, ...);
+} // namespace fi
+
+using fi::BoxList;
+using fi::FusedGeom;
+using fi::Geo;
+
+struct fi_model_s {
+    fi_model_desc desc;
+    Geo g;
+    hipStream_t stream = nullptr;
+    double* base[FI_F_COUNT] = {};
+    double* f[FI_F_COUNT] = {};
+    // second buffer of the stress (P..TYZ) and velocity (VX..VZ) state
+    double* sbase[10] = {};
+    double* sf[10] = {};
+    int sig_cur = 0; // 1: current stress lives in sf
+    int v_cur = 0;   // 1: current velocity lives in sf
+    int eta_cur = 0; // 0: logical eta lives in buffer FI_F_ETA, 1: in FI_F_ETA_NEXT
+    bool shadow_dirty = true;
+    uint8_t* flags_base = nullptr;
+    uint8_t* flags = nullptr;
+    bool flags_dirty = true;
+    int mask_gen = 0; // bumped whenever a mask component changes
+    int use_fused = 1;
+    int kch = 32;
+    FusedGeom fg;
+    BoxList shell;
+    bool has_interior = false;
+    fi::Refine* rf = nullptr; // FP32 defect-correction workspace (refine.hip)
+    unsigned long long* d_norm = nullptr;
+    unsigned long long* h_norm = nullptr;
+    double* d_dem = nullptr;
+    size_t dem_elems = 0;
+
+    static bool is_sig(int id) { return id >= FI_F_P && id <= FI_F_TYZ; }
+    static bool is_vel(int id) { return id >= FI_F_VX && id <= FI_F_VZ; }
+    double* eta() { return f[eta_cur ? FI_F_ETA_NEXT : FI_F_ETA]; }
+    double* eta_next() { return f[eta_cur ? FI_F_ETA : FI_F_ETA_NEXT]; }
+    // physical buffer holding the current value of a logical field id
+    double* buf(int id)
+    {
+        if (id == FI_F_ETA) return eta();
+        if (id == FI_F_ETA_NEXT) return eta_next();
+        if (is_sig(id)) return sig_cur ? sf[id] : f[id];
+        if (is_vel(id)) return v_cur ? sf[id] : f[id];
+        return f[id];
+    }
+    // buffer the next iteration writes
+    double* alt(int id)
+    {
+        if (is_sig(id)) return sig_cur ? f[id] : sf[id];
+        if (is_vel(id)) return v_cur ? f[id] : sf[id];
+        return f[id];
+    }
+};
+
+namespace fi {
+
+StokesArgs<double> make_args(fi_model_s* m, bool in_place);
+void run_bc(fi_model_s* m, int group);
+int prepare(fi_model_s* m);
+void iterate(fi_model_s* m, hipEvent_t* ev);
+int advance(fi_model_s* m, int niter);
+int residual_norms(fi_model_s* m, double out[4]);
+double bits_to_double(unsigned long long b);
+BcBatch neumann_batch(double* p, int L, int d);
+void free_refine(fi_model_s* m);
+
+inline void add_op(BcBatch& b, void* p, int L, int kind, double v, int f32 = 0)
+{
+    BcOp& o = b.op[b.nop++];
+    o.ptr = p;
+    o.L = L;
+    o.kind = kind;
+    o.f32 = f32;
+    o.v = v;
+}
+
+// Both sides of one dimension share a launch; an op that belongs to one side
+// only carries the side in its kind: 0/1 = both sides, 2/3 = low, 4/5 = high.
+inline void add_side_op(BcBatch& b, void* p, int L, int kind, double v, int side, int f32 = 0)
+{
+    add_op(b, p, L, kind + 2 * (side + 1), v, f32);
+}
+
+// stress batch (boundary_conditions.jl:26-100) for the fields get(id); homog
+// sets every prescribed value to zero (the batch of a correction)
+template <class G>
+BcBatch stress_batch_g(const fi_model_desc& ds, int d, G get, bool homog, int f32)
+{
+    BcBatch b;
+    b.nop = 0;
+    b.d = d;
+    for (int s = 0; s < 2; ++s) {
+        const int kind = ds.bc_kind[d][s];
+        double c[3];
+        for (int q = 0; q < 3; ++q) c[q] = homog ? 0.0 : ds.bc_value[d][s][q];
+        if (ds.ndims == 3) {
+            if (kind == FI_BC_TRACTION) {
+                add_side_op(b, get(FI_F_P), L_CCC, 0, c[d], s, f32);
+                add_side_op(b, get(d == 0 ? FI_F_TXX : d == 1 ? FI_F_TYY : FI_F_TZZ), L_CCC, 0, c[d], s, f32);
+            }
+            if (kind == FI_BC_TRACTION || kind == FI_BC_SLIP) {
+                if (d == 0) {
+                    add_side_op(b, get(FI_F_TXY), L_VVC, 0, c[1], s, f32);
+                    add_side_op(b, get(FI_F_TXZ), L_VCV, 0, c[2], s, f32);
+                } else if (d == 1) {
+                    add_side_op(b, get(FI_F_TXY), L_VVC, 0, c[0], s, f32);
+                    add_side_op(b, get(FI_F_TYZ), L_CVV, 0, c[2], s, f32);
+                } else {
+                    add_side_op(b, get(FI_F_TXZ), L_VCV, 0, c[0], s, f32);
+                    add_side_op(b, get(FI_F_TYZ), L_CVV, 0, c[1], s, f32);
+                }
+            }
+        } else {
+            if (kind == FI_BC_TRACTION) {
+                add_side_op(b, get(FI_F_P), L_CCC, 0, c[d], s, f32);
+                add_side_op(b, get(d == 0 ? FI_F_TXX : FI_F_TYY), L_CCC, 0, c[d], s, f32);
+            }
+            if (kind == FI_BC_TRACTION || kind == FI_BC_SLIP) add_side_op(b, get(FI_F_TXY), L_VVC, 0, c[1 - d], s, f32);
+        }
+    }
+    return b;
+}
+
+// velocity batch (boundary_conditions.jl:104-115); eta_next Neumann is added by the caller
+template <class G>
+BcBatch velocity_batch_g(const fi_model_desc& ds, int d, G get, bool homog, int f32)
+{
+    BcBatch b;
+    b.nop = 0;
+    b.d = d;
+    const int idV[3] = {FI_F_VX, FI_F_VY, FI_F_VZ};
+    const int LV[3] = {L_VCC, L_CVC, L_CCV};
+    for (int s = 0; s < 2; ++s) {
+        const int kind = ds.bc_kind[d][s];
+        double c[3];
+        for (int q = 0; q < 3; ++q) c[q] = homog ? 0.0 : ds.bc_value[d][s][q];
+        if (kind == FI_BC_VELOCITY) {
+            for (int q = 0; q < ds.ndims; ++q) add_side_op(b, get(idV[q]), LV[q], 0, c[q], s, f32);
+        } else if (kind == FI_BC_SLIP) {
+            add_side_op(b, get(idV[d]), LV[d], 0, c[d], s, f32);
+        }
+    }
+    return b;
+}
+
+// residual batch (boundary_conditions.jl:120-128)
+template <class G>
+BcBatch residual_batch_g(const fi_model_desc& ds, int d, G get, int f32)
+{
+    BcBatch b;
+    b.nop = 0;
+    b.d = d;
+    const int idR[3] = {FI_F_RVX, FI_F_RVY, FI_F_RVZ};
+    const int LV[3] = {L_VCC, L_CVC, L_CCV};
+    for (int s = 0; s < 2; ++s)
+        if (ds.bc_kind[d][s] != FI_BC_TRACTION) add_side_op(b, get(idR[d]), LV[d], 0, 0.0, s, f32);
+    return b;
+}
+
+} // namespace fi

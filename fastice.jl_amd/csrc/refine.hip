@@ -1,0 +1,426 @@
+// refine.hip -- FP32 pseudo-transient sweeps inside an FP64 defect-correction
+// (iterative-refinement) loop.
+//
+// The FP64 model state x = (P, tau, V, eta) is the master.  Each outer step,
+// all in FP64 on the GPU:
+//   1. eta <- rheology(tau_II(tau), eta)          (nonlinear rheologies only)
+//   2. tau <- 2 eta (edot(V) - div V / 3)          (compute_tau!: zero constitutive defect)
+//   3. r_P, r_V <- compute_residuals!(x); Inf-norms; stop when below tol
+// then the correction d solves the same PT iteration in FP32 with frozen eta,
+// homogeneous boundary values and the FP64 defect as forcing:
+//   P_d   -= (div V_d + r_P) eta dtau_P
+//   tau_d -= (tau_d - 2 eta edot'(V_d)) dtau_tau
+//   V_d   += (-grad P_d + div tau_d + r_V) / eta dtau_V
+// (r_V enters as gravity -r_V), until its own residual has dropped by
+// inner_rtol; finally x += d in FP64.  Because every operator is linear in the
+// state for frozen eta, x + d satisfies the FP64 equations up to the inner
+// tolerance and FP32 rounding, and the outer loop removes what remains.
+#include <cstring>
+#include <vector>
+
+#include "model.h"
+
+namespace fi {
+
+struct Refine {
+    float* d[10][2] = {};   // correction state P..VZ, two buffers
+    float* eta = nullptr;   // frozen viscosity
+    float* fP = nullptr;    // continuity forcing r_P
+    float* g[3] = {};       // momentum forcing -r_V
+    float* wna[8] = {};     // FP32 masks (IB)
+    float* wns[8] = {};
+    float* r[4] = {};       // inner residuals
+    std::vector<void*> allocs;
+    int sig_cur = 0, v_cur = 0;
+    FusedGeom fg;
+    BoxList shell;
+    bool has_interior = false;
+    int mask_gen = -1; // model mask generation the FP32 masks were cast from
+    unsigned long long* d_norm = nullptr;
+    unsigned long long* h_norm = nullptr;
+
+    float* cur(int id) { return d[id][(id <= FI_F_TYZ ? sig_cur : v_cur)]; }
+    float* alt(int id) { return d[id][1 - (id <= FI_F_TYZ ? sig_cur : v_cur)]; }
+};
+
+// ---------------------------------------------------------------------------
+// elementwise kernels over whole allocations
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_cast(float* dst, const double* src, double scale, size_t n)
+{
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
+        dst[t] = (float)(scale * src[t]);
+}
+
+__global__ __launch_bounds__(256) void k_fill32(float* p, size_t n, float v)
+{
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) p[t] = v;
+}
+
+__global__ __launch_bounds__(256) void k_add_correction(double* x, const float* d, size_t n)
+{
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
+        x[t] = x[t] + (double)d[t];
+}
+
+static dim3 ew_grid(size_t n)
+{
+    size_t b = (n + 255) / 256;
+    if (b > 8192) b = 8192;
+    return dim3((unsigned)(b ? b : 1));
+}
+
+// Inf-norm of FP32 fields over their interiors (|x| bit patterns as uint32;
+// NaN sorts above +Inf)
+struct Norm32Args {
+    const float* f[4];
+    int hi[4][3];
+};
+
+__global__ __launch_bounds__(256) void k_norms32(Norm32Args a, long sy, long sz, int nd, unsigned long long* out)
+{
+    const int q = blockIdx.y;
+    const float* f = a.f[q];
+    const int hx = a.hi[q][0], hy = a.hi[q][1], hz = a.hi[q][2];
+    const long total = (long)hx * hy * hz;
+    unsigned m = 0u;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
+        long i = t % hx + 1, r = t / hx;
+        long j = r % hy + 1, k = (nd == 3) ? r / hy + 1 : 0;
+        unsigned b = __float_as_uint(f[i + sy * j + sz * k]) & 0x7fffffffu;
+        m = b > m ? b : m;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        unsigned v = __shfl_xor(m, o);
+        m = v > m ? v : m;
+    }
+    __shared__ unsigned red[4];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned r = red[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) r = red[w] > r ? red[w] : r;
+        atomicMax(out + q, (unsigned long long)r);
+    }
+}
+
+__global__ void k_zero4(unsigned long long* p)
+{
+    if (threadIdx.x < 4) p[threadIdx.x] = 0ull;
+}
+
+// ---------------------------------------------------------------------------
+// workspace
+// ---------------------------------------------------------------------------
+void free_refine(fi_model_s* m)
+{
+    Refine* r = m->rf;
+    if (!r) return;
+    for (void* p : r->allocs) (void)hipFree(p);
+    if (r->h_norm) (void)hipHostFree(r->h_norm);
+    delete r;
+    m->rf = nullptr;
+}
+
+static int alloc32(fi_model_s* m, float** out)
+{
+    Refine* r = m->rf;
+    float* base = nullptr;
+    if (hipMalloc(&base, m->g.elems * sizeof(float)) != hipSuccess)
+        return fail(FI_ERR_OOM, "FP32 workspace allocation failed");
+    r->allocs.push_back(base);
+    HIPCHK(hipMemsetAsync(base, 0, m->g.elems * sizeof(float), m->stream));
+    *out = base + m->g.off;
+    return FI_OK;
+}
+
+static int ensure_refine(fi_model_s* m)
+{
+    if (m->rf) return FI_OK;
+    m->rf = new Refine();
+    Refine* r = m->rf;
+    const int nd = m->g.nd;
+    int rc = FI_OK;
+    for (int id = FI_F_P; id <= FI_F_VZ && !rc; ++id)
+        if (m->f[id])
+            for (int b = 0; b < 2 && !rc; ++b) rc = alloc32(m, &r->d[id][b]);
+    if (!rc) rc = alloc32(m, &r->eta);
+    if (!rc) rc = alloc32(m, &r->fP);
+    for (int q = 0; q < nd && !rc; ++q) rc = alloc32(m, &r->g[q]);
+    for (int q = 0; q < (nd == 3 ? 4 : 3) && !rc; ++q) rc = alloc32(m, &r->r[q]);
+    if (!rc && m->desc.immersed_boundary) {
+        // the masks the sweeps read: na at stress locations, ns at velocity locations
+        const int na3[4] = {OM_CCC, OM_VVC, OM_VCV, OM_CVV}, ns3[3] = {OM_VCC, OM_CVC, OM_CCV};
+        const int na2[2] = {OM2_CC, OM2_VV}, ns2[2] = {OM2_VC, OM2_CV};
+        const int nna = nd == 3 ? 4 : 2, nns = nd == 3 ? 3 : 2;
+        for (int q = 0; q < nna && !rc; ++q) rc = alloc32(m, &r->wna[nd == 3 ? na3[q] : na2[q]]);
+        for (int q = 0; q < nns && !rc; ++q) rc = alloc32(m, &r->wns[nd == 3 ? ns3[q] : ns2[q]]);
+    }
+    if (!rc && (hipMalloc(&r->d_norm, 4 * sizeof(unsigned long long)) != hipSuccess ||
+                hipHostMalloc(&r->h_norm, 4 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess))
+        rc = fail(FI_ERR_OOM, "norm buffers");
+    if (!rc && r->d_norm) r->allocs.push_back(r->d_norm);
+    if (rc) {
+        free_refine(m);
+        return rc;
+    }
+    if (nd == 3 && m->g.elems * sizeof(float) < (1ull << 32)) r->has_interior = fused_split(m->g, 16, m->kch, r->fg, r->shell);
+    return FI_OK;
+}
+
+static StokesArgs<float> make_args32(fi_model_s* m, bool in_place)
+{
+    Refine* r = m->rf;
+    const fi_model_desc& d = m->desc;
+    StokesArgs<float> a;
+    std::memset(&a, 0, sizeof(a));
+    auto in = [&](int id) { return r->cur(id); };
+    auto out = [&](int id) { return in_place ? r->cur(id) : r->alt(id); };
+    a.P = in(FI_F_P); a.txx = in(FI_F_TXX); a.tyy = in(FI_F_TYY); a.tzz = in(FI_F_TZZ);
+    a.txy = in(FI_F_TXY); a.txz = in(FI_F_TXZ); a.tyz = in(FI_F_TYZ);
+    a.Vx = in(FI_F_VX); a.Vy = in(FI_F_VY); a.Vz = in(FI_F_VZ);
+    a.Pn = out(FI_F_P); a.txxn = out(FI_F_TXX); a.tyyn = out(FI_F_TYY); a.tzzn = out(FI_F_TZZ);
+    a.txyn = out(FI_F_TXY); a.txzn = out(FI_F_TXZ); a.tyzn = out(FI_F_TYZ);
+    a.Vxn = out(FI_F_VX); a.Vyn = out(FI_F_VY); a.Vzn = out(FI_F_VZ);
+    a.eta = r->eta;
+    a.eta_next = nullptr;
+    a.frozen_eta = 1;
+    a.fP = r->fP;
+    a.gx = r->g[0]; a.gy = r->g[1]; a.gz = r->g[2];
+    a.grav_field = 1;
+    a.rP = r->r[0]; a.rVx = r->r[1]; a.rVy = r->r[2]; a.rVz = r->r[3];
+    for (int q = 0; q < 8; ++q) {
+        a.na[q] = r->wna[q];
+        a.ns[q] = r->wns[q];
+    }
+    a.flags = m->flags;
+    a.nx = m->g.n[0]; a.ny = m->g.n[1]; a.nz = m->g.n[2];
+    a.sy = m->g.sy; a.sz = m->g.sz;
+    a.ix = (float)m->g.idx[0]; a.iy = (float)m->g.idx[1]; a.iz = (float)m->g.idx[2];
+    a.dPr = (float)d.dtau_P;
+    for (int q = 0; q < 6; ++q) a.dtau[q] = (float)d.dtau_tau[q];
+    for (int q = 0; q < 3; ++q) a.dV[q] = (float)d.dtau_V[q];
+    a.rh_kind = FI_RHEO_LINEAR_CONST;
+    return a;
+}
+
+static void bc32(fi_model_s* m, int group)
+{
+    Refine* r = m->rf;
+    for (int d = 0; d < m->g.nd; ++d) {
+        BcBatch b;
+        if (group == FI_BCGROUP_STRESS)
+            b = stress_batch_g(m->desc, d, [r](int id) { return (void*)r->cur(id); }, true, 1);
+        else if (group == FI_BCGROUP_VELOCITY)
+            b = velocity_batch_g(m->desc, d, [r](int id) { return (void*)r->cur(id); }, true, 1);
+        else
+            b = residual_batch_g(m->desc, d, [r](int id) { return (void*)r->r[id - FI_F_RP]; }, 1);
+        launch_bc_dim(m->g, b, m->stream);
+    }
+}
+
+// one FP32 PT iteration of the correction problem
+static void iterate32(fi_model_s* m, hipEvent_t* ev)
+{
+    Refine* r = m->rf;
+    const bool ib = m->desc.immersed_boundary != 0;
+    StokesArgs<float> a = make_args32(m, false);
+    const bool fused = m->use_fused && r->has_interior;
+    const BoxList boxes = fused ? r->shell : full_box(m->g);
+    if (ev) (void)hipEventRecord(ev[0], m->stream);
+    if (fused) launch_fused(m->g, a, r->fg, ib, m->stream);
+    if (ev) (void)hipEventRecord(ev[1], m->stream);
+    launch_sigma_boxes(m->g, a, boxes, ib, m->stream);
+    r->sig_cur ^= 1;
+    if (ev) (void)hipEventRecord(ev[2], m->stream);
+    bc32(m, FI_BCGROUP_STRESS);
+    if (ev) (void)hipEventRecord(ev[3], m->stream);
+    launch_velocity_boxes(m->g, a, boxes, ib, m->stream);
+    r->v_cur ^= 1;
+    if (ev) (void)hipEventRecord(ev[4], m->stream);
+    bc32(m, FI_BCGROUP_VELOCITY);
+    if (ev) (void)hipEventRecord(ev[5], m->stream);
+}
+
+static int inner_norm(fi_model_s* m, double* out)
+{
+    Refine* r = m->rf;
+    const bool ib = m->desc.immersed_boundary != 0;
+    StokesArgs<float> a = make_args32(m, true);
+    launch_residuals(m->g, a, ib, m->stream);
+    bc32(m, FI_BCGROUP_RESIDUAL);
+    Norm32Args na;
+    const int L[4] = {L_CCC, L_VCC, L_CVC, L_CCV};
+    const int nf = m->g.nd == 3 ? 4 : 3;
+    for (int q = 0; q < 4; ++q) {
+        const int qq = q < nf ? q : 0;
+        na.f[q] = r->r[qq];
+        for (int d = 0; d < 3; ++d) na.hi[q][d] = (d < m->g.nd) ? m->g.n[d] + ((L[qq] >> d) & 1) : 1;
+    }
+    hipLaunchKernelGGL(k_zero4, dim3(1), dim3(64), 0, m->stream, r->d_norm);
+    long total = (long)na.hi[0][0] * na.hi[0][1] * na.hi[0][2];
+    int blocks = (int)((total + 255) / 256);
+    blocks = blocks > 1024 ? 1024 : (blocks < 1 ? 1 : blocks);
+    hipLaunchKernelGGL(k_norms32, dim3(blocks, nf, 1), dim3(256), 0, m->stream, na, m->g.sy, m->g.sz, m->g.nd, r->d_norm);
+    HIPCHK(hipMemcpyAsync(r->h_norm, r->d_norm, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));
+    double mx = 0.0;
+    for (int q = 0; q < nf; ++q) {
+        unsigned b = (unsigned)r->h_norm[q];
+        float v;
+        std::memcpy(&v, &b, sizeof(v));
+        if (!(v == v) || b >= 0x7f800000u) {
+            *out = (double)v;
+            return FI_OK;
+        }
+        mx = (double)v > mx ? (double)v : mx;
+    }
+    *out = mx;
+    return FI_OK;
+}
+
+static void cast(fi_model_s* m, float* dst, const double* src, double scale)
+{
+    hipLaunchKernelGGL(k_cast, ew_grid(m->g.elems), dim3(256), 0, m->stream, dst - m->g.off, src - m->g.off, scale,
+                       m->g.elems);
+}
+
+static void fill32(fi_model_s* m, float* p, float v)
+{
+    hipLaunchKernelGGL(k_fill32, ew_grid(m->g.elems), dim3(256), 0, m->stream, p - m->g.off, m->g.elems, v);
+}
+
+} // namespace fi
+
+using namespace fi;
+
+extern "C" {
+
+int fi_refined_solve(fi_model_t m, int max_outer, int max_inner, int ncheck, double inner_rtol, double tol,
+                     int counts_out[2], double norms_inf[4])
+{
+    if (!m || !norms_inf) return fail(FI_ERR_ARG, "null argument");
+    if (max_outer < 1 || max_inner < 1 || ncheck < 1 || !(inner_rtol > 0.0) || !(tol > 0.0))
+        return fail(FI_ERR_ARG, "bad refinement parameters");
+    int rc = ensure_refine(m);
+    if (rc) return rc;
+    rc = prepare(m);
+    if (rc) return rc;
+    Refine* r = m->rf;
+    const bool ib = m->desc.immersed_boundary != 0;
+    const int nd = m->g.nd;
+    if (ib && r->mask_gen != m->mask_gen) {
+        for (int q = 0; q < 8; ++q) {
+            if (r->wna[q]) cast(m, r->wna[q], m->f[FI_F_OM_NA0 + q], 1.0);
+            if (r->wns[q]) cast(m, r->wns[q], m->f[FI_F_OM_NS0 + q], 1.0);
+        }
+        r->mask_gen = m->mask_gen;
+    }
+    const fi_rheology& rh = m->desc.rheology;
+    const bool nonlinear = rh.kind == FI_RHEO_POWERLAW || rh.relax != FI_RELAX_NONE;
+    int outer = 0, inner_total = 0;
+    for (;; ++outer) {
+        if (nonlinear && outer > 0) {
+            StokesArgs<double> a = make_args(m, true);
+            launch_eta_update(m->g, a, m->stream);
+            m->eta_cur ^= 1;
+            for (int d = 0; d < nd; ++d) launch_bc_dim(m->g, neumann_batch(m->eta(), L_CCC, d), m->stream);
+        }
+        {
+            // tau <- 2 eta edot'(V), then the stress boundary conditions the PT
+            // iteration applies after every stress update (boundary_conditions.jl:26-100)
+            StokesArgs<double> a = make_args(m, true);
+            launch_tau(m->g, a, ib, m->stream);
+            run_bc(m, FI_BCGROUP_STRESS);
+            m->shadow_dirty = true;
+        }
+        rc = residual_norms(m, norms_inf);
+        if (rc) return rc;
+        double r0 = 0.0;
+        for (int q = 0; q < 4; ++q) {
+            if (!(norms_inf[q] == norms_inf[q]) || norms_inf[q] > 1.79e308) {
+                if (counts_out) { counts_out[0] = outer; counts_out[1] = inner_total; }
+                return fail(FI_ERR_NONFINITE, "simulation failed at outer step %d: non-finite residual", outer);
+            }
+            r0 = norms_inf[q] > r0 ? norms_inf[q] : r0;
+        }
+        if (r0 < tol || outer >= max_outer) break;
+        // correction problem: frozen eta, forcing from the FP64 defect, zero state
+        cast(m, r->eta, m->eta(), 1.0);
+        cast(m, r->fP, m->f[FI_F_RP], 1.0);
+        const int idR[3] = {FI_F_RVX, FI_F_RVY, FI_F_RVZ};
+        for (int q = 0; q < nd; ++q) cast(m, r->g[q], m->f[idR[q]], -1.0);
+        for (int id = FI_F_P; id <= FI_F_VZ; ++id)
+            if (m->f[id]) {
+                fill32(m, r->d[id][0], 0.0f);
+                fill32(m, r->d[id][1], 0.0f);
+            }
+        r->sig_cur = r->v_cur = 0;
+        int done = 0;
+        while (done < max_inner) {
+            int chunk = ncheck < max_inner - done ? ncheck : max_inner - done;
+            for (int it = 0; it < chunk; ++it) iterate32(m, nullptr);
+            done += chunk;
+            double rin = 0.0;
+            rc = inner_norm(m, &rin);
+            if (rc) return rc;
+            if (!(rin == rin) || rin > 3.4e38) {
+                if (counts_out) { counts_out[0] = outer; counts_out[1] = inner_total + done; }
+                return fail(FI_ERR_NONFINITE, "FP32 correction diverged at outer step %d", outer);
+            }
+            if (rin <= inner_rtol * r0) break;
+        }
+        inner_total += done;
+        for (int id = FI_F_P; id <= FI_F_VZ; ++id)
+            if (m->f[id])
+                hipLaunchKernelGGL(k_add_correction, ew_grid(m->g.elems), dim3(256), 0, m->stream, m->buf(id) - m->g.off,
+                                   r->cur(id) - m->g.off, m->g.elems);
+        m->shadow_dirty = true;
+    }
+    if (counts_out) {
+        counts_out[0] = outer;
+        counts_out[1] = inner_total;
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(m->stream));
+    return FI_OK;
+}
+
+int fi_refined_time_inner(fi_model_t m, int n_iter, double out_ms[4])
+{
+    if (!m || !out_ms || n_iter < 1) return fail(FI_ERR_ARG, "bad arguments");
+    int rc = ensure_refine(m);
+    if (rc) return rc;
+    rc = prepare(m);
+    if (rc) return rc;
+    hipEvent_t ev[6];
+    for (int q = 0; q < 6; ++q) HIPCHK(hipEventCreate(&ev[q]));
+    double acc[4] = {0, 0, 0, 0};
+    for (int it = 0; it < n_iter; ++it) {
+        iterate32(m, ev);
+        HIPCHK(hipEventSynchronize(ev[5]));
+        float t[5];
+        for (int q = 0; q < 5; ++q) HIPCHK(hipEventElapsedTime(&t[q], ev[q], ev[q + 1]));
+        acc[0] += t[0] + t[1] + t[2] + t[3] + t[4];
+        acc[1] += t[0];
+        acc[2] += t[1] + t[3];
+        acc[3] += t[2] + t[4];
+    }
+    for (int q = 0; q < 6; ++q) (void)hipEventDestroy(ev[q]);
+    for (int q = 0; q < 4; ++q) out_ms[q] = acc[q] / n_iter;
+    return FI_OK;
+}
+
+int fi_refined_advance_inner(fi_model_t m, int n_iter)
+{
+    if (!m || n_iter < 0) return fail(FI_ERR_ARG, "bad arguments");
+    int rc = ensure_refine(m);
+    if (rc) return rc;
+    rc = prepare(m);
+    if (rc) return rc;
+    for (int it = 0; it < n_iter; ++it) iterate32(m, nullptr);
+    HIPCHK(hipGetLastError());
+    return FI_OK;
+}
+
+} // extern "C"

@@ -141,7 +141,7 @@ __device__ __forceinline__ void sigma_point3(const StokesArgs<T>& a, long c, lon
         T exz = T(0.5) * (dif(Vz[c], Vz[c - 1], ix) + dif(Vx[c], Vx[c - sz], iz));
         T eyz = T(0.5) * (dif(Vz[c], Vz[c - sy], iy) + dif(Vy[c], Vy[c - sz], iz));
         T divV = exx + eyy + ezz;
-        a.Pn[c] = upd_P(a.P[c], divV, ec, a.dPr);
+        a.Pn[c] = upd_P(a.P[c], a.fP ? divV + a.fP[c] : divV, ec, a.dPr);
         a.txxn[c] = upd_td(a.txx[c], ec, exx, divV, a.dtau[0]);
         a.tyyn[c] = upd_td(a.tyy[c], ec, eyy, divV, a.dtau[1]);
         a.tzzn[c] = upd_td(a.tzz[c], ec, ezz, divV, a.dtau[2]);
@@ -156,7 +156,7 @@ __device__ __forceinline__ void sigma_point3(const StokesArgs<T>& a, long c, lon
             T eyy = dif(wy[c + sy] * Vy[c + sy], wy[c] * Vy[c], iy);
             T ezz = dif(wz[c + sz] * Vz[c + sz], wz[c] * Vz[c], iz);
             T divV = exx + eyy + ezz;
-            a.Pn[c] = upd_P(a.P[c], divV, ec, a.dPr);
+            a.Pn[c] = upd_P(a.P[c], a.fP ? divV + a.fP[c] : divV, ec, a.dPr);
             a.txxn[c] = upd_td(a.txx[c], ec, exx, divV, a.dtau[0]);
             a.tyyn[c] = upd_td(a.tyy[c], ec, eyy, divV, a.dtau[1]);
             a.tzzn[c] = upd_td(a.tzz[c], ec, ezz, divV, a.dtau[2]);
@@ -217,6 +217,7 @@ __device__ __forceinline__ void velocity_point3(const StokesArgs<T>& a, long c, 
                                   gz, ez, a.ns[OM_CCV][c], a.dV[2])
                        : T(0);
     }
+    if (a.frozen_eta) return;
     const T t = tauII(txx[c], tyy[c], tzz[c], lerp4(txy[c], txy[c + 1], txy[c + sy], txy[c + 1 + sy]),
                       lerp4(txz[c], txz[c + 1], txz[c + sz], txz[c + 1 + sz]),
                       lerp4(tyz[c], tyz[c + sy], tyz[c + sz], tyz[c + sy + sz]));
@@ -360,7 +361,7 @@ __global__ __launch_bounds__(64 * TY) void k_fused3(StokesArgs<T> a, FusedGeom f
                 T eyy = dif(sVy[ty + 1][tx], vy0, iy);
                 T ezz = dif(vzp, vz0, iz);
                 T divV = exx + eyy + ezz;
-                Pc = upd_P(oP, divV, e0, a.dPr);
+                Pc = upd_P(oP, a.fP ? divV + ldo(a.fP, c) : divV, e0, a.dPr);
                 xxc = upd_td(oxx, e0, exx, divV, a.dtau[0]);
                 yyc = upd_td(oyy, e0, eyy, divV, a.dtau[1]);
                 zzc = upd_td(ozz, e0, ezz, divV, a.dtau[2]);
@@ -389,7 +390,7 @@ __global__ __launch_bounds__(64 * TY) void k_fused3(StokesArgs<T> a, FusedGeom f
                 T eyy = dif(ldo(wy, c + bsy) * sVy[ty + 1][tx], wy0 * vy0, iy);
                 T ezz = dif(ldo(wz, c + bsz) * vzp, wz0 * vz0, iz);
                 T divV = exx + eyy + ezz;
-                Pc = upd_P(oP, divV, e0, a.dPr);
+                Pc = upd_P(oP, a.fP ? divV + ldo(a.fP, c) : divV, e0, a.dPr);
                 xxc = upd_td(oxx, e0, exx, divV, a.dtau[0]);
                 yyc = upd_td(oyy, e0, eyy, divV, a.dtau[1]);
                 zzc = upd_td(ozz, e0, ezz, divV, a.dtau[2]);
@@ -468,9 +469,11 @@ __global__ __launch_bounds__(64 * TY) void k_fused3(StokesArgs<T> a, FusedGeom f
                                                       g2, ez, ldo(a.ns[OM_CCV], cv), a.dV[2])
                                            : T(0));
             }
-            const T t = tauII(xxk, yyk, zzk, lerp4(xy, xy_ip, xy_jp, xy_ijp), lerp4(xz, xz_ip, xz_kp, xz_ikp),
-                              lerp4(yz, yz_jp, yz_kp, yz_jkp));
-            sto(a.eta_next, cv, rheology_o(a, cv, t));
+            if (!a.frozen_eta) {
+                const T t = tauII(xxk, yyk, zzk, lerp4(xy, xy_ip, xy_jp, xy_ijp), lerp4(xz, xz_ip, xz_kp, xz_ikp),
+                                  lerp4(yz, yz_jp, yz_kp, yz_jkp));
+                sto(a.eta_next, cv, rheology_o(a, cv, t));
+            }
         }
 
         // rotate the column registers
@@ -527,6 +530,7 @@ __global__ __launch_bounds__(256) void k_residuals3(StokesArgs<T> a)
     T gz = a.grav_field ? a.gz[c] : a.grav[2];
     if (!IB) {
         a.rP[c] = (Vx[c + 1] - Vx[c]) * ix + (Vy[c + sy] - Vy[c]) * iy + (Vz[c + sz] - Vz[c]) * iz;
+        if (a.fP) a.rP[c] = a.rP[c] + a.fP[c];
         a.rVx[c] = -((P[c] - P[c - 1]) * ix) + (txx[c] - txx[c - 1]) * ix + (txy[c + sy] - txy[c]) * iy +
                    (txz[c + sz] - txz[c]) * iz - gx;
         a.rVy[c] = -((P[c] - P[c - sy]) * iy) + (tyy[c] - tyy[c - sy]) * iy + (txy[c + 1] - txy[c]) * ix +
@@ -541,7 +545,7 @@ __global__ __launch_bounds__(256) void k_residuals3(StokesArgs<T> a)
         const unsigned fl = a.flags[c];
         if (fl & F_CCC)
             a.rP[c] = (wx[c + 1] * Vx[c + 1] - wx[c] * Vx[c]) * ix + (wy[c + sy] * Vy[c + sy] - wy[c] * Vy[c]) * iy +
-                      (wz[c + sz] * Vz[c + sz] - wz[c] * Vz[c]) * iz;
+                      (wz[c + sz] * Vz[c + sz] - wz[c] * Vz[c]) * iz + (a.fP ? a.fP[c] : T(0));
         else a.rP[c] = T(0);
         if (fl & F_VCC)
             a.rVx[c] = -((wc[c] * P[c] - wc[c - 1] * P[c - 1]) * ix) + (wc[c] * txx[c] - wc[c - 1] * txx[c - 1]) * ix +
@@ -609,6 +613,40 @@ __global__ __launch_bounds__(256) void k_tau3(StokesArgs<T> a)
         else eyz = T(0.5) * ((Vz[c] - Vz[c - sy]) * iy + (Vy[c] - Vy[c - sz]) * iz);
         a.tyzn[c] = T(2) * lerp4(eta[c - sy - sz], eta[c - sz], eta[c - sy], eta[c]) * eyz;
     } else a.tyzn[c] = T(0);
+}
+
+static inline dim3 block2();
+
+// eta_next <- rheology(tau_II(tau), I) at the current stress (the viscosity half of
+// update_V!, kernels_3d.jl:162-166 / kernels_2d.jl:229-230)
+template <typename T>
+__global__ __launch_bounds__(256) void k_eta_update(StokesArgs<T> a, int nd)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = (nd == 3) ? (int)blockIdx.z + 1 : 0;
+    if (i > a.nx + 1 || j > a.ny + 1) return;
+    const long sy = a.sy, sz = a.sz;
+    const long c = (long)i + sy * j + sz * k;
+    const T *txx = a.txx, *tyy = a.tyy, *txy = a.txy;
+    T t;
+    if (nd == 3) {
+        const T *tzz = a.tzz, *txz = a.txz, *tyz = a.tyz;
+        t = tauII(txx[c], tyy[c], tzz[c], lerp4(txy[c], txy[c + 1], txy[c + sy], txy[c + 1 + sy]),
+                  lerp4(txz[c], txz[c + 1], txz[c + sz], txz[c + 1 + sz]), lerp4(tyz[c], tyz[c + sy], tyz[c + sz], tyz[c + sy + sz]));
+    } else {
+        T p = txx[c], q = tyy[c];
+        T l = lerp4(txy[c], txy[c + 1], txy[c + sy], txy[c + 1 + sy]);
+        t = sqrt(T(0.5) * (p * p + q * q) + l * l);
+    }
+    a.eta_next[c] = rheology(a, c, t);
+}
+
+template <typename T>
+void launch_eta_update(const Geo& g, const StokesArgs<T>& a, hipStream_t s)
+{
+    dim3 grd((g.n[0] + 1 + 63) / 64, (g.n[1] + 1 + 3) / 4, g.nd == 3 ? g.n[2] + 1 : 1);
+    hipLaunchKernelGGL((k_eta_update<T>), grd, block2(), 0, s, a, g.nd);
 }
 
 // IB validity bits, once per geometry (null_spaces.jl:1-43)
@@ -715,7 +753,7 @@ __global__ __launch_bounds__(256) void k_sigma2(StokesArgs<T> a)
         T eyy = dif(Vy[c + sy], Vy[c], iy);
         T exy = T(0.5) * (dif(Vy[c], Vy[c - 1], ix) + dif(Vx[c], Vx[c - sy], iy));
         T divV = exx + eyy;
-        a.Pn[c] = upd_P(a.P[c], divV, ec, a.dPr);
+        a.Pn[c] = upd_P(a.P[c], a.fP ? divV + a.fP[c] : divV, ec, a.dPr);
         a.txxn[c] = upd_td(a.txx[c], ec, exx, divV, a.dtau[0]);
         a.tyyn[c] = upd_td(a.tyy[c], ec, eyy, divV, a.dtau[1]);
         a.txyn[c] = upd_to(a.txy[c], exy_eta, exy, a.dtau[2]);
@@ -726,7 +764,7 @@ __global__ __launch_bounds__(256) void k_sigma2(StokesArgs<T> a)
             T exx = dif(wx[c + 1] * Vx[c + 1], wx[c] * Vx[c], ix);
             T eyy = dif(wy[c + sy] * Vy[c + sy], wy[c] * Vy[c], iy);
             T divV = exx + eyy;
-            a.Pn[c] = upd_P(a.P[c], divV, ec, a.dPr);
+            a.Pn[c] = upd_P(a.P[c], a.fP ? divV + a.fP[c] : divV, ec, a.dPr);
             a.txxn[c] = upd_td(a.txx[c], ec, exx, divV, a.dtau[0]);
             a.tyyn[c] = upd_td(a.tyy[c], ec, eyy, divV, a.dtau[1]);
         } else {
@@ -765,6 +803,7 @@ __global__ __launch_bounds__(256) void k_velocity2(StokesArgs<T> a)
                                            dif(wv[c + 1] * txy[c + 1], wv[c] * txy[c], ix), zero, gy, ey, a.ns[OM2_CV][c], a.dV[1])
                                 : T(0);
     }
+    if (a.frozen_eta) return;
     // kernels_2d.jl:229-230
     T p = txx[c], q = tyy[c];
     T l = lerp4(txy[c], txy[c + 1], txy[c + sy], txy[c + 1 + sy]);
@@ -782,13 +821,15 @@ __global__ __launch_bounds__(256) void k_residuals2(StokesArgs<T> a)
     if (!IB) {
         // kernels_2d.jl:248-256
         a.rP[c] = (Vx[c + 1] - Vx[c]) * ix + (Vy[c + sy] - Vy[c]) * iy;
+        if (a.fP) a.rP[c] = a.rP[c] + a.fP[c];
         a.rVx[c] = -((P[c] - P[c - 1]) * ix) + (txx[c] - txx[c - 1]) * ix + (txy[c + sy] - txy[c]) * iy - gx;
         a.rVy[c] = -((P[c] - P[c - sy]) * iy) + (tyy[c] - tyy[c - sy]) * iy + (txy[c + 1] - txy[c]) * ix - gy;
     } else {
         const T *wx = a.ns[OM2_VC], *wy = a.ns[OM2_CV], *wc = a.na[OM2_CC], *wv = a.na[OM2_VV];
         const unsigned fl = a.flags[c];
         if (fl & F_CCC)
-            a.rP[c] = (wx[c + 1] * Vx[c + 1] - wx[c] * Vx[c]) * ix + (wy[c + sy] * Vy[c + sy] - wy[c] * Vy[c]) * iy;
+            a.rP[c] = (wx[c + 1] * Vx[c + 1] - wx[c] * Vx[c]) * ix + (wy[c + sy] * Vy[c + sy] - wy[c] * Vy[c]) * iy +
+                      (a.fP ? a.fP[c] : T(0));
         else a.rP[c] = T(0);
         if (fl & F_VCC)
             a.rVx[c] = -((wc[c] * P[c] - wc[c - 1] * P[c - 1]) * ix) + (wc[c] * txx[c] - wc[c - 1] * txx[c - 1]) * ix +
@@ -973,7 +1014,8 @@ void launch_tau(const Geo& g, const StokesArgs<T>& a, bool ib, hipStream_t s)
     template void launch_velocity_boxes<T>(const Geo&, const StokesArgs<T>&, const BoxList&, bool, hipStream_t); \
     template void launch_fused<T>(const Geo&, const StokesArgs<T>&, const FusedGeom&, bool, hipStream_t);        \
     template void launch_residuals<T>(const Geo&, const StokesArgs<T>&, bool, hipStream_t);                      \
-    template void launch_tau<T>(const Geo&, const StokesArgs<T>&, bool, hipStream_t);
+    template void launch_tau<T>(const Geo&, const StokesArgs<T>&, bool, hipStream_t);                      \
+    template void launch_eta_update<T>(const Geo&, const StokesArgs<T>&, hipStream_t);
 INST(double)
 INST(float)
 

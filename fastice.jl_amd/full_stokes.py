@@ -298,6 +298,22 @@ class IsothermalFullStokesModel:
         self.residual.norms = out
         return it.value, out
 
+    def refined_solve(self, tol, max_outer=50, max_inner=None, ncheck=None, inner_rtol=1e-3):
+        """Solve with FP32 PT sweeps inside an FP64 defect-correction loop.
+
+        Returns (outer steps, FP32 iterations, Inf-norms).  Same discrete
+        solution as solve(); the outer residual is evaluated in FP64.
+        """
+        n = max(self.grid.dims)
+        ncheck = ncheck or n
+        max_inner = max_inner or 100 * n
+        counts = (C.c_int * 2)()
+        out = np.zeros(4)
+        A.check(A.lib().fi_refined_solve(self._h, int(max_outer), int(max_inner), int(ncheck), float(inner_rtol),
+                                         float(tol), counts, A.dptr(out)))
+        self.residual.norms = out
+        return counts[0], counts[1], out
+
     def apply_bc(self, group):
         g = {"stress": A.FI_BCGROUP_STRESS, "velocity": A.FI_BCGROUP_VELOCITY, "viscosity": A.FI_BCGROUP_ETA,
              "eta": A.FI_BCGROUP_ETA, "residual": A.FI_BCGROUP_RESIDUAL}.get(group, group)

@@ -156,3 +156,44 @@ def manufactured_errors(grid, V, tau):
     return (n(vm - v, np.inf) / n(vm, np.inf), n(tm - t, np.inf) / n(tm, np.inf),
             n(vm - v, 1) / n(vm, 1), n(tm - t, 1) / n(tm, 1),
             n(vm - v, 2) / n(vm, 2), n(tm - t, 2) / n(tm, 2))
+
+
+def glacier_3d(dims, seed=1, glen=True, eta_reg=30.0, relax_rate=0.05, lx=2.0, ly=2.0, H=1.0):
+    """Seeded synthetic glacier on [-lx/2, lx/2] x [-ly/2, ly/2] x [0, H].
+
+    Set-up order of scripts_future_API/test_vavilov.jl:244-295: level sets of
+    the ice surface (na) and of the bed (ns, inverted), volume fractions,
+    Traction(0) on the box, gravity (0, 0, 1), PT parameters r = 0.9,
+    re = 5 pi.  Rheology: Glen n = 3 (PowerLawRheology(1, 3)) capped by
+    ViscoplasticRegularisation(eta_reg) and relaxed in log space, or linear
+    eta = 1 when glen is False.  Returns (model, cadence).
+    """
+    from .full_stokes import IsothermalImmersedBoundaryFullStokesModel, TBC
+    from .geometries import GaussianGlacier
+    from .level_sets import compute_levelset_from_dem, invert_levelset, omega_from_psi
+    from .physics import PowerLawRheology, Relaxation, ViscoplasticRegularisation
+
+    dims = tuple(dims)
+    gl = GaussianGlacier(seed=seed, lx=lx, ly=ly, H=H)
+    grid = UniformGrid(origin=(-lx / 2, -ly / 2, 0.0), extent=(lx, ly, H), dims=dims)
+    dgrid = UniformGrid(origin=(-lx / 2, -ly / 2), extent=(lx, ly), dims=dims[:2])
+    tr = TBC(0.0, 0.0, 0.0)
+    if glen:
+        rheology = Relaxation("eta", ViscoplasticRegularisation(PowerLawRheology(1.0, 3), eta_reg), relax_rate,
+                              scale="log")
+    else:
+        rheology = LinearViscousRheology(1.0)
+    model = IsothermalImmersedBoundaryFullStokesModel(
+        grid=grid, boundary_conditions=dict(x=(tr, tr), y=(tr, tr), z=(tr, tr)),
+        gravity=dict(x=0.0, y=0.0, z=1.0), rheology=rheology,
+        solver_params=pt_params(grid, r=0.9, re_mech=5 * PI, vdtau_div=1.1))
+    compute_levelset_from_dem(model.level_sets.na, gl.surface, dgrid, grid)
+    compute_levelset_from_dem(model.level_sets.ns, gl.bed, dgrid, grid)
+    invert_levelset(model.level_sets.ns)
+    omega_from_psi(model.field_masks.ns, model.level_sets.ns)
+    omega_from_psi(model.field_masks.na, model.level_sets.na)
+    model.viscosity.eta.set(1.0)
+    model.viscosity.eta_next.set(1.0)
+    model.apply_bc("eta")
+    n = max(dims)
+    return model, dict(niter=50 * n, ncheck=5 * n, dem_grid=dgrid, glacier=gl)

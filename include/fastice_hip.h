@@ -173,6 +173,22 @@ int fi_levelset_from_dem(fi_model_t m, int phase, const double* dem_parent, int 
 int fi_invert_levelset(fi_model_t m, int phase);
 int fi_omega_from_psi(fi_model_t m, int phase);
 
+/* FP32 sweeps inside an FP64 defect-correction loop (no counterpart in the
+ * reference; same solution as fi_solve).  Outer steps run on the GPU in FP64:
+ * eta <- rheology(tau_II) (nonlinear rheologies), tau <- 2 eta edot'(V),
+ * residuals and Inf-norms; stop when all norms < tol or after max_outer steps.
+ * Between them the correction solves the PT iteration in FP32 with frozen
+ * eta, homogeneous boundary values and the FP64 defect as forcing, checked
+ * every ncheck iterations, until its residual falls below inner_rtol times the
+ * outer one (at most max_inner iterations), and is added to the FP64 state.
+ * counts_out = {outer steps, total FP32 iterations}. */
+int fi_refined_solve(fi_model_t m, int max_outer, int max_inner, int ncheck, double inner_rtol, double tol,
+                     int counts_out[2], double norms_inf[4]);
+/* n_iter FP32 correction iterations on the current correction state (timing) */
+int fi_refined_advance_inner(fi_model_t m, int n_iter);
+/* as fi_time_iterations, for the FP32 correction iteration */
+int fi_refined_time_inner(fi_model_t m, int n_iter, double out_ms[4]);
+
 /* free / total device memory in bytes (hipMemGetInfo) */
 int fi_mem_info(size_t* free_bytes, size_t* total_bytes);
 

@@ -1,0 +1,98 @@
+"""FP32 sweeps inside FP64 defect correction (through the C ABI).
+
+The refined solve must reach the same FP64 residual tolerance as the plain
+FP64 solve; the per-field difference of the two converged states is bounded
+by what that tolerance allows (stated per test).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from helpers import state_names
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _field_errors(a, b, names):
+    out = {}
+    for n in names:
+        x, y = a.field(n).interior(), b.field(n).interior()
+        out[n] = float(np.max(np.abs(x - y)) / max(np.max(np.abs(y)), 1e-300))
+    return out
+
+
+def test_refined_manufactured_3d_matches_fp64(pkg):
+    _say("case: refined vs FP64, 3D manufactured 70x36x24 (fused interior), tol 1e-8")
+    kw, cad = pkg.manufactured_3d((70, 36, 24))
+    a = pkg.IsothermalFullStokesModel(**kw)
+    b = pkg.IsothermalFullStokesModel(**kw)
+    for m in (a, b):
+        m.viscosity.eta.set(1.0)
+        m.viscosity.eta_next.set(1.0)
+    it, nb = b.solve(400 * 70, cad["ncheck"], 1e-8)
+    assert max(nb) < 1e-8
+    outer, inner, na = a.refined_solve(1e-8)
+    assert max(na) < 1e-8, na
+    errs = _field_errors(a, b, state_names(3))
+    _say("outer %d inner %d fp64 iters %d errors %s" % (outer, inner, it, errs))
+    assert max(errs.values()) < 1e-5
+    a.close()
+    b.close()
+
+
+def test_refined_manufactured_2d(pkg):
+    _say("case: refined vs FP64, 2D manufactured 96^2, tol 1e-9")
+    kw, cad = pkg.manufactured_2d(96)
+    a = pkg.IsothermalFullStokesModel(**kw)
+    b = pkg.IsothermalFullStokesModel(**kw)
+    for m in (a, b):
+        m.viscosity.eta.set(1.0)
+        m.viscosity.eta_next.set(1.0)
+    it, nb = b.solve(400 * 96, 96, 1e-9)
+    outer, inner, na = a.refined_solve(1e-9)
+    assert max(na) < 1e-9 and max(nb) < 1e-9
+    errs = _field_errors(a, b, state_names(2))
+    assert max(errs.values()) < 1e-6, errs
+
+
+@pytest.mark.parametrize("glen", [False, True])
+def test_refined_glacier_matches_fp64(pkg, glen):
+    _say("case: refined vs FP64, immersed-boundary glacier 70x40x20, glen=%s" % glen)
+    a, cad = pkg.glacier_3d((70, 40, 20), seed=2, glen=glen)
+    b, _ = pkg.glacier_3d((70, 40, 20), seed=2, glen=glen)
+    it, nb = b.solve(400 * 70, cad["ncheck"], 1e-8)
+    assert max(nb) < 1e-8, nb
+    outer, inner, na = a.refined_solve(1e-8, max_outer=200)
+    _say("outer %d inner %d fp64 iters %d" % (outer, inner, it))
+    assert max(na) < 1e-8, na
+    errs = _field_errors(a, b, ["P", "TXX", "TXY", "VX", "VY", "VZ"])
+    _say("errors %s" % errs)
+    assert max(errs.values()) < 1e-4
+    a.close()
+    b.close()
+
+
+def test_refined_fused_equals_pointwise(pkg):
+    _say("case: FP32 correction sweep, fused vs pointwise, 3D manufactured 70x36x24")
+    kw, cad = pkg.manufactured_3d((70, 36, 24))
+    ms = []
+    for fused in (1, 0):
+        m = pkg.IsothermalFullStokesModel(**kw)
+        m.viscosity.eta.set(1.0)
+        m.viscosity.eta_next.set(1.0)
+        m.set_option("fused", fused)
+        out = m.refined_solve(1e-6, max_outer=3, max_inner=3 * 70, ncheck=35)
+        ms.append((m, out))
+    (a, oa), (b, ob) = ms
+    assert oa[0] == ob[0] and oa[1] == ob[1]
+    for name in state_names(3):
+        fa = a.field(name).parent()
+        assert np.all(np.isfinite(fa)), name
+        assert np.array_equal(fa, b.field(name).parent()), name
+    a.close()
+    b.close()
