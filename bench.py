@@ -112,10 +112,12 @@ def solve_comparison(pkg, dims):
     t0 = time.perf_counter()
     outer, inner, na = a.refined_solve(1e-8, max_outer=300, ncheck=cad["ncheck"])
     t32 = time.perf_counter() - t0
-    errs = {}
-    for name in ("P", "TXX", "TXY", "VX", "VY", "VZ"):
-        x, y = a.field(name).interior(), b.field(name).interior()
-        errs[name] = float(np.max(np.abs(x - y)) / max(np.max(np.abs(y)), 1e-300))
+    # per-field difference relative to the FP64 solution's stress / velocity scale
+    names = ("P", "TXX", "TYY", "TZZ", "TXY", "TXZ", "TYZ", "VX", "VY", "VZ")
+    vals = {k: (a.field(k).interior(), b.field(k).interior()) for k in names}
+    s_scale = max(np.max(np.abs(vals[k][1])) for k in names[:7])
+    v_scale = max(np.max(np.abs(vals[k][1])) for k in names[7:])
+    errs = {k: float(np.max(np.abs(x - y)) / (s_scale if k[0] in "PT" else v_scale)) for k, (x, y) in vals.items()}
     a.close()
     b.close()
     return {"grid": list(dims), "tol": 1e-8,

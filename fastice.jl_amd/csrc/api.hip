@@ -606,6 +606,11 @@ int fi_set_option(fi_model_t m, int key, int value)
         m->kch = value;
         if (m->g.nd == 3 && m->g.elems * sizeof(double) < (1ull << 32))
             m->has_interior = fused_split(m->g, 8, m->kch, m->fg, m->shell);
+    } else if (key == FI_OPT_REFINE_FULL_PHASE) {
+        m->refine_full_phase = value ? 1 : 0;
+    } else if (key == FI_OPT_REFINE_ETA_RATE) {
+        if (value < 1 || value > 1000) return fail(FI_ERR_ARG, "viscosity relaxation rate must be in 1..1000 per mille");
+        m->refine_eta_rate = value / 1000.0;
     } else {
         return fail(FI_ERR_ARG, "unknown option %d", key);
     }

@@ -38,6 +38,8 @@ struct fi_model_s {
     bool flags_dirty = true;
     int mask_gen = 0; // bumped whenever a mask component changes
     int use_fused = 1;
+    int refine_full_phase = 1;      // nonlinear refined solves start with a full FP32 PT phase
+    double refine_eta_rate = 0.5;   // log-space relaxation of the outer viscosity update
     int kch = 32;
     FusedGeom fg;
     BoxList shell;

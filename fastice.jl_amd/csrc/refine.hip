@@ -24,7 +24,12 @@ namespace fi {
 
 struct Refine {
     float* d[10][2] = {};   // correction state P..VZ, two buffers
-    float* eta = nullptr;   // frozen viscosity
+    float* eta = nullptr;   // frozen viscosity (correction phase) / viscosity (full phase)
+    float* eta2 = nullptr;  // second viscosity buffer of the full nonlinear FP32 phase
+    float* rh = nullptr;    // FP32 rheology field (LINEAR_FIELD eta or POWERLAW A)
+    float* gf[3] = {};      // FP32 gravity fields of the full phase
+    int eta_cur = 0;        // full phase: 1 when the current viscosity is eta2
+    bool full = false;      // sweeps solve the full state (true) or the correction (false)
     float* fP = nullptr;    // continuity forcing r_P
     float* g[3] = {};       // momentum forcing -r_V
     float* wna[8] = {};     // FP32 masks (IB)
@@ -55,6 +60,12 @@ __global__ __launch_bounds__(256) void k_cast(float* dst, const double* src, dou
 __global__ __launch_bounds__(256) void k_fill32(float* p, size_t n, float v)
 {
     for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) p[t] = v;
+}
+
+__global__ __launch_bounds__(256) void k_widen(double* dst, const float* src, size_t n)
+{
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
+        dst[t] = (double)src[t];
 }
 
 __global__ __launch_bounds__(256) void k_add_correction(double* x, const float* d, size_t n)
@@ -145,6 +156,10 @@ static int ensure_refine(fi_model_s* m)
         if (m->f[id])
             for (int b = 0; b < 2 && !rc; ++b) rc = alloc32(m, &r->d[id][b]);
     if (!rc) rc = alloc32(m, &r->eta);
+    if (!rc) rc = alloc32(m, &r->eta2);
+    if (!rc && m->f[FI_F_RHEO]) rc = alloc32(m, &r->rh);
+    for (int q = 0; q < nd && !rc; ++q)
+        if (m->f[FI_F_GX + q]) rc = alloc32(m, &r->gf[q]);
     if (!rc) rc = alloc32(m, &r->fP);
     for (int q = 0; q < nd && !rc; ++q) rc = alloc32(m, &r->g[q]);
     for (int q = 0; q < (nd == 3 ? 4 : 3) && !rc; ++q) rc = alloc32(m, &r->r[q]);
@@ -182,12 +197,23 @@ static StokesArgs<float> make_args32(fi_model_s* m, bool in_place)
     a.Pn = out(FI_F_P); a.txxn = out(FI_F_TXX); a.tyyn = out(FI_F_TYY); a.tzzn = out(FI_F_TZZ);
     a.txyn = out(FI_F_TXY); a.txzn = out(FI_F_TXZ); a.tyzn = out(FI_F_TYZ);
     a.Vxn = out(FI_F_VX); a.Vyn = out(FI_F_VY); a.Vzn = out(FI_F_VZ);
-    a.eta = r->eta;
-    a.eta_next = nullptr;
-    a.frozen_eta = 1;
-    a.fP = r->fP;
-    a.gx = r->g[0]; a.gy = r->g[1]; a.gz = r->g[2];
-    a.grav_field = 1;
+    if (r->full) {
+        // the full nonlinear PT iteration in FP32 (real forcing, boundary values and rheology)
+        a.eta = r->eta_cur ? r->eta2 : r->eta;
+        a.eta_next = r->eta_cur ? r->eta : r->eta2;
+        a.frozen_eta = 0;
+        a.fP = nullptr;
+        a.gx = r->gf[0]; a.gy = r->gf[1]; a.gz = r->gf[2];
+        a.grav_field = d.gravity_is_field;
+        for (int q = 0; q < 3; ++q) a.grav[q] = (float)d.gravity[q];
+    } else {
+        a.eta = r->eta;
+        a.eta_next = nullptr;
+        a.frozen_eta = 1;
+        a.fP = r->fP;
+        a.gx = r->g[0]; a.gy = r->g[1]; a.gz = r->g[2];
+        a.grav_field = 1;
+    }
     a.rP = r->r[0]; a.rVx = r->r[1]; a.rVy = r->r[2]; a.rVz = r->r[3];
     for (int q = 0; q < 8; ++q) {
         a.na[q] = r->wna[q];
@@ -200,7 +226,15 @@ static StokesArgs<float> make_args32(fi_model_s* m, bool in_place)
     a.dPr = (float)d.dtau_P;
     for (int q = 0; q < 6; ++q) a.dtau[q] = (float)d.dtau_tau[q];
     for (int q = 0; q < 3; ++q) a.dV[q] = (float)d.dtau_V[q];
-    a.rh_kind = FI_RHEO_LINEAR_CONST;
+    a.rh_kind = d.rheology.kind;
+    a.rh_has_reg = d.rheology.has_reg;
+    a.rh_relax = d.rheology.relax;
+    a.rh_eta = (float)d.rheology.eta;
+    a.rh_A = (float)d.rheology.A;
+    a.rh_n = (float)d.rheology.n;
+    a.rh_inv_eta_reg = d.rheology.has_reg ? (float)(1.0 / d.rheology.eta_reg) : 0.0f;
+    a.rh_rate = (float)d.rheology.rate;
+    a.rh = r->rh;
     return a;
 }
 
@@ -209,11 +243,13 @@ static void bc32(fi_model_s* m, int group)
     Refine* r = m->rf;
     for (int d = 0; d < m->g.nd; ++d) {
         BcBatch b;
+        const bool homog = !r->full;
         if (group == FI_BCGROUP_STRESS)
-            b = stress_batch_g(m->desc, d, [r](int id) { return (void*)r->cur(id); }, true, 1);
-        else if (group == FI_BCGROUP_VELOCITY)
-            b = velocity_batch_g(m->desc, d, [r](int id) { return (void*)r->cur(id); }, true, 1);
-        else
+            b = stress_batch_g(m->desc, d, [r](int id) { return (void*)r->cur(id); }, homog, 1);
+        else if (group == FI_BCGROUP_VELOCITY) {
+            b = velocity_batch_g(m->desc, d, [r](int id) { return (void*)r->cur(id); }, homog, 1);
+            if (r->full) add_op(b, r->eta_cur ? r->eta : r->eta2, L_CCC, 1, 0.0, 1); // eta_next Neumann
+        } else
             b = residual_batch_g(m->desc, d, [r](int id) { return (void*)r->r[id - FI_F_RP]; }, 1);
         launch_bc_dim(m->g, b, m->stream);
     }
@@ -240,6 +276,7 @@ static void iterate32(fi_model_s* m, hipEvent_t* ev)
     if (ev) (void)hipEventRecord(ev[4], m->stream);
     bc32(m, FI_BCGROUP_VELOCITY);
     if (ev) (void)hipEventRecord(ev[5], m->stream);
+    if (r->full) r->eta_cur ^= 1;
 }
 
 static int inner_norm(fi_model_s* m, double* out)
@@ -285,6 +322,59 @@ static void cast(fi_model_s* m, float* dst, const double* src, double scale)
                        m->g.elems);
 }
 
+static void widen(fi_model_s* m, double* dst, const float* src)
+{
+    hipLaunchKernelGGL(k_widen, ew_grid(m->g.elems), dim3(256), 0, m->stream, dst - m->g.off, src - m->g.off, m->g.elems);
+}
+
+// Phase 1 of a nonlinear refined solve: the full nonlinear PT iteration (eta
+// updated every sweep, real boundary values and forcing) on the whole state in
+// FP32, from the FP64 state, until the FP32 residual has dropped by rtol or
+// stops decreasing; the result becomes the FP64 state.
+static int full_phase(fi_model_s* m, int max_iter, int ncheck, double rtol, int* iters)
+{
+    Refine* r = m->rf;
+    r->full = true;
+    for (int id = FI_F_P; id <= FI_F_VZ; ++id)
+        if (m->f[id]) {
+            cast(m, r->d[id][0], m->buf(id), 1.0);
+            cast(m, r->d[id][1], m->buf(id), 1.0);
+        }
+    r->sig_cur = r->v_cur = 0;
+    cast(m, r->eta, m->eta(), 1.0);
+    cast(m, r->eta2, m->eta_next(), 1.0);
+    r->eta_cur = 0;
+    if (r->rh) cast(m, r->rh, m->f[FI_F_RHEO], 1.0);
+    for (int q = 0; q < 3; ++q)
+        if (r->gf[q]) cast(m, r->gf[q], m->f[FI_F_GX + q], 1.0);
+    double first = -1.0, prev = -1.0;
+    int done = 0, rc = FI_OK;
+    while (done < max_iter) {
+        int chunk = ncheck < max_iter - done ? ncheck : max_iter - done;
+        for (int it = 0; it < chunk; ++it) iterate32(m, nullptr);
+        done += chunk;
+        double rin = 0.0;
+        rc = inner_norm(m, &rin);
+        if (rc) break;
+        if (!(rin == rin) || rin > 3.4e38) {
+            rc = fail(FI_ERR_NONFINITE, "FP32 nonlinear phase diverged after %d iterations", done);
+            break;
+        }
+        if (first < 0.0) first = rin;
+        if (rin <= rtol * first || (prev > 0.0 && rin > 0.9 * prev)) break;
+        prev = rin;
+    }
+    if (!rc) {
+        for (int id = FI_F_P; id <= FI_F_VZ; ++id)
+            if (m->f[id]) widen(m, m->buf(id), r->cur(id));
+        widen(m, m->eta(), r->eta_cur ? r->eta2 : r->eta);
+        m->shadow_dirty = true;
+    }
+    r->full = false;
+    *iters = done;
+    return rc;
+}
+
 static void fill32(fi_model_s* m, float* p, float v)
 {
     hipLaunchKernelGGL(k_fill32, ew_grid(m->g.elems), dim3(256), 0, m->stream, p - m->g.off, m->g.elems, v);
@@ -319,9 +409,21 @@ int fi_refined_solve(fi_model_t m, int max_outer, int max_inner, int ncheck, dou
     const fi_rheology& rh = m->desc.rheology;
     const bool nonlinear = rh.kind == FI_RHEO_POWERLAW || rh.relax != FI_RELAX_NONE;
     int outer = 0, inner_total = 0;
+    if (nonlinear && m->refine_full_phase) {
+        int it = 0;
+        rc = full_phase(m, max_inner, ncheck, 1e-5, &it);
+        inner_total += it;
+        if (rc) {
+            if (counts_out) { counts_out[0] = 0; counts_out[1] = inner_total; }
+            return rc;
+        }
+    }
     for (;; ++outer) {
-        if (nonlinear && outer > 0) {
+        if (nonlinear && (outer > 0 || m->refine_full_phase)) {
+            // Picard update of the frozen viscosity, relaxed in log space
             StokesArgs<double> a = make_args(m, true);
+            a.rh_relax = FI_RELAX_LOG;
+            a.rh_rate = m->refine_eta_rate;
             launch_eta_update(m->g, a, m->stream);
             m->eta_cur ^= 1;
             for (int d = 0; d < nd; ++d) launch_bc_dim(m->g, neumann_batch(m->eta(), L_CCC, d), m->stream);

@@ -330,7 +330,8 @@ class IsothermalFullStokesModel:
 
     def set_option(self, key, value):
         """Kernel-path option (fused interior sweep on/off, z planes per workgroup)."""
-        k = {"fused": A.FI_OPT_FUSED, "zchunk": A.FI_OPT_ZCHUNK}.get(key, key)
+        k = {"fused": A.FI_OPT_FUSED, "zchunk": A.FI_OPT_ZCHUNK, "refine_full_phase": A.FI_OPT_REFINE_FULL_PHASE,
+             "refine_eta_rate_permil": A.FI_OPT_REFINE_ETA_RATE}.get(key, key)
         A.check(A.lib().fi_set_option(self._h, int(k), int(value)))
 
 

@@ -161,11 +161,14 @@ int fi_time_iterations(fi_model_t m, int n_iter, double out_ms[4]);
  * returns 1 when the fused sweep is active, 0 otherwise. */
 int fi_fused_region(fi_model_t m, int lo[3], int hi[3]);
 
-/* Kernel-path options (no counterpart in the reference; results do not depend
- * on them):  FI_OPT_FUSED 1/0 selects the fused interior sweep or the
+/* Options (no counterpart in the reference).  Kernel path, results do not
+ * depend on them: FI_OPT_FUSED 1/0 selects the fused interior sweep or the
  * pointwise sweep everywhere; FI_OPT_ZCHUNK sets the z planes per workgroup of
- * the fused sweep. */
-enum { FI_OPT_FUSED = 1, FI_OPT_ZCHUNK = 2 };
+ * the fused sweep.  Refined solver: FI_OPT_REFINE_FULL_PHASE 1/0 starts a
+ * nonlinear solve with full FP32 PT sweeps (default 1); FI_OPT_REFINE_ETA_RATE
+ * is the log-space relaxation of the outer viscosity update in per mille
+ * (default 500). */
+enum { FI_OPT_FUSED = 1, FI_OPT_ZCHUNK = 2, FI_OPT_REFINE_FULL_PHASE = 3, FI_OPT_REFINE_ETA_RATE = 4 };
 int fi_set_option(fi_model_t m, int key, int value);
 
 int fi_levelset_from_dem(fi_model_t m, int phase, const double* dem_parent, int dem_nx, int dem_ny,

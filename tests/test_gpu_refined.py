@@ -19,10 +19,19 @@ def _say(msg):
 
 
 def _field_errors(a, b, names):
+    """Max difference per field, relative to the scale of its group (stress or
+    velocity); pressure is compared modulo its mean (a free-slip or traction
+    box leaves a constant pressure mode free)."""
+    vals = {n: (a.field(n).interior(), b.field(n).interior()) for n in names}
+    stress = [n for n in names if n == "P" or n.startswith("T")]
+    vel = [n for n in names if n.startswith("V")]
+    s_scale = max(np.max(np.abs(vals[n][1])) for n in stress) if stress else 1.0
+    v_scale = max(np.max(np.abs(vals[n][1])) for n in vel) if vel else 1.0
     out = {}
-    for n in names:
-        x, y = a.field(n).interior(), b.field(n).interior()
-        out[n] = float(np.max(np.abs(x - y)) / max(np.max(np.abs(y)), 1e-300))
+    for n, (x, y) in vals.items():
+        if n == "P":
+            x, y = x - x.mean(), y - y.mean()
+        out[n] = float(np.max(np.abs(x - y)) / (s_scale if n in stress else v_scale))
     return out
 
 
