@@ -347,7 +347,10 @@ static int full_phase(fi_model_s* m, int max_iter, int ncheck, double rtol, int*
     if (r->rh) cast(m, r->rh, m->f[FI_F_RHEO], 1.0);
     for (int q = 0; q < 3; ++q)
         if (r->gf[q]) cast(m, r->gf[q], m->f[FI_F_GX + q], 1.0);
-    double first = -1.0, prev = -1.0;
+    // PT residuals are not monotone: stop when the best residual has not halved
+    // over four consecutive checks (the FP32 floor), or after a drop by rtol
+    double first = -1.0, best = -1.0;
+    int stall = 0;
     int done = 0, rc = FI_OK;
     while (done < max_iter) {
         int chunk = ncheck < max_iter - done ? ncheck : max_iter - done;
@@ -360,9 +363,14 @@ static int full_phase(fi_model_s* m, int max_iter, int ncheck, double rtol, int*
             rc = fail(FI_ERR_NONFINITE, "FP32 nonlinear phase diverged after %d iterations", done);
             break;
         }
-        if (first < 0.0) first = rin;
-        if (rin <= rtol * first || (prev > 0.0 && rin > 0.9 * prev)) break;
-        prev = rin;
+        if (first < 0.0) first = best = rin;
+        if (rin <= rtol * first) break;
+        if (rin < 0.5 * best) {
+            best = rin;
+            stall = 0;
+        } else if (++stall >= 4) {
+            break;
+        }
     }
     if (!rc) {
         for (int id = FI_F_P; id <= FI_F_VZ; ++id)
@@ -411,7 +419,7 @@ int fi_refined_solve(fi_model_t m, int max_outer, int max_inner, int ncheck, dou
     int outer = 0, inner_total = 0;
     if (nonlinear && m->refine_full_phase) {
         int it = 0;
-        rc = full_phase(m, max_inner, ncheck, 1e-5, &it);
+        rc = full_phase(m, max_inner, ncheck, 1e-6, &it);
         inner_total += it;
         if (rc) {
             if (counts_out) { counts_out[0] = 0; counts_out[1] = inner_total; }

@@ -506,6 +506,236 @@ __global__ __launch_bounds__(64 * TY) void k_fused3(StokesArgs<T> a, FusedGeom f
     }
 }
 
+// Immersed-boundary fused sweep.  Same marching scheme as k_fused3; every
+// thread loads the seven mask weights of its own position once per plane and
+// publishes weighted values to LDS: ns-weighted velocity for the masked strain
+// rates and na-weighted stress for the masked momentum fluxes (the products
+// are the operands the pointwise kernels form, so results are identical).
+// Raw off-diagonal stress is kept beside it for tau_II when eta is updated.
+template <typename T, int TY>
+__global__ __launch_bounds__(64 * TY) void k_fused3_ib(StokesArgs<T> a, FusedGeom fg)
+{
+    constexpr int TX = 64;
+    __shared__ T sVx[TY][TX], sVy[TY][TX], sVz[TY][TX];
+    __shared__ T sE[3][TY][TX];
+    __shared__ T sS[2][9][TY][TX]; // na-weighted P txx tyy txy txz tyz; raw txy txz tyz
+
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int i = fg.lo[0] - 1 + (int)blockIdx.x * (TX - 2) + tx;
+    const int j = fg.lo[1] - 1 + (int)blockIdx.y * (TY - 2) + ty;
+    const int ks = fg.lo[2] + (int)blockIdx.z * fg.kch;
+    const int ke = min(ks + fg.kch - 1, fg.hi[2]);
+    const bool inner = tx >= 1 && tx <= TX - 2 && ty >= 1 && ty <= TY - 2 && i <= fg.hi[0] && j <= fg.hi[1];
+    const bool can_c = tx < TX - 1 && ty < TY - 1;
+    const bool can_xy = tx > 0 && ty > 0;
+    const bool can_xz = tx > 0;
+    const bool can_yz = ty > 0;
+    const bool need_raw = !a.frozen_eta;
+    constexpr uint32_t E = sizeof(T);
+    const uint32_t bsz = (uint32_t)a.sz * E;
+    const int ic = min(i, fg.hi[0] + 1), jc = min(j, fg.hi[1] + 1);
+    const uint32_t col = ((uint32_t)ic + (uint32_t)a.sy * (uint32_t)jc) * E;
+    const T ix = a.ix, iy = a.iy, iz = a.iz;
+    const T *wvx = a.ns[OM_VCC], *wvy = a.ns[OM_CVC], *wvz = a.ns[OM_CCV];
+    const T *wc = a.na[OM_CCC], *wxy = a.na[OM_VVC], *wxz = a.na[OM_VCV], *wyz = a.na[OM_CVV];
+
+    const uint32_t c2 = col + bsz * (uint32_t)(ks - 2), c1 = c2 + bsz, c0 = c1 + bsz, c3 = c2 - bsz;
+    // raw velocity and ns weights of the own column
+    T vxm = ldo(a.Vx, c2), vym = ldo(a.Vy, c2), vzm = ldo(a.Vz, c2);
+    T mxm = ldo(wvx, c2), mym = ldo(wvy, c2), mzm = ldo(wvz, c2);
+    unsigned flm = ldo(a.flags, c2 / E);
+    T qxm = mxm * vxm, qym = mym * vym;
+    T vx0 = ldo(a.Vx, c1), vy0 = ldo(a.Vy, c1), vz0 = ldo(a.Vz, c1);
+    T mx0 = ldo(wvx, c1), my0 = ldo(wvy, c1), mz0 = ldo(wvz, c1);
+    T qx0 = mx0 * vx0, qy0 = my0 * vy0, qz0 = mz0 * vz0;
+    T vzp = ldo(a.Vz, c0), mzp = ldo(wvz, c0);
+    T qzp = mzp * vzp;
+    T mc0 = ldo(wc, c1), mxy0 = ldo(wxy, c1), mxz0 = ldo(wxz, c1), myz0 = ldo(wyz, c1);
+    unsigned fl0 = ldo(a.flags, c1 / E);
+    T emm = ldo(a.eta, c3), em = ldo(a.eta, c2), e0 = ldo(a.eta, c1);
+    T oP = ldo(a.P, c1), oxx = ldo(a.txx, c1), oyy = ldo(a.tyy, c1), ozz = ldo(a.tzz, c1), oxy = ldo(a.txy, c1),
+      oxz = ldo(a.txz, c1), oyz = ldo(a.tyz, c1);
+    T gx = T(0), gy = T(0), gz = T(0);
+    // own new stress: weighted at k and k-1, raw diagonal at k
+    T Pw = T(0), Pwm = T(0), xxw = T(0), yyw = T(0), zzw = T(0), zzwm = T(0), xxr = T(0), yyr = T(0), zzr = T(0);
+    sE[(ks - 2) % 3][ty][tx] = em;
+
+    for (int kk = ks - 1; kk <= ke + 1; ++kk) {
+        const uint32_t c = col + bsz * (uint32_t)kk;
+        sVx[ty][tx] = qx0;
+        sVy[ty][tx] = qy0;
+        sVz[ty][tx] = qz0;
+        sE[kk % 3][ty][tx] = e0;
+        T nvx = T(0), nvy = T(0), nvz2 = T(0), nmx = T(0), nmy = T(0), nmz2 = T(0), nmc = T(0), nmxy = T(0),
+          nmxz = T(0), nmyz = T(0), ne = T(0), nP = T(0), nxx = T(0), nyy = T(0), nzz = T(0), nxy = T(0), nxz = T(0),
+          nyz = T(0), ngx = T(0), ngy = T(0), ngz = T(0);
+        unsigned nfl = 0;
+        if (kk <= ke) {
+            const uint32_t cn = c + bsz;
+            nvx = ldo(a.Vx, cn);
+            nvy = ldo(a.Vy, cn);
+            nvz2 = ldo(a.Vz, cn + bsz);
+            nmx = ldo(wvx, cn);
+            nmy = ldo(wvy, cn);
+            nmz2 = ldo(wvz, cn + bsz);
+            nmc = ldo(wc, cn);
+            nmxy = ldo(wxy, cn);
+            nmxz = ldo(wxz, cn);
+            nmyz = ldo(wyz, cn);
+            nfl = ldo(a.flags, cn / E);
+            ne = ldo(a.eta, cn);
+            nP = ldo(a.P, cn);
+            nxx = ldo(a.txx, cn);
+            nyy = ldo(a.tyy, cn);
+            nzz = ldo(a.tzz, cn);
+            nxy = ldo(a.txy, cn);
+            nxz = ldo(a.txz, cn);
+            nyz = ldo(a.tyz, cn);
+        }
+        if (a.grav_field) {
+            ngx = ldo(a.gx, c);
+            ngy = ldo(a.gy, c);
+            ngz = ldo(a.gz, c);
+        }
+        __syncthreads();
+
+        // new stress at plane kk (IB kernels_3d.jl:144-181)
+        const int e_k = kk % 3, e_km = (kk + 2) % 3;
+        T Pc = T(0), xxc = T(0), yyc = T(0), zzc = T(0), xyc = T(0), xzc = T(0), yzc = T(0);
+        if (can_c && (fl0 & F_CCC)) {
+            T exx = dif(sVx[ty][tx + 1], qx0, ix);
+            T eyy = dif(sVy[ty + 1][tx], qy0, iy);
+            T ezz = dif(qzp, qz0, iz);
+            T divV = exx + eyy + ezz;
+            Pc = upd_P(oP, a.fP ? divV + ldo(a.fP, c) : divV, e0, a.dPr);
+            xxc = upd_td(oxx, e0, exx, divV, a.dtau[0]);
+            yyc = upd_td(oyy, e0, eyy, divV, a.dtau[1]);
+            zzc = upd_td(ozz, e0, ezz, divV, a.dtau[2]);
+        }
+        if (can_xy && (fl0 & F_VVC)) {
+            T exy = T(0.5) * (dif(qy0, sVy[ty][tx - 1], ix) + dif(qx0, sVx[ty - 1][tx], iy));
+            T el = lerp4(sE[e_k][ty - 1][tx - 1], sE[e_k][ty - 1][tx], sE[e_k][ty][tx - 1], e0);
+            xyc = upd_to(oxy, el, exy, a.dtau[3]);
+        }
+        if (can_xz && (fl0 & F_VCV)) {
+            T exz = T(0.5) * (dif(qz0, sVz[ty][tx - 1], ix) + dif(qx0, qxm, iz));
+            T el = lerp4(sE[e_km][ty][tx - 1], em, sE[e_k][ty][tx - 1], e0);
+            xzc = upd_to(oxz, el, exz, a.dtau[4]);
+        }
+        if (can_yz && (fl0 & F_CVV)) {
+            T eyz = T(0.5) * (dif(qz0, sVz[ty - 1][tx], iy) + dif(qy0, qym, iz));
+            T el = lerp4(sE[e_km][ty - 1][tx], em, sE[e_k][ty - 1][tx], e0);
+            yzc = upd_to(oyz, el, eyz, a.dtau[5]);
+        }
+        const int b = kk & 1;
+        const T Pwc = mc0 * Pc, zzwc = mc0 * zzc;
+        sS[b][0][ty][tx] = Pwc;
+        sS[b][1][ty][tx] = mc0 * xxc;
+        sS[b][2][ty][tx] = mc0 * yyc;
+        sS[b][3][ty][tx] = mxy0 * xyc;
+        sS[b][4][ty][tx] = mxz0 * xzc;
+        sS[b][5][ty][tx] = myz0 * yzc;
+        if (need_raw) {
+            sS[b][6][ty][tx] = xyc;
+            sS[b][7][ty][tx] = xzc;
+            sS[b][8][ty][tx] = yzc;
+        }
+        if (inner && kk >= ks && kk <= ke) {
+            sto(a.Pn, c, Pc);
+            sto(a.txxn, c, xxc);
+            sto(a.tyyn, c, yyc);
+            sto(a.tzzn, c, zzc);
+            sto(a.txyn, c, xyc);
+            sto(a.txzn, c, xzc);
+            sto(a.tyzn, c, yzc);
+        }
+        __syncthreads();
+
+        // velocity (IB kernels_3d.jl:188-205) and viscosity at plane k = kk-1
+        const int k = kk - 1;
+        if (inner && k >= ks) {
+            const uint32_t cv = c - bsz;
+            const int bo = k & 1, bn = kk & 1, ek = k % 3;
+            const T ex = T(0.5) * (sE[ek][ty][tx - 1] + em);
+            const T ey = T(0.5) * (sE[ek][ty - 1][tx] + em);
+            const T ez = T(0.5) * (emm + em);
+            const T g0 = a.grav_field ? gx : a.grav[0];
+            const T g1 = a.grav_field ? gy : a.grav[1];
+            const T g2 = a.grav_field ? gz : a.grav[2];
+            const T xyw = sS[bo][3][ty][tx], xzw = sS[bo][4][ty][tx], yzw = sS[bo][5][ty][tx];
+            sto(a.Vxn, cv, (flm & F_VCC) ? upd_V_ib(vxm, dif(Pw, sS[bo][0][ty][tx - 1], ix), dif(xxw, sS[bo][1][ty][tx - 1], ix),
+                                                    dif(sS[bo][3][ty + 1][tx], xyw, iy), dif(sS[bn][4][ty][tx], xzw, iz),
+                                                    g0, ex, mxm, a.dV[0])
+                                         : T(0));
+            sto(a.Vyn, cv, (flm & F_CVC) ? upd_V_ib(vym, dif(Pw, sS[bo][0][ty - 1][tx], iy), dif(yyw, sS[bo][2][ty - 1][tx], iy),
+                                                    dif(sS[bo][3][ty][tx + 1], xyw, ix), dif(sS[bn][5][ty][tx], yzw, iz),
+                                                    g1, ey, mym, a.dV[1])
+                                         : T(0));
+            sto(a.Vzn, cv, (flm & F_CCV) ? upd_V_ib(vzm, dif(Pw, Pwm, iz), dif(zzw, zzwm, iz),
+                                                    dif(sS[bo][4][ty][tx + 1], xzw, ix), dif(sS[bo][5][ty + 1][tx], yzw, iy),
+                                                    g2, ez, mzm, a.dV[2])
+                                         : T(0));
+            if (need_raw) {
+                const T t = tauII(xxr, yyr, zzr,
+                                  lerp4(sS[bo][6][ty][tx], sS[bo][6][ty][tx + 1], sS[bo][6][ty + 1][tx], sS[bo][6][ty + 1][tx + 1]),
+                                  lerp4(sS[bo][7][ty][tx], sS[bo][7][ty][tx + 1], sS[bn][7][ty][tx], sS[bn][7][ty][tx + 1]),
+                                  lerp4(sS[bo][8][ty][tx], sS[bo][8][ty + 1][tx], sS[bn][8][ty][tx], sS[bn][8][ty + 1][tx]));
+                sto(a.eta_next, cv, rheology_o(a, cv, t));
+            }
+        }
+
+        // rotate the column registers
+        Pwm = Pw;
+        zzwm = zzw;
+        Pw = Pwc;
+        xxw = mc0 * xxc;
+        yyw = mc0 * yyc;
+        zzw = zzwc;
+        xxr = xxc;
+        yyr = yyc;
+        zzr = zzc;
+        vxm = vx0;
+        vym = vy0;
+        vzm = vz0;
+        qxm = qx0;
+        qym = qy0;
+        mxm = mx0;
+        mym = my0;
+        mzm = mz0;
+        flm = fl0;
+        vx0 = nvx;
+        vy0 = nvy;
+        vz0 = vzp;
+        mx0 = nmx;
+        my0 = nmy;
+        mz0 = mzp;
+        qx0 = mx0 * vx0;
+        qy0 = my0 * vy0;
+        qz0 = qzp;
+        vzp = nvz2;
+        mzp = nmz2;
+        qzp = mzp * vzp;
+        mc0 = nmc;
+        mxy0 = nmxy;
+        mxz0 = nmxz;
+        myz0 = nmyz;
+        fl0 = nfl;
+        emm = em;
+        em = e0;
+        e0 = ne;
+        oP = nP;
+        oxx = nxx;
+        oyy = nyy;
+        ozz = nzz;
+        oxy = nxy;
+        oxz = nxz;
+        oyz = nyz;
+        gx = ngx;
+        gy = ngy;
+        gz = ngz;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // residuals and compute_tau (3D pointwise, full work range)
 // ---------------------------------------------------------------------------
@@ -981,7 +1211,7 @@ void launch_fused(const Geo& g, const StokesArgs<T>& a, const FusedGeom& fg, boo
     const int nby = (fg.hi[1] - fg.lo[1] + 1) / (TY - 2);
     const int nbz = (fg.hi[2] - fg.lo[2] + fg.kch) / fg.kch;
     dim3 grd(nbx, nby, nbz), blk(64, TY, 1);
-    if (ib) hipLaunchKernelGGL((k_fused3<T, true, TY>), grd, blk, 0, s, a, fg);
+    if (ib) hipLaunchKernelGGL((k_fused3_ib<T, TY>), grd, blk, 0, s, a, fg);
     else hipLaunchKernelGGL((k_fused3<T, false, TY>), grd, blk, 0, s, a, fg);
 }
 
