@@ -26,7 +26,7 @@ FI_PREC_FP64, FI_PREC_FP32_REFINED = 0, 1
 FI_LAYOUT_PARENT, FI_LAYOUT_INTERIOR = 0, 1
 FI_BCGROUP_STRESS, FI_BCGROUP_VELOCITY, FI_BCGROUP_ETA, FI_BCGROUP_RESIDUAL = 0, 1, 2, 3
 FI_PHASE_NA, FI_PHASE_NS = 0, 1
-FI_OPT_FUSED, FI_OPT_ZCHUNK, FI_OPT_REFINE_FULL_PHASE, FI_OPT_REFINE_ETA_RATE = 1, 2, 3, 4
+FI_OPT_FUSED, FI_OPT_ZCHUNK, FI_OPT_REFINE_FULL_PHASE, FI_OPT_REFINE_ETA_RATE, FI_OPT_REFINE_PHASE1_DIGITS = 1, 2, 3, 4, 5
 
 FIELD_NAMES = [
     "P", "TXX", "TYY", "TZZ", "TXY", "TXZ", "TYZ", "VX", "VY", "VZ", "ETA", "ETA_NEXT",

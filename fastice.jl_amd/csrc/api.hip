@@ -611,6 +611,9 @@ int fi_set_option(fi_model_t m, int key, int value)
     } else if (key == FI_OPT_REFINE_ETA_RATE) {
         if (value < 1 || value > 1000) return fail(FI_ERR_ARG, "viscosity relaxation rate must be in 1..1000 per mille");
         m->refine_eta_rate = value / 1000.0;
+    } else if (key == FI_OPT_REFINE_PHASE1_DIGITS) {
+        if (value < 1 || value > 7) return fail(FI_ERR_ARG, "nonlinear phase digits must be in 1..7");
+        m->refine_phase1_digits = value;
     } else {
         return fail(FI_ERR_ARG, "unknown option %d", key);
     }

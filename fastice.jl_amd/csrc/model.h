@@ -39,7 +39,8 @@ struct fi_model_s {
     int mask_gen = 0; // bumped whenever a mask component changes
     int use_fused = 1;
     int refine_full_phase = 1;      // nonlinear refined solves start with a full FP32 PT phase
-    double refine_eta_rate = 0.5;   // log-space relaxation of the outer viscosity update
+    double refine_eta_rate = 1.0;   // log-space relaxation of the outer viscosity update (initial)
+    int refine_phase1_digits = 3;   // the FP32 nonlinear phase stops after this many digits of residual drop
     int kch = 32;
     FusedGeom fg;
     BoxList shell;

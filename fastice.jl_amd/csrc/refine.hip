@@ -15,6 +15,9 @@
 // inner_rtol; finally x += d in FP64.  Because every operator is linear in the
 // state for frozen eta, x + d satisfies the FP64 equations up to the inner
 // tolerance and FP32 rounding, and the outer loop removes what remains.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -395,9 +398,11 @@ using namespace fi;
 extern "C" {
 
 int fi_refined_solve(fi_model_t m, int max_outer, int max_inner, int ncheck, double inner_rtol, double tol,
-                     int counts_out[2], double norms_inf[4])
+                     int counts_out[3], double norms_inf[4])
 {
+    const bool trace = std::getenv("FASTICE_TRACE") != nullptr;
     if (!m || !norms_inf) return fail(FI_ERR_ARG, "null argument");
+    if (counts_out) counts_out[2] = 0;
     if (max_outer < 1 || max_inner < 1 || ncheck < 1 || !(inner_rtol > 0.0) || !(tol > 0.0))
         return fail(FI_ERR_ARG, "bad refinement parameters");
     int rc = ensure_refine(m);
@@ -417,10 +422,15 @@ int fi_refined_solve(fi_model_t m, int max_outer, int max_inner, int ncheck, dou
     const fi_rheology& rh = m->desc.rheology;
     const bool nonlinear = rh.kind == FI_RHEO_POWERLAW || rh.relax != FI_RELAX_NONE;
     int outer = 0, inner_total = 0;
+    // outer viscosity relaxation: the configured rate, halved whenever an outer
+    // step fails to reduce the residual by 10 %
+    double omega = m->refine_eta_rate, r_prev = -1.0;
     if (nonlinear && m->refine_full_phase) {
         int it = 0;
-        rc = full_phase(m, max_inner, ncheck, 1e-6, &it);
+        rc = full_phase(m, max_inner, ncheck, std::pow(10.0, -m->refine_phase1_digits), &it);
         inner_total += it;
+        if (counts_out) counts_out[2] = it;
+        if (trace) std::fprintf(stderr, "[fastice] refined: FP32 nonlinear phase %d iterations\n", it);
         if (rc) {
             if (counts_out) { counts_out[0] = 0; counts_out[1] = inner_total; }
             return rc;
@@ -431,7 +441,7 @@ int fi_refined_solve(fi_model_t m, int max_outer, int max_inner, int ncheck, dou
             // Picard update of the frozen viscosity, relaxed in log space
             StokesArgs<double> a = make_args(m, true);
             a.rh_relax = FI_RELAX_LOG;
-            a.rh_rate = m->refine_eta_rate;
+            a.rh_rate = omega;
             launch_eta_update(m->g, a, m->stream);
             m->eta_cur ^= 1;
             for (int d = 0; d < nd; ++d) launch_bc_dim(m->g, neumann_batch(m->eta(), L_CCC, d), m->stream);
@@ -454,6 +464,11 @@ int fi_refined_solve(fi_model_t m, int max_outer, int max_inner, int ncheck, dou
             }
             r0 = norms_inf[q] > r0 ? norms_inf[q] : r0;
         }
+        if (trace)
+            std::fprintf(stderr, "[fastice] refined: outer %d residual %.3e eta rate %.3f FP32 iterations so far %d\n",
+                         outer, r0, omega, inner_total);
+        if (r_prev > 0.0 && r0 > 0.9 * r_prev && omega > 1.0 / 64) omega *= 0.5;
+        r_prev = r0;
         if (r0 < tol || outer >= max_outer) break;
         // correction problem: frozen eta, forcing from the FP64 defect, zero state
         cast(m, r->eta, m->eta(), 1.0);

@@ -224,20 +224,22 @@ __device__ __forceinline__ void velocity_point3(const StokesArgs<T>& a, long c, 
     a.eta_next[c] = rheology(a, c, t);
 }
 
+// flat (i fastest) indexing over the boxes of a list: every lane of a block
+// is busy however thin the box, and wide boxes keep coalesced x rows
 __device__ __forceinline__ bool box_index(const BoxList& bl, int& i, int& j, int& k)
 {
     const int bid = blockIdx.x;
     int q = 0;
     while (q + 1 < bl.nb && bid >= bl.start[q + 1]) ++q;
     const Box b = bl.b[q];
-    const int nbx = (b.i1 - b.i0 + 64) / 64, nby = (b.j1 - b.j0 + 4) / 4;
-    const int loc = bid - bl.start[q];
-    const int bx = loc % nbx, r = loc / nbx;
-    const int by = r % nby, bz = r / nby;
-    i = b.i0 + bx * 64 + (int)threadIdx.x;
-    j = b.j0 + by * 4 + (int)threadIdx.y;
-    k = b.k0 + bz;
-    return i <= b.i1 && j <= b.j1;
+    const long nx = b.i1 - b.i0 + 1, ny = b.j1 - b.j0 + 1, nz = b.k1 - b.k0 + 1;
+    const long t = (long)(bid - bl.start[q]) * blockDim.x + threadIdx.x;
+    if (t >= nx * ny * nz) return false;
+    i = b.i0 + (int)(t % nx);
+    const long r = t / nx;
+    j = b.j0 + (int)(r % ny);
+    k = b.k0 + (int)(r / ny);
+    return true;
 }
 
 template <typename T, bool IB>
@@ -1117,8 +1119,8 @@ static void finish_boxes(BoxList& bl)
     bl.start[0] = 0;
     for (int q = 0; q < bl.nb; ++q) {
         const Box& b = bl.b[q];
-        const int nbx = (b.i1 - b.i0 + 64) / 64, nby = (b.j1 - b.j0 + 4) / 4, nbz = b.k1 - b.k0 + 1;
-        bl.start[q + 1] = bl.start[q] + nbx * nby * nbz;
+        const long n = (long)(b.i1 - b.i0 + 1) * (b.j1 - b.j0 + 1) * (b.k1 - b.k0 + 1);
+        bl.start[q + 1] = bl.start[q] + (int)((n + 255) / 256);
     }
 }
 
@@ -1133,16 +1135,13 @@ BoxList full_box(const Geo& g)
 
 bool fused_split(const Geo& g, int ty, int kch, FusedGeom& fg, BoxList& shell)
 {
-    // interior [3, n-1] in every dimension, trimmed to whole workgroup tiles in x and y
-    const int tile[2] = {64 - 2, ty - 2};
+    // interior [3, n-1] in every dimension; the last workgroup tile in x and y
+    // may be partial (its footprint is clamped to hi+1)
+    (void)ty;
     int lo[3], hi[3];
     for (int d = 0; d < 3; ++d) {
         lo[d] = 3;
         hi[d] = g.n[d] - 1;
-        if (d < 2) {
-            const int m = (hi[d] - lo[d] + 1) / tile[d];
-            hi[d] = lo[d] + m * tile[d] - 1;
-        }
         if (hi[d] < lo[d]) {
             shell = full_box(g);
             return false;
@@ -1175,8 +1174,8 @@ void launch_sigma_boxes(const Geo& g, const StokesArgs<T>& a, const BoxList& bl,
     if (g.nd == 3) {
         if (bl.nb == 0 || bl.start[bl.nb] == 0) return;
         dim3 grd(bl.start[bl.nb]);
-        if (ib) hipLaunchKernelGGL((k_sigma_box<T, true>), grd, dim3(64, 4, 1), 0, s, a, bl);
-        else hipLaunchKernelGGL((k_sigma_box<T, false>), grd, dim3(64, 4, 1), 0, s, a, bl);
+        if (ib) hipLaunchKernelGGL((k_sigma_box<T, true>), grd, dim3(256), 0, s, a, bl);
+        else hipLaunchKernelGGL((k_sigma_box<T, false>), grd, dim3(256), 0, s, a, bl);
     } else {
         if (ib) hipLaunchKernelGGL((k_sigma2<T, true>), grid2(g), block2(), 0, s, a);
         else hipLaunchKernelGGL((k_sigma2<T, false>), grid2(g), block2(), 0, s, a);
@@ -1189,8 +1188,8 @@ void launch_velocity_boxes(const Geo& g, const StokesArgs<T>& a, const BoxList& 
     if (g.nd == 3) {
         if (bl.nb == 0 || bl.start[bl.nb] == 0) return;
         dim3 grd(bl.start[bl.nb]);
-        if (ib) hipLaunchKernelGGL((k_velocity_box<T, true>), grd, dim3(64, 4, 1), 0, s, a, bl);
-        else hipLaunchKernelGGL((k_velocity_box<T, false>), grd, dim3(64, 4, 1), 0, s, a, bl);
+        if (ib) hipLaunchKernelGGL((k_velocity_box<T, true>), grd, dim3(256), 0, s, a, bl);
+        else hipLaunchKernelGGL((k_velocity_box<T, false>), grd, dim3(256), 0, s, a, bl);
     } else {
         if (ib) hipLaunchKernelGGL((k_velocity2<T, true>), grid2(g), block2(), 0, s, a);
         else hipLaunchKernelGGL((k_velocity2<T, false>), grid2(g), block2(), 0, s, a);
@@ -1207,8 +1206,8 @@ template <typename T>
 void launch_fused(const Geo& g, const StokesArgs<T>& a, const FusedGeom& fg, bool ib, hipStream_t s)
 {
     constexpr int TY = fused_ty<T>();
-    const int nbx = (fg.hi[0] - fg.lo[0] + 1) / (64 - 2);
-    const int nby = (fg.hi[1] - fg.lo[1] + 1) / (TY - 2);
+    const int nbx = (fg.hi[0] - fg.lo[0] + 1 + 64 - 3) / (64 - 2);
+    const int nby = (fg.hi[1] - fg.lo[1] + 1 + TY - 3) / (TY - 2);
     const int nbz = (fg.hi[2] - fg.lo[2] + fg.kch) / fg.kch;
     dim3 grd(nbx, nby, nbz), blk(64, TY, 1);
     if (ib) hipLaunchKernelGGL((k_fused3_ib<T, TY>), grd, blk, 0, s, a, fg);

@@ -165,10 +165,18 @@ int fi_fused_region(fi_model_t m, int lo[3], int hi[3]);
  * depend on them: FI_OPT_FUSED 1/0 selects the fused interior sweep or the
  * pointwise sweep everywhere; FI_OPT_ZCHUNK sets the z planes per workgroup of
  * the fused sweep.  Refined solver: FI_OPT_REFINE_FULL_PHASE 1/0 starts a
- * nonlinear solve with full FP32 PT sweeps (default 1); FI_OPT_REFINE_ETA_RATE
- * is the log-space relaxation of the outer viscosity update in per mille
- * (default 500). */
-enum { FI_OPT_FUSED = 1, FI_OPT_ZCHUNK = 2, FI_OPT_REFINE_FULL_PHASE = 3, FI_OPT_REFINE_ETA_RATE = 4 };
+ * nonlinear solve with full FP32 PT sweeps (default 1) that stop after
+ * FI_OPT_REFINE_PHASE1_DIGITS digits of residual drop (1..7, default 3) or
+ * at the FP32 floor; FI_OPT_REFINE_ETA_RATE is the initial log-space
+ * relaxation of the outer viscosity update in per mille (default 1000),
+ * halved whenever an outer step reduces the residual by less than 10 %. */
+enum {
+    FI_OPT_FUSED = 1,
+    FI_OPT_ZCHUNK = 2,
+    FI_OPT_REFINE_FULL_PHASE = 3,
+    FI_OPT_REFINE_ETA_RATE = 4,
+    FI_OPT_REFINE_PHASE1_DIGITS = 5
+};
 int fi_set_option(fi_model_t m, int key, int value);
 
 int fi_levelset_from_dem(fi_model_t m, int phase, const double* dem_parent, int dem_nx, int dem_ny,
@@ -184,9 +192,13 @@ int fi_omega_from_psi(fi_model_t m, int phase);
  * eta, homogeneous boundary values and the FP64 defect as forcing, checked
  * every ncheck iterations, until its residual falls below inner_rtol times the
  * outer one (at most max_inner iterations), and is added to the FP64 state.
- * counts_out = {outer steps, total FP32 iterations}. */
+ * Nonlinear rheologies first run full FP32 PT sweeps (eta updated every
+ * sweep) until the FP32 residual drops by 1e-6 or stalls, and relax the outer
+ * viscosity update in log space (FI_OPT_REFINE_*).
+ * counts_out = {outer steps, total FP32 iterations, FP32 iterations of the
+ * nonlinear phase}. */
 int fi_refined_solve(fi_model_t m, int max_outer, int max_inner, int ncheck, double inner_rtol, double tol,
-                     int counts_out[2], double norms_inf[4]);
+                     int counts_out[3], double norms_inf[4]);
 /* n_iter FP32 correction iterations on the current correction state (timing) */
 int fi_refined_advance_inner(fi_model_t m, int n_iter);
 /* as fi_time_iterations, for the FP32 correction iteration */
