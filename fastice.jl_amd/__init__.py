@@ -12,6 +12,7 @@ include/fastice_hip.h).  The directory name contains a dot, so load it with
 """
 from . import _abi
 from ._abi import FastIceError, NonFiniteError
+from .batch import Batch, model_bytes
 from .full_stokes import (BoundaryCondition, IsothermalFullStokesModel, IsothermalImmersedBoundaryFullStokesModel,
                           SBC, Slip, TBC, Traction, VBC, Velocity, advance_iteration, build_desc, compute_residuals,
                           norm_inf)

@@ -27,6 +27,8 @@ FI_LAYOUT_PARENT, FI_LAYOUT_INTERIOR = 0, 1
 FI_BCGROUP_STRESS, FI_BCGROUP_VELOCITY, FI_BCGROUP_ETA, FI_BCGROUP_RESIDUAL = 0, 1, 2, 3
 FI_PHASE_NA, FI_PHASE_NS = 0, 1
 FI_OPT_FUSED, FI_OPT_ZCHUNK, FI_OPT_REFINE_FULL_PHASE, FI_OPT_REFINE_ETA_RATE, FI_OPT_REFINE_PHASE1_DIGITS = 1, 2, 3, 4, 5
+FI_BATCH_RUNNING, FI_BATCH_CONVERGED, FI_BATCH_MAX_ITER, FI_BATCH_FAILED = 0, 1, 2, 3
+FI_BOPT_TILE_Y, FI_BOPT_ZCHUNK, FI_BOPT_AUTOTUNE, FI_BOPT_GRAPH_ITERS = 1, 2, 3, 4
 
 FIELD_NAMES = [
     "P", "TXX", "TYY", "TZZ", "TXY", "TXZ", "TYZ", "VX", "VY", "VZ", "ETA", "ETA_NEXT",
@@ -145,6 +147,16 @@ def _declare(lib):
         "fi_refined_solve": (C.c_int, [H, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.POINTER(C.c_int), _DP]),
         "fi_refined_advance_inner": (C.c_int, [H, C.c_int]),
         "fi_refined_time_inner": (C.c_int, [H, C.c_int, _DP]),
+        "fi_model_bytes": (C.c_size_t, [C.POINTER(fi_model_desc)]),
+        "fi_batch_create": (C.c_int, [C.POINTER(C.c_void_p)]),
+        "fi_batch_destroy": (C.c_int, [C.c_void_p]),
+        "fi_batch_add": (C.c_int, [C.c_void_p, H, C.c_int, C.c_int, C.c_double]),
+        "fi_batch_remove": (C.c_int, [C.c_void_p, H]),
+        "fi_batch_step": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+        "fi_batch_solve": (C.c_int, [C.c_void_p]),
+        "fi_batch_status": (C.c_int, [C.c_void_p, H, C.POINTER(C.c_int), C.POINTER(C.c_int), _DP]),
+        "fi_batch_set_option": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+        "fi_batch_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

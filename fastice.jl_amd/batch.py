@@ -1,0 +1,100 @@
+"""Batched solves over libfasticehip.so (fi_batch_* in include/fastice_hip.h).
+
+No counterpart in the reference: FastIce.jl solves one model at a time in the
+driver loop (scripts_future_API/tm_stokes_chmy.jl:96-108).  A Batch advances
+every attached FP64 model with packed launches (one fused-sweep launch over
+the tiles of all instances, one launch per pointwise shell kind, one per
+boundary dimension) replayed as HIP graphs; each model keeps the semantics of
+IsothermalFullStokesModel.solve exactly, so its result is bitwise the one it
+gets alone.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _abi as A
+from .full_stokes import build_desc
+
+STATES = {A.FI_BATCH_RUNNING: "running", A.FI_BATCH_CONVERGED: "converged", A.FI_BATCH_MAX_ITER: "max_iter",
+          A.FI_BATCH_FAILED: "failed"}
+
+
+def model_bytes(grid, boundary_conditions, gravity, rheology, solver_params, immersed_boundary=False):
+    """Device bytes a model of this description allocates (admission control)."""
+    desc, _, _ = build_desc(grid, boundary_conditions, gravity, rheology, solver_params,
+                            immersed_boundary=immersed_boundary)
+    return int(A.lib().fi_model_bytes(C.byref(desc)))
+
+
+class Batch:
+    """A set of models solved together.
+
+    add(model, max_iter, ncheck, tol) attaches a model and starts its solve;
+    step() advances the running models to the next residual check of the
+    nearest one and returns how many finished; solve() steps until none runs;
+    status(model) gives (state, iterations, Inf-norms).
+    """
+
+    def __init__(self, autotune=True, tile_y=None, zchunk=None, graph_iters=None):
+        h = C.c_void_p()
+        A.check(A.lib().fi_batch_create(C.byref(h)))
+        self._h = h
+        self._models = {}
+        self.set_option(A.FI_BOPT_AUTOTUNE, 1 if autotune else 0)
+        if tile_y is not None:
+            self.set_option(A.FI_BOPT_TILE_Y, tile_y)
+        if zchunk is not None:
+            self.set_option(A.FI_BOPT_ZCHUNK, zchunk)
+        if graph_iters is not None:
+            self.set_option(A.FI_BOPT_GRAPH_ITERS, graph_iters)
+
+    def set_option(self, key, value):
+        A.check(A.lib().fi_batch_set_option(self._h, int(key), int(value)))
+
+    def add(self, model, max_iter, ncheck, tol):
+        A.check(A.lib().fi_batch_add(self._h, model._h, int(max_iter), int(ncheck), float(tol)))
+        self._models[id(model)] = model
+
+    def remove(self, model):
+        A.check(A.lib().fi_batch_remove(self._h, model._h))
+        self._models.pop(id(model), None)
+
+    def step(self):
+        n = C.c_int(0)
+        A.check(A.lib().fi_batch_step(self._h, C.byref(n)))
+        return n.value
+
+    def solve(self):
+        A.check(A.lib().fi_batch_solve(self._h))
+
+    def status(self, model):
+        st, it = C.c_int(0), C.c_int(0)
+        norms = np.zeros(4)
+        A.check(A.lib().fi_batch_status(self._h, model._h, C.byref(st), C.byref(it), A.dptr(norms)))
+        return STATES[st.value], it.value, norms
+
+    def info(self):
+        out = (C.c_int * 8)()
+        A.check(A.lib().fi_batch_info(self._h, out))
+        keys = ("attached", "running", "fused_items", "tile_y", "zchunk", "graph_iters", "autotune_runs",
+                "rebuilds")
+        return dict(zip(keys, list(out)))
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            A.lib().fi_batch_destroy(h)
+            self._h = C.c_void_p()
+        self._models = {}
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

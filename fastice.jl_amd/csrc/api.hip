@@ -228,6 +228,7 @@ void iterate(fi_model_s* m, hipEvent_t* ev)
 
 int advance(fi_model_s* m, int niter)
 {
+    if (m->batch && niter > 0) batch_touch(m); // the batch's tables assume the buffer parity
     int rc = prepare(m);
     if (rc) return rc;
     for (int it = 0; it < niter; ++it) iterate(m, nullptr);
@@ -241,7 +242,8 @@ double bits_to_double(unsigned long long b)
     return v;
 }
 
-int residual_norms(fi_model_s* m, double out[4])
+// enqueue compute_residuals! and the Inf-norms into the pinned h_norm
+int residual_norms_enqueue(fi_model_s* m)
 {
     const bool ib = m->desc.immersed_boundary != 0;
     int rc = prepare(m);
@@ -254,8 +256,22 @@ int residual_norms(fi_model_s* m, double out[4])
     const int nf = m->g.nd == 3 ? 4 : 3;
     launch_norms(m->g, f, L, nf, m->d_norm, m->stream);
     HIPCHK(hipMemcpyAsync(m->h_norm, m->d_norm, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, m->stream));
-    HIPCHK(hipStreamSynchronize(m->stream));
+    return FI_OK;
+}
+
+// the norms of the last residual_norms_enqueue, once its stream has synchronised
+void residual_norms_read(fi_model_s* m, double out[4])
+{
+    const int nf = m->g.nd == 3 ? 4 : 3;
     for (int q = 0; q < 4; ++q) out[q] = (q < nf) ? bits_to_double(m->h_norm[q]) : 0.0;
+}
+
+int residual_norms(fi_model_s* m, double out[4])
+{
+    int rc = residual_norms_enqueue(m);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(m->stream));
+    residual_norms_read(m, out);
     return FI_OK;
 }
 
@@ -317,23 +333,16 @@ int fi_abi_version(void) { return FI_ABI_VERSION; }
 
 const char* fi_last_error(void) { return g_err; }
 
-int fi_model_create(const fi_model_desc* desc, fi_model_t* out)
+// device layout: rows padded to 128-byte lines, Chmy index 1 of every row
+// 128-byte aligned
+static Geo make_geo(const fi_model_desc& desc)
 {
-    if (!out) return fail(FI_ERR_ARG, "null output handle");
-    *out = nullptr;
-    int rc = check_desc(desc);
-    if (rc) return rc;
-    if (desc->precision == FI_PREC_FP32_REFINED)
-        return fail(FI_ERR_UNSUPPORTED, "FP32-refined models are created through fi_refined_* entry points");
-    fi_model_s* m = new (std::nothrow) fi_model_s();
-    if (!m) return fail(FI_ERR_OOM, "host allocation failed");
-    m->desc = *desc;
-    Geo& g = m->g;
-    g.nd = desc->ndims;
+    Geo g;
+    g.nd = desc.ndims;
     for (int d = 0; d < 3; ++d) {
         if (d < g.nd) {
-            g.n[d] = desc->n[d];
-            g.dx[d] = desc->extent[d] / (double)desc->n[d];
+            g.n[d] = desc.n[d];
+            g.dx[d] = desc.extent[d] / (double)desc.n[d];
             g.idx[d] = 1.0 / g.dx[d];
         } else {
             g.n[d] = 0;
@@ -347,6 +356,34 @@ int fi_model_create(const fi_model_desc* desc, fi_model_t* out)
     g.off = align - 1;
     const long rows = (g.nd == 3) ? (long)(g.n[1] + 3) * (g.n[2] + 3) : (long)(g.n[1] + 3);
     g.elems = (size_t)(g.off + g.sy * rows + align);
+    return g;
+}
+
+size_t fi_model_bytes(const fi_model_desc* desc)
+{
+    if (check_desc(desc) != FI_OK) return 0;
+    const Geo g = make_geo(*desc);
+    size_t fields = 0;
+    for (int id = 0; id < FI_F_COUNT; ++id)
+        if (field_used(*desc, id)) fields += (id >= FI_F_P && id <= FI_F_VZ) ? 2 : 1;
+    size_t bytes = fields * g.elems * sizeof(double) + 4 * sizeof(unsigned long long);
+    if (desc->immersed_boundary) bytes += g.elems;
+    return bytes;
+}
+
+int fi_model_create(const fi_model_desc* desc, fi_model_t* out)
+{
+    if (!out) return fail(FI_ERR_ARG, "null output handle");
+    *out = nullptr;
+    int rc = check_desc(desc);
+    if (rc) return rc;
+    if (desc->precision == FI_PREC_FP32_REFINED)
+        return fail(FI_ERR_UNSUPPORTED, "FP32-refined models are created through fi_refined_* entry points");
+    fi_model_s* m = new (std::nothrow) fi_model_s();
+    if (!m) return fail(FI_ERR_OOM, "host allocation failed");
+    m->desc = *desc;
+    m->g = make_geo(*desc);
+    Geo& g = m->g;
     if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess) {
         free_model(m);
         return fail(FI_ERR_HIP, "hipStreamCreate failed");
@@ -401,6 +438,7 @@ int fi_model_create(const fi_model_desc* desc, fi_model_t* out)
 int fi_model_destroy(fi_model_t m)
 {
     if (!m) return FI_OK;
+    if (m->batch) batch_detach(m);
     (void)hipStreamSynchronize(m->stream);
     free_model(m);
     return FI_OK;
@@ -421,6 +459,7 @@ static void mark_dirty(fi_model_t m, int id)
         m->flags_dirty = true;
         ++m->mask_gen;
     }
+    if (m->batch) batch_touch(m); // the batch re-prepares the model before its next step
 }
 
 static int field_copy(fi_model_t m, int id, double* host, size_t count, int layout, bool up)

@@ -1,0 +1,651 @@
+// batch.hip -- batched PT solves: many FP64 models advanced together by
+// packed launches over (instance, tile) work lists, replayed as HIP graphs.
+//
+// Per iteration of the running set the batch launches
+//   fused interior tiles of every 3D instance       (k_fused3_packed, per IB kind)
+//   pointwise stress update of shells / 2D grids    (k_point_packed, per kind)
+//   stress boundary conditions, x then y then z     (k_bc_packed)
+//   pointwise velocity update of shells / 2D grids  (k_point_packed)
+//   velocity + eta_next boundary conditions         (k_bc_packed)
+// which is the single-model iteration (api.hip: iterate) with every launch
+// widened to all instances, so each instance computes exactly what it
+// computes alone.  The kernel arguments of an instance depend on its buffer
+// parity; the tables hold both parities and the iterations alternate them.
+#include "model.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+namespace fi {
+
+struct BatchInst {
+    fi_model_s* m = nullptr;
+    hipStream_t home = nullptr; // the model's own stream, restored on removal
+    int max_iter = 0, ncheck = 1;
+    double tol = 0.0;
+    int iters = 0;
+    int state = FI_BATCH_RUNNING;
+    double norms[4] = {};
+};
+
+enum { K_3D_PLAIN = 0, K_3D_IB = 1, K_2D_PLAIN = 2, K_2D_IB = 3 };
+
+// device tables of the running set for one tile configuration
+struct BatchTables {
+    char* dev = nullptr;
+    size_t cap = 0;
+    PackedInst<double>* inst[2] = {};
+    int4* items[2] = {}; // plain, IB
+    int nitems[2] = {};
+    PackedBox* box[4] = {};
+    int* blk2box[4] = {};
+    int nblk_box[4] = {};
+    PackedBcOp* bc[2][2][3] = {}; // parity, group (0 stress, 1 velocity), dim
+    int* blk2bc[2][2][3] = {};
+    int nblk_bc[2][2][3] = {};
+};
+
+} // namespace fi
+
+using namespace fi;
+
+struct fi_batch_s {
+    hipStream_t stream = nullptr;
+    std::vector<BatchInst> inst;
+    std::vector<int> run; // indices of running instances in table order
+    bool dirty = true;
+    int phase = 0; // iterations since the tables were built, mod 2
+    BatchTables t;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    int ty = 8, kch = 32;
+    int autotune = 1;
+    int graph_iters = 8;
+    long tuned_cells = 0; // size of the working set the tile choice was made for
+    int tune_runs = 0;
+    int rebuilds = 0;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+};
+
+namespace fi {
+
+static void destroy_graphs(fi_batch_s* b)
+{
+    for (int p = 0; p < 2; ++p)
+        if (b->graph[p]) {
+            (void)hipGraphExecDestroy(b->graph[p]);
+            b->graph[p] = nullptr;
+        }
+}
+
+void batch_touch(fi_model_s* m)
+{
+    if (m->batch) m->batch->dirty = true;
+}
+
+static int find_inst(fi_batch_s* b, fi_model_s* m)
+{
+    for (size_t q = 0; q < b->inst.size(); ++q)
+        if (b->inst[q].m == m) return (int)q;
+    return -1;
+}
+
+static int detach(fi_batch_s* b, int q)
+{
+    HIPCHK(hipStreamSynchronize(b->stream));
+    fi_model_s* m = b->inst[q].m;
+    m->stream = b->inst[q].home;
+    m->batch = nullptr;
+    b->inst.erase(b->inst.begin() + q);
+    b->dirty = true;
+    return FI_OK;
+}
+
+void batch_detach(fi_model_s* m)
+{
+    fi_batch_s* b = m->batch;
+    int q = find_inst(b, m);
+    if (q >= 0) (void)detach(b, q);
+}
+
+// the model's kernel arguments and boundary batches after p iterations
+// (buffer parities flipped p times)
+struct ParityView {
+    fi_model_s* m;
+    int s0, v0, e0;
+    ParityView(fi_model_s* mm, int p) : m(mm), s0(mm->sig_cur), v0(mm->v_cur), e0(mm->eta_cur)
+    {
+        m->sig_cur ^= p;
+        m->v_cur ^= p;
+        m->eta_cur ^= p;
+    }
+    ~ParityView()
+    {
+        m->sig_cur = s0;
+        m->v_cur = v0;
+        m->eta_cur = e0;
+    }
+};
+
+static bool fused_ok(const fi_model_s* m)
+{
+    return m->g.nd == 3 && m->g.elems * sizeof(double) < (1ull << 32);
+}
+
+static int kind_of(const fi_model_s* m)
+{
+    const bool ib = m->desc.immersed_boundary != 0;
+    return m->g.nd == 3 ? (ib ? K_3D_IB : K_3D_PLAIN) : (ib ? K_2D_IB : K_2D_PLAIN);
+}
+
+template <typename X>
+static size_t put(std::vector<char>& h, const std::vector<X>& v)
+{
+    size_t off = (h.size() + 255) / 256 * 256;
+    h.resize(off + v.size() * sizeof(X));
+    if (!v.empty()) std::memcpy(h.data() + off, v.data(), v.size() * sizeof(X));
+    return off;
+}
+
+static void add_bc_ops(std::vector<PackedBcOp>& ops, std::vector<int>& blk, const fi_model_s* m, const BcBatch& bb)
+{
+    const Geo& g = m->g;
+    int e1 = -1, e2 = -1;
+    for (int e = 0; e < g.nd; ++e) {
+        if (e == bb.d) continue;
+        if (e1 < 0) e1 = e;
+        else e2 = e;
+    }
+    for (int o = 0; o < bb.nop; ++o) {
+        PackedBcOp p;
+        std::memset(&p, 0, sizeof(p));
+        p.op = bb.op[o];
+        p.st[0] = 1;
+        p.st[1] = g.sy;
+        p.st[2] = g.sz;
+        p.d = bb.d;
+        p.nn = g.n[bb.d];
+        p.e1 = e1;
+        p.e2 = e2;
+        p.ext1 = g.n[e1] + ((p.op.L >> e1) & 1) + 2;
+        p.ext2 = (e2 >= 0) ? g.n[e2] + ((p.op.L >> e2) & 1) + 2 : 1;
+        p.start = (int)blk.size();
+        const long work = 2L * p.ext1 * p.ext2;
+        const int nb = (int)((work + 255) / 256);
+        const int idx = (int)ops.size();
+        ops.push_back(p);
+        for (int k = 0; k < nb; ++k) blk.push_back(idx);
+    }
+}
+
+// build the tables of the running set for tile rows ty and z chunk kch
+static int build_tables(fi_batch_s* b, int ty, int kch)
+{
+    const int nrun = (int)b->run.size();
+    std::vector<PackedInst<double>> pinst[2];
+    std::vector<int4> items[2];
+    std::vector<PackedBox> boxes[4];
+    std::vector<int> blk2box[4];
+    std::vector<PackedBcOp> bcops[2][2][3];
+    std::vector<int> bcblk[2][2][3];
+    for (int r = 0; r < nrun; ++r) {
+        fi_model_s* m = b->inst[b->run[r]].m;
+        const int kind = kind_of(m);
+        FusedGeom fg;
+        BoxList shell;
+        bool interior = fused_ok(m) && fused_split(m->g, ty, kch, fg, shell);
+        if (!interior) {
+            shell = full_box(m->g);
+            std::memset(&fg, 0, sizeof(fg));
+        }
+        for (int p = 0; p < 2; ++p) {
+            ParityView pv(m, p);
+            PackedInst<double> pi;
+            pi.a = make_args(m, false);
+            pi.fg = fg;
+            pinst[p].push_back(pi);
+            {
+                // stress BCs see the new stress buffers
+                ParityView ps(m, 0);
+                m->sig_cur ^= 1;
+                for (int d = 0; d < m->g.nd; ++d) add_bc_ops(bcops[p][0][d], bcblk[p][0][d], m, stress_batch(m, d));
+                m->v_cur ^= 1;
+                for (int d = 0; d < m->g.nd; ++d)
+                    add_bc_ops(bcops[p][1][d], bcblk[p][1][d], m, velocity_batch(m, d, true));
+            }
+        }
+        if (interior) {
+            const int ib = m->desc.immersed_boundary ? 1 : 0;
+            const int nbx = (fg.hi[0] - fg.lo[0] + 1 + 64 - 3) / (64 - 2);
+            const int nby = (fg.hi[1] - fg.lo[1] + 1 + ty - 3) / (ty - 2);
+            const int nbz = (fg.hi[2] - fg.lo[2] + fg.kch) / fg.kch;
+            for (int bz = 0; bz < nbz; ++bz)
+                for (int by = 0; by < nby; ++by)
+                    for (int bx = 0; bx < nbx; ++bx) items[ib].push_back(make_int4(r, bx, by, bz));
+        }
+        for (int q = 0; q < shell.nb; ++q) {
+            const Box& x = shell.b[q];
+            PackedBox pb;
+            pb.inst = r;
+            pb.i0 = x.i0;
+            pb.i1 = x.i1;
+            pb.j0 = x.j0;
+            pb.j1 = x.j1;
+            pb.k0 = m->g.nd == 3 ? x.k0 : 0;
+            pb.k1 = m->g.nd == 3 ? x.k1 : 0;
+            pb.start = (int)blk2box[kind].size();
+            const long n = (long)(pb.i1 - pb.i0 + 1) * (pb.j1 - pb.j0 + 1) * (pb.k1 - pb.k0 + 1);
+            const int nb = (int)((n + 255) / 256);
+            const int idx = (int)boxes[kind].size();
+            boxes[kind].push_back(pb);
+            for (int k = 0; k < nb; ++k) blk2box[kind].push_back(idx);
+        }
+    }
+    std::vector<char> h;
+    size_t o_inst[2], o_items[2], o_box[4], o_blk[4], o_bc[2][2][3], o_bcblk[2][2][3];
+    for (int p = 0; p < 2; ++p) o_inst[p] = put(h, pinst[p]);
+    for (int q = 0; q < 2; ++q) o_items[q] = put(h, items[q]);
+    for (int k = 0; k < 4; ++k) {
+        o_box[k] = put(h, boxes[k]);
+        o_blk[k] = put(h, blk2box[k]);
+    }
+    for (int p = 0; p < 2; ++p)
+        for (int gr = 0; gr < 2; ++gr)
+            for (int d = 0; d < 3; ++d) {
+                o_bc[p][gr][d] = put(h, bcops[p][gr][d]);
+                o_bcblk[p][gr][d] = put(h, bcblk[p][gr][d]);
+            }
+    BatchTables& t = b->t;
+    if (h.size() > t.cap) {
+        HIPCHK(hipStreamSynchronize(b->stream)); // graphs in flight may still read the old tables
+        destroy_graphs(b);
+        if (t.dev) HIPCHK(hipFree(t.dev));
+        t.dev = nullptr;
+        t.cap = 0;
+        size_t cap = h.size() + h.size() / 2 + 4096;
+        if (hipMalloc(&t.dev, cap) != hipSuccess) return fail(FI_ERR_OOM, "batch tables (%zu bytes)", cap);
+        t.cap = cap;
+    } else {
+        HIPCHK(hipStreamSynchronize(b->stream));
+    }
+    if (!h.empty()) HIPCHK(hipMemcpy(t.dev, h.data(), h.size(), hipMemcpyHostToDevice));
+    for (int p = 0; p < 2; ++p) t.inst[p] = reinterpret_cast<PackedInst<double>*>(t.dev + o_inst[p]);
+    for (int q = 0; q < 2; ++q) {
+        t.items[q] = reinterpret_cast<int4*>(t.dev + o_items[q]);
+        t.nitems[q] = (int)items[q].size();
+    }
+    for (int k = 0; k < 4; ++k) {
+        t.box[k] = reinterpret_cast<PackedBox*>(t.dev + o_box[k]);
+        t.blk2box[k] = reinterpret_cast<int*>(t.dev + o_blk[k]);
+        t.nblk_box[k] = (int)blk2box[k].size();
+    }
+    for (int p = 0; p < 2; ++p)
+        for (int gr = 0; gr < 2; ++gr)
+            for (int d = 0; d < 3; ++d) {
+                t.bc[p][gr][d] = reinterpret_cast<PackedBcOp*>(t.dev + o_bc[p][gr][d]);
+                t.blk2bc[p][gr][d] = reinterpret_cast<int*>(t.dev + o_bcblk[p][gr][d]);
+                t.nblk_bc[p][gr][d] = (int)bcblk[p][gr][d].size();
+            }
+    b->ty = ty;
+    b->kch = kch;
+    b->phase = 0;
+    b->rebuilds++;
+    return FI_OK;
+}
+
+// persistent grid of the fused sweep: enough workgroups for every CU
+static int fused_grid(int nitems) { return nitems < 8192 ? nitems : 8192; }
+
+static void launch_fused_all(fi_batch_s* b, int p)
+{
+    const BatchTables& t = b->t;
+    for (int q = 0; q < 2; ++q)
+        launch_fused_packed<double>(t.inst[p], t.items[q], t.nitems[q], q == 1, b->ty, fused_grid(t.nitems[q]),
+                                    b->stream);
+}
+
+// one PT iteration of every running instance at table parity p
+static void launch_iteration(fi_batch_s* b, int p)
+{
+    const BatchTables& t = b->t;
+    launch_fused_all(b, p);
+    const int kib[4] = {0, 1, 0, 1}, knd[4] = {3, 3, 2, 2};
+    for (int k = 0; k < 4; ++k)
+        launch_point_packed<double>(t.inst[p], t.box[k], t.blk2box[k], t.nblk_box[k], kib[k] != 0, knd[k], true,
+                                    b->stream);
+    for (int d = 0; d < 3; ++d) launch_bc_packed(t.bc[p][0][d], t.blk2bc[p][0][d], t.nblk_bc[p][0][d], b->stream);
+    for (int k = 0; k < 4; ++k)
+        launch_point_packed<double>(t.inst[p], t.box[k], t.blk2box[k], t.nblk_box[k], kib[k] != 0, knd[k], false,
+                                    b->stream);
+    for (int d = 0; d < 3; ++d) launch_bc_packed(t.bc[p][1][d], t.blk2bc[p][1][d], t.nblk_bc[p][1][d], b->stream);
+}
+
+static int capture_graphs(fi_batch_s* b)
+{
+    destroy_graphs(b);
+    if (b->graph_iters <= 0) return FI_OK;
+    for (int p = 0; p < 2; ++p) {
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        for (int it = 0; it < b->graph_iters; ++it) launch_iteration(b, (p + it) & 1);
+        HIPCHK(hipStreamEndCapture(b->stream, &g));
+        hipError_t e = hipGraphInstantiate(&b->graph[p], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return fail(FI_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    }
+    return FI_OK;
+}
+
+static long running_cells(fi_batch_s* b)
+{
+    long c = 0;
+    for (int r : b->run) {
+        const Geo& g = b->inst[r].m->g;
+        c += (long)g.n[0] * g.n[1] * (g.nd == 3 ? g.n[2] : 1);
+    }
+    return c;
+}
+
+// Time the fused sweep of every tile candidate on genuine iterations of the
+// running set (the tile shape does not change any result) and keep the
+// fastest.  Uses at most `budget` iterations; returns how many it ran.
+static int autotune(fi_batch_s* b, int budget, int* used)
+{
+    *used = 0;
+    struct Cand {
+        int ty, kch;
+    };
+    const Cand cands[] = {{8, 32}, {8, 16}, {8, 64}, {4, 32}, {4, 64}};
+    const int nc = sizeof(cands) / sizeof(cands[0]);
+    const int reps = 2; // one warm-up iteration, one timed
+    if (budget < nc * reps) return FI_OK;
+    int best = -1;
+    float best_ms = 0.0f;
+    for (int c = 0; c < nc; ++c) {
+        int rc = build_tables(b, cands[c].ty, cands[c].kch);
+        if (rc) return rc;
+        float ms = 0.0f;
+        for (int k = 0; k < reps; ++k) {
+            // full iterations; only the fused launches of the last one are timed
+            const int p = b->phase;
+            if (k == reps - 1) HIPCHK(hipEventRecord(b->ev[0], b->stream));
+            launch_fused_all(b, p);
+            if (k == reps - 1) HIPCHK(hipEventRecord(b->ev[1], b->stream));
+            const BatchTables& t = b->t;
+            const int kib[4] = {0, 1, 0, 1}, knd[4] = {3, 3, 2, 2};
+            for (int q = 0; q < 4; ++q)
+                launch_point_packed<double>(t.inst[p], t.box[q], t.blk2box[q], t.nblk_box[q], kib[q] != 0, knd[q],
+                                            true, b->stream);
+            for (int d = 0; d < 3; ++d)
+                launch_bc_packed(t.bc[p][0][d], t.blk2bc[p][0][d], t.nblk_bc[p][0][d], b->stream);
+            for (int q = 0; q < 4; ++q)
+                launch_point_packed<double>(t.inst[p], t.box[q], t.blk2box[q], t.nblk_box[q], kib[q] != 0, knd[q],
+                                            false, b->stream);
+            for (int d = 0; d < 3; ++d)
+                launch_bc_packed(t.bc[p][1][d], t.blk2bc[p][1][d], t.nblk_bc[p][1][d], b->stream);
+            // the tables were rebuilt for this candidate: keep the models' parity
+            // in step with every iteration run here
+            for (int r : b->run) {
+                fi_model_s* m = b->inst[r].m;
+                m->sig_cur ^= 1;
+                m->v_cur ^= 1;
+                m->eta_cur ^= 1;
+                b->inst[r].iters++;
+            }
+            b->phase ^= 1;
+            (*used)++;
+        }
+        HIPCHK(hipEventSynchronize(b->ev[1]));
+        HIPCHK(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
+        if (best < 0 || ms < best_ms) {
+            best = c;
+            best_ms = ms;
+        }
+        // the next candidate's tables start from the models' current parity
+        b->phase = 0;
+    }
+    b->tune_runs++;
+    b->tuned_cells = running_cells(b);
+    int rc = build_tables(b, cands[best].ty, cands[best].kch);
+    if (rc) return rc;
+    return capture_graphs(b);
+}
+
+static int rebuild(fi_batch_s* b)
+{
+    b->run.clear();
+    for (size_t q = 0; q < b->inst.size(); ++q)
+        if (b->inst[q].state == FI_BATCH_RUNNING) b->run.push_back((int)q);
+    for (int r : b->run) {
+        int rc = prepare(b->inst[r].m);
+        if (rc) return rc;
+    }
+    int rc = build_tables(b, b->ty, b->kch);
+    if (rc) return rc;
+    rc = capture_graphs(b);
+    if (rc) return rc;
+    b->dirty = false;
+    return FI_OK;
+}
+
+static int finish_check(BatchInst& bi)
+{
+    residual_norms_read(bi.m, bi.norms);
+    bool conv = true;
+    for (int q = 0; q < 4; ++q) {
+        if (!std::isfinite(bi.norms[q])) {
+            bi.state = FI_BATCH_FAILED;
+            return 1;
+        }
+        if (!(bi.norms[q] < bi.tol)) conv = false;
+    }
+    if (conv) {
+        bi.state = FI_BATCH_CONVERGED;
+        return 1;
+    }
+    return 0;
+}
+
+static int step(fi_batch_s* b, int* n_finished)
+{
+    *n_finished = 0;
+    if (b->dirty) {
+        int rc = rebuild(b);
+        if (rc) return rc;
+    }
+    if (b->run.empty()) return FI_OK;
+    // iterations to the next residual check (or the iteration cap) of the nearest instance
+    int delta = 1 << 30;
+    for (int r : b->run) {
+        const BatchInst& bi = b->inst[r];
+        const int to_check = bi.ncheck - bi.iters % bi.ncheck;
+        delta = std::min(delta, std::min(to_check, bi.max_iter - bi.iters));
+    }
+    int done = 0;
+    const long cells = running_cells(b);
+    if (b->autotune && (b->tuned_cells == 0 || cells > 2 * b->tuned_cells || 2 * cells < b->tuned_cells)) {
+        int rc = autotune(b, delta, &done);
+        if (rc) return rc;
+    }
+    int left = delta - done;
+    while (b->graph_iters > 0 && left >= b->graph_iters) {
+        HIPCHK(hipGraphLaunch(b->graph[b->phase], b->stream));
+        left -= b->graph_iters;
+        b->phase ^= (b->graph_iters & 1);
+    }
+    while (left > 0) {
+        launch_iteration(b, b->phase);
+        b->phase ^= 1;
+        --left;
+    }
+    HIPCHK(hipGetLastError());
+    const int adv = delta - done;
+    std::vector<int> due;
+    for (int r : b->run) {
+        BatchInst& bi = b->inst[r];
+        fi_model_s* m = bi.m;
+        m->sig_cur ^= (adv & 1);
+        m->v_cur ^= (adv & 1);
+        m->eta_cur ^= (adv & 1);
+        bi.iters += adv;
+        if (bi.iters % bi.ncheck == 0) {
+            int rc = residual_norms_enqueue(m);
+            if (rc) return rc;
+            due.push_back(r);
+        }
+    }
+    HIPCHK(hipStreamSynchronize(b->stream));
+    for (int r : b->run) {
+        BatchInst& bi = b->inst[r];
+        bool fin = false;
+        if (std::find(due.begin(), due.end(), r) != due.end()) fin = finish_check(bi) != 0;
+        if (!fin && bi.iters >= bi.max_iter) {
+            bi.state = FI_BATCH_MAX_ITER;
+            fin = true;
+        }
+        if (fin) {
+            (*n_finished)++;
+            b->dirty = true;
+        }
+    }
+    return FI_OK;
+}
+
+} // namespace fi
+
+extern "C" {
+
+int fi_batch_create(fi_batch_t* out)
+{
+    if (!out) return fail(FI_ERR_ARG, "null output handle");
+    *out = nullptr;
+    fi_batch_s* b = new (std::nothrow) fi_batch_s();
+    if (!b) return fail(FI_ERR_OOM, "host allocation failed");
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&b->ev[0]) != hipSuccess || hipEventCreate(&b->ev[1]) != hipSuccess) {
+        delete b;
+        return fail(FI_ERR_HIP, "stream/event creation failed");
+    }
+    *out = b;
+    return FI_OK;
+}
+
+int fi_batch_destroy(fi_batch_t b)
+{
+    if (!b) return FI_OK;
+    (void)hipStreamSynchronize(b->stream);
+    while (!b->inst.empty()) (void)detach(b, (int)b->inst.size() - 1);
+    destroy_graphs(b);
+    if (b->t.dev) (void)hipFree(b->t.dev);
+    for (int k = 0; k < 2; ++k)
+        if (b->ev[k]) (void)hipEventDestroy(b->ev[k]);
+    (void)hipStreamDestroy(b->stream);
+    delete b;
+    return FI_OK;
+}
+
+int fi_batch_add(fi_batch_t b, fi_model_t m, int max_iter, int ncheck, double tol)
+{
+    if (!b || !m) return fail(FI_ERR_ARG, "null argument");
+    if (max_iter < 0 || ncheck < 1) return fail(FI_ERR_ARG, "bad max_iter/ncheck");
+    if (m->batch) return fail(FI_ERR_ARG, "model already belongs to a batch");
+    // the model's pending work completes before the batch stream takes over
+    HIPCHK(hipStreamSynchronize(m->stream));
+    BatchInst bi;
+    bi.m = m;
+    bi.home = m->stream;
+    bi.max_iter = max_iter;
+    bi.ncheck = ncheck;
+    bi.tol = tol;
+    bi.state = max_iter == 0 ? FI_BATCH_MAX_ITER : FI_BATCH_RUNNING;
+    m->stream = b->stream;
+    m->batch = b;
+    b->inst.push_back(bi);
+    b->dirty = true;
+    return FI_OK;
+}
+
+int fi_batch_remove(fi_batch_t b, fi_model_t m)
+{
+    if (!b || !m) return fail(FI_ERR_ARG, "null argument");
+    int q = find_inst(b, m);
+    if (q < 0) return fail(FI_ERR_ARG, "model is not in this batch");
+    return detach(b, q);
+}
+
+int fi_batch_step(fi_batch_t b, int* n_finished)
+{
+    if (!b) return fail(FI_ERR_ARG, "null batch");
+    int nf = 0;
+    int rc = step(b, &nf);
+    if (n_finished) *n_finished = nf;
+    return rc;
+}
+
+int fi_batch_solve(fi_batch_t b)
+{
+    if (!b) return fail(FI_ERR_ARG, "null batch");
+    for (;;) {
+        int nf = 0;
+        int rc = step(b, &nf);
+        if (rc) return rc;
+        bool any = false;
+        for (const BatchInst& bi : b->inst) any = any || bi.state == FI_BATCH_RUNNING;
+        if (!any) break;
+    }
+    return FI_OK;
+}
+
+int fi_batch_status(fi_batch_t b, fi_model_t m, int* state, int* iters, double norms_inf[4])
+{
+    if (!b || !m) return fail(FI_ERR_ARG, "null argument");
+    int q = find_inst(b, m);
+    if (q < 0) return fail(FI_ERR_ARG, "model is not in this batch");
+    const BatchInst& bi = b->inst[q];
+    if (state) *state = bi.state;
+    if (iters) *iters = bi.iters;
+    if (norms_inf)
+        for (int k = 0; k < 4; ++k) norms_inf[k] = bi.norms[k];
+    return FI_OK;
+}
+
+int fi_batch_set_option(fi_batch_t b, int key, int value)
+{
+    if (!b) return fail(FI_ERR_ARG, "null batch");
+    if (key == FI_BOPT_TILE_Y) {
+        if (value != 4 && value != 8) return fail(FI_ERR_ARG, "tile rows must be 4 or 8");
+        b->ty = value;
+    } else if (key == FI_BOPT_ZCHUNK) {
+        if (value < 1) return fail(FI_ERR_ARG, "z chunk must be >= 1");
+        b->kch = value;
+    } else if (key == FI_BOPT_AUTOTUNE) {
+        b->autotune = value ? 1 : 0;
+    } else if (key == FI_BOPT_GRAPH_ITERS) {
+        if (value < 0 || value > 64) return fail(FI_ERR_ARG, "graph iterations must be in 0..64");
+        b->graph_iters = value & ~1; // even, so a replay keeps the table parity
+    } else {
+        return fail(FI_ERR_ARG, "unknown batch option %d", key);
+    }
+    b->dirty = true;
+    return FI_OK;
+}
+
+int fi_batch_info(fi_batch_t b, int out[8])
+{
+    if (!b || !out) return fail(FI_ERR_ARG, "null argument");
+    int running = 0;
+    for (const BatchInst& bi : b->inst) running += bi.state == FI_BATCH_RUNNING;
+    out[0] = (int)b->inst.size();
+    out[1] = running;
+    out[2] = b->t.nitems[0] + b->t.nitems[1];
+    out[3] = b->ty;
+    out[4] = b->kch;
+    out[5] = b->graph_iters;
+    out[6] = b->tune_runs;
+    out[7] = b->rebuilds;
+    return FI_OK;
+}
+
+} // extern "C"

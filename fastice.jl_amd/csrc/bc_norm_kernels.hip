@@ -74,6 +74,32 @@ void launch_bc_dim(const Geo& g, const BcBatch& b, hipStream_t s)
     hipLaunchKernelGGL(k_bc_dim, grd, blk, 0, s, b, g.nd, g.n[0], g.n[1], g.n[2], g.sy, g.sz);
 }
 
+// Packed boundary conditions of many instances in one dimension: each op
+// covers both sides of its face plane (full padded tangential extent).
+__global__ __launch_bounds__(256) void k_bc_packed(const PackedBcOp* __restrict__ ops, const int* __restrict__ blk2op)
+{
+    const PackedBcOp& o = ops[blk2op[blockIdx.x]];
+    const long plane = (long)o.ext1 * o.ext2;
+    const long t = (long)(blockIdx.x - o.start) * 256 + threadIdx.x;
+    if (t >= 2 * plane) return;
+    const int s = (int)(t / plane);
+    const long rem = t - s * plane;
+    const int sides = o.op.kind >> 1;
+    if ((sides == 1 && s != 0) || (sides == 2 && s != 1)) return;
+    const long q1 = rem % o.ext1, q2 = rem / o.ext1;
+    const long base = q1 * o.st[o.e1] + (o.e2 >= 0 ? q2 * o.st[o.e2] : 0);
+    const int vert = (o.op.L >> o.d) & 1;
+    const int kind = o.op.kind & 1;
+    if (o.op.f32) apply_face(static_cast<float*>(o.op.ptr), kind, vert, s, o.nn, base, o.st[o.d], o.op.v);
+    else apply_face(static_cast<double*>(o.op.ptr), kind, vert, s, o.nn, base, o.st[o.d], o.op.v);
+}
+
+void launch_bc_packed(const PackedBcOp* ops, const int* blk2op, int nblocks, hipStream_t s)
+{
+    if (nblocks <= 0) return;
+    hipLaunchKernelGGL(k_bc_packed, dim3(nblocks), dim3(256), 0, s, ops, blk2op);
+}
+
 // Inf-norm over the interior.  |x| is compared through its bit pattern:
 // for non-negative doubles the unsigned order equals the numeric order and
 // every NaN sorts above +Inf, so a NaN anywhere propagates to the result and

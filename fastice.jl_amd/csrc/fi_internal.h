@@ -81,6 +81,21 @@ struct FusedGeom {
     int kch; // planes per workgroup
 };
 
+// one instance of a packed (batched) sweep: its kernel arguments at one
+// state parity and its fused interior region
+template <typename T>
+struct PackedInst {
+    StokesArgs<T> a;
+    FusedGeom fg;
+};
+
+// a box of pointwise work of one instance; start = its first workgroup
+struct PackedBox {
+    int inst;
+    int i0, i1, j0, j1, k0, k1;
+    int start;
+};
+
 struct BcOp {
     void* ptr;
     int L;    // location bits
@@ -117,6 +132,27 @@ void launch_eta_update(const Geo& g, const StokesArgs<T>& a, hipStream_t s);
 
 // boundary conditions, norms, fills (bc_norm_kernels.hip)
 void launch_bc_dim(const Geo& g, const BcBatch& b, hipStream_t s);
+
+// packed sweeps over many instances (batch.hip builds the tables)
+template <typename T>
+void launch_fused_packed(const PackedInst<T>* inst, const int4* items, int nitems, bool ib, int ty, int grid,
+                         hipStream_t s);
+template <typename T>
+void launch_point_packed(const PackedInst<T>* inst, const PackedBox* boxes, const int* blk2box, int nblocks, bool ib,
+                         int nd, bool sigma, hipStream_t s);
+
+// one boundary-condition face operation of one instance in one dimension,
+// with the geometry it needs; start = its first workgroup
+struct PackedBcOp {
+    BcOp op;
+    long st[3];  // element strides of x, y, z
+    int d;       // normal dimension
+    int nn;      // cells in the normal dimension
+    int e1, e2;  // tangential dimensions (e2 = -1 in 2D)
+    int ext1, ext2;
+    int start;
+};
+void launch_bc_packed(const PackedBcOp* ops, const int* blk2op, int nblocks, hipStream_t s);
 // Inf-norm of up to 4 fields over their interiors; out = 4 uint64 (|x| bit patterns, NaN largest)
 void launch_norms(const Geo& g, const double* const f[4], const int L[4], int nf, unsigned long long* out,
                   hipStream_t s);

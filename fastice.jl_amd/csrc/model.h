@@ -50,6 +50,7 @@ struct fi_model_s {
     unsigned long long* h_norm = nullptr;
     double* d_dem = nullptr;
     size_t dem_elems = 0;
+    fi_batch_s* batch = nullptr; // the batch this model is attached to (batch.hip)
 
     static bool is_sig(int id) { return id >= FI_F_P && id <= FI_F_TYZ; }
     static bool is_vel(int id) { return id >= FI_F_VX && id <= FI_F_VZ; }
@@ -76,11 +77,17 @@ struct fi_model_s {
 namespace fi {
 
 StokesArgs<double> make_args(fi_model_s* m, bool in_place);
+BcBatch stress_batch(fi_model_s* m, int d);
+BcBatch velocity_batch(fi_model_s* m, int d, bool with_eta);
 void run_bc(fi_model_s* m, int group);
 int prepare(fi_model_s* m);
 void iterate(fi_model_s* m, hipEvent_t* ev);
 int advance(fi_model_s* m, int niter);
 int residual_norms(fi_model_s* m, double out[4]);
+int residual_norms_enqueue(fi_model_s* m);
+void residual_norms_read(fi_model_s* m, double out[4]);
+void batch_touch(fi_model_s* m);  // the model's buffer parity changed outside its batch
+void batch_detach(fi_model_s* m); // remove the model from its batch (model destruction)
 double bits_to_double(unsigned long long b);
 BcBatch neumann_batch(double* p, int L, int d);
 void free_refine(fi_model_s* m);

@@ -287,7 +287,7 @@ __device__ __forceinline__ void sto(T* base, uint32_t bo, T v)
 }
 
 template <typename T, bool IB, int TY>
-__global__ __launch_bounds__(64 * TY) void k_fused3(StokesArgs<T> a, FusedGeom fg)
+__device__ __forceinline__ void fused3_tile(const StokesArgs<T>& a, const FusedGeom& fg, int bx, int by, int bz)
 {
     constexpr int TX = 64;
     __shared__ T sVx[TY][TX], sVy[TY][TX], sVz[TY][TX];
@@ -295,9 +295,9 @@ __global__ __launch_bounds__(64 * TY) void k_fused3(StokesArgs<T> a, FusedGeom f
     __shared__ T sS[2][6][TY][TX]; // P txx tyy txy txz tyz
 
     const int tx = threadIdx.x, ty = threadIdx.y;
-    const int i = fg.lo[0] - 1 + (int)blockIdx.x * (TX - 2) + tx;
-    const int j = fg.lo[1] - 1 + (int)blockIdx.y * (TY - 2) + ty;
-    const int ks = fg.lo[2] + (int)blockIdx.z * fg.kch;
+    const int i = fg.lo[0] - 1 + bx * (TX - 2) + tx;
+    const int j = fg.lo[1] - 1 + by * (TY - 2) + ty;
+    const int ks = fg.lo[2] + bz * fg.kch;
     const int ke = min(ks + fg.kch - 1, fg.hi[2]);
     const bool inner = tx >= 1 && tx <= TX - 2 && ty >= 1 && ty <= TY - 2 && i <= fg.hi[0] && j <= fg.hi[1];
     const bool can_c = tx < TX - 1 && ty < TY - 1;
@@ -515,7 +515,7 @@ __global__ __launch_bounds__(64 * TY) void k_fused3(StokesArgs<T> a, FusedGeom f
 // are the operands the pointwise kernels form, so results are identical).
 // Raw off-diagonal stress is kept beside it for tau_II when eta is updated.
 template <typename T, int TY>
-__global__ __launch_bounds__(64 * TY) void k_fused3_ib(StokesArgs<T> a, FusedGeom fg)
+__device__ __forceinline__ void fused3_ib_tile(const StokesArgs<T>& a, const FusedGeom& fg, int bx, int by, int bz)
 {
     constexpr int TX = 64;
     __shared__ T sVx[TY][TX], sVy[TY][TX], sVz[TY][TX];
@@ -523,9 +523,9 @@ __global__ __launch_bounds__(64 * TY) void k_fused3_ib(StokesArgs<T> a, FusedGeo
     __shared__ T sS[2][9][TY][TX]; // na-weighted P txx tyy txy txz tyz; raw txy txz tyz
 
     const int tx = threadIdx.x, ty = threadIdx.y;
-    const int i = fg.lo[0] - 1 + (int)blockIdx.x * (TX - 2) + tx;
-    const int j = fg.lo[1] - 1 + (int)blockIdx.y * (TY - 2) + ty;
-    const int ks = fg.lo[2] + (int)blockIdx.z * fg.kch;
+    const int i = fg.lo[0] - 1 + bx * (TX - 2) + tx;
+    const int j = fg.lo[1] - 1 + by * (TY - 2) + ty;
+    const int ks = fg.lo[2] + bz * fg.kch;
     const int ke = min(ks + fg.kch - 1, fg.hi[2]);
     const bool inner = tx >= 1 && tx <= TX - 2 && ty >= 1 && ty <= TY - 2 && i <= fg.hi[0] && j <= fg.hi[1];
     const bool can_c = tx < TX - 1 && ty < TY - 1;
@@ -735,6 +735,30 @@ __global__ __launch_bounds__(64 * TY) void k_fused3_ib(StokesArgs<T> a, FusedGeo
         gx = ngx;
         gy = ngy;
         gz = ngz;
+    }
+}
+
+template <typename T, bool IB, int TY>
+__global__ __launch_bounds__(64 * TY) void k_fused3(StokesArgs<T> a, FusedGeom fg)
+{
+    if constexpr (IB) fused3_ib_tile<T, TY>(a, fg, blockIdx.x, blockIdx.y, blockIdx.z);
+    else fused3_tile<T, false, TY>(a, fg, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// Packed fused sweep over many instances: a persistent grid walks the
+// (instance, tile) work list; each item is one workgroup tile of one
+// instance, swept exactly as the single-instance kernel sweeps it.
+template <typename T, bool IB, int TY>
+__global__ __launch_bounds__(64 * TY) void k_fused3_packed(const PackedInst<T>* __restrict__ inst,
+                                                           const int4* __restrict__ items, int nitems)
+{
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const int4 w = items[it];
+        const StokesArgs<T> a = inst[w.x].a;
+        const FusedGeom fg = inst[w.x].fg;
+        __syncthreads(); // the previous item's LDS reads are done
+        if constexpr (IB) fused3_ib_tile<T, TY>(a, fg, w.y, w.z, w.w);
+        else fused3_tile<T, false, TY>(a, fg, w.y, w.z, w.w);
     }
 }
 
@@ -972,9 +996,8 @@ void launch_ib_flags(const Geo& g, const double* const na[8], const double* cons
     const long c = (long)i + sy * j;
 
 template <typename T, bool IB>
-__global__ __launch_bounds__(256) void k_sigma2(StokesArgs<T> a)
+__device__ __forceinline__ void sigma_point2(const StokesArgs<T>& a, long c, long sy)
 {
-    IDX2
     const T ix = a.ix, iy = a.iy;
     const T *Vx = a.Vx, *Vy = a.Vy, *eta = a.eta;
     const T ec = eta[c];
@@ -1010,9 +1033,8 @@ __global__ __launch_bounds__(256) void k_sigma2(StokesArgs<T> a)
 }
 
 template <typename T, bool IB>
-__global__ __launch_bounds__(256) void k_velocity2(StokesArgs<T> a)
+__device__ __forceinline__ void velocity_point2(const StokesArgs<T>& a, long c, long sy)
 {
-    IDX2
     const T ix = a.ix, iy = a.iy;
     const T *P = a.Pn, *txx = a.txxn, *tyy = a.tyyn, *txy = a.txyn, *eta = a.eta;
     T gx = a.grav_field ? a.gx[c] : a.grav[0];
@@ -1040,6 +1062,48 @@ __global__ __launch_bounds__(256) void k_velocity2(StokesArgs<T> a)
     T p = txx[c], q = tyy[c];
     T l = lerp4(txy[c], txy[c + 1], txy[c + sy], txy[c + 1 + sy]);
     a.eta_next[c] = rheology(a, c, sqrt(T(0.5) * (p * p + q * q) + l * l));
+}
+
+template <typename T, bool IB>
+__global__ __launch_bounds__(256) void k_sigma2(StokesArgs<T> a)
+{
+    IDX2
+    sigma_point2<T, IB>(a, c, sy);
+}
+
+template <typename T, bool IB>
+__global__ __launch_bounds__(256) void k_velocity2(StokesArgs<T> a)
+{
+    IDX2
+    velocity_point2<T, IB>(a, c, sy);
+}
+
+// Packed pointwise sweeps over the boxes of many instances (3D shells, whole
+// 2D grids): blk2box maps a workgroup to its box, each box covers whole
+// workgroups of 256 points (i fastest).
+template <typename T, bool IB, int ND, bool SIGMA>
+__global__ __launch_bounds__(256) void k_point_packed(const PackedInst<T>* __restrict__ inst,
+                                                      const PackedBox* __restrict__ boxes,
+                                                      const int* __restrict__ blk2box)
+{
+    const PackedBox b = boxes[blk2box[blockIdx.x]];
+    const long nx = b.i1 - b.i0 + 1, ny = b.j1 - b.j0 + 1, nz = b.k1 - b.k0 + 1;
+    const long t = (long)(blockIdx.x - b.start) * 256 + threadIdx.x;
+    if (t >= nx * ny * nz) return;
+    const long i = b.i0 + t % nx;
+    const long r = t / nx;
+    const long j = b.j0 + r % ny;
+    const long k = b.k0 + r / ny;
+    const StokesArgs<T>& a = inst[b.inst].a;
+    if constexpr (ND == 3) {
+        const long sy = a.sy, sz = a.sz;
+        if constexpr (SIGMA) sigma_point3<T, IB>(a, i + sy * j + sz * k, sy, sz);
+        else velocity_point3<T, IB>(a, i + sy * j + sz * k, sy, sz);
+    } else {
+        const long sy = a.sy;
+        if constexpr (SIGMA) sigma_point2<T, IB>(a, i + sy * j, sy);
+        else velocity_point2<T, IB>(a, i + sy * j, sy);
+    }
 }
 
 template <typename T, bool IB>
@@ -1210,7 +1274,7 @@ void launch_fused(const Geo& g, const StokesArgs<T>& a, const FusedGeom& fg, boo
     const int nby = (fg.hi[1] - fg.lo[1] + 1 + TY - 3) / (TY - 2);
     const int nbz = (fg.hi[2] - fg.lo[2] + fg.kch) / fg.kch;
     dim3 grd(nbx, nby, nbz), blk(64, TY, 1);
-    if (ib) hipLaunchKernelGGL((k_fused3_ib<T, TY>), grd, blk, 0, s, a, fg);
+    if (ib) hipLaunchKernelGGL((k_fused3<T, true, TY>), grd, blk, 0, s, a, fg);
     else hipLaunchKernelGGL((k_fused3<T, false, TY>), grd, blk, 0, s, a, fg);
 }
 
@@ -1238,13 +1302,50 @@ void launch_tau(const Geo& g, const StokesArgs<T>& a, bool ib, hipStream_t s)
     }
 }
 
+template <typename T>
+void launch_fused_packed(const PackedInst<T>* inst, const int4* items, int nitems, bool ib, int ty, int grid,
+                         hipStream_t s)
+{
+    if (nitems <= 0) return;
+    dim3 grd(nitems < grid ? nitems : grid);
+    // tile heights: 4 or 8 rows in FP64, 8 or 16 in FP32 (LDS and VGPR budget)
+    constexpr int TA = sizeof(T) == 8 ? 4 : 8, TB = sizeof(T) == 8 ? 8 : 16;
+    if (ty == TA) {
+        if (ib) hipLaunchKernelGGL((k_fused3_packed<T, true, TA>), grd, dim3(64, TA), 0, s, inst, items, nitems);
+        else hipLaunchKernelGGL((k_fused3_packed<T, false, TA>), grd, dim3(64, TA), 0, s, inst, items, nitems);
+    } else {
+        if (ib) hipLaunchKernelGGL((k_fused3_packed<T, true, TB>), grd, dim3(64, TB), 0, s, inst, items, nitems);
+        else hipLaunchKernelGGL((k_fused3_packed<T, false, TB>), grd, dim3(64, TB), 0, s, inst, items, nitems);
+    }
+}
+
+template <typename T>
+void launch_point_packed(const PackedInst<T>* inst, const PackedBox* boxes, const int* blk2box, int nblocks, bool ib,
+                         int nd, bool sigma, hipStream_t s)
+{
+    if (nblocks <= 0) return;
+    dim3 grd(nblocks), blk(256);
+#define FI_PP(IB_, ND_, S_) hipLaunchKernelGGL((k_point_packed<T, IB_, ND_, S_>), grd, blk, 0, s, inst, boxes, blk2box)
+    if (nd == 3) {
+        if (ib) { if (sigma) FI_PP(true, 3, true); else FI_PP(true, 3, false); }
+        else { if (sigma) FI_PP(false, 3, true); else FI_PP(false, 3, false); }
+    } else {
+        if (ib) { if (sigma) FI_PP(true, 2, true); else FI_PP(true, 2, false); }
+        else { if (sigma) FI_PP(false, 2, true); else FI_PP(false, 2, false); }
+    }
+#undef FI_PP
+}
+
 #define INST(T)                                                                                               \
     template void launch_sigma_boxes<T>(const Geo&, const StokesArgs<T>&, const BoxList&, bool, hipStream_t);    \
     template void launch_velocity_boxes<T>(const Geo&, const StokesArgs<T>&, const BoxList&, bool, hipStream_t); \
     template void launch_fused<T>(const Geo&, const StokesArgs<T>&, const FusedGeom&, bool, hipStream_t);        \
     template void launch_residuals<T>(const Geo&, const StokesArgs<T>&, bool, hipStream_t);                      \
     template void launch_tau<T>(const Geo&, const StokesArgs<T>&, bool, hipStream_t);                      \
-    template void launch_eta_update<T>(const Geo&, const StokesArgs<T>&, hipStream_t);
+    template void launch_eta_update<T>(const Geo&, const StokesArgs<T>&, hipStream_t);                     \
+    template void launch_fused_packed<T>(const PackedInst<T>*, const int4*, int, bool, int, int, hipStream_t);  \
+    template void launch_point_packed<T>(const PackedInst<T>*, const PackedBox*, const int*, int, bool, int, bool, \
+                                         hipStream_t);
 INST(double)
 INST(float)
 

@@ -193,8 +193,9 @@ int fi_omega_from_psi(fi_model_t m, int phase);
  * every ncheck iterations, until its residual falls below inner_rtol times the
  * outer one (at most max_inner iterations), and is added to the FP64 state.
  * Nonlinear rheologies first run full FP32 PT sweeps (eta updated every
- * sweep) until the FP32 residual drops by 1e-6 or stalls, and relax the outer
- * viscosity update in log space (FI_OPT_REFINE_*).
+ * sweep) until the FP32 residual drops by FI_OPT_REFINE_PHASE1_DIGITS digits
+ * or stalls, and relax the outer viscosity update in log space
+ * (FI_OPT_REFINE_ETA_RATE).
  * counts_out = {outer steps, total FP32 iterations, FP32 iterations of the
  * nonlinear phase}. */
 int fi_refined_solve(fi_model_t m, int max_outer, int max_inner, int ncheck, double inner_rtol, double tol,
@@ -206,6 +207,45 @@ int fi_refined_time_inner(fi_model_t m, int n_iter, double out_ms[4]);
 
 /* free / total device memory in bytes (hipMemGetInfo) */
 int fi_mem_info(size_t* free_bytes, size_t* total_bytes);
+/* device bytes fi_model_create would allocate for desc (0 on a bad desc);
+ * admission control of the serving loop */
+size_t fi_model_bytes(const fi_model_desc* desc);
+
+/* Batched solves (no counterpart in the reference, whose models are solved
+ * one at a time by the driver loop tm_stokes_chmy.jl:96-108).  A batch holds
+ * FP64 models; every PT iteration of all running models is a few packed
+ * launches (fused interior tiles of every instance, their pointwise shells,
+ * boundary conditions per dimension) replayed as a HIP graph.  Each model
+ * keeps fi_solve's semantics exactly: its own iteration count, residual
+ * check every ncheck of its iterations, stop at the first check with all
+ * norms < tol or at max_iter; results are bitwise those of fi_solve on the
+ * model alone.  While attached, the model's work runs on the batch's stream.
+ * The batch does not own its models: fi_model_destroy detaches one. */
+typedef struct fi_batch_s* fi_batch_t;
+enum { FI_BATCH_RUNNING = 0, FI_BATCH_CONVERGED = 1, FI_BATCH_MAX_ITER = 2, FI_BATCH_FAILED = 3 };
+int fi_batch_create(fi_batch_t* out);
+int fi_batch_destroy(fi_batch_t b);
+/* attach a model and start its solve (state FI_BATCH_RUNNING) */
+int fi_batch_add(fi_batch_t b, fi_model_t m, int max_iter, int ncheck, double tol);
+int fi_batch_remove(fi_batch_t b, fi_model_t m);
+/* advance every running model to the next residual check of the nearest one,
+ * check the models due, and return how many finished in *n_finished */
+int fi_batch_step(fi_batch_t b, int* n_finished);
+/* step until no model is running */
+int fi_batch_solve(fi_batch_t b);
+/* state (FI_BATCH_*), iterations done and last residual norms of a model;
+ * a failed model (non-finite residual) reports FI_BATCH_FAILED */
+int fi_batch_status(fi_batch_t b, fi_model_t m, int* state, int* iters, double norms_inf[4]);
+/* FI_BOPT_TILE_Y: rows of the fused tile (FP64: 4 or 8), FI_BOPT_ZCHUNK: z
+ * planes per tile, FI_BOPT_AUTOTUNE 1/0: time the tile candidates on the
+ * first step of a new working set and keep the fastest (default 1),
+ * FI_BOPT_GRAPH_ITERS: iterations per graph replay (0: no graph).  Results do
+ * not depend on any of them. */
+enum { FI_BOPT_TILE_Y = 1, FI_BOPT_ZCHUNK = 2, FI_BOPT_AUTOTUNE = 3, FI_BOPT_GRAPH_ITERS = 4 };
+int fi_batch_set_option(fi_batch_t b, int key, int value);
+/* out = {attached, running, fused work items, tile rows, z planes, graph
+ * iterations, autotune runs, table rebuilds} */
+int fi_batch_info(fi_batch_t b, int out[8]);
 
 #ifdef __cplusplus
 }
