@@ -12,6 +12,14 @@ Workloads
       FP64, both to FP64 residual 1e-8) runs on the same glacier at 127x127x63.
   stokes3d_fp64_511x511x255: isoviscous free-slip box with manufactured
       forcing, plain FP64 iteration.
+  batch96_glacier: 96 seeded glaciers, each dimension uniform in 48..192,
+      Glen n = 3, solved in FP64 to residual 1e-8 as one batch (packed
+      launches, HIP graphs); value = solves/s; the same models are then reset
+      and a sample of them solved one at a time for the comparison.
+  serving_poisson40: Poisson arrivals at 40/s, half 2D manufactured
+      (64..511 squared), half 3D glaciers (32..127 squared x half), FP64 to
+      1e-8, continuously batched; value = completed solves/s with
+      p50/p95/p99 latency.
 
 value = T_eff = A_eff * K / t with A_eff = 23 * sizeof(T) * nx*ny*nz bytes per
 iteration (scripts_future_API/tm_stokes_mpi_wip.jl:180), inputs resident in
@@ -36,6 +44,8 @@ DEFAULT = "glacier_511x511x255"
 WORKLOADS = {
     "glacier_511x511x255": (511, 511, 255),
     "stokes3d_fp64_511x511x255": (511, 511, 255),
+    "batch96_glacier": None,
+    "serving_poisson40": None,
 }
 
 
@@ -49,6 +59,9 @@ def parse():
     p.add_argument("--no-solve", action="store_true", help="skip the converged-solve comparison")
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--solve-dims", default="255,255,127")
+    p.add_argument("--alone-sample", type=int, default=16,
+                   help="batch96_glacier: models solved one at a time for the comparison")
+    p.add_argument("--requests", type=int, default=200, help="serving_poisson40: number of requests")
     p.add_argument("--copy-calib", action="store_true",
                    help="run a 1 GiB device copy first (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     return p.parse_args()
@@ -131,6 +144,120 @@ def solve_comparison(pkg, dims):
             "rel_error_vs_fp64": {k: float("%.3e" % v) for k, v in errs.items()}}
 
 
+def _line(metric, value, unit, ms_per_step, steps, warmup, dtype, config, higher=True, scaling="weak"):
+    return {"metric": metric, "value": round(value, 4), "unit": unit, "n_gpus": 1, "steps": steps,
+            "warmup": warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": higher,
+            "scaling": scaling, "vs_baseline": None, "dtype": dtype, "data": "synthetic", "config": config}
+
+
+def _reset(model):
+    """Back to the glacier's initial state: zero stress and velocity, eta = 1."""
+    for name in ("P", "TXX", "TYY", "TZZ", "TXY", "TXZ", "TYZ", "VX", "VY", "VZ"):
+        model.field(name).set(0.0)
+    model.viscosity.eta.set(1.0)
+    model.viscosity.eta_next.set(1.0)
+    model.apply_bc("eta")
+
+
+def bench_batch(pkg, args):
+    """96 uneven glaciers solved as one batch vs one at a time."""
+    rng = np.random.default_rng(2024)
+    dims = [tuple(int(x) for x in rng.integers(48, 193, 3)) for _ in range(96)]
+    cells = sum(d[0] * d[1] * d[2] for d in dims)
+    log("setting up 96 glaciers, %.1f M cells in total" % (cells / 1e6))
+    t0 = time.perf_counter()
+    inst = []
+    for i, d in enumerate(dims):
+        m, cad = pkg.glacier_3d(d, seed=100 + i, glen=True)
+        inst.append((m, cad["ncheck"], cad["niter"]))
+    for m, _, _ in inst:
+        m.synchronize()
+    setup_s = time.perf_counter() - t0
+    log("setup %.1f s" % setup_s)
+    batch = pkg.Batch()
+    for m, nc, mi in inst:
+        batch.add(m, mi, nc, 1e-8)
+    t0 = time.perf_counter()
+    last = t0
+    while batch.info()["running"] > 0 or batch.info()["rebuilds"] == 0:
+        batch.step()
+        if time.perf_counter() - last > 10.0:
+            last = time.perf_counter()
+            log("batch: %.1f s, %d running" % (last - t0, batch.info()["running"]))
+    t_batch = time.perf_counter() - t0
+    st = [batch.status(m) for m, _, _ in inst]
+    info = batch.info()
+    batch.close()
+    iters = [s[1] for s in st]
+    log("batch: %.2f s, %.2f solves/s, info %s" % (t_batch, 96 / t_batch, info))
+    # one at a time on a sample of the same models, reset to their initial state
+    sample = list(range(0, 96, max(1, 96 // max(1, args.alone_sample))))[:args.alone_sample]
+    for q in sample:
+        _reset(inst[q][0])
+    inst[0][0].synchronize()
+    t0 = time.perf_counter()
+    same = True
+    for q in sample:
+        m, nc, mi = inst[q]
+        it, _ = m.solve(mi, nc, 1e-8)
+        same = same and it == iters[q]
+        log("alone: model %d %s, %d iterations, %.1f s so far" % (q, dims[q], it, time.perf_counter() - t0))
+    t_alone = time.perf_counter() - t0
+    alone_rate = len(sample) / t_alone
+    sample_batch_iters = sum(iters[q] for q in sample)
+    log("alone: %d models in %.2f s, %.2f solves/s" % (len(sample), t_alone, alone_rate))
+    for m, _, _ in inst:
+        m.close()
+    states = {k: sum(s[0] == k for s in st) for k in ("converged", "max_iter", "failed")}
+    config = {"workload": "batch96_glacier",
+              "instances": 96, "dims_range": [48, 192], "total_cells": cells,
+              "description": "seeded synthetic glaciers (seeds 100..195), dims uniform in 48..192 per axis, Glen n=3 "
+                             "(A=1, eta_reg=30, log relaxation 0.05), FP64 PT to residual 1e-8 or the driver's "
+                             "niter = 50n, ncheck = 5n (test_vavilov.jl)",
+              "batch_seconds": round(t_batch, 3), "states": states,
+              "iterations": {"min": int(min(iters)), "max": int(max(iters)), "sum": int(sum(iters))},
+              "one_at_a_time": {"models": len(sample), "seconds": round(t_alone, 3),
+                                "solves_per_s": round(alone_rate, 4), "iterations": int(sample_batch_iters),
+                                "same_iterations_as_batch": bool(same)},
+              "speedup_vs_one_at_a_time": round((96 / t_batch) / alone_rate, 3),
+              "batch_info": info, "setup_s": round(setup_s, 2)}
+    line = _line("batched solves/s (96 glaciers 48-192 per axis, FP64, tol 1e-8 / niter 50n)", 96 / t_batch, "solves/s",
+                 t_batch * 1e3, 1, 0, "f64", config)
+    line["roofline"] = None
+    line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+
+
+def bench_serving(pkg, args):
+    """Continuous-batching service under Poisson arrivals at 40/s."""
+    reqs = pkg.poisson_requests(args.requests, 40.0, seed=7)
+    log("serving %d requests (%d 3D), arrivals over %.1f s" % (len(reqs), sum(r.ndims == 3 for r in reqs),
+                                                                reqs[-1].t_arrival))
+    srv = pkg.Server(pkg, max_slots=96)
+    try:
+        s = srv.run(reqs, realtime=True, log=log)
+        recs = list(srv.results.values())
+    finally:
+        srv.close()
+    lat3 = [r["latency_s"] for r in recs if r["ndims"] == 3]
+    lat2 = [r["latency_s"] for r in recs if r["ndims"] == 2]
+    config = {"workload": "serving_poisson40", "requests": len(reqs), "arrival_rate_per_s": 40.0,
+              "offered_window_s": round(reqs[-1].t_arrival, 3),
+              "mix": "p=0.5 3D glacier (n log-uniform 32..127, n x n x ceil(n/2), Glen n=3, niter 50n, ncheck 5n) / "
+                     "2D manufactured free slip (n log-uniform 64..511, n x n, ncheck n, cap 200n); FP64 to "
+                     "residual 1e-8",
+              "latency_s": {k: (round(v, 4) if v is not None else None) for k, v in s["latency_s"].items()},
+              "latency_s_3d": {k: round(v, 4) for k, v in pkg.percentiles(lat3).items()} if lat3 else None,
+              "latency_s_2d": {k: round(v, 4) for k, v in pkg.percentiles(lat2).items()} if lat2 else None,
+              "makespan_s": round(s["makespan_s"], 3), "converged": s["converged"],
+              "batch_steps": s["batch_steps"], "setup_s_mean": round(s["setup_s_mean"], 4)}
+    line = _line("serving solves/s (p99 latency in config), Poisson 40/s", s["solves_per_s"], "solves/s",
+                 s["makespan_s"] * 1e3, 1, 0, "f64", config)
+    line["roofline"] = None
+    line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -147,6 +274,10 @@ def main():
         del src, dst
     pkg = ge.load_package()
     A = pkg._abi
+    if args.workload == "batch96_glacier":
+        return bench_batch(pkg, args)
+    if args.workload == "serving_poisson40":
+        return bench_serving(pkg, args)
     dims = WORKLOADS[args.workload]
     cells = dims[0] * dims[1] * dims[2]
     glacier = args.workload.startswith("glacier")

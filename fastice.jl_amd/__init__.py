@@ -21,7 +21,8 @@ from .geometries import (AABB, GaussianGlacier, SyntheticElevation, center, dila
 from .grids import Center, UniformGrid, Vertex, flip
 from .level_sets import compute_levelset_from_dem, invert_levelset, omega_from_psi
 from .physics import LinearViscousRheology, PowerLawRheology, Relaxation, ViscoplasticRegularisation
-from .problems import glacier_3d, manufactured_2d, manufactured_3d, pt_params
+from .problems import glacier_3d, glacier_3d_spec, manufactured_2d, manufactured_3d, pt_params
+from .serving import Request, Server, percentiles, poisson_requests
 
 
 def mem_info():

@@ -53,9 +53,9 @@ int field_loc(int nd, int id)
 
 bool field_used(const fi_model_desc& d, int id)
 {
-    if (id == FI_F_TZZ || id == FI_F_TXZ || id == FI_F_TYZ || id == FI_F_VZ || id == FI_F_RVZ || id == FI_F_GZ)
-        return d.ndims == 3;
+    if (id == FI_F_TZZ || id == FI_F_TXZ || id == FI_F_TYZ || id == FI_F_VZ || id == FI_F_RVZ) return d.ndims == 3;
     if (id == FI_F_GX || id == FI_F_GY) return d.gravity_is_field != 0;
+    if (id == FI_F_GZ) return d.ndims == 3 && d.gravity_is_field != 0;
     if (id == FI_F_RHEO)
         return d.rheology.kind == FI_RHEO_LINEAR_FIELD || (d.rheology.kind == FI_RHEO_POWERLAW && d.rheology.A_is_field);
     if (id == FI_F_PSI_NA || id == FI_F_PSI_NS) return d.immersed_boundary != 0;

@@ -158,19 +158,11 @@ def manufactured_errors(grid, V, tau):
             n(vm - v, 2) / n(vm, 2), n(tm - t, 2) / n(tm, 2))
 
 
-def glacier_3d(dims, seed=1, glen=True, eta_reg=30.0, relax_rate=0.05, lx=2.0, ly=2.0, H=1.0):
-    """Seeded synthetic glacier on [-lx/2, lx/2] x [-ly/2, ly/2] x [0, H].
-
-    Set-up order of scripts_future_API/test_vavilov.jl:244-295: level sets of
-    the ice surface (na) and of the bed (ns, inverted), volume fractions,
-    Traction(0) on the box, gravity (0, 0, 1), PT parameters r = 0.9,
-    re = 5 pi.  Rheology: Glen n = 3 (PowerLawRheology(1, 3)) capped by
-    ViscoplasticRegularisation(eta_reg) and relaxed in log space, or linear
-    eta = 1 when glen is False.  Returns (model, cadence).
-    """
-    from .full_stokes import IsothermalImmersedBoundaryFullStokesModel, TBC
+def glacier_3d_spec(dims, seed=1, glen=True, eta_reg=30.0, relax_rate=0.05, lx=2.0, ly=2.0, H=1.0):
+    """Model arguments, driver cadence and geometry of the seeded synthetic
+    glacier (see glacier_3d), without allocating anything on the device."""
+    from .full_stokes import TBC
     from .geometries import GaussianGlacier
-    from .level_sets import compute_levelset_from_dem, invert_levelset, omega_from_psi
     from .physics import PowerLawRheology, Relaxation, ViscoplasticRegularisation
 
     dims = tuple(dims)
@@ -183,10 +175,29 @@ def glacier_3d(dims, seed=1, glen=True, eta_reg=30.0, relax_rate=0.05, lx=2.0, l
                               scale="log")
     else:
         rheology = LinearViscousRheology(1.0)
-    model = IsothermalImmersedBoundaryFullStokesModel(
-        grid=grid, boundary_conditions=dict(x=(tr, tr), y=(tr, tr), z=(tr, tr)),
-        gravity=dict(x=0.0, y=0.0, z=1.0), rheology=rheology,
-        solver_params=pt_params(grid, r=0.9, re_mech=5 * PI, vdtau_div=1.1))
+    kw = dict(grid=grid, boundary_conditions=dict(x=(tr, tr), y=(tr, tr), z=(tr, tr)),
+              gravity=dict(x=0.0, y=0.0, z=1.0), rheology=rheology,
+              solver_params=pt_params(grid, r=0.9, re_mech=5 * PI, vdtau_div=1.1))
+    n = max(dims)
+    return kw, dict(niter=50 * n, ncheck=5 * n, dem_grid=dgrid, glacier=gl)
+
+
+def glacier_3d(dims, seed=1, glen=True, eta_reg=30.0, relax_rate=0.05, lx=2.0, ly=2.0, H=1.0):
+    """Seeded synthetic glacier on [-lx/2, lx/2] x [-ly/2, ly/2] x [0, H].
+
+    Set-up order of scripts_future_API/test_vavilov.jl:244-295: level sets of
+    the ice surface (na) and of the bed (ns, inverted), volume fractions,
+    Traction(0) on the box, gravity (0, 0, 1), PT parameters r = 0.9,
+    re = 5 pi.  Rheology: Glen n = 3 (PowerLawRheology(1, 3)) capped by
+    ViscoplasticRegularisation(eta_reg) and relaxed in log space, or linear
+    eta = 1 when glen is False.  Returns (model, cadence).
+    """
+    from .full_stokes import IsothermalImmersedBoundaryFullStokesModel
+    from .level_sets import compute_levelset_from_dem, invert_levelset, omega_from_psi
+
+    kw, cad = glacier_3d_spec(dims, seed, glen, eta_reg, relax_rate, lx, ly, H)
+    gl, dgrid, grid = cad["glacier"], cad["dem_grid"], kw["grid"]
+    model = IsothermalImmersedBoundaryFullStokesModel(**kw)
     compute_levelset_from_dem(model.level_sets.na, gl.surface, dgrid, grid)
     compute_levelset_from_dem(model.level_sets.ns, gl.bed, dgrid, grid)
     invert_levelset(model.level_sets.ns)
@@ -195,5 +206,4 @@ def glacier_3d(dims, seed=1, glen=True, eta_reg=30.0, relax_rate=0.05, lx=2.0, l
     model.viscosity.eta.set(1.0)
     model.viscosity.eta_next.set(1.0)
     model.apply_bc("eta")
-    n = max(dims)
-    return model, dict(niter=50 * n, ncheck=5 * n, dem_grid=dgrid, glacier=gl)
+    return model, cad

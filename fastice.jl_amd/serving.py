@@ -1,0 +1,191 @@
+"""Serving loop: solve requests as they arrive, continuously batched.
+
+No counterpart in the reference (its drivers solve one model per run,
+scripts_future_API/tm_stokes_chmy.jl:96-108).  Requests arrive as a Poisson
+process; each is a 2D manufactured free-slip problem
+(stokes_manufactured_solution_free_slip_2D.jl) or a 3D seeded synthetic
+glacier (test_vavilov.jl set-up) solved in FP64 to a residual tolerance with
+the reference's check cadence.  The server
+
+  * admits a waiting request when a batch slot is free and the device has the
+    model's bytes (fi_model_bytes) plus a reserve free (hipMemGetInfo),
+  * builds its model (fields, level sets, volume fractions) and attaches it to
+    the running Batch, which advances all running models together,
+  * records its completion at the batch step in which it converges, then frees
+    its model (its HBM is available to the next admission).
+
+A model's result does not depend on what else is in the batch: every model
+ends bitwise equal to its solve alone.
+"""
+import math
+import time
+
+import numpy as np
+
+from .batch import Batch, model_bytes
+from .problems import glacier_3d, glacier_3d_spec, manufactured_2d
+
+
+class Request:
+    __slots__ = ("rid", "t_arrival", "ndims", "dims", "seed")
+
+    def __init__(self, rid, t_arrival, ndims, dims, seed):
+        self.rid, self.t_arrival, self.ndims, self.dims, self.seed = rid, t_arrival, ndims, tuple(dims), seed
+
+    def __repr__(self):
+        return "Request(%d, t=%.3f, %s)" % (self.rid, self.t_arrival, "x".join(map(str, self.dims)))
+
+
+def poisson_requests(n, rate, seed=0, p3d=0.5, range2d=(64, 511), range3d=(32, 127)):
+    """n requests with exponential inter-arrival times (rate per second); 3D
+    with probability p3d; sizes log-uniform in range2d (n x n) or range3d
+    (n x n x ceil(n/2))."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, n))
+    out = []
+    for i in range(n):
+        three = rng.random() < p3d
+        lo, hi = range3d if three else range2d
+        s = int(round(math.exp(rng.uniform(math.log(lo), math.log(hi)))))
+        dims = (s, s, (s + 1) // 2) if three else (s, s)
+        out.append(Request(i, float(t[i]), 3 if three else 2, dims, int(rng.integers(1, 2 ** 31))))
+    return out
+
+
+def request_spec(req):
+    """(model arguments, immersed boundary, ncheck, max_iter) of a request:
+    the reference drivers' cadences, glacier niter = 50 n and ncheck = 5 n
+    (test_vavilov.jl), manufactured ncheck = n with a 200 n cap."""
+    if req.ndims == 3:
+        kw, cad = glacier_3d_spec(req.dims, seed=req.seed, glen=True)
+        return kw, True, cad["ncheck"], cad["niter"]
+    kw, cad = manufactured_2d(req.dims)
+    return kw, False, cad["ncheck"], 200 * max(req.dims)
+
+
+def build_model(pkg, req):
+    """The request's model with its initial state (on its own stream)."""
+    if req.ndims == 3:
+        m, _ = glacier_3d(req.dims, seed=req.seed, glen=True)
+        return m
+    kw, _ = manufactured_2d(req.dims)
+    m = pkg.IsothermalFullStokesModel(**kw)
+    m.viscosity.eta.set(1.0)
+    m.viscosity.eta_next.set(1.0)
+    return m
+
+
+def percentiles(x, qs=(50, 95, 99)):
+    x = np.asarray(x, dtype=float)
+    if x.size == 0:
+        return {("p%d" % q): None for q in qs}
+    return {("p%d" % q): float(np.percentile(x, q)) for q in qs}
+
+
+class Server:
+    """Continuous-batching solver service over one device.
+
+    max_slots: models attached to the batch at once; hbm_reserve: bytes kept
+    free beyond the admitted models; tol: residual tolerance of every solve;
+    keep_models: keep finished models open (tests read their fields).
+    """
+
+    def __init__(self, pkg, max_slots=64, hbm_reserve=4 << 30, tol=1e-8, keep_models=False, batch_options=None):
+        self.pkg = pkg
+        self.max_slots = max_slots
+        self.hbm_reserve = hbm_reserve
+        self.tol = tol
+        self.keep_models = keep_models
+        self.batch = Batch(**(batch_options or {}))
+        self.models = {}
+        self.results = {}
+
+    def _fits(self, req):
+        kw, ib, _, _ = request_spec(req)
+        need = model_bytes(immersed_boundary=ib, **kw)
+        free, _ = self.pkg.mem_info()
+        return free >= need + self.hbm_reserve
+
+    def _admit(self, req, t0):
+        ta = time.perf_counter() - t0
+        _, _, ncheck, max_iter = request_spec(req)
+        m = build_model(self.pkg, req)
+        self.batch.add(m, max_iter, ncheck, self.tol)
+        self.models[req.rid] = (req, m)
+        self.results[req.rid] = dict(rid=req.rid, ndims=req.ndims, dims=list(req.dims), t_arrival=req.t_arrival,
+                                     t_admit=ta, setup_s=time.perf_counter() - t0 - ta)
+
+    def _collect(self, t0):
+        done = []
+        for rid, (req, m) in self.models.items():
+            state, iters, norms = self.batch.status(m)
+            if state == "running":
+                continue
+            r = self.results[rid]
+            r.update(t_done=time.perf_counter() - t0, state=state, iterations=iters,
+                     residual=float(np.max(norms)))
+            r["latency_s"] = r["t_done"] - r["t_arrival"]
+            done.append(rid)
+        for rid in done:
+            req, m = self.models.pop(rid)
+            self.batch.remove(m)
+            if self.keep_models:
+                self.results[rid]["model"] = m
+            else:
+                m.close()
+        return len(done)
+
+    def run(self, requests, realtime=True, log=None):
+        """Serve requests (sorted by arrival).  realtime: requests become
+        eligible at their arrival time; otherwise one becomes eligible per
+        batch step (a deterministic interleaving for tests).  Returns the
+        summary; per-request records are in self.results."""
+        reqs = sorted(requests, key=lambda r: r.t_arrival)
+        t0 = time.perf_counter()
+        nxt = 0
+        waiting = []
+        steps = 0
+        while True:
+            now = time.perf_counter() - t0
+            if realtime:
+                while nxt < len(reqs) and reqs[nxt].t_arrival <= now:
+                    waiting.append(reqs[nxt])
+                    nxt += 1
+            elif nxt < len(reqs):
+                r = reqs[nxt]
+                r.t_arrival = now
+                waiting.append(r)
+                nxt += 1
+            # admission: free slot and enough free HBM, first come first served
+            while waiting and len(self.models) < self.max_slots and self._fits(waiting[0]):
+                self._admit(waiting.pop(0), t0)
+            if self.models:
+                self.batch.step()
+                steps += 1
+                n = self._collect(t0)
+                if log and n:
+                    log("t=%.2f s done %d/%d running %d waiting %d" % (time.perf_counter() - t0, len(self.results) -
+                        len(self.models), len(reqs), len(self.models), len(waiting)))
+            elif waiting:
+                raise RuntimeError("request %r does not fit in device memory" % (waiting[0],))
+            elif nxt < len(reqs):
+                time.sleep(max(0.0, reqs[nxt].t_arrival - (time.perf_counter() - t0)))
+            else:
+                break
+        makespan = time.perf_counter() - t0
+        recs = [r for r in self.results.values()]
+        lat = [r["latency_s"] for r in recs]
+        return dict(requests=len(recs), makespan_s=makespan, solves_per_s=len(recs) / makespan if makespan else 0.0,
+                    latency_s=percentiles(lat), batch_steps=steps,
+                    converged=sum(r["state"] == "converged" for r in recs),
+                    setup_s_mean=float(np.mean([r["setup_s"] for r in recs])) if recs else 0.0)
+
+    def close(self):
+        for _, m in self.models.values():
+            m.close()
+        self.models = {}
+        if self.keep_models:
+            for r in self.results.values():
+                if "model" in r:
+                    r["model"].close()
+        self.batch.close()
