@@ -13,6 +13,33 @@
  * index of the 1-based Chmy index (i,j,k) is i + sy*j + sz*k with
  * sy = nx+3, sz = (nx+3)*(ny+3); index 0 is the low halo, 1..m the
  * interior (m = n for Center, n+1 for Vertex), m+1 the high halo.
+ *
+ * Parity status of each function (tests/test_oracle_pins.py; the reference
+ * ships no golden vectors for this path and its Julia/Chmy code cannot run
+ * here, so every pin is a closed form, an invariant or brute force):
+ *   or_init_grid            grid KATs of test/test_chmy_grids.jl:113-122
+ *   or_update_sigma/_V,     exact divergence/gradient on linear fields,
+ *   or_advance, or_solve    fixed-point invariance, linearity in the forcing,
+ *                           O(h^2) convergence to the 2D and 3D manufactured
+ *                           solutions (127^2 unit-box case included)
+ *   or_compute_tau          closed form on a linear velocity field
+ *   or_bc_*                 free-slip / no-slip / traction invariants after
+ *                           every iteration
+ *   or_compute_residuals,   residual of the manufactured solutions; NaN
+ *   or_residual_norms       propagation of the Inf-norm
+ *   or_rheology             scalar closed forms of Physics.jl:7-47
+ *   or_psi2omega3/2,        exact fractions of planar level sets, complement
+ *   or_omega_from_psi       rule, O(h^2) sphere volume
+ *   or_levelset_from_dem,   flat and inclined DEMs exactly, brute-force
+ *   or_invert_levelset      distance over every DEM triangle
+ * Parity unpinned (choices this library specifies where the reference
+ * delegates to unvendored Chmy.jl and no check can reach the reference's own
+ * bits): the ghost values at edges and corners that the x-then-y-then-z
+ * order of the boundary conditions produces, the masked derivative
+ * (w_r f_r - w_l f_l) / dx of the immersed-boundary kernels and the validity
+ * bits derived from it, zero halos of the volume-fraction masks, the Inf-norm
+ * over interiors only, and the bitwise trajectory of the KernelAbstractions
+ * CPU path (difference times inverse spacing, operation order).
  */
 #ifndef FASTICE_ORACLE_H
 #define FASTICE_ORACLE_H
