@@ -25,7 +25,8 @@ value = T_eff = A_eff * K / t with A_eff = 23 * sizeof(T) * nx*ny*nz bytes per
 iteration (scripts_future_API/tm_stokes_mpi_wip.jl:180), inputs resident in
 HBM.  roofline: the dominant kernel's algorithmic bytes per launch (each
 field it must read or write, once, over its cells; masks not credited) over
-its HIP-event duration on the model's stream.  cpu_baseline: the CPU oracle
+its HIP-event duration on the model's stream; traffic: its measured HBM bytes
+per launch from the rocprofv3 summary in profiles/.  cpu_baseline: the CPU oracle
 (OpenMP) on a bounded number of FP64 iterations of the same grid.
 """
 import argparse
@@ -65,6 +66,21 @@ def parse():
     p.add_argument("--copy-calib", action="store_true",
                    help="run a 1 GiB device copy first (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     return p.parse_args()
+
+
+def profile_traffic(workload, role):
+    """HBM bytes per launch of a kernel as measured by rocprofv3 (FETCH_SIZE x 2 +
+    WRITE_SIZE, calibrated on a device copy) and recorded in profiles/*.md."""
+    import glob
+    here = os.path.dirname(os.path.abspath(__file__))
+    for path in sorted(glob.glob(os.path.join(here, "profiles", "*.md"))):
+        with open(path) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 4 and parts[0] == "traffic_bytes_per_launch" and parts[1] == workload \
+                        and parts[2] == role:
+                    return int(parts[3]), os.path.relpath(path, here)
+    return None, None
 
 
 def log(msg):
@@ -123,7 +139,7 @@ def solve_comparison(pkg, dims):
     t64 = time.perf_counter() - t0
     a.synchronize()
     t0 = time.perf_counter()
-    outer, inner, na = a.refined_solve(1e-8, max_outer=300)
+    outer, inner, na = a.refined_solve(1e-8, max_outer=300, ncheck=cad["ncheck"] // 2)
     t32 = time.perf_counter() - t0
     # per-field difference relative to the FP64 solution's stress / velocity scale
     names = ("P", "TXX", "TYY", "TZZ", "TXY", "TXZ", "TYZ", "VX", "VY", "VZ")
@@ -324,14 +340,17 @@ def main():
         value, ms_step, dtype = teff32, ms32, "f32 sweeps in f64 defect correction"
         # fused FP32 correction sweep: R P tau(6) V(3) eta fP g(3), W P tau(6) V(3)
         kname, kms, kbytes = "k_fused3<float, IB>", ev32[1], 25 * 4 * icells
+        role = "fp32_fused"
         events = {"fp32_iteration": ev32[0], "fp32_fused": ev32[1], "fp32_shell": ev32[2], "fp32_bc": ev32[3],
                   "fp64_iteration": ev64[0], "fp64_fused": ev64[1], "fp64_shell": ev64[2], "fp64_bc": ev64[3]}
     else:
         value, ms_step, dtype = teff64, ms64, "f64"
         # fused FP64 sweep: R P tau(6) V(3) eta [g(3)], W P tau(6) V(3) eta_next
         kname, kms, kbytes = "k_fused3<double>", ev64[1], (22 + gfields) * 8 * icells
+        role = "fp64_fused"
         events = {"fp64_iteration": ev64[0], "fp64_fused": ev64[1], "fp64_shell": ev64[2], "fp64_bc": ev64[3]}
     achieved = kbytes / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
+    traffic, tsrc = profile_traffic(args.workload, role)
 
     solve = None
     if glacier and not args.no_solve:
@@ -371,7 +390,8 @@ def main():
         "data": "synthetic",
         "config": config,
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_source": tsrc,
                      "kernel_ms": round(float(kms), 4), "algorithmic_bytes_per_launch": int(kbytes),
                      "event_ms": {k: round(float(v), 4) for k, v in events.items()}},
         "cpu_baseline": cpu,

@@ -27,8 +27,9 @@ FI_LAYOUT_PARENT, FI_LAYOUT_INTERIOR = 0, 1
 FI_BCGROUP_STRESS, FI_BCGROUP_VELOCITY, FI_BCGROUP_ETA, FI_BCGROUP_RESIDUAL = 0, 1, 2, 3
 FI_PHASE_NA, FI_PHASE_NS = 0, 1
 FI_OPT_FUSED, FI_OPT_ZCHUNK, FI_OPT_REFINE_FULL_PHASE, FI_OPT_REFINE_ETA_RATE, FI_OPT_REFINE_PHASE1_DIGITS = 1, 2, 3, 4, 5
+FI_OPT_FUSED_VARIANT, FI_OPT_FUSED_VARIANT_FP32, FI_OPT_TILE_ORDER = 6, 7, 8
 FI_BATCH_RUNNING, FI_BATCH_CONVERGED, FI_BATCH_MAX_ITER, FI_BATCH_FAILED = 0, 1, 2, 3
-FI_BOPT_TILE_Y, FI_BOPT_ZCHUNK, FI_BOPT_AUTOTUNE, FI_BOPT_GRAPH_ITERS = 1, 2, 3, 4
+FI_BOPT_VARIANT, FI_BOPT_ZCHUNK, FI_BOPT_AUTOTUNE, FI_BOPT_GRAPH_ITERS, FI_BOPT_TILE_ORDER = 1, 2, 3, 4, 5
 
 FIELD_NAMES = [
     "P", "TXX", "TYY", "TZZ", "TXY", "TXZ", "TYZ", "VX", "VY", "VZ", "ETA", "ETA_NEXT",
@@ -148,6 +149,7 @@ def _declare(lib):
         "fi_refined_advance_inner": (C.c_int, [H, C.c_int]),
         "fi_refined_time_inner": (C.c_int, [H, C.c_int, _DP]),
         "fi_model_bytes": (C.c_size_t, [C.POINTER(fi_model_desc)]),
+        "fi_fused_variants": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.c_int]),
         "fi_batch_create": (C.c_int, [C.POINTER(C.c_void_p)]),
         "fi_batch_destroy": (C.c_int, [C.c_void_p]),
         "fi_batch_add": (C.c_int, [C.c_void_p, H, C.c_int, C.c_int, C.c_double]),
@@ -186,6 +188,14 @@ def check(rc):
     if rc == FI_ERR_NONFINITE:
         raise NonFiniteError(rc, msg)
     raise FastIceError(rc, msg)
+
+
+def fused_variants(fp32=False):
+    """[(tile lanes in x, tile rows, min waves per SIMD)] of the compiled
+    fused-sweep variants."""
+    buf = (C.c_int * 48)()
+    n = lib().fi_fused_variants(1 if fp32 else 0, buf, 16)
+    return [(buf[3 * q], buf[3 * q + 1], buf[3 * q + 2]) for q in range(n)]
 
 
 def dptr(a):

@@ -35,18 +35,20 @@ class Batch:
     status(model) gives (state, iterations, Inf-norms).
     """
 
-    def __init__(self, autotune=True, tile_y=None, zchunk=None, graph_iters=None):
+    def __init__(self, autotune=True, variant=None, zchunk=None, graph_iters=None, order=None):
         h = C.c_void_p()
         A.check(A.lib().fi_batch_create(C.byref(h)))
         self._h = h
         self._models = {}
         self.set_option(A.FI_BOPT_AUTOTUNE, 1 if autotune else 0)
-        if tile_y is not None:
-            self.set_option(A.FI_BOPT_TILE_Y, tile_y)
+        if variant is not None:
+            self.set_option(A.FI_BOPT_VARIANT, variant)
         if zchunk is not None:
             self.set_option(A.FI_BOPT_ZCHUNK, zchunk)
         if graph_iters is not None:
             self.set_option(A.FI_BOPT_GRAPH_ITERS, graph_iters)
+        if order is not None:
+            self.set_option(A.FI_BOPT_TILE_ORDER, order)
 
     def set_option(self, key, value):
         A.check(A.lib().fi_batch_set_option(self._h, int(key), int(value)))
@@ -76,9 +78,12 @@ class Batch:
     def info(self):
         out = (C.c_int * 8)()
         A.check(A.lib().fi_batch_info(self._h, out))
-        keys = ("attached", "running", "fused_items", "tile_y", "zchunk", "graph_iters", "autotune_runs",
+        keys = ("attached", "running", "fused_items", "variant", "zchunk", "graph_iters", "autotune_runs",
                 "rebuilds")
-        return dict(zip(keys, list(out)))
+        d = dict(zip(keys, list(out)))
+        v = d["variant"]  # (tile lanes in x, tile rows, min waves per SIMD)
+        d["variant"] = (v // 16 // 32, v // 16 % 32, v % 16)
+        return d
 
     def close(self):
         h = getattr(self, "_h", None)

@@ -211,7 +211,11 @@ void iterate(fi_model_s* m, hipEvent_t* ev)
     const bool fused = m->use_fused && m->has_interior;
     const BoxList boxes = fused ? m->shell : full_box(m->g);
     if (ev) (void)hipEventRecord(ev[0], m->stream);
-    if (fused) launch_fused(m->g, a, m->fg, ib, m->stream);
+    if (fused) {
+        FusedGeom fg = m->fg;
+        fg.order = m->forder;
+        launch_fused(m->g, a, fg, ib, m->ftx, m->fty, m->fminw, m->stream);
+    }
     if (ev) (void)hipEventRecord(ev[1], m->stream);
     launch_sigma_boxes(m->g, a, boxes, ib, m->stream);
     m->sig_cur ^= 1;
@@ -333,8 +337,7 @@ int fi_abi_version(void) { return FI_ABI_VERSION; }
 
 const char* fi_last_error(void) { return g_err; }
 
-// device layout: rows padded to 128-byte lines, Chmy index 1 of every row
-// 128-byte aligned
+// device layout: padded rows, Chmy index 1 of every row 128-byte aligned
 static Geo make_geo(const fi_model_desc& desc)
 {
     Geo g;
@@ -350,7 +353,9 @@ static Geo make_geo(const fi_model_desc& desc)
             g.idx[d] = 1.0;
         }
     }
-    const long align = 16; // doubles per 128-byte line
+    // rows padded to 32 elements and 31 leading elements: Chmy index 1 of
+    // every row starts a 128-byte line in FP64 and in FP32 storage
+    const long align = 32;
     g.sy = ((long)(g.n[0] + 3) + align - 1) / align * align;
     g.sz = (g.nd == 3) ? g.sy * (g.n[1] + 3) : 0;
     g.off = align - 1;
@@ -641,7 +646,7 @@ int fi_set_option(fi_model_t m, int key, int value)
     if (key == FI_OPT_FUSED) {
         m->use_fused = value ? 1 : 0;
     } else if (key == FI_OPT_ZCHUNK) {
-        if (value < 1) return fail(FI_ERR_ARG, "z chunk must be >= 1");
+        if (value < 0) return fail(FI_ERR_ARG, "z chunk must be >= 1 (0: sized to the grid)");
         m->kch = value;
         if (m->g.nd == 3 && m->g.elems * sizeof(double) < (1ull << 32))
             m->has_interior = fused_split(m->g, 8, m->kch, m->fg, m->shell);
@@ -650,6 +655,23 @@ int fi_set_option(fi_model_t m, int key, int value)
     } else if (key == FI_OPT_REFINE_ETA_RATE) {
         if (value < 1 || value > 1000) return fail(FI_ERR_ARG, "viscosity relaxation rate must be in 1..1000 per mille");
         m->refine_eta_rate = value / 1000.0;
+    } else if (key == FI_OPT_FUSED_VARIANT || key == FI_OPT_FUSED_VARIANT_FP32) {
+        const bool f32 = key == FI_OPT_FUSED_VARIANT_FP32;
+        int cfg[16][3];
+        const int n = fused_configs(f32, cfg, 16);
+        if (value < 0 || value >= n) return fail(FI_ERR_ARG, "fused variant must be in 0..%d", n - 1);
+        if (f32) {
+            m->ftx32 = cfg[value][0];
+            m->fty32 = cfg[value][1];
+            m->fminw32 = cfg[value][2];
+        } else {
+            m->ftx = cfg[value][0];
+            m->fty = cfg[value][1];
+            m->fminw = cfg[value][2];
+        }
+    } else if (key == FI_OPT_TILE_ORDER) {
+        if (value < 0 || value > 2) return fail(FI_ERR_ARG, "tile order must be 0, 1 or 2");
+        m->forder = value;
     } else if (key == FI_OPT_REFINE_PHASE1_DIGITS) {
         if (value < 1 || value > 7) return fail(FI_ERR_ARG, "nonlinear phase digits must be in 1..7");
         m->refine_phase1_digits = value;
@@ -657,6 +679,16 @@ int fi_set_option(fi_model_t m, int key, int value)
         return fail(FI_ERR_ARG, "unknown option %d", key);
     }
     return FI_OK;
+}
+
+int fi_fused_variants(int fp32, int* tx_ty_minw, int max_variants)
+{
+    int cfg[16][3];
+    const int n = fused_configs(fp32 != 0, cfg, 16);
+    if (tx_ty_minw)
+        for (int q = 0; q < n && q < max_variants; ++q)
+            for (int k = 0; k < 3; ++k) tx_ty_minw[3 * q + k] = cfg[q][k];
+    return n;
 }
 
 int fi_synchronize(fi_model_t m)

@@ -60,7 +60,8 @@ struct fi_batch_s {
     int phase = 0; // iterations since the tables were built, mod 2
     BatchTables t;
     hipGraphExec_t graph[2] = {nullptr, nullptr};
-    int ty = 8, kch = 32;
+    int tx = 64, ty = 8, minw = 1, kch = 32; // fused variant and z chunk
+    int order = 2;                  // workgroup-to-tile order of the packed fused sweep
     int autotune = 1;
     int graph_iters = 8;
     long tuned_cells = 0; // size of the working set the tile choice was made for
@@ -200,6 +201,8 @@ static int build_tables(fi_batch_s* b, int ty, int kch)
             shell = full_box(m->g);
             std::memset(&fg, 0, sizeof(fg));
         }
+        fg.order = b->order;
+        if (interior) fused_xtiles(fg, b->tx, 8, m->g.off);
         for (int p = 0; p < 2; ++p) {
             ParityView pv(m, p);
             PackedInst<double> pi;
@@ -218,7 +221,7 @@ static int build_tables(fi_batch_s* b, int ty, int kch)
         }
         if (interior) {
             const int ib = m->desc.immersed_boundary ? 1 : 0;
-            const int nbx = (fg.hi[0] - fg.lo[0] + 1 + 64 - 3) / (64 - 2);
+            const int nbx = fg.nbx;
             const int nby = (fg.hi[1] - fg.lo[1] + 1 + ty - 3) / (ty - 2);
             const int nbz = (fg.hi[2] - fg.lo[2] + fg.kch) / fg.kch;
             for (int bz = 0; bz < nbz; ++bz)
@@ -302,7 +305,7 @@ static void launch_fused_all(fi_batch_s* b, int p)
 {
     const BatchTables& t = b->t;
     for (int q = 0; q < 2; ++q)
-        launch_fused_packed<double>(t.inst[p], t.items[q], t.nitems[q], q == 1, b->ty, fused_grid(t.nitems[q]),
+        launch_fused_packed<double>(t.inst[p], t.items[q], t.nitems[q], q == 1, b->tx, b->ty, b->minw, fused_grid(t.nitems[q]),
                                     b->stream);
 }
 
@@ -355,15 +358,21 @@ static int autotune(fi_batch_s* b, int budget, int* used)
 {
     *used = 0;
     struct Cand {
-        int ty, kch;
+        int tx, ty, minw, kch;
     };
-    const Cand cands[] = {{8, 32}, {8, 16}, {8, 64}, {4, 32}, {4, 64}};
-    const int nc = sizeof(cands) / sizeof(cands[0]);
+    std::vector<Cand> cands;
+    int cfg[16][3];
+    const int nv = fused_configs(false, cfg, 16);
+    for (int v = 0; v < nv; ++v)
+        for (int kch : {32, 64}) cands.push_back(Cand{cfg[v][0], cfg[v][1], cfg[v][2], kch});
+    const int nc = (int)cands.size();
     const int reps = 2; // one warm-up iteration, one timed
     if (budget < nc * reps) return FI_OK;
     int best = -1;
     float best_ms = 0.0f;
     for (int c = 0; c < nc; ++c) {
+        b->tx = cands[c].tx;
+        b->minw = cands[c].minw;
         int rc = build_tables(b, cands[c].ty, cands[c].kch);
         if (rc) return rc;
         float ms = 0.0f;
@@ -408,6 +417,8 @@ static int autotune(fi_batch_s* b, int budget, int* used)
     }
     b->tune_runs++;
     b->tuned_cells = running_cells(b);
+    b->tx = cands[best].tx;
+    b->minw = cands[best].minw;
     int rc = build_tables(b, cands[best].ty, cands[best].kch);
     if (rc) return rc;
     return capture_graphs(b);
@@ -614,9 +625,16 @@ int fi_batch_status(fi_batch_t b, fi_model_t m, int* state, int* iters, double n
 int fi_batch_set_option(fi_batch_t b, int key, int value)
 {
     if (!b) return fail(FI_ERR_ARG, "null batch");
-    if (key == FI_BOPT_TILE_Y) {
-        if (value != 4 && value != 8) return fail(FI_ERR_ARG, "tile rows must be 4 or 8");
-        b->ty = value;
+    if (key == FI_BOPT_VARIANT) {
+        int cfg[16][3];
+        const int n = fused_configs(false, cfg, 16);
+        if (value < 0 || value >= n) return fail(FI_ERR_ARG, "fused variant must be in 0..%d", n - 1);
+        b->tx = cfg[value][0];
+        b->ty = cfg[value][1];
+        b->minw = cfg[value][2];
+    } else if (key == FI_BOPT_TILE_ORDER) {
+        if (value < 0 || value > 2) return fail(FI_ERR_ARG, "tile order must be 0, 1 or 2");
+        b->order = value;
     } else if (key == FI_BOPT_ZCHUNK) {
         if (value < 1) return fail(FI_ERR_ARG, "z chunk must be >= 1");
         b->kch = value;
@@ -640,7 +658,7 @@ int fi_batch_info(fi_batch_t b, int out[8])
     out[0] = (int)b->inst.size();
     out[1] = running;
     out[2] = b->t.nitems[0] + b->t.nitems[1];
-    out[3] = b->ty;
+    out[3] = (b->tx * 32 + b->ty) * 16 + b->minw;
     out[4] = b->kch;
     out[5] = b->graph_iters;
     out[6] = b->tune_runs;

@@ -78,7 +78,11 @@ struct BoxList {
 // interior region of the fused sweep
 struct FusedGeom {
     int lo[3], hi[3];
-    int kch; // planes per workgroup
+    int kch;   // planes per workgroup
+    int order; // workgroup-to-tile order (0: x fastest, 1: contiguous runs per XCD)
+    int xa;    // first column of the line-aligned x tiles (see fused_xrange)
+    int nx0;   // 1: a leading tile covers [lo, xa-1]
+    int nbx;   // tiles in x
 };
 
 // one instance of a packed (batched) sweep: its kernel arguments at one
@@ -117,7 +121,14 @@ template <typename T>
 void launch_velocity_boxes(const Geo& g, const StokesArgs<T>& a, const BoxList& bl, bool ib, hipStream_t s);
 // fused stress + velocity sweep over the interior region (3D)
 template <typename T>
-void launch_fused(const Geo& g, const StokesArgs<T>& a, const FusedGeom& fg, bool ib, hipStream_t s);
+void launch_fused(const Geo& g, const StokesArgs<T>& a, const FusedGeom& fg, bool ib, int tx, int ty, int minw,
+                  hipStream_t s);
+// compiled (tile lanes in x, tile rows, min waves per SIMD) variants of the fused sweep
+int fused_configs(bool fp32, int cfg[][3], int max);
+// x tiling of a fused region for tiles of tx lanes (sets xa, nx0, nbx)
+void fused_xtiles(FusedGeom& fg, int tx, int esize, long off);
+// z planes per workgroup of the fused sweep when none is set (kch <= 0)
+int fused_auto_kch(const FusedGeom& fg, int tiles_per_plane);
 // interior region and the shell boxes around it; false when the interior is empty
 bool fused_split(const Geo& g, int ty, int kch, FusedGeom& fg, BoxList& shell);
 BoxList full_box(const Geo& g);
@@ -135,8 +146,8 @@ void launch_bc_dim(const Geo& g, const BcBatch& b, hipStream_t s);
 
 // packed sweeps over many instances (batch.hip builds the tables)
 template <typename T>
-void launch_fused_packed(const PackedInst<T>* inst, const int4* items, int nitems, bool ib, int ty, int grid,
-                         hipStream_t s);
+void launch_fused_packed(const PackedInst<T>* inst, const int4* items, int nitems, bool ib, int tx, int ty, int minw,
+                         int grid, hipStream_t s);
 template <typename T>
 void launch_point_packed(const PackedInst<T>* inst, const PackedBox* boxes, const int* blk2box, int nblocks, bool ib,
                          int nd, bool sigma, hipStream_t s);

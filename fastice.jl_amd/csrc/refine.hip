@@ -267,7 +267,11 @@ static void iterate32(fi_model_s* m, hipEvent_t* ev)
     const bool fused = m->use_fused && r->has_interior;
     const BoxList boxes = fused ? r->shell : full_box(m->g);
     if (ev) (void)hipEventRecord(ev[0], m->stream);
-    if (fused) launch_fused(m->g, a, r->fg, ib, m->stream);
+    if (fused) {
+        FusedGeom fg = r->fg;
+        fg.order = m->forder;
+        launch_fused(m->g, a, fg, ib, m->ftx32, m->fty32, m->fminw32, m->stream);
+    }
     if (ev) (void)hipEventRecord(ev[1], m->stream);
     launch_sigma_boxes(m->g, a, boxes, ib, m->stream);
     r->sig_cur ^= 1;

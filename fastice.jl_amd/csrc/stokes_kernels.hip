@@ -286,30 +286,51 @@ __device__ __forceinline__ void sto(T* base, uint32_t bo, T v)
     *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + bo) = v;
 }
 
-template <typename T, bool IB, int TY>
+// x placement of the fused tiles.  A tile of TX lanes sweeps TX-2 inner
+// columns with one halo column on each side.  With fg.nx0 = 1 the first tile
+// covers the columns [lo, xa-1] and every further tile starts at
+// xa + (bx-1)(TX-2); xa is chosen so that those tiles start on a 128-byte
+// line (whole-line stores, no line shared by two tiles' stores).
+__host__ __device__ __forceinline__ void fused_xrange(const FusedGeom& fg, int tx, int bx, int& xs, int& xe)
+{
+    if (bx < fg.nx0) {
+        xs = fg.lo[0];
+        xe = fg.xa - 1;
+    } else {
+        xs = fg.xa + (bx - fg.nx0) * (tx - 2);
+        xe = min(xs + tx - 3, fg.hi[0]);
+    }
+}
+
+template <typename T, bool IB, int TX, int TY>
 __device__ __forceinline__ void fused3_tile(const StokesArgs<T>& a, const FusedGeom& fg, int bx, int by, int bz)
 {
-    constexpr int TX = 64;
     __shared__ T sVx[TY][TX], sVy[TY][TX], sVz[TY][TX];
     __shared__ T sE[3][TY][TX];
     __shared__ T sS[2][6][TY][TX]; // P txx tyy txy txz tyz
 
     const int tx = threadIdx.x, ty = threadIdx.y;
-    const int i = fg.lo[0] - 1 + bx * (TX - 2) + tx;
+    int xs, xe; // inner x range of this tile
+    fused_xrange(fg, TX, bx, xs, xe);
+    const int i = xs - 1 + tx;
     const int j = fg.lo[1] - 1 + by * (TY - 2) + ty;
     const int ks = fg.lo[2] + bz * fg.kch;
     const int ke = min(ks + fg.kch - 1, fg.hi[2]);
-    const bool inner = tx >= 1 && tx <= TX - 2 && ty >= 1 && ty <= TY - 2 && i <= fg.hi[0] && j <= fg.hi[1];
+    const bool inner = tx >= 1 && tx <= TX - 2 && ty >= 1 && ty <= TY - 2 && i <= xe && j <= fg.hi[1];
     const bool can_c = tx < TX - 1 && ty < TY - 1;
     const bool can_xy = tx > 0 && ty > 0;
     const bool can_xz = tx > 0;
     const bool can_yz = ty > 0;
     constexpr uint32_t E = sizeof(T);
     const uint32_t bsy = (uint32_t)a.sy * E, bsz = (uint32_t)a.sz * E;
-    // footprint positions past hi+1 are never read by inner threads: clamp their addresses
-    const int ic = min(i, fg.hi[0] + 1), jc = min(j, fg.hi[1] + 1);
+    // footprint positions past the inner range + 1 are never read by inner threads: clamp their addresses
+    const int ic = min(i, xe + 1), jc = min(j, fg.hi[1] + 1);
     const uint32_t col = ((uint32_t)ic + (uint32_t)a.sy * (uint32_t)jc) * E;
     const T ix = a.ix, iy = a.iy, iz = a.iz;
+    // old-stress components this thread's row needs: the halo rows only feed
+    // the inner rows P, tyy (row 0) or txy, tyz (row TY-1); the halo planes
+    // ks-1 and ke+1 only P, tzz and txz, tyz (bits P xx yy zz xy xz yz)
+    const unsigned rneed = ty == 0 ? 0x05u : ty == TY - 1 ? 0x50u : 0x7fu;
 
     // registers of the own column
     const uint32_t c2 = col + bsz * (uint32_t)(ks - 2), c1 = c2 + bsz, c3 = c2 - bsz;
@@ -317,8 +338,9 @@ __device__ __forceinline__ void fused3_tile(const StokesArgs<T>& a, const FusedG
     T vx0 = ldo(a.Vx, c1), vy0 = ldo(a.Vy, c1), vz0 = ldo(a.Vz, c1);
     T vzp = ldo(a.Vz, c1 + bsz);
     T emm = ldo(a.eta, c3), em = ldo(a.eta, c2), e0 = ldo(a.eta, c1);
-    T oP = ldo(a.P, c1), oxx = ldo(a.txx, c1), oyy = ldo(a.tyy, c1), ozz = ldo(a.tzz, c1), oxy = ldo(a.txy, c1),
-      oxz = ldo(a.txz, c1), oyz = ldo(a.tyz, c1);
+    const unsigned n0 = rneed & 0x09u;
+    T oP = (n0 & 1u) ? ldo(a.P, c1) : T(0), oxx = T(0), oyy = T(0), ozz = (n0 & 8u) ? ldo(a.tzz, c1) : T(0),
+      oxy = T(0), oxz = T(0), oyz = T(0);
     T gx = T(0), gy = T(0), gz = T(0);
     T Pk = T(0), xxk = T(0), yyk = T(0), zzk = T(0), Pkm = T(0), zzkm = T(0);
     sE[(ks - 2) % 3][ty][tx] = em;
@@ -339,13 +361,17 @@ __device__ __forceinline__ void fused3_tile(const StokesArgs<T>& a, const FusedG
             nvy = ldo(a.Vy, cn);
             nvz2 = ldo(a.Vz, cn + bsz);
             ne = ldo(a.eta, cn);
-            nP = ldo(a.P, cn);
-            nxx = ldo(a.txx, cn);
-            nyy = ldo(a.tyy, cn);
-            nzz = ldo(a.tzz, cn);
-            nxy = ldo(a.txy, cn);
-            nxz = ldo(a.txz, cn);
-            nyz = ldo(a.tyz, cn);
+            // unneeded components read the line of a needed one (no branch, no
+            // extra HBM traffic, value never used)
+            const unsigned nd = rneed & (kk + 1 == ke + 1 ? 0x60u : 0x7fu);
+            const T* fb = (nd & 1u) ? a.P : (nd & 16u) ? a.txy : (nd & 32u) ? a.txz : (nd & 64u) ? a.tyz : a.P;
+            nP = ldo((nd & 1u) ? a.P : fb, cn);
+            nxx = ldo((nd & 2u) ? a.txx : fb, cn);
+            nyy = ldo((nd & 4u) ? a.tyy : fb, cn);
+            nzz = ldo((nd & 8u) ? a.tzz : fb, cn);
+            nxy = ldo((nd & 16u) ? a.txy : fb, cn);
+            nxz = ldo((nd & 32u) ? a.txz : fb, cn);
+            nyz = ldo((nd & 64u) ? a.tyz : fb, cn);
         }
         if (a.grav_field) {
             ngx = ldo(a.gx, c);
@@ -514,20 +540,21 @@ __device__ __forceinline__ void fused3_tile(const StokesArgs<T>& a, const FusedG
 // rates and na-weighted stress for the masked momentum fluxes (the products
 // are the operands the pointwise kernels form, so results are identical).
 // Raw off-diagonal stress is kept beside it for tau_II when eta is updated.
-template <typename T, int TY>
+template <typename T, int TX, int TY>
 __device__ __forceinline__ void fused3_ib_tile(const StokesArgs<T>& a, const FusedGeom& fg, int bx, int by, int bz)
 {
-    constexpr int TX = 64;
     __shared__ T sVx[TY][TX], sVy[TY][TX], sVz[TY][TX];
     __shared__ T sE[3][TY][TX];
     __shared__ T sS[2][9][TY][TX]; // na-weighted P txx tyy txy txz tyz; raw txy txz tyz
 
     const int tx = threadIdx.x, ty = threadIdx.y;
-    const int i = fg.lo[0] - 1 + bx * (TX - 2) + tx;
+    int xs, xe; // inner x range of this tile
+    fused_xrange(fg, TX, bx, xs, xe);
+    const int i = xs - 1 + tx;
     const int j = fg.lo[1] - 1 + by * (TY - 2) + ty;
     const int ks = fg.lo[2] + bz * fg.kch;
     const int ke = min(ks + fg.kch - 1, fg.hi[2]);
-    const bool inner = tx >= 1 && tx <= TX - 2 && ty >= 1 && ty <= TY - 2 && i <= fg.hi[0] && j <= fg.hi[1];
+    const bool inner = tx >= 1 && tx <= TX - 2 && ty >= 1 && ty <= TY - 2 && i <= xe && j <= fg.hi[1];
     const bool can_c = tx < TX - 1 && ty < TY - 1;
     const bool can_xy = tx > 0 && ty > 0;
     const bool can_xz = tx > 0;
@@ -535,11 +562,13 @@ __device__ __forceinline__ void fused3_ib_tile(const StokesArgs<T>& a, const Fus
     const bool need_raw = !a.frozen_eta;
     constexpr uint32_t E = sizeof(T);
     const uint32_t bsz = (uint32_t)a.sz * E;
-    const int ic = min(i, fg.hi[0] + 1), jc = min(j, fg.hi[1] + 1);
+    const int ic = min(i, xe + 1), jc = min(j, fg.hi[1] + 1);
     const uint32_t col = ((uint32_t)ic + (uint32_t)a.sy * (uint32_t)jc) * E;
     const T ix = a.ix, iy = a.iy, iz = a.iz;
     const T *wvx = a.ns[OM_VCC], *wvy = a.ns[OM_CVC], *wvz = a.ns[OM_CCV];
     const T *wc = a.na[OM_CCC], *wxy = a.na[OM_VVC], *wxz = a.na[OM_VCV], *wyz = a.na[OM_CVV];
+    // old-stress components this thread's row needs (as in fused3_tile)
+    const unsigned rneed = ty == 0 ? 0x05u : ty == TY - 1 ? 0x50u : 0x7fu;
 
     const uint32_t c2 = col + bsz * (uint32_t)(ks - 2), c1 = c2 + bsz, c0 = c1 + bsz, c3 = c2 - bsz;
     // raw velocity and ns weights of the own column
@@ -555,22 +584,28 @@ __device__ __forceinline__ void fused3_ib_tile(const StokesArgs<T>& a, const Fus
     T mc0 = ldo(wc, c1), mxy0 = ldo(wxy, c1), mxz0 = ldo(wxz, c1), myz0 = ldo(wyz, c1);
     unsigned fl0 = ldo(a.flags, c1 / E);
     T emm = ldo(a.eta, c3), em = ldo(a.eta, c2), e0 = ldo(a.eta, c1);
-    T oP = ldo(a.P, c1), oxx = ldo(a.txx, c1), oyy = ldo(a.tyy, c1), ozz = ldo(a.tzz, c1), oxy = ldo(a.txy, c1),
-      oxz = ldo(a.txz, c1), oyz = ldo(a.tyz, c1);
-    T gx = T(0), gy = T(0), gz = T(0);
     // own new stress: weighted at k and k-1, raw diagonal at k
     T Pw = T(0), Pwm = T(0), xxw = T(0), yyw = T(0), zzw = T(0), zzwm = T(0), xxr = T(0), yyr = T(0), zzr = T(0);
     sE[(ks - 2) % 3][ty][tx] = em;
 
     for (int kk = ks - 1; kk <= ke + 1; ++kk) {
         const uint32_t c = col + bsz * (uint32_t)kk;
+        // previous-iteration stress at plane kk (used after the barrier); the
+        // halo planes ks-1 and ke+1 only feed P, tzz and txz, tyz
+        // (an unneeded component is read from the line of a needed one
+        // instead: no branch, no extra HBM traffic, its value is never used)
+        const unsigned nd = rneed & (kk == ks - 1 ? 0x09u : kk == ke + 1 ? 0x60u : 0x7fu);
+        const T* fb = (nd & 1u) ? a.P : (nd & 16u) ? a.txy : (nd & 32u) ? a.txz : (nd & 64u) ? a.tyz : a.P;
+        const T oP = ldo((nd & 1u) ? a.P : fb, c), oxx = ldo((nd & 2u) ? a.txx : fb, c),
+                oyy = ldo((nd & 4u) ? a.tyy : fb, c), ozz = ldo((nd & 8u) ? a.tzz : fb, c),
+                oxy = ldo((nd & 16u) ? a.txy : fb, c), oxz = ldo((nd & 32u) ? a.txz : fb, c),
+                oyz = ldo((nd & 64u) ? a.tyz : fb, c);
         sVx[ty][tx] = qx0;
         sVy[ty][tx] = qy0;
         sVz[ty][tx] = qz0;
         sE[kk % 3][ty][tx] = e0;
         T nvx = T(0), nvy = T(0), nvz2 = T(0), nmx = T(0), nmy = T(0), nmz2 = T(0), nmc = T(0), nmxy = T(0),
-          nmxz = T(0), nmyz = T(0), ne = T(0), nP = T(0), nxx = T(0), nyy = T(0), nzz = T(0), nxy = T(0), nxz = T(0),
-          nyz = T(0), ngx = T(0), ngy = T(0), ngz = T(0);
+          nmxz = T(0), nmyz = T(0), ne = T(0);
         unsigned nfl = 0;
         if (kk <= ke) {
             const uint32_t cn = c + bsz;
@@ -586,18 +621,6 @@ __device__ __forceinline__ void fused3_ib_tile(const StokesArgs<T>& a, const Fus
             nmyz = ldo(wyz, cn);
             nfl = ldo(a.flags, cn / E);
             ne = ldo(a.eta, cn);
-            nP = ldo(a.P, cn);
-            nxx = ldo(a.txx, cn);
-            nyy = ldo(a.tyy, cn);
-            nzz = ldo(a.tzz, cn);
-            nxy = ldo(a.txy, cn);
-            nxz = ldo(a.txz, cn);
-            nyz = ldo(a.tyz, cn);
-        }
-        if (a.grav_field) {
-            ngx = ldo(a.gx, c);
-            ngy = ldo(a.gy, c);
-            ngz = ldo(a.gz, c);
         }
         __syncthreads();
 
@@ -661,9 +684,9 @@ __device__ __forceinline__ void fused3_ib_tile(const StokesArgs<T>& a, const Fus
             const T ex = T(0.5) * (sE[ek][ty][tx - 1] + em);
             const T ey = T(0.5) * (sE[ek][ty - 1][tx] + em);
             const T ez = T(0.5) * (emm + em);
-            const T g0 = a.grav_field ? gx : a.grav[0];
-            const T g1 = a.grav_field ? gy : a.grav[1];
-            const T g2 = a.grav_field ? gz : a.grav[2];
+            const T g0 = a.grav_field ? ldo(a.gx, cv) : a.grav[0];
+            const T g1 = a.grav_field ? ldo(a.gy, cv) : a.grav[1];
+            const T g2 = a.grav_field ? ldo(a.gz, cv) : a.grav[2];
             const T xyw = sS[bo][3][ty][tx], xzw = sS[bo][4][ty][tx], yzw = sS[bo][5][ty][tx];
             sto(a.Vxn, cv, (flm & F_VCC) ? upd_V_ib(vxm, dif(Pw, sS[bo][0][ty][tx - 1], ix), dif(xxw, sS[bo][1][ty][tx - 1], ix),
                                                     dif(sS[bo][3][ty + 1][tx], xyw, iy), dif(sS[bn][4][ty][tx], xzw, iz),
@@ -725,40 +748,83 @@ __device__ __forceinline__ void fused3_ib_tile(const StokesArgs<T>& a, const Fus
         emm = em;
         em = e0;
         e0 = ne;
-        oP = nP;
-        oxx = nxx;
-        oyy = nyy;
-        ozz = nzz;
-        oxy = nxy;
-        oxz = nxz;
-        oyz = nyz;
-        gx = ngx;
-        gy = ngy;
-        gz = ngz;
     }
 }
 
-template <typename T, bool IB, int TY>
-__global__ __launch_bounds__(64 * TY) void k_fused3(StokesArgs<T> a, FusedGeom fg)
+// Tile order.  Workgroups are dispatched round-robin over the 8 XCDs
+// (workgroup b runs on XCD b % 8), each with its own L2.  Tiles are numbered
+// x fastest.  order 0: tile b (the XCDs interleave tile by tile); order 1:
+// every XCD takes a contiguous run of 1/8 of the tiles, so the tiles running
+// side by side on one XCD include x and y neighbours whose shared halo lines
+// meet in that XCD's L2; order 2: every 64 workgroups, each XCD takes 8
+// consecutive tiles (x neighbours share their halo lines in one L2 while the
+// XCDs still interleave finely, which keeps uneven work balanced).  The grid
+// is padded to a multiple of 64; padding workgroups exit at once.
+constexpr int NXCD = 8;
+
+__device__ __forceinline__ int xcd_run_tile(int b, int order, int ntiles)
 {
-    if constexpr (IB) fused3_ib_tile<T, TY>(a, fg, blockIdx.x, blockIdx.y, blockIdx.z);
-    else fused3_tile<T, false, TY>(a, fg, blockIdx.x, blockIdx.y, blockIdx.z);
+    if (order == 1) {
+        const int per = (ntiles + NXCD - 1) / NXCD;
+        return (b % NXCD) * per + b / NXCD;
+    }
+    if (order == 2) {
+        // groups of 64 workgroups: XCD x takes 8 consecutive tiles of the group
+        const int g = b / (NXCD * 8), r = b % (NXCD * 8);
+        return g * (NXCD * 8) + (r % NXCD) * 8 + r / NXCD;
+    }
+    return b;
+}
+
+__device__ __forceinline__ bool fused_tile_of(int b, int order, int nbx, int nby, int nbz, int& bx, int& by, int& bz)
+{
+    const int ntiles = nbx * nby * nbz;
+    const int t = xcd_run_tile(b, order, ntiles);
+    if (t >= ntiles) return false;
+    bx = t % nbx;
+    const int r = t / nbx;
+    by = r % nby;
+    bz = r / nby;
+    return true;
+}
+
+// MINW: minimum waves per SIMD the register allocation must allow (1: none)
+template <typename T, bool IB, int TX, int TY, int MINW>
+__global__ __launch_bounds__(TX * TY, MINW) void k_fused3(StokesArgs<T> a, FusedGeom fg)
+{
+    const int nby = (fg.hi[1] - fg.lo[1] + 1 + TY - 3) / (TY - 2);
+    const int nbz = (fg.hi[2] - fg.lo[2] + fg.kch) / fg.kch;
+    int bx, by, bz;
+    if (!fused_tile_of(blockIdx.x, fg.order, fg.nbx, nby, nbz, bx, by, bz)) return;
+    if constexpr (IB) fused3_ib_tile<T, TX, TY>(a, fg, bx, by, bz);
+    else fused3_tile<T, false, TX, TY>(a, fg, bx, by, bz);
 }
 
 // Packed fused sweep over many instances: a persistent grid walks the
 // (instance, tile) work list; each item is one workgroup tile of one
 // instance, swept exactly as the single-instance kernel sweeps it.
-template <typename T, bool IB, int TY>
-__global__ __launch_bounds__(64 * TY) void k_fused3_packed(const PackedInst<T>* __restrict__ inst,
-                                                           const int4* __restrict__ items, int nitems)
+template <typename T, bool IB, int TX, int TY, int MINW>
+__global__ __launch_bounds__(TX * TY, MINW) void k_fused3_packed(const PackedInst<T>* __restrict__ inst,
+                                                                 const int4* __restrict__ items, int nitems)
 {
-    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+    // items are in x-fastest tile order per instance, mapped to workgroups as
+    // in fused_tile_of; gridDim.x is a multiple of 8 * NXCD
+    const int order = inst[0].fg.order;
+    const int per = (nitems + NXCD - 1) / NXCD;
+    const int x = blockIdx.x % NXCD;
+    const int first = order == 1 ? blockIdx.x / NXCD : blockIdx.x;
+    const int stride = order == 1 ? gridDim.x / NXCD : gridDim.x;
+    const int nslot = order == 1 ? per : order == 2 ? (nitems + 8 * NXCD - 1) / (8 * NXCD) * (8 * NXCD) : nitems;
+    for (int slot = first; slot < nslot; slot += stride) {
+        const int it = order == 1 ? x * per + slot : xcd_run_tile(slot, order, nitems);
+        if (order == 2 && it >= nitems) continue;
+        if (it >= nitems) break;
         const int4 w = items[it];
         const StokesArgs<T> a = inst[w.x].a;
         const FusedGeom fg = inst[w.x].fg;
         __syncthreads(); // the previous item's LDS reads are done
-        if constexpr (IB) fused3_ib_tile<T, TY>(a, fg, w.y, w.z, w.w);
-        else fused3_tile<T, false, TY>(a, fg, w.y, w.z, w.w);
+        if constexpr (IB) fused3_ib_tile<T, TX, TY>(a, fg, w.y, w.z, w.w);
+        else fused3_tile<T, false, TX, TY>(a, fg, w.y, w.z, w.w);
     }
 }
 
@@ -1216,6 +1282,8 @@ bool fused_split(const Geo& g, int ty, int kch, FusedGeom& fg, BoxList& shell)
         fg.hi[d] = hi[d];
     }
     fg.kch = kch;
+    fg.order = 0;
+    fused_xtiles(fg, 64, 8, g.off);
     const int N0 = g.n[0] + 1, N1 = g.n[1] + 1, N2 = g.n[2] + 1;
     BoxList& bl = shell;
     bl.nb = 0;
@@ -1260,22 +1328,71 @@ void launch_velocity_boxes(const Geo& g, const StokesArgs<T>& a, const BoxList& 
     }
 }
 
-template <typename T>
-constexpr int fused_ty()
+// (tile lanes in x, tile rows, minimum waves per SIMD) variants of the fused
+// sweep; the first entry is the default.  64-lane tiles sweep 62 columns at
+// any alignment; 66-lane tiles sweep 64 line-aligned columns.
+#define FI_FUSED_CFG_D(X) X(64, 8, 1) X(64, 8, 4) X(64, 4, 1) X(64, 4, 3) X(66, 7, 1) X(66, 8, 3)
+#define FI_FUSED_CFG_F(X) X(64, 16, 1) X(64, 8, 1) X(66, 15, 1) X(66, 15, 4) X(66, 7, 1)
+
+int fused_configs(bool fp32, int cfg[][3], int max)
 {
-    return sizeof(T) == 8 ? 8 : 16;
+    int n = 0;
+#define FI_LIST_(TX_, TY_, W_) if (n < max) { cfg[n][0] = TX_; cfg[n][1] = TY_; cfg[n][2] = W_; ++n; }
+    if (fp32) { FI_FUSED_CFG_F(FI_LIST_) }
+    else { FI_FUSED_CFG_D(FI_LIST_) }
+#undef FI_LIST_
+    return n;
+}
+
+// x tiling of fg for tiles of tx lanes over fields of esize-byte elements
+// whose Chmy index 0 sits `off` elements past a 128-byte line: sets xa, nx0,
+// nbx (see fused_xrange)
+void fused_xtiles(FusedGeom& fg, int tx, int esize, long off)
+{
+    const int lo = fg.lo[0], hi = fg.hi[0];
+    if (tx == 64) {
+        fg.xa = lo;
+        fg.nx0 = 0;
+    } else {
+        const long line = 128 / esize;
+        long c = lo + 1; // a leading tile needs at least the columns [lo, lo]
+        while ((off + c) % line != 0) ++c;
+        fg.xa = (int)(c <= hi ? c : hi + 1);
+        fg.nx0 = 1;
+    }
+    fg.nbx = fg.nx0 + (hi - fg.xa + 1 + tx - 3) / (tx - 2);
+}
+
+// z planes per workgroup when not set: long chunks (fewer halo planes) as
+// long as the launch keeps about 1024 workgroups for the 256 CUs
+int fused_auto_kch(const FusedGeom& fg, int tiles_per_plane)
+{
+    const long nz = fg.hi[2] - fg.lo[2] + 1;
+    long k = 64;
+    while (k > 8 && nz * tiles_per_plane / k < 1024) k /= 2;
+    return (int)k;
 }
 
 template <typename T>
-void launch_fused(const Geo& g, const StokesArgs<T>& a, const FusedGeom& fg, bool ib, hipStream_t s)
+void launch_fused(const Geo& g, const StokesArgs<T>& a, const FusedGeom& fg0, bool ib, int tx, int ty, int minw,
+                  hipStream_t s)
 {
-    constexpr int TY = fused_ty<T>();
-    const int nbx = (fg.hi[0] - fg.lo[0] + 1 + 64 - 3) / (64 - 2);
-    const int nby = (fg.hi[1] - fg.lo[1] + 1 + TY - 3) / (TY - 2);
+    FusedGeom fg = fg0;
+    fused_xtiles(fg, tx, (int)sizeof(T), g.off);
+    const int nby = (fg.hi[1] - fg.lo[1] + 1 + ty - 3) / (ty - 2);
+    if (fg.kch <= 0) fg.kch = fused_auto_kch(fg, fg.nbx * nby);
     const int nbz = (fg.hi[2] - fg.lo[2] + fg.kch) / fg.kch;
-    dim3 grd(nbx, nby, nbz), blk(64, TY, 1);
-    if (ib) hipLaunchKernelGGL((k_fused3<T, true, TY>), grd, blk, 0, s, a, fg);
-    else hipLaunchKernelGGL((k_fused3<T, false, TY>), grd, blk, 0, s, a, fg);
+    const int ntiles = fg.nbx * nby * nbz;
+    dim3 grd((ntiles + 8 * NXCD - 1) / (8 * NXCD) * (8 * NXCD)), blk(tx, ty, 1);
+#define FI_LAUNCH_(TX_, TY_, W_)                                                                \
+    if (tx == TX_ && ty == TY_ && minw == W_) {                                                 \
+        if (ib) hipLaunchKernelGGL((k_fused3<T, true, TX_, TY_, W_>), grd, blk, 0, s, a, fg);  \
+        else hipLaunchKernelGGL((k_fused3<T, false, TX_, TY_, W_>), grd, blk, 0, s, a, fg);    \
+        return;                                                                                 \
+    }
+    if constexpr (sizeof(T) == 8) { FI_FUSED_CFG_D(FI_LAUNCH_) }
+    else { FI_FUSED_CFG_F(FI_LAUNCH_) }
+#undef FI_LAUNCH_
 }
 
 template <typename T>
@@ -1303,20 +1420,21 @@ void launch_tau(const Geo& g, const StokesArgs<T>& a, bool ib, hipStream_t s)
 }
 
 template <typename T>
-void launch_fused_packed(const PackedInst<T>* inst, const int4* items, int nitems, bool ib, int ty, int grid,
-                         hipStream_t s)
+void launch_fused_packed(const PackedInst<T>* inst, const int4* items, int nitems, bool ib, int tx, int ty, int minw,
+                         int grid, hipStream_t s)
 {
     if (nitems <= 0) return;
-    dim3 grd(nitems < grid ? nitems : grid);
-    // tile heights: 4 or 8 rows in FP64, 8 or 16 in FP32 (LDS and VGPR budget)
-    constexpr int TA = sizeof(T) == 8 ? 4 : 8, TB = sizeof(T) == 8 ? 8 : 16;
-    if (ty == TA) {
-        if (ib) hipLaunchKernelGGL((k_fused3_packed<T, true, TA>), grd, dim3(64, TA), 0, s, inst, items, nitems);
-        else hipLaunchKernelGGL((k_fused3_packed<T, false, TA>), grd, dim3(64, TA), 0, s, inst, items, nitems);
-    } else {
-        if (ib) hipLaunchKernelGGL((k_fused3_packed<T, true, TB>), grd, dim3(64, TB), 0, s, inst, items, nitems);
-        else hipLaunchKernelGGL((k_fused3_packed<T, false, TB>), grd, dim3(64, TB), 0, s, inst, items, nitems);
+    int gsz = nitems < grid ? nitems : grid;
+    dim3 grd((gsz + 8 * NXCD - 1) / (8 * NXCD) * (8 * NXCD)), blk(tx, ty);
+#define FI_LAUNCH_(TX_, TY_, W_)                                                                                 \
+    if (tx == TX_ && ty == TY_ && minw == W_) {                                                                  \
+        if (ib) hipLaunchKernelGGL((k_fused3_packed<T, true, TX_, TY_, W_>), grd, blk, 0, s, inst, items, nitems);  \
+        else hipLaunchKernelGGL((k_fused3_packed<T, false, TX_, TY_, W_>), grd, blk, 0, s, inst, items, nitems);    \
+        return;                                                                                                  \
     }
+    if constexpr (sizeof(T) == 8) { FI_FUSED_CFG_D(FI_LAUNCH_) }
+    else { FI_FUSED_CFG_F(FI_LAUNCH_) }
+#undef FI_LAUNCH_
 }
 
 template <typename T>
@@ -1339,11 +1457,13 @@ void launch_point_packed(const PackedInst<T>* inst, const PackedBox* boxes, cons
 #define INST(T)                                                                                               \
     template void launch_sigma_boxes<T>(const Geo&, const StokesArgs<T>&, const BoxList&, bool, hipStream_t);    \
     template void launch_velocity_boxes<T>(const Geo&, const StokesArgs<T>&, const BoxList&, bool, hipStream_t); \
-    template void launch_fused<T>(const Geo&, const StokesArgs<T>&, const FusedGeom&, bool, hipStream_t);        \
+    template void launch_fused<T>(const Geo&, const StokesArgs<T>&, const FusedGeom&, bool, int, int, int,       \
+                                  hipStream_t);                                                                 \
     template void launch_residuals<T>(const Geo&, const StokesArgs<T>&, bool, hipStream_t);                      \
     template void launch_tau<T>(const Geo&, const StokesArgs<T>&, bool, hipStream_t);                      \
     template void launch_eta_update<T>(const Geo&, const StokesArgs<T>&, hipStream_t);                     \
-    template void launch_fused_packed<T>(const PackedInst<T>*, const int4*, int, bool, int, int, hipStream_t);  \
+    template void launch_fused_packed<T>(const PackedInst<T>*, const int4*, int, bool, int, int, int, int,         \
+                                         hipStream_t);                                                          \
     template void launch_point_packed<T>(const PackedInst<T>*, const PackedBox*, const int*, int, bool, int, bool, \
                                          hipStream_t);
 INST(double)

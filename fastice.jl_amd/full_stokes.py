@@ -302,12 +302,13 @@ class IsothermalFullStokesModel:
         """Solve with FP32 PT sweeps inside an FP64 defect-correction loop.
 
         Returns (outer steps, FP32 iterations, Inf-norms).  Same discrete
-        solution as solve(); the outer residual is evaluated in FP64.  The
-        defaults (inner checks every n/2 sweeps, inexact inner solves to a
-        3e-2 drop) were the fastest on the Glen-law glacier.
+        solution as solve(); the outer residual is evaluated in FP64.
+        ncheck: FP32 sweeps between inner residual checks (default n; half
+        the driver's check cadence was the fastest on the Glen-law glacier,
+        whose cadence is 5n); inner_rtol: inexact inner solves (3e-2 drop).
         """
         n = max(self.grid.dims)
-        ncheck = ncheck or max(n // 2, 1)
+        ncheck = ncheck or n
         max_inner = max_inner or 100 * n
         counts = (C.c_int * 3)()
         out = np.zeros(4)
@@ -335,7 +336,8 @@ class IsothermalFullStokesModel:
         """Kernel-path option (fused interior sweep on/off, z planes per workgroup)."""
         k = {"fused": A.FI_OPT_FUSED, "zchunk": A.FI_OPT_ZCHUNK, "refine_full_phase": A.FI_OPT_REFINE_FULL_PHASE,
              "refine_eta_rate_permil": A.FI_OPT_REFINE_ETA_RATE,
-             "refine_phase1_digits": A.FI_OPT_REFINE_PHASE1_DIGITS}.get(key, key)
+             "refine_phase1_digits": A.FI_OPT_REFINE_PHASE1_DIGITS, "fused_variant": A.FI_OPT_FUSED_VARIANT,
+             "fused_variant_fp32": A.FI_OPT_FUSED_VARIANT_FP32, "tile_order": A.FI_OPT_TILE_ORDER}.get(key, key)
         A.check(A.lib().fi_set_option(self._h, int(k), int(value)))
 
 

@@ -164,7 +164,10 @@ int fi_fused_region(fi_model_t m, int lo[3], int hi[3]);
 /* Options (no counterpart in the reference).  Kernel path, results do not
  * depend on them: FI_OPT_FUSED 1/0 selects the fused interior sweep or the
  * pointwise sweep everywhere; FI_OPT_ZCHUNK sets the z planes per workgroup of
- * the fused sweep.  Refined solver: FI_OPT_REFINE_FULL_PHASE 1/0 starts a
+ * the fused sweep; FI_OPT_FUSED_VARIANT(_FP32) selects a compiled variant of
+ * the FP64 (FP32) fused sweep; FI_OPT_TILE_ORDER maps workgroups to tiles:
+ * 0 x-fastest, 1 one contiguous run per XCD, 2 runs of 8 tiles per XCD
+ * (default 2).  Refined solver: FI_OPT_REFINE_FULL_PHASE 1/0 starts a
  * nonlinear solve with full FP32 PT sweeps (default 1) that stop after
  * FI_OPT_REFINE_PHASE1_DIGITS digits of residual drop (1..7, default 3) or
  * at the FP32 floor; FI_OPT_REFINE_ETA_RATE is the initial log-space
@@ -175,9 +178,19 @@ enum {
     FI_OPT_ZCHUNK = 2,
     FI_OPT_REFINE_FULL_PHASE = 3,
     FI_OPT_REFINE_ETA_RATE = 4,
-    FI_OPT_REFINE_PHASE1_DIGITS = 5
+    FI_OPT_REFINE_PHASE1_DIGITS = 5,
+    FI_OPT_FUSED_VARIANT = 6,
+    FI_OPT_FUSED_VARIANT_FP32 = 7,
+    FI_OPT_TILE_ORDER = 8
 };
 int fi_set_option(fi_model_t m, int key, int value);
+/* Compiled variants of the fused sweep (FI_OPT_FUSED_VARIANT* take an index
+ * into this list): triples (tile lanes in x: 64 for 62 columns at any
+ * alignment or 66 for 64 line-aligned columns, tile rows, minimum waves per
+ * SIMD the register allocation allows, 1 = unconstrained); returns the
+ * number of variants and writes at most max_variants triples.  Results do
+ * not depend on the variant. */
+int fi_fused_variants(int fp32, int* tx_ty_minw, int max_variants);
 
 int fi_levelset_from_dem(fi_model_t m, int phase, const double* dem_parent, int dem_nx, int dem_ny,
                          const double dem_origin[2], const double dem_extent[2], const double R[9]);
@@ -236,15 +249,17 @@ int fi_batch_solve(fi_batch_t b);
 /* state (FI_BATCH_*), iterations done and last residual norms of a model;
  * a failed model (non-finite residual) reports FI_BATCH_FAILED */
 int fi_batch_status(fi_batch_t b, fi_model_t m, int* state, int* iters, double norms_inf[4]);
-/* FI_BOPT_TILE_Y: rows of the fused tile (FP64: 4 or 8), FI_BOPT_ZCHUNK: z
- * planes per tile, FI_BOPT_AUTOTUNE 1/0: time the tile candidates on the
- * first step of a new working set and keep the fastest (default 1),
- * FI_BOPT_GRAPH_ITERS: iterations per graph replay (0: no graph).  Results do
+/* FI_BOPT_VARIANT: fused sweep variant (index into fi_fused_variants(0)),
+ * FI_BOPT_ZCHUNK: z planes per tile, FI_BOPT_AUTOTUNE 1/0: time every
+ * (variant, z chunk) candidate on the first step of a new working set and
+ * keep the fastest (default 1), FI_BOPT_GRAPH_ITERS: iterations per graph
+ * replay (0: no graph), FI_BOPT_TILE_ORDER: as FI_OPT_TILE_ORDER.  Results do
  * not depend on any of them. */
-enum { FI_BOPT_TILE_Y = 1, FI_BOPT_ZCHUNK = 2, FI_BOPT_AUTOTUNE = 3, FI_BOPT_GRAPH_ITERS = 4 };
+enum { FI_BOPT_VARIANT = 1, FI_BOPT_ZCHUNK = 2, FI_BOPT_AUTOTUNE = 3, FI_BOPT_GRAPH_ITERS = 4, FI_BOPT_TILE_ORDER = 5 };
 int fi_batch_set_option(fi_batch_t b, int key, int value);
-/* out = {attached, running, fused work items, tile rows, z planes, graph
- * iterations, autotune runs, table rebuilds} */
+/* out = {attached, running, fused work items, (32 * tile lanes + tile rows)
+ * * 16 + min waves, z planes, graph iterations, autotune runs, table
+ * rebuilds} */
 int fi_batch_info(fi_batch_t b, int out[8]);
 
 #ifdef __cplusplus

@@ -80,8 +80,10 @@ def alone(pkg):
     return _alone(pkg, CASES, 1e-8)
 
 
-@pytest.mark.parametrize("opts", [dict(), dict(autotune=False, tile_y=4, zchunk=5, graph_iters=0),
-                                  dict(autotune=False, tile_y=8, zchunk=16, graph_iters=6)])
+@pytest.mark.parametrize("opts", [dict(), dict(autotune=False, variant=2, zchunk=5, graph_iters=0),
+                                  dict(autotune=False, variant=1, zchunk=16, graph_iters=6),
+                                  dict(autotune=False, variant=4, zchunk=32, graph_iters=8, order=2),
+                                  dict(autotune=False, variant=5, zchunk=9, graph_iters=2, order=1)])
 def test_batch_equals_alone(pkg, alone, opts):
     _say("case: batch of %d mixed models, options %s" % (len(CASES), opts))
     models = []

@@ -105,3 +105,25 @@ def test_refined_fused_equals_pointwise(pkg):
         assert np.array_equal(fa, b.field(name).parent()), name
     a.close()
     b.close()
+
+
+def test_every_fp32_variant_equals_pointwise(pkg):
+    """Every compiled FP32 fused-sweep variant gives the pointwise FP32
+    correction sweep's results bitwise (glacier, immersed boundary)."""
+    variants = pkg._abi.fused_variants(True)
+    _say("case: %d FP32 fused variants, refined glacier 70x40x20, 2 outer steps" % len(variants))
+
+    def run(option):
+        m, _ = pkg.glacier_3d((70, 40, 20), seed=6, glen=True)
+        option(m)
+        out = m.refined_solve(1e-12, max_outer=2, max_inner=140, ncheck=35)
+        fields = {n: m.field(n).parent() for n in state_names(3) + ["ETA"]}
+        m.close()
+        return out, fields
+
+    (o_ref, want) = run(lambda m: m.set_option("fused", 0))
+    for v, cfg in enumerate(variants):
+        (o, got) = run(lambda m: m.set_option("fused_variant_fp32", v))
+        assert o[0] == o_ref[0] and o[1] == o_ref[1], cfg
+        for n in want:
+            assert np.array_equal(got[n], want[n]), (cfg, n)

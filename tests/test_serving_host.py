@@ -34,12 +34,12 @@ def test_percentiles(pkg):
 
 
 def _expected_bytes(dims, nfields_single, immersed):
-    # padded layout: rows of n0+3 doubles rounded up to 16, 15 leading doubles,
-    # one trailing line; the ten stress/velocity fields are double buffered
+    # padded layout: rows of n0+3 elements rounded up to 32, 31 leading
+    # elements, 32 trailing; the ten stress/velocity fields are double buffered
     n = list(dims)
-    sy = (n[0] + 3 + 15) // 16 * 16
+    sy = (n[0] + 3 + 31) // 32 * 32
     rows = (n[1] + 3) * (n[2] + 3) if len(n) == 3 else n[1] + 3
-    elems = 15 + sy * rows + 16
+    elems = 31 + sy * rows + 32
     b = (nfields_single + 10) * elems * 8 + 32
     if immersed:
         b += elems
@@ -52,8 +52,8 @@ def test_model_bytes_matches_layout(pkg):
     assert pkg.model_bytes(**kw) == _expected_bytes((30, 20, 10), 10 + 2 + 4 + 3, False)
     kw, _ = pkg.manufactured_2d((50, 40))
     # P tau(3) V(2) eta eta_next rP rV(2) gravity fields(2), second buffers of 6 fields
-    sy = (50 + 3 + 15) // 16 * 16
-    elems = 15 + sy * 43 + 16
+    sy = (50 + 3 + 31) // 32 * 32
+    elems = 31 + sy * 43 + 32
     assert pkg.model_bytes(**kw) == (6 + 2 + 3 + 2 + 6) * elems * 8 + 32
     kw, _ = pkg.glacier_3d_spec((40, 30, 20), seed=3)
     # + psi(2) + omega na/ns (8 each), constant gravity, no rheology field
