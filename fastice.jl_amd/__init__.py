@@ -19,9 +19,10 @@ from .full_stokes import (BoundaryCondition, IsothermalFullStokesModel, Isotherm
 from .geometries import (AABB, GaussianGlacier, SyntheticElevation, center, dilate, extents, make_synthetic,
                          splitmix64, union)
 from .grids import Center, UniformGrid, Vertex, flip
-from .level_sets import compute_levelset_from_dem, invert_levelset, omega_from_psi
+from .level_sets import compute_levelset_from_dem, compute_levelset_from_polygon, invert_levelset, omega_from_psi
 from .physics import LinearViscousRheology, PowerLawRheology, Relaxation, ViscoplasticRegularisation
-from .problems import glacier_3d, glacier_3d_spec, manufactured_2d, manufactured_3d, pt_params
+from .problems import (glacier_2d, glacier_2d_spec, glacier_3d, glacier_3d_spec, manufactured_2d, manufactured_3d,
+                       pt_params)
 from .serving import Request, Server, percentiles, poisson_requests
 
 

@@ -143,6 +143,7 @@ def _declare(lib):
         "fi_fused_region": (C.c_int, [H, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "fi_levelset_from_dem": (C.c_int, [H, C.c_int, _DP, C.c_int, C.c_int, _DP, _DP, _DP]),
         "fi_invert_levelset": (C.c_int, [H, C.c_int]),
+        "fi_levelset_from_polygon": (C.c_int, [H, C.c_int, _DP, C.c_int]),
         "fi_omega_from_psi": (C.c_int, [H, C.c_int]),
         "fi_mem_info": (C.c_int, [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
         "fi_refined_solve": (C.c_int, [H, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.POINTER(C.c_int), _DP]),

@@ -83,6 +83,7 @@ class Batch:
         d = dict(zip(keys, list(out)))
         v = d["variant"]  # (tile lanes in x, tile rows, min waves per SIMD)
         d["variant"] = (v // 16 // 32, v // 16 % 32, v % 16)
+        d["zchunk"], d["tile_order"] = d["zchunk"] // 4, d["zchunk"] % 4
         return d
 
     def close(self):

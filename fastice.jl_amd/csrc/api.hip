@@ -737,6 +737,38 @@ int fi_levelset_from_dem(fi_model_t m, int phase, const double* dem_parent, int 
     return FI_OK;
 }
 
+int fi_levelset_from_polygon(fi_model_t m, int phase, const double* xy, int npts)
+{
+    if (!m) return fail(FI_ERR_ARG, "null model");
+    if (!m->desc.immersed_boundary) return fail(FI_ERR_STATE, "model has no immersed boundary");
+    if (m->g.nd != 2) return fail(FI_ERR_UNSUPPORTED, "polygon level sets are defined for 2D grids");
+    if (phase != FI_PHASE_NA && phase != FI_PHASE_NS) return fail(FI_ERR_ARG, "bad phase %d", phase);
+    if (!xy || npts < 3) return fail(FI_ERR_ARG, "a polygon needs at least 3 points");
+    const size_t elems = 2 * (size_t)npts;
+    for (size_t q = 0; q < elems; ++q)
+        if (!std::isfinite(xy[q])) return fail(FI_ERR_ARG, "polygon coordinates must be finite");
+    for (int q = 0; q < npts; ++q) {
+        const int r = (q + 1) % npts;
+        if (xy[2 * q] == xy[2 * r] && xy[2 * q + 1] == xy[2 * r + 1])
+            return fail(FI_ERR_ARG, "polygon has a zero-length edge at point %d", q);
+    }
+    if (elems > m->dem_elems) {
+        if (m->d_dem) HIPCHK(hipFree(m->d_dem));
+        m->d_dem = nullptr;
+        m->dem_elems = 0;
+        HIPCHK(hipMalloc(&m->d_dem, elems * sizeof(double)));
+        m->dem_elems = elems;
+    }
+    HIPCHK(hipStreamSynchronize(m->stream));
+    HIPCHK(hipMemcpy(m->d_dem, xy, elems * sizeof(double), hipMemcpyHostToDevice));
+    double* psi = m->f[phase == FI_PHASE_NA ? FI_F_PSI_NA : FI_F_PSI_NS];
+    launch_levelset_from_polygon(m->g, m->desc.origin, psi, m->d_dem, npts, m->stream);
+    for (int d = 0; d < 2; ++d) launch_bc_dim(m->g, neumann_batch(psi, L_VVV, d), m->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(m->stream));
+    return FI_OK;
+}
+
 int fi_invert_levelset(fi_model_t m, int phase)
 {
     if (!m) return fail(FI_ERR_ARG, "null model");

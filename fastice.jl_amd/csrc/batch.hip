@@ -358,13 +358,14 @@ static int autotune(fi_batch_s* b, int budget, int* used)
 {
     *used = 0;
     struct Cand {
-        int tx, ty, minw, kch;
+        int tx, ty, minw, kch, order;
     };
     std::vector<Cand> cands;
     int cfg[16][3];
     const int nv = fused_configs(false, cfg, 16);
     for (int v = 0; v < nv; ++v)
-        for (int kch : {32, 64}) cands.push_back(Cand{cfg[v][0], cfg[v][1], cfg[v][2], kch});
+        for (int kch : {32, 64})
+            for (int order : {0, 2}) cands.push_back(Cand{cfg[v][0], cfg[v][1], cfg[v][2], kch, order});
     const int nc = (int)cands.size();
     const int reps = 2; // one warm-up iteration, one timed
     if (budget < nc * reps) return FI_OK;
@@ -373,6 +374,7 @@ static int autotune(fi_batch_s* b, int budget, int* used)
     for (int c = 0; c < nc; ++c) {
         b->tx = cands[c].tx;
         b->minw = cands[c].minw;
+        b->order = cands[c].order;
         int rc = build_tables(b, cands[c].ty, cands[c].kch);
         if (rc) return rc;
         float ms = 0.0f;
@@ -419,6 +421,7 @@ static int autotune(fi_batch_s* b, int budget, int* used)
     b->tuned_cells = running_cells(b);
     b->tx = cands[best].tx;
     b->minw = cands[best].minw;
+    b->order = cands[best].order;
     int rc = build_tables(b, cands[best].ty, cands[best].kch);
     if (rc) return rc;
     return capture_graphs(b);
@@ -659,7 +662,7 @@ int fi_batch_info(fi_batch_t b, int out[8])
     out[1] = running;
     out[2] = b->t.nitems[0] + b->t.nitems[1];
     out[3] = (b->tx * 32 + b->ty) * 16 + b->minw;
-    out[4] = b->kch;
+    out[4] = b->kch * 4 + b->order;
     out[5] = b->graph_iters;
     out[6] = b->tune_runs;
     out[7] = b->rebuilds;

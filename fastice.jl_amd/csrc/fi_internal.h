@@ -180,6 +180,8 @@ struct DemArgs {
     int has_R;
 };
 void launch_levelset_from_dem(const Geo& g, const double origin[3], double* psi, const DemArgs& dem, hipStream_t s);
+void launch_levelset_from_polygon(const Geo& g, const double origin[3], double* psi, const double* xy, int npts,
+                                  hipStream_t s);
 void launch_negate_vertex(const Geo& g, double* psi, hipStream_t s);
 
 } // namespace fi

@@ -317,6 +317,48 @@ void launch_levelset_from_dem(const Geo& g, const double origin[3], double* psi,
                        g.dx[0], g.dx[1], g.dx[2], 4.0 * dmax);
 }
 
+// Signed distance of (px, py) to a closed polygon (xy pairs), negative
+// inside by the even-odd rule: sd_poly, signed_distances.jl:3-19.
+__device__ double sd_poly(double px, double py, const double* __restrict__ xy, int npts)
+{
+    const double dx0 = px - xy[0], dy0 = py - xy[1];
+    double d = dx0 * dx0 + dy0 * dy0;
+    double s = 1.0;
+    int j = npts - 1;
+    for (int i = 0; i < npts; ++i) {
+        const double ex = xy[2 * j] - xy[2 * i], ey = xy[2 * j + 1] - xy[2 * i + 1];
+        const double wx = px - xy[2 * i], wy = py - xy[2 * i + 1];
+        double t = (wx * ex + wy * ey) / (ex * ex + ey * ey);
+        t = t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t);
+        const double bx = wx - ex * t, by = wy - ey * t;
+        const double bb = bx * bx + by * by;
+        d = bb < d ? bb : d;
+        const bool c1 = py >= xy[2 * i + 1], c2 = py < xy[2 * j + 1], c3 = ex * wy > ey * wx;
+        if ((c1 && c2 && c3) || (!c1 && !c2 && !c3)) s = -s;
+        j = i;
+    }
+    return s * sqrt(d);
+}
+
+__global__ __launch_bounds__(256) void k_levelset_poly(const double* __restrict__ xy, int npts, double* psi, int nx,
+                                                       int ny, long sy, double ox, double oy, double dx, double dy)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    if (i > nx + 1 || j > ny + 1) return;
+    const double x = ox + (double)(i - 1) * dx;
+    const double y = oy + (double)(j - 1) * dy;
+    psi[(long)i + sy * j] = sd_poly(x, y, xy, npts);
+}
+
+void launch_levelset_from_polygon(const Geo& g, const double origin[3], double* psi, const double* xy, int npts,
+                                  hipStream_t s)
+{
+    dim3 blk(64, 4, 1), grd((g.n[0] + 1 + 63) / 64, (g.n[1] + 1 + 3) / 4, 1);
+    hipLaunchKernelGGL(k_levelset_poly, grd, blk, 0, s, xy, npts, psi, g.n[0], g.n[1], g.sy, origin[0], origin[1],
+                       g.dx[0], g.dx[1]);
+}
+
 __global__ __launch_bounds__(256) void k_negate(double* psi, int nx, int ny, long sy, long sz)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;

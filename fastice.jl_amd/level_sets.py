@@ -49,6 +49,13 @@ def compute_levelset_from_dem(psi, dem, dem_grid, grid=None, R=None):
                                          A.dptr(o), A.dptr(e), None if r is None else A.dptr(r)))
 
 
+def compute_levelset_from_polygon(psi, poly):
+    """2D level set of a closed polygon (n x 2 points): signed distance of
+    every vertex, negative inside (sd_poly, signed_distances.jl:3-19)."""
+    xy = np.ascontiguousarray(np.asarray(poly, dtype=np.float64).reshape(-1, 2))
+    A.check(A.lib().fi_levelset_from_polygon(psi.model._h, psi.phase, A.dptr(xy), xy.shape[0]))
+
+
 def invert_levelset(psi):
     """psi <- -psi (compute_levelsets.jl:27-35)."""
     A.check(A.lib().fi_invert_levelset(psi.model._h, psi.phase))

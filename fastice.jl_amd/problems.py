@@ -207,3 +207,52 @@ def glacier_3d(dims, seed=1, glen=True, eta_reg=30.0, relax_rate=0.05, lx=2.0, l
     model.viscosity.eta_next.set(1.0)
     model.apply_bc("eta")
     return model, cad
+
+
+def glacier_2d_spec(dims, seed=1, glen=True, eta_reg=30.0, relax_rate=0.05, lx=2.0, H=1.0, y0=0.0):
+    """Model arguments, cadence and the two polygons of a 2D glacier: the
+    vertical section y = y0 of the seeded synthetic glacier on
+    [-lx/2, lx/2] x [0, H].  na is the region below the ice surface, ns the
+    region above the bed (both closed well outside the grid and its halo)."""
+    from .full_stokes import TBC
+    from .geometries import GaussianGlacier
+    from .physics import PowerLawRheology, Relaxation, ViscoplasticRegularisation
+
+    dims = tuple(dims)
+    gl = GaussianGlacier(seed=seed, lx=lx, ly=lx, H=H)
+    grid = UniformGrid(origin=(-lx / 2, 0.0), extent=(lx, H), dims=dims)
+    tr = TBC(0.0, 0.0)
+    if glen:
+        rheology = Relaxation("eta", ViscoplasticRegularisation(PowerLawRheology(1.0, 3), eta_reg), relax_rate,
+                              scale="log")
+    else:
+        rheology = LinearViscousRheology(1.0)
+    kw = dict(grid=grid, boundary_conditions=dict(x=(tr, tr), y=(tr, tr)), gravity=dict(x=0.0, y=1.0),
+              rheology=rheology, solver_params=pt_params(grid, r=0.9, re_mech=5 * PI, vdtau_div=1.1))
+    pad = 3 * max(lx / dims[0], H / dims[1])
+    xs = np.linspace(-lx / 2 - pad, lx / 2 + pad, 2 * dims[0] + 1)
+    ys = np.full_like(xs, y0)
+    surf = np.stack([xs, gl.surface(xs, ys)], axis=1)
+    bed = np.stack([xs, gl.bed(xs, ys)], axis=1)
+    na = np.concatenate([[[-lx / 2 - pad, -pad]], surf, [[lx / 2 + pad, -pad]]])
+    ns = np.concatenate([[[-lx / 2 - pad, H + pad]], bed, [[lx / 2 + pad, H + pad]]])
+    n = max(dims)
+    return kw, dict(niter=50 * n, ncheck=5 * n, glacier=gl, poly_na=na, poly_ns=ns)
+
+
+def glacier_2d(dims, seed=1, glen=True, **kw_spec):
+    """2D immersed-boundary glacier section (glacier_2d_spec) with its level
+    sets and volume fractions set up.  Returns (model, cadence)."""
+    from .full_stokes import IsothermalImmersedBoundaryFullStokesModel
+    from .level_sets import compute_levelset_from_polygon, omega_from_psi
+
+    kw, cad = glacier_2d_spec(dims, seed, glen, **kw_spec)
+    model = IsothermalImmersedBoundaryFullStokesModel(**kw)
+    compute_levelset_from_polygon(model.level_sets.na, cad["poly_na"])
+    compute_levelset_from_polygon(model.level_sets.ns, cad["poly_ns"])
+    omega_from_psi(model.field_masks.ns, model.level_sets.ns)
+    omega_from_psi(model.field_masks.na, model.level_sets.na)
+    model.viscosity.eta.set(1.0)
+    model.viscosity.eta_next.set(1.0)
+    model.apply_bc("eta")
+    return model, cad

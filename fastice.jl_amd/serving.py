@@ -2,10 +2,10 @@
 
 No counterpart in the reference (its drivers solve one model per run,
 scripts_future_API/tm_stokes_chmy.jl:96-108).  Requests arrive as a Poisson
-process; each is a 2D manufactured free-slip problem
-(stokes_manufactured_solution_free_slip_2D.jl) or a 3D seeded synthetic
-glacier (test_vavilov.jl set-up) solved in FP64 to a residual tolerance with
-the reference's check cadence.  The server
+process; each is a seeded synthetic glacier, 3D (test_vavilov.jl set-up) or
+a 2D section of one (polygon level sets), solved in FP64 to a residual
+tolerance with the reference's glacier cadence (niter = 50n, ncheck = 5n).
+The server
 
   * admits a waiting request when a batch slot is free and the device has the
     model's bytes (fi_model_bytes) plus a reserve free (hipMemGetInfo),
@@ -23,7 +23,7 @@ import time
 import numpy as np
 
 from .batch import Batch, model_bytes
-from .problems import glacier_3d, glacier_3d_spec, manufactured_2d
+from .problems import glacier_2d, glacier_2d_spec, glacier_3d, glacier_3d_spec
 
 
 class Request:
@@ -38,8 +38,8 @@ class Request:
 
 def poisson_requests(n, rate, seed=0, p3d=0.5, range2d=(64, 511), range3d=(32, 127)):
     """n requests with exponential inter-arrival times (rate per second); 3D
-    with probability p3d; sizes log-uniform in range2d (n x n) or range3d
-    (n x n x ceil(n/2))."""
+    with probability p3d; sizes log-uniform in range2d (n x ceil(n/2)) or
+    range3d (n x n x ceil(n/2))."""
     rng = np.random.default_rng(seed)
     t = np.cumsum(rng.exponential(1.0 / rate, n))
     out = []
@@ -47,31 +47,24 @@ def poisson_requests(n, rate, seed=0, p3d=0.5, range2d=(64, 511), range3d=(32, 1
         three = rng.random() < p3d
         lo, hi = range3d if three else range2d
         s = int(round(math.exp(rng.uniform(math.log(lo), math.log(hi)))))
-        dims = (s, s, (s + 1) // 2) if three else (s, s)
+        dims = (s, s, (s + 1) // 2) if three else (s, (s + 1) // 2)
         out.append(Request(i, float(t[i]), 3 if three else 2, dims, int(rng.integers(1, 2 ** 31))))
     return out
 
 
 def request_spec(req):
     """(model arguments, immersed boundary, ncheck, max_iter) of a request:
-    the reference drivers' cadences, glacier niter = 50 n and ncheck = 5 n
-    (test_vavilov.jl), manufactured ncheck = n with a 200 n cap."""
-    if req.ndims == 3:
-        kw, cad = glacier_3d_spec(req.dims, seed=req.seed, glen=True)
-        return kw, True, cad["ncheck"], cad["niter"]
-    kw, cad = manufactured_2d(req.dims)
-    return kw, False, cad["ncheck"], 200 * max(req.dims)
+    the glacier driver's cadence, niter = 50 n and ncheck = 5 n
+    (test_vavilov.jl)."""
+    spec = glacier_3d_spec if req.ndims == 3 else glacier_2d_spec
+    kw, cad = spec(req.dims, seed=req.seed, glen=True)
+    return kw, True, cad["ncheck"], cad["niter"]
 
 
 def build_model(pkg, req):
     """The request's model with its initial state (on its own stream)."""
-    if req.ndims == 3:
-        m, _ = glacier_3d(req.dims, seed=req.seed, glen=True)
-        return m
-    kw, _ = manufactured_2d(req.dims)
-    m = pkg.IsothermalFullStokesModel(**kw)
-    m.viscosity.eta.set(1.0)
-    m.viscosity.eta_next.set(1.0)
+    build = glacier_3d if req.ndims == 3 else glacier_2d
+    m, _ = build(req.dims, seed=req.seed, glen=True)
     return m
 
 
@@ -135,19 +128,22 @@ class Server:
                 m.close()
         return len(done)
 
-    def run(self, requests, realtime=True, log=None):
+    def run(self, requests, realtime=True, log=None, offline=False):
         """Serve requests (sorted by arrival).  realtime: requests become
         eligible at their arrival time; otherwise one becomes eligible per
-        batch step (a deterministic interleaving for tests).  Returns the
-        summary; per-request records are in self.results."""
+        batch step (a deterministic interleaving for tests); offline: all are
+        eligible at once (the offline batched throughput of the same mix).
+        Returns the summary; per-request records are in self.results."""
         reqs = sorted(requests, key=lambda r: r.t_arrival)
+        if offline:
+            reqs = [Request(r.rid, 0.0, r.ndims, r.dims, r.seed) for r in reqs]
         t0 = time.perf_counter()
         nxt = 0
         waiting = []
         steps = 0
         while True:
             now = time.perf_counter() - t0
-            if realtime:
+            if realtime or offline:
                 while nxt < len(reqs) and reqs[nxt].t_arrival <= now:
                     waiting.append(reqs[nxt])
                     nxt += 1

@@ -194,6 +194,12 @@ int fi_fused_variants(int fp32, int* tx_ty_minw, int max_variants);
 
 int fi_levelset_from_dem(fi_model_t m, int phase, const double* dem_parent, int dem_nx, int dem_ny,
                          const double dem_origin[2], const double dem_extent[2], const double R[9]);
+/* 2D level set of a closed polygon (npts points, x y pairs): signed distance
+ * of every vertex, negative inside (even-odd rule), then Neumann on psi.
+ * Replaces the 2D distance sd_poly (src/LevelSets/signed_distances.jl:3-19),
+ * the 2D counterpart of compute_levelset_from_dem! (compute_levelsets.jl:16-25);
+ * the reference defines it but no 2D driver calls it. */
+int fi_levelset_from_polygon(fi_model_t m, int phase, const double* xy, int npts);
 int fi_invert_levelset(fi_model_t m, int phase);
 int fi_omega_from_psi(fi_model_t m, int phase);
 
@@ -250,16 +256,14 @@ int fi_batch_solve(fi_batch_t b);
  * a failed model (non-finite residual) reports FI_BATCH_FAILED */
 int fi_batch_status(fi_batch_t b, fi_model_t m, int* state, int* iters, double norms_inf[4]);
 /* FI_BOPT_VARIANT: fused sweep variant (index into fi_fused_variants(0)),
- * FI_BOPT_ZCHUNK: z planes per tile, FI_BOPT_AUTOTUNE 1/0: time every
- * (variant, z chunk) candidate on the first step of a new working set and
- * keep the fastest (default 1), FI_BOPT_GRAPH_ITERS: iterations per graph
+ * FI_BOPT_ZCHUNK: z planes per tile, FI_BOPT_AUTOTUNE 1/0: time every FI_BOPT_GRAPH_ITERS: iterations per graph
  * replay (0: no graph), FI_BOPT_TILE_ORDER: as FI_OPT_TILE_ORDER.  Results do
  * not depend on any of them. */
 enum { FI_BOPT_VARIANT = 1, FI_BOPT_ZCHUNK = 2, FI_BOPT_AUTOTUNE = 3, FI_BOPT_GRAPH_ITERS = 4, FI_BOPT_TILE_ORDER = 5 };
 int fi_batch_set_option(fi_batch_t b, int key, int value);
 /* out = {attached, running, fused work items, (32 * tile lanes + tile rows)
- * * 16 + min waves, z planes, graph iterations, autotune runs, table
- * rebuilds} */
+ * * 16 + min waves, 4 * z planes + tile order, graph iterations, autotune
+ * runs, table rebuilds} */
 int fi_batch_info(fi_batch_t b, int out[8]);
 
 #ifdef __cplusplus

@@ -1128,6 +1128,43 @@ void or_levelset_from_dem(const or_model* m, double* psi, const double* dem, con
     or_bc_neumann_vertex(m, psi);
 }
 
+/* sd_poly: signed distance of p to a closed polygon, negative inside
+ * (even-odd rule), signed_distances.jl:3-19 */
+double or_sd_poly(double px, double py, const double* xy, int npts)
+{
+    double dx0 = px - xy[0], dy0 = py - xy[1];
+    double d = dx0 * dx0 + dy0 * dy0;
+    double s = 1.0;
+    int j = npts - 1;
+    for (int i = 0; i < npts; ++i) {
+        const double ex = xy[2 * j] - xy[2 * i], ey = xy[2 * j + 1] - xy[2 * i + 1];
+        const double wx = px - xy[2 * i], wy = py - xy[2 * i + 1];
+        double t = (wx * ex + wy * ey) / (ex * ex + ey * ey);
+        t = t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t);
+        const double bx = wx - ex * t, by = wy - ey * t;
+        const double bb = bx * bx + by * by;
+        d = bb < d ? bb : d;
+        const int c1 = py >= xy[2 * i + 1], c2 = py < xy[2 * j + 1], c3 = ex * wy > ey * wx;
+        if ((c1 && c2 && c3) || (!c1 && !c2 && !c3)) s = -s;
+        j = i;
+    }
+    return s * sqrt(d);
+}
+
+/* 2D level set of a polygon at every vertex, then Neumann on psi (the 2D
+ * counterpart of compute_levelsets.jl:16-25 with sd_poly as the distance) */
+void or_levelset_from_polygon(const or_model* m, double* psi, const double* xy, int npts)
+{
+    const long sy = m->sy;
+    for (int j = 1; j <= m->n[1] + 1; ++j)
+        for (int i = 1; i <= m->n[0] + 1; ++i) {
+            const double x = m->origin[0] + (double)(i - 1) * m->dx[0];
+            const double y = m->origin[1] + (double)(j - 1) * m->dx[1];
+            psi[i + sy * j] = or_sd_poly(x, y, xy, npts);
+        }
+    or_bc_neumann_vertex(m, psi);
+}
+
 void or_invert_levelset(const or_model* m, double* psi)
 {
     /* compute_levelsets.jl:10-14,32-35 */

@@ -32,6 +32,9 @@
  *   or_omega_from_psi       rule, O(h^2) sphere volume
  *   or_levelset_from_dem,   flat and inclined DEMs exactly, brute-force
  *   or_invert_levelset      distance over every DEM triangle
+ *   or_sd_poly,             distances to axis-aligned squares in closed
+ *   or_levelset_from_polygon form, brute-force point-segment distances and a
+ *                           winding-number inside test on random polygons
  * Parity unpinned (choices this library specifies where the reference
  * delegates to unvendored Chmy.jl and no check can reach the reference's own
  * bits): the ghost values at edges and corners that the x-then-y-then-z
@@ -125,6 +128,8 @@ void or_levelset_from_dem(const or_model* m, double* psi, const double* dem,
                           const int dem_n[2], const double dem_origin[2],
                           const double dem_extent[2], const double R[9]);
 void or_invert_levelset(const or_model* m, double* psi);
+double or_sd_poly(double px, double py, const double* xy, int npts);
+void or_levelset_from_polygon(const or_model* m, double* psi, const double* xy, int npts);
 
 #ifdef __cplusplus
 }

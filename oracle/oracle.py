@@ -102,6 +102,8 @@ def lib():
             "or_psi2omega3": (C.c_double, [_DP, _DP]),
             "or_psi2omega2": (C.c_double, [_DP, _DP]),
             "or_levelset_from_dem": (None, [M, _DP, _DP, C.POINTER(C.c_int), _DP, _DP, _DP]),
+            "or_levelset_from_polygon": (None, [M, _DP, _DP, C.c_int]),
+            "or_sd_poly": (C.c_double, [C.c_double, C.c_double, _DP, C.c_int]),
             "or_invert_levelset": (None, [M, _DP]),
         }
         for name, (res, args) in sigs.items():
@@ -126,6 +128,12 @@ def psi2omega3(psi8, d):
     p = np.ascontiguousarray(psi8, dtype=np.float64)
     dd = np.ascontiguousarray(d, dtype=np.float64)
     return lib().or_psi2omega3(_p(p), _p(dd))
+
+
+def sd_poly(p, xy):
+    """or_sd_poly: signed distance of point p to the closed polygon xy (n x 2)."""
+    pts = np.ascontiguousarray(np.asarray(xy, dtype=np.float64).reshape(-1, 2))
+    return lib().or_sd_poly(float(p[0]), float(p[1]), _p(pts), pts.shape[0])
 
 
 def psi2omega2(psi4, d):
@@ -288,6 +296,11 @@ class OracleModel:
         e = np.asarray(dem_extent, dtype=np.float64)
         r = None if R is None else np.ascontiguousarray(np.asarray(R, dtype=np.float64).reshape(9))
         lib().or_levelset_from_dem(C.byref(self.m), _p(psi), _p(dem), n, _p(o), _p(e), _p(r))
+
+    def levelset_from_polygon(self, phase, xy):
+        psi = self.arrays["PSI_NA" if phase == 0 else "PSI_NS"]
+        pts = np.ascontiguousarray(np.asarray(xy, dtype=np.float64).reshape(-1, 2))
+        lib().or_levelset_from_polygon(C.byref(self.m), _p(psi), _p(pts), pts.shape[0])
 
     def invert_levelset(self, phase):
         lib().or_invert_levelset(C.byref(self.m), _p(self.arrays["PSI_NA" if phase == 0 else "PSI_NS"]))

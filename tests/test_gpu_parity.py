@@ -276,3 +276,33 @@ def test_every_fused_variant_equals_pointwise(pkg, order):
                 g = m.field(n).parent()
                 assert np.array_equal(g, f), (make.__name__, cfg, n, float(np.nanmax(np.abs(g - f))))
             m.close()
+
+
+def test_glacier_2d_levelsets_fractions_iterations(pkg, orc):
+    _say("case: 2D glacier section 96x40: polygon level sets, volume fractions, 300 IB iterations")
+    kw, cad = pkg.glacier_2d_spec((96, 40), seed=4, glen=True)
+    model, ref = make_pair(pkg, orc, kw, ib=True)
+    pkg.compute_levelset_from_polygon(model.level_sets.na, cad["poly_na"])
+    pkg.compute_levelset_from_polygon(model.level_sets.ns, cad["poly_ns"])
+    ref.levelset_from_polygon(0, cad["poly_na"])
+    ref.levelset_from_polygon(1, cad["poly_ns"])
+    for name in ("PSI_NA", "PSI_NS"):
+        assert rel_err(model.field(name).parent(), ref.get(name)) <= TOL, name
+    pkg.omega_from_psi(model.field_masks.ns, model.level_sets.ns)
+    pkg.omega_from_psi(model.field_masks.na, model.level_sets.na)
+    ref.omega_from_psi(1)
+    ref.omega_from_psi(0)
+    for ph, base in (("NA", pkg._abi.FI_F_OM_NA0), ("NS", pkg._abi.FI_F_OM_NS0)):
+        for q in range(4):
+            g = pkg.full_stokes.DeviceField(model, base + q).parent()
+            assert rel_err(g, ref.get("OM_%s%d" % (ph, q))) <= TOL, (ph, q)
+    om = ref.get("OM_NA0")
+    assert 0.05 < om[1:97, 1:41].mean() < 0.95  # the section cuts the ice
+    for m in (model, ref):
+        m.apply_bc("eta")
+    model.advance(300)
+    ref.advance(300)
+    worst, bad = compare_state(model, ref, state_names(2) + ["ETA"], TOL)
+    assert not bad, worst
+    np.testing.assert_allclose(model.compute_residuals(), ref.compute_residuals(), rtol=TOL, atol=0)
+    model.close()

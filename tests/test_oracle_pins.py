@@ -435,3 +435,69 @@ def test_residual_norm_propagates_nan(pkg, orc):
     P[3, 3, 3] = np.nan
     norms = ref.compute_residuals()
     assert np.isnan(norms[1])
+
+
+def _seg_dist(p, a, b):
+    """Point-segment distance from the endpoint distances and the
+    perpendicular distance |e x w| / |e| when the foot lies on the segment."""
+    e, w = b - a, p - a
+    le = math.hypot(e[0], e[1])
+    along = (e[0] * w[0] + e[1] * w[1]) / le
+    if along <= 0:
+        return math.hypot(w[0], w[1])
+    if along >= le:
+        return math.hypot(p[0] - b[0], p[1] - b[1])
+    return abs(e[0] * w[1] - e[1] * w[0]) / le
+
+
+def _winding(p, poly):
+    """Total signed angle swept around p by the polygon's edges (2 pi inside, 0 outside)."""
+    ang = 0.0
+    for i in range(len(poly)):
+        a, b = poly[i] - p, poly[(i + 1) % len(poly)] - p
+        ang += math.atan2(a[0] * b[1] - a[1] * b[0], a[0] * b[0] + a[1] * b[1])
+    return ang
+
+
+def test_sd_poly_closed_forms_and_brute_force(orc):
+    sq = np.array([[0.0, 0.0], [1.0, 0.0], [1.0, 1.0], [0.0, 1.0]])
+    # inside: minus the distance to the nearest side; outside: to the side or the corner
+    assert abs(orc.sd_poly((0.3, 0.4), sq) - (-0.3)) < 1e-15
+    assert abs(orc.sd_poly((0.5, 0.9), sq) - (-0.1)) < 1e-15
+    assert abs(orc.sd_poly((1.5, 0.5), sq) - 0.5) < 1e-15
+    assert abs(orc.sd_poly((1.3, 1.4), sq) - 0.5) < 1e-15
+    assert abs(orc.sd_poly((-0.6, -0.8), sq) - 1.0) < 1e-15
+    # random star-shaped polygons: |psi| = brute-force segment distance, sign from the winding number
+    rng = np.random.default_rng(21)
+    for trial in range(6):
+        n = int(rng.integers(3, 12))
+        ang = np.sort(rng.uniform(0, 2 * math.pi, n))
+        rad = rng.uniform(0.3, 1.0, n)
+        poly = np.stack([0.1 + rad * np.cos(ang), -0.2 + rad * np.sin(ang)], axis=1)
+        for p in rng.uniform(-1.3, 1.3, (40, 2)):
+            d = min(_seg_dist(p, poly[i], poly[(i + 1) % n]) for i in range(n))
+            inside = abs(_winding(p, poly)) > math.pi
+            got = orc.sd_poly(p, poly)
+            assert abs(abs(got) - d) < 1e-12, (trial, p)
+            if d > 1e-9:
+                assert (got < 0) == inside, (trial, p)
+
+
+def test_levelset_from_polygon_half_plane(pkg, orc):
+    """A polygon whose only edge near the grid is the line y = h: psi = y - h
+    exactly, with Neumann halos."""
+    grid = pkg.UniformGrid(origin=(-1, 0), extent=(2, 1), dims=(12, 9))
+    tr = pkg.TBC(0.0, 0.0)
+    kw = dict(grid=grid, boundary_conditions=dict(x=(tr, tr), y=(tr, tr)), gravity=dict(x=0.0, y=1.0),
+              rheology=pkg.LinearViscousRheology(1.0), solver_params=pkg.pt_params(grid))
+    ref = _oracle(pkg, orc, kw, ib=True)
+    h = 0.37
+    poly = np.array([[-50.0, -50.0], [50.0, -50.0], [50.0, h], [-50.0, h]])
+    ref.levelset_from_polygon(0, poly)
+    X, Y = grid.parent_coords(0b11)
+    psi = ref.get("PSI_NA")
+    inner = (slice(1, 14), slice(1, 11))
+    assert np.allclose(psi[inner], (Y - h)[inner], atol=1e-14)
+    # Neumann vertex ghosts mirror the node next to the boundary node
+    assert np.array_equal(psi[0, 1:11], psi[2, 1:11]) and np.array_equal(psi[14, 1:11], psi[12, 1:11])
+    assert np.array_equal(psi[1:14, 0], psi[1:14, 2]) and np.array_equal(psi[1:14, 11], psi[1:14, 9])
