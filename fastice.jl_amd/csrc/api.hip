@@ -669,6 +669,9 @@ int fi_set_option(fi_model_t m, int key, int value)
             m->fty = cfg[value][1];
             m->fminw = cfg[value][2];
         }
+    } else if (key == FI_OPT_REFINE_ANDERSON) {
+        if (value < 0 || value > 3) return fail(FI_ERR_ARG, "Anderson depth must be in 0..3");
+        m->refine_anderson = value;
     } else if (key == FI_OPT_TILE_ORDER) {
         if (value < 0 || value > 2) return fail(FI_ERR_ARG, "tile order must be 0, 1 or 2");
         m->forder = value;

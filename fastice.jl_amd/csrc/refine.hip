@@ -15,6 +15,14 @@
 // inner_rtol; finally x += d in FP64.  Because every operator is linear in the
 // state for frozen eta, x + d satisfies the FP64 equations up to the inner
 // tolerance and FP32 rounding, and the outer loop removes what remains.
+// With a nonlinear rheology the outer loop is a Picard iteration on eta; its
+// updates are mixed by Anderson acceleration: with the corrections f_j and
+// the applied updates u_j of the last m steps, the update is
+//   u_k = f_k - sum_j g_j (u_j + f_{j+1} - f_j),
+//   g = argmin || f_k - sum_j g_j (f_{j+1} - f_j) ||   (weighted L2)
+// (Walker & Ni 2011, type II with unit mixing), restarted when a step fails
+// to reduce the outer residual.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -46,6 +54,14 @@ struct Refine {
     int mask_gen = -1; // model mask generation the FP32 masks were cast from
     unsigned long long* d_norm = nullptr;
     unsigned long long* h_norm = nullptr;
+    // Anderson history: previous correction f_{k-1} and, per ring slot, the
+    // applied update u_j (dX) and the correction difference f_{j+1} - f_j (dF)
+    static constexpr int AA_SLOTS = 4;
+    float* aa_f[10] = {};
+    float* aa_x[AA_SLOTS][10] = {};
+    float* aa_df[AA_SLOTS][10] = {};
+    double* d_part = nullptr; // per-block partial dot products
+    double* h_part = nullptr;
 
     float* cur(int id) { return d[id][(id <= FI_F_TYZ ? sig_cur : v_cur)]; }
     float* alt(int id) { return d[id][1 - (id <= FI_F_TYZ ? sig_cur : v_cur)]; }
@@ -75,6 +91,74 @@ __global__ __launch_bounds__(256) void k_add_correction(double* x, const float* 
 {
     for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
         x[t] = x[t] + (double)d[t];
+}
+
+// Anderson: per-block partial sums of dF_a . dF_b (a <= b < c) and dF_a . f
+// for one field, in a fixed block order (deterministic after the host sum)
+constexpr int AA_DOT_BLOCKS = 512;
+constexpr int AA_MAXC = 3;
+constexpr int AA_NDOT = AA_MAXC * (AA_MAXC + 1) / 2 + AA_MAXC + 1;
+
+struct AaDotArgs {
+    const float* df[AA_MAXC];
+    const float* f;
+    int c; // columns in use
+};
+
+__global__ __launch_bounds__(256) void k_aa_dots(AaDotArgs a, size_t n, double* part)
+{
+    double acc[AA_NDOT];
+    for (int q = 0; q < AA_NDOT; ++q) acc[q] = 0.0;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+        double v[AA_MAXC];
+        for (int q = 0; q < AA_MAXC; ++q) v[q] = q < a.c ? (double)a.df[q][t] : 0.0;
+        const double fv = (double)a.f[t];
+        int o = 0;
+        for (int p = 0; p < AA_MAXC; ++p)
+            for (int q = p; q < AA_MAXC; ++q) acc[o++] += v[p] * v[q];
+        for (int p = 0; p < AA_MAXC; ++p) acc[o++] += v[p] * fv;
+        acc[o] += fv * fv;
+    }
+    __shared__ double red[256];
+    for (int q = 0; q < AA_NDOT; ++q) {
+        red[threadIdx.x] = acc[q];
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) part[(size_t)blockIdx.x * AA_NDOT + q] = red[0];
+        __syncthreads();
+    }
+}
+
+// dF_new = f - f_prev; f_prev = f (before the mixing)
+__global__ __launch_bounds__(256) void k_aa_diff(float* df, float* fprev, const float* f, size_t n, int have_prev)
+{
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+        const float fv = f[t];
+        if (have_prev) df[t] = fv - fprev[t];
+        fprev[t] = fv;
+    }
+}
+
+// u = f - sum_j g_j (dX_j + dF_j); x += u; the update is kept as the next dX
+struct AaMixArgs {
+    const float* dx[AA_MAXC];
+    const float* df[AA_MAXC];
+    float g[AA_MAXC];
+    int c;
+};
+
+__global__ __launch_bounds__(256) void k_aa_mix(AaMixArgs a, const float* f, float* u_out, double* x, size_t n)
+{
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+        float u = f[t];
+        for (int q = 0; q < AA_MAXC; ++q)
+            if (q < a.c) u -= a.g[q] * (a.dx[q][t] + a.df[q][t]);
+        u_out[t] = u;
+        x[t] = x[t] + (double)u;
+    }
 }
 
 static dim3 ew_grid(size_t n)
@@ -132,6 +216,7 @@ void free_refine(fi_model_s* m)
     if (!r) return;
     for (void* p : r->allocs) (void)hipFree(p);
     if (r->h_norm) (void)hipHostFree(r->h_norm);
+    if (r->h_part) (void)hipHostFree(r->h_part);
     delete r;
     m->rf = nullptr;
 }
@@ -334,6 +419,142 @@ static void widen(fi_model_s* m, double* dst, const float* src)
     hipLaunchKernelGGL(k_widen, ew_grid(m->g.elems), dim3(256), 0, m->stream, dst - m->g.off, src - m->g.off, m->g.elems);
 }
 
+struct AaState {
+    int depth = 0;     // columns kept (0: plain defect correction)
+    int cols = 0;      // columns in use
+    int have_prev = 0; // a previous correction f_{k-1} is stored
+    int prev_slot = 0; // ring slot of the last applied update u_{k-1}
+};
+
+static int ensure_anderson(fi_model_s* m, int depth)
+{
+    Refine* r = m->rf;
+    int rc = FI_OK;
+    for (int id = FI_F_P; id <= FI_F_VZ && !rc; ++id) {
+        if (!m->f[id]) continue;
+        if (!r->aa_f[id]) rc = alloc32(m, &r->aa_f[id]);
+        for (int sl = 0; sl <= depth && !rc; ++sl) {
+            if (!r->aa_x[sl][id]) rc = alloc32(m, &r->aa_x[sl][id]);
+            if (!rc && !r->aa_df[sl][id]) rc = alloc32(m, &r->aa_df[sl][id]);
+        }
+    }
+    if (!rc && !r->d_part) {
+        const size_t bytes = (size_t)AA_DOT_BLOCKS * AA_NDOT * sizeof(double);
+        if (hipMalloc(&r->d_part, bytes) != hipSuccess || hipHostMalloc(&r->h_part, bytes, hipHostMallocDefault) != hipSuccess)
+            return fail(FI_ERR_OOM, "Anderson workspace");
+        r->allocs.push_back(r->d_part);
+    }
+    return rc;
+}
+
+// Mix the new correction (in the current correction buffers) with the
+// history and add the update to the FP64 state.  restart: drop the history.
+static int anderson_apply(fi_model_s* m, AaState& st, bool restart)
+{
+    Refine* r = m->rf;
+    const size_t n = m->g.elems;
+    const int slots = st.depth + 1;
+    if (restart) st.cols = 0;
+    // new column: the update of the previous step and f_k - f_{k-1} in its slot
+    for (int id = FI_F_P; id <= FI_F_VZ; ++id)
+        if (m->f[id])
+            hipLaunchKernelGGL(k_aa_diff, ew_grid(n), dim3(256), 0, m->stream, r->aa_df[st.prev_slot][id] - m->g.off,
+                               r->aa_f[id] - m->g.off, r->cur(id) - m->g.off, n, st.have_prev);
+    if (st.have_prev && !restart) st.cols = st.cols + 1 < st.depth ? st.cols + 1 : st.depth;
+    const int c = st.cols;
+    int col[AA_MAXC];
+    for (int q = 0; q < c; ++q) col[q] = ((st.prev_slot - q) % slots + slots) % slots;
+    float gam[AA_MAXC] = {0.0f, 0.0f, 0.0f};
+    int used = 0;
+    if (c > 0) {
+        // weighted least squares over the fields: stress and velocity groups
+        // each weighted by the inverse of their |f|^2
+        double G[AA_MAXC][AA_MAXC] = {}, b[AA_MAXC] = {}, ff[2] = {0.0, 0.0};
+        double Gf[2][AA_MAXC][AA_MAXC] = {}, bf[2][AA_MAXC] = {};
+        for (int id = FI_F_P; id <= FI_F_VZ; ++id) {
+            if (!m->f[id]) continue;
+            AaDotArgs a;
+            for (int q = 0; q < AA_MAXC; ++q) a.df[q] = q < c ? r->aa_df[col[q]][id] - m->g.off : r->cur(id) - m->g.off;
+            a.f = r->cur(id) - m->g.off;
+            a.c = c;
+            hipLaunchKernelGGL(k_aa_dots, dim3(AA_DOT_BLOCKS), dim3(256), 0, m->stream, a, n, r->d_part);
+            HIPCHK(hipMemcpyAsync(r->h_part, r->d_part, (size_t)AA_DOT_BLOCKS * AA_NDOT * sizeof(double),
+                                  hipMemcpyDeviceToHost, m->stream));
+            HIPCHK(hipStreamSynchronize(m->stream));
+            double dots[AA_NDOT] = {};
+            for (int bl = 0; bl < AA_DOT_BLOCKS; ++bl)
+                for (int q = 0; q < AA_NDOT; ++q) dots[q] += r->h_part[(size_t)bl * AA_NDOT + q];
+            const int grp = fi_model_s::is_vel(id) ? 1 : 0;
+            int o = 0;
+            for (int p = 0; p < AA_MAXC; ++p)
+                for (int q = p; q < AA_MAXC; ++q, ++o)
+                    if (p < c && q < c) {
+                        Gf[grp][p][q] += dots[o];
+                        if (p != q) Gf[grp][q][p] += dots[o];
+                    }
+            for (int p = 0; p < AA_MAXC; ++p, ++o)
+                if (p < c) bf[grp][p] += dots[o];
+            ff[grp] += dots[o];
+        }
+        for (int grp = 0; grp < 2; ++grp) {
+            const double w = ff[grp] > 0.0 ? 1.0 / ff[grp] : 0.0;
+            for (int p = 0; p < c; ++p) {
+                b[p] += w * bf[grp][p];
+                for (int q = 0; q < c; ++q) G[p][q] += w * Gf[grp][p][q];
+            }
+        }
+        // Tikhonov-regularised normal equations, Gauss elimination
+        double dmax = 0.0;
+        for (int p = 0; p < c; ++p) dmax = G[p][p] > dmax ? G[p][p] : dmax;
+        for (int p = 0; p < c; ++p) G[p][p] += 1e-10 * dmax;
+        double x[AA_MAXC] = {};
+        bool ok = dmax > 0.0;
+        for (int p = 0; p < c && ok; ++p) {
+            int piv = p;
+            for (int q = p + 1; q < c; ++q)
+                if (std::fabs(G[q][p]) > std::fabs(G[piv][p])) piv = q;
+            if (!(std::fabs(G[piv][p]) > 0.0)) { ok = false; break; }
+            for (int q = 0; q < c; ++q) std::swap(G[p][q], G[piv][q]);
+            std::swap(b[p], b[piv]);
+            for (int q = p + 1; q < c; ++q) {
+                const double f = G[q][p] / G[p][p];
+                for (int t = p; t < c; ++t) G[q][t] -= f * G[p][t];
+                b[q] -= f * b[p];
+            }
+        }
+        for (int p = c - 1; p >= 0 && ok; --p) {
+            double v = b[p];
+            for (int q = p + 1; q < c; ++q) v -= G[p][q] * x[q];
+            x[p] = v / G[p][p];
+            if (!std::isfinite(x[p]) || std::fabs(x[p]) > 1e3) ok = false;
+        }
+        if (ok) {
+            used = c;
+            for (int p = 0; p < c; ++p) gam[p] = (float)x[p];
+        } else {
+            st.cols = 0;
+        }
+    }
+    const int uslot = (st.prev_slot + 1) % slots;
+    for (int id = FI_F_P; id <= FI_F_VZ; ++id) {
+        if (!m->f[id]) continue;
+        AaMixArgs a;
+        for (int q = 0; q < AA_MAXC; ++q) {
+            const int sl = q < used ? col[q] : uslot;
+            a.dx[q] = r->aa_x[sl][id] - m->g.off;
+            a.df[q] = r->aa_df[sl][id] - m->g.off;
+            a.g[q] = q < used ? gam[q] : 0.0f;
+        }
+        a.c = used;
+        hipLaunchKernelGGL(k_aa_mix, ew_grid(n), dim3(256), 0, m->stream, a, r->cur(id) - m->g.off,
+                           r->aa_x[uslot][id] - m->g.off, m->buf(id) - m->g.off, n);
+    }
+    st.have_prev = 1;
+    st.prev_slot = uslot;
+    HIPCHK(hipGetLastError());
+    return FI_OK;
+}
+
 // Phase 1 of a nonlinear refined solve: the full nonlinear PT iteration (eta
 // updated every sweep, real boundary values and forcing) on the whole state in
 // FP32, from the FP64 state, until the FP32 residual has dropped by rtol or
@@ -429,6 +650,13 @@ int fi_refined_solve(fi_model_t m, int max_outer, int max_inner, int ncheck, dou
     // outer viscosity relaxation: the configured rate, halved whenever an outer
     // step fails to reduce the residual by 10 %
     double omega = m->refine_eta_rate, r_prev = -1.0;
+    AaState aa;
+    aa.depth = nonlinear ? m->refine_anderson : 0;
+    if (aa.depth > 0) {
+        rc = ensure_anderson(m, aa.depth);
+        if (rc) return rc;
+    }
+    bool aa_restart = false;
     if (nonlinear && m->refine_full_phase) {
         int it = 0;
         rc = full_phase(m, max_inner, ncheck, std::pow(10.0, -m->refine_phase1_digits), &it);
@@ -471,7 +699,10 @@ int fi_refined_solve(fi_model_t m, int max_outer, int max_inner, int ncheck, dou
         if (trace)
             std::fprintf(stderr, "[fastice] refined: outer %d residual %.3e eta rate %.3f FP32 iterations so far %d\n",
                          outer, r0, omega, inner_total);
-        if (r_prev > 0.0 && r0 > 0.9 * r_prev && omega > 1.0 / 64) omega *= 0.5;
+        // (with Anderson mixing the rate stays >= 1/2: a frozen viscosity
+        // update would stall the mixed iteration)
+        if (r_prev > 0.0 && r0 > 0.9 * r_prev && omega > (aa.depth > 0 ? 0.5 : 1.0 / 64)) omega *= 0.5;
+        aa_restart = r_prev > 0.0 && r0 > r_prev; // the last mixed update did not help
         r_prev = r0;
         if (r0 < tol || outer >= max_outer) break;
         // correction problem: frozen eta, forcing from the FP64 defect, zero state
@@ -500,10 +731,15 @@ int fi_refined_solve(fi_model_t m, int max_outer, int max_inner, int ncheck, dou
             if (rin <= inner_rtol * r0) break;
         }
         inner_total += done;
-        for (int id = FI_F_P; id <= FI_F_VZ; ++id)
-            if (m->f[id])
-                hipLaunchKernelGGL(k_add_correction, ew_grid(m->g.elems), dim3(256), 0, m->stream, m->buf(id) - m->g.off,
-                                   r->cur(id) - m->g.off, m->g.elems);
+        if (aa.depth > 0) {
+            rc = anderson_apply(m, aa, aa_restart);
+            if (rc) return rc;
+        } else {
+            for (int id = FI_F_P; id <= FI_F_VZ; ++id)
+                if (m->f[id])
+                    hipLaunchKernelGGL(k_add_correction, ew_grid(m->g.elems), dim3(256), 0, m->stream,
+                                       m->buf(id) - m->g.off, r->cur(id) - m->g.off, m->g.elems);
+        }
         m->shadow_dirty = true;
     }
     if (counts_out) {

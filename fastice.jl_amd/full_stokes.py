@@ -337,7 +337,8 @@ class IsothermalFullStokesModel:
         k = {"fused": A.FI_OPT_FUSED, "zchunk": A.FI_OPT_ZCHUNK, "refine_full_phase": A.FI_OPT_REFINE_FULL_PHASE,
              "refine_eta_rate_permil": A.FI_OPT_REFINE_ETA_RATE,
              "refine_phase1_digits": A.FI_OPT_REFINE_PHASE1_DIGITS, "fused_variant": A.FI_OPT_FUSED_VARIANT,
-             "fused_variant_fp32": A.FI_OPT_FUSED_VARIANT_FP32, "tile_order": A.FI_OPT_TILE_ORDER}.get(key, key)
+             "fused_variant_fp32": A.FI_OPT_FUSED_VARIANT_FP32, "tile_order": A.FI_OPT_TILE_ORDER,
+             "refine_anderson": A.FI_OPT_REFINE_ANDERSON}.get(key, key)
         A.check(A.lib().fi_set_option(self._h, int(k), int(value)))
 
 

@@ -172,7 +172,9 @@ int fi_fused_region(fi_model_t m, int lo[3], int hi[3]);
  * FI_OPT_REFINE_PHASE1_DIGITS digits of residual drop (1..7, default 3) or
  * at the FP32 floor; FI_OPT_REFINE_ETA_RATE is the initial log-space
  * relaxation of the outer viscosity update in per mille (default 1000),
- * halved whenever an outer step reduces the residual by less than 10 %. */
+ * halved whenever an outer step reduces the residual by less than 10 %;
+ * FI_OPT_REFINE_ANDERSON is the depth (0..3, default 2) of the Anderson
+ * mixing of the nonlinear outer updates (0: plain defect correction). */
 enum {
     FI_OPT_FUSED = 1,
     FI_OPT_ZCHUNK = 2,
@@ -181,7 +183,8 @@ enum {
     FI_OPT_REFINE_PHASE1_DIGITS = 5,
     FI_OPT_FUSED_VARIANT = 6,
     FI_OPT_FUSED_VARIANT_FP32 = 7,
-    FI_OPT_TILE_ORDER = 8
+    FI_OPT_TILE_ORDER = 8,
+    FI_OPT_REFINE_ANDERSON = 9
 };
 int fi_set_option(fi_model_t m, int key, int value);
 /* Compiled variants of the fused sweep (FI_OPT_FUSED_VARIANT* take an index
