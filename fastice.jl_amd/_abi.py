@@ -155,6 +155,8 @@ def _declare(lib):
         "fi_batch_destroy": (C.c_int, [C.c_void_p]),
         "fi_batch_add": (C.c_int, [C.c_void_p, H, C.c_int, C.c_int, C.c_double]),
         "fi_batch_remove": (C.c_int, [C.c_void_p, H]),
+        "fi_batch_add_refined": (C.c_int, [C.c_void_p, H, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double]),
+        "fi_batch_refined_counts": (C.c_int, [C.c_void_p, H, C.POINTER(C.c_int)]),
         "fi_batch_step": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
         "fi_batch_solve": (C.c_int, [C.c_void_p]),
         "fi_batch_status": (C.c_int, [C.c_void_p, H, C.POINTER(C.c_int), C.POINTER(C.c_int), _DP]),

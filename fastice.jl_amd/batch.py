@@ -6,7 +6,10 @@ every attached FP64 model with packed launches (one fused-sweep launch over
 the tiles of all instances, one launch per pointwise shell kind, one per
 boundary dimension) replayed as HIP graphs; each model keeps the semantics of
 IsothermalFullStokesModel.solve exactly, so its result is bitwise the one it
-gets alone.
+gets alone.  A batch is either all FP64 (add) or all FP32-refined
+(add_refined): refined models share the packed FP32 sweeps, and each runs
+its own outer defect-correction steps between chunks, so it also ends
+bitwise equal to IsothermalFullStokesModel.refined_solve alone.
 """
 import ctypes as C
 
@@ -56,6 +59,22 @@ class Batch:
     def add(self, model, max_iter, ncheck, tol):
         A.check(A.lib().fi_batch_add(self._h, model._h, int(max_iter), int(ncheck), float(tol)))
         self._models[id(model)] = model
+
+    def add_refined(self, model, tol, max_outer=100, max_inner=None, ncheck=None, inner_rtol=3e-2):
+        """Attach a model and start its FP32-refined solve (the arguments of
+        IsothermalFullStokesModel.refined_solve)."""
+        n = max(model.grid.dims)
+        ncheck = ncheck or n
+        max_inner = max_inner or 100 * n
+        A.check(A.lib().fi_batch_add_refined(self._h, model._h, int(max_outer), int(max_inner), int(ncheck),
+                                             float(inner_rtol), float(tol)))
+        self._models[id(model)] = model
+
+    def refined_counts(self, model):
+        """{outer, fp32_iterations, nonlinear_phase} of a refined solve."""
+        out = (C.c_int * 3)()
+        A.check(A.lib().fi_batch_refined_counts(self._h, model._h, out))
+        return dict(outer=out[0], fp32_iterations=out[1], nonlinear_phase=out[2])
 
     def remove(self, model):
         A.check(A.lib().fi_batch_remove(self._h, model._h))

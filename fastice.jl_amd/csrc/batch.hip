@@ -11,7 +11,7 @@
 // widened to all instances, so each instance computes exactly what it
 // computes alone.  The kernel arguments of an instance depend on its buffer
 // parity; the tables hold both parities and the iterations alternate them.
-#include "model.h"
+#include "refine.h"
 
 #include <algorithm>
 #include <cmath>
@@ -23,6 +23,9 @@ namespace fi {
 
 struct BatchInst {
     fi_model_s* m = nullptr;
+    bool refined = false; // solved by FP32 sweeps in FP64 defect correction
+    RefineJob job;
+    int left = 0;         // sweeps of the job's pending chunk still to run
     hipStream_t home = nullptr; // the model's own stream, restored on removal
     int max_iter = 0, ncheck = 1;
     double tol = 0.0;
@@ -37,7 +40,7 @@ enum { K_3D_PLAIN = 0, K_3D_IB = 1, K_2D_PLAIN = 2, K_2D_IB = 3 };
 struct BatchTables {
     char* dev = nullptr;
     size_t cap = 0;
-    PackedInst<double>* inst[2] = {};
+    void* inst[2] = {}; // PackedInst<double> (FP64 batches) or PackedInst<float> (refined batches)
     int4* items[2] = {}; // plain, IB
     int nitems[2] = {};
     PackedBox* box[4] = {};
@@ -61,6 +64,7 @@ struct fi_batch_s {
     BatchTables t;
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     int tx = 64, ty = 8, minw = 1, kch = 32; // fused variant and z chunk
+    int refined = -1; // -1: no model yet, 0: FP64 solves, 1: refined solves
     int order = 2;                  // workgroup-to-tile order of the packed fused sweep
     int autotune = 1;
     int graph_iters = 8;
@@ -111,24 +115,6 @@ void batch_detach(fi_model_s* m)
     if (q >= 0) (void)detach(b, q);
 }
 
-// the model's kernel arguments and boundary batches after p iterations
-// (buffer parities flipped p times)
-struct ParityView {
-    fi_model_s* m;
-    int s0, v0, e0;
-    ParityView(fi_model_s* mm, int p) : m(mm), s0(mm->sig_cur), v0(mm->v_cur), e0(mm->eta_cur)
-    {
-        m->sig_cur ^= p;
-        m->v_cur ^= p;
-        m->eta_cur ^= p;
-    }
-    ~ParityView()
-    {
-        m->sig_cur = s0;
-        m->v_cur = v0;
-        m->eta_cur = e0;
-    }
-};
 
 static bool fused_ok(const fi_model_s* m)
 {
@@ -181,11 +167,67 @@ static void add_bc_ops(std::vector<PackedBcOp>& ops, std::vector<int>& blk, cons
     }
 }
 
+// the FP64 sweep of a model (its PT iteration) and the FP32 sweep of a
+// refined model (phase 1 or correction), as the table builder sees them
+template <typename T>
+struct SweepOf;
+
+template <>
+struct SweepOf<double> {
+    using Parity = RefineParity;
+    static Parity get(fi_model_s* m) { return Parity{m->sig_cur, m->v_cur, m->eta_cur}; }
+    static void set(fi_model_s* m, Parity p)
+    {
+        m->sig_cur = p.sig;
+        m->v_cur = p.v;
+        m->eta_cur = p.eta;
+    }
+    static Parity flipped(fi_model_s* m, Parity p, int n)
+    {
+        (void)m;
+        return Parity{p.sig ^ (n & 1), p.v ^ (n & 1), p.eta ^ (n & 1)};
+    }
+    static StokesArgs<double> args(fi_model_s* m) { return make_args(m, false); }
+    static BcBatch bc(fi_model_s* m, int group, int d)
+    {
+        return group == 0 ? stress_batch(m, d) : velocity_batch(m, d, true);
+    }
+    static bool fused(fi_model_s* m, int ty, int kch, FusedGeom& fg, BoxList& shell)
+    {
+        return fused_ok(m) && fused_split(m->g, ty, kch, fg, shell);
+    }
+};
+
+template <>
+struct SweepOf<float> {
+    using Parity = RefineParity;
+    static Parity get(fi_model_s* m) { return refine_parity(m); }
+    static void set(fi_model_s* m, Parity p) { refine_set_parity(m, p); }
+    static Parity flipped(fi_model_s* m, Parity p, int n)
+    {
+        return Parity{p.sig ^ (n & 1), p.v ^ (n & 1), p.eta ^ (refine_full(m) ? (n & 1) : 0)};
+    }
+    static StokesArgs<float> args(fi_model_s* m) { return refine_args(m, false); }
+    static BcBatch bc(fi_model_s* m, int group, int d)
+    {
+        return refine_bc_batch(m, group == 0 ? FI_BCGROUP_STRESS : FI_BCGROUP_VELOCITY, d);
+    }
+    static bool fused(fi_model_s* m, int ty, int kch, FusedGeom& fg, BoxList& shell)
+    {
+        (void)ty;
+        const bool f = refine_fused(m, fg, shell) && m->g.elems * sizeof(float) < (1ull << 32);
+        fg.kch = kch;
+        return f;
+    }
+};
+
 // build the tables of the running set for tile rows ty and z chunk kch
-static int build_tables(fi_batch_s* b, int ty, int kch)
+template <typename T>
+static int build_tables_t(fi_batch_s* b, int ty, int kch)
 {
+    using S = SweepOf<T>;
     const int nrun = (int)b->run.size();
-    std::vector<PackedInst<double>> pinst[2];
+    std::vector<PackedInst<T>> pinst[2];
     std::vector<int4> items[2];
     std::vector<PackedBox> boxes[4];
     std::vector<int> blk2box[4];
@@ -196,28 +238,29 @@ static int build_tables(fi_batch_s* b, int ty, int kch)
         const int kind = kind_of(m);
         FusedGeom fg;
         BoxList shell;
-        bool interior = fused_ok(m) && fused_split(m->g, ty, kch, fg, shell);
+        bool interior = S::fused(m, ty, kch, fg, shell);
         if (!interior) {
             shell = full_box(m->g);
             std::memset(&fg, 0, sizeof(fg));
         }
         fg.order = b->order;
-        if (interior) fused_xtiles(fg, b->tx, 8, m->g.off);
+        if (interior) fused_xtiles(fg, b->tx, (int)sizeof(T), m->g.off);
+        const typename S::Parity p0 = S::get(m);
         for (int p = 0; p < 2; ++p) {
-            ParityView pv(m, p);
-            PackedInst<double> pi;
-            pi.a = make_args(m, false);
+            typename S::Parity pp = S::flipped(m, p0, p);
+            S::set(m, pp);
+            PackedInst<T> pi;
+            pi.a = S::args(m);
             pi.fg = fg;
             pinst[p].push_back(pi);
-            {
-                // stress BCs see the new stress buffers
-                ParityView ps(m, 0);
-                m->sig_cur ^= 1;
-                for (int d = 0; d < m->g.nd; ++d) add_bc_ops(bcops[p][0][d], bcblk[p][0][d], m, stress_batch(m, d));
-                m->v_cur ^= 1;
-                for (int d = 0; d < m->g.nd; ++d)
-                    add_bc_ops(bcops[p][1][d], bcblk[p][1][d], m, velocity_batch(m, d, true));
-            }
+            // stress BCs see the new stress buffers, velocity BCs the new velocity
+            pp.sig ^= 1;
+            S::set(m, pp);
+            for (int d = 0; d < m->g.nd; ++d) add_bc_ops(bcops[p][0][d], bcblk[p][0][d], m, S::bc(m, 0, d));
+            pp.v ^= 1;
+            S::set(m, pp);
+            for (int d = 0; d < m->g.nd; ++d) add_bc_ops(bcops[p][1][d], bcblk[p][1][d], m, S::bc(m, 1, d));
+            S::set(m, p0);
         }
         if (interior) {
             const int ib = m->desc.immersed_boundary ? 1 : 0;
@@ -274,7 +317,7 @@ static int build_tables(fi_batch_s* b, int ty, int kch)
         HIPCHK(hipStreamSynchronize(b->stream));
     }
     if (!h.empty()) HIPCHK(hipMemcpy(t.dev, h.data(), h.size(), hipMemcpyHostToDevice));
-    for (int p = 0; p < 2; ++p) t.inst[p] = reinterpret_cast<PackedInst<double>*>(t.dev + o_inst[p]);
+    for (int p = 0; p < 2; ++p) t.inst[p] = t.dev + o_inst[p];
     for (int q = 0; q < 2; ++q) {
         t.items[q] = reinterpret_cast<int4*>(t.dev + o_items[q]);
         t.nitems[q] = (int)items[q].size();
@@ -298,31 +341,60 @@ static int build_tables(fi_batch_s* b, int ty, int kch)
     return FI_OK;
 }
 
+static int build_tables(fi_batch_s* b, int ty, int kch)
+{
+    return b->refined == 1 ? build_tables_t<float>(b, ty, kch) : build_tables_t<double>(b, ty, kch);
+}
+
 // persistent grid of the fused sweep: enough workgroups for every CU
 static int fused_grid(int nitems) { return nitems < 8192 ? nitems : 8192; }
 
-static void launch_fused_all(fi_batch_s* b, int p)
+template <typename T>
+static void launch_fused_all_t(fi_batch_s* b, int p)
 {
     const BatchTables& t = b->t;
     for (int q = 0; q < 2; ++q)
-        launch_fused_packed<double>(t.inst[p], t.items[q], t.nitems[q], q == 1, b->tx, b->ty, b->minw, fused_grid(t.nitems[q]),
-                                    b->stream);
+        launch_fused_packed<T>(static_cast<const PackedInst<T>*>(t.inst[p]), t.items[q], t.nitems[q], q == 1, b->tx,
+                               b->ty, b->minw, fused_grid(t.nitems[q]), b->stream);
+}
+
+template <typename T>
+static void launch_points_t(fi_batch_s* b, int p, bool sigma)
+{
+    const BatchTables& t = b->t;
+    const int kib[4] = {0, 1, 0, 1}, knd[4] = {3, 3, 2, 2};
+    for (int k = 0; k < 4; ++k)
+        launch_point_packed<T>(static_cast<const PackedInst<T>*>(t.inst[p]), t.box[k], t.blk2box[k], t.nblk_box[k],
+                               kib[k] != 0, knd[k], sigma, b->stream);
+}
+
+static void launch_fused_all(fi_batch_s* b, int p)
+{
+    if (b->refined == 1) launch_fused_all_t<float>(b, p);
+    else launch_fused_all_t<double>(b, p);
+}
+
+static void launch_points(fi_batch_s* b, int p, bool sigma)
+{
+    if (b->refined == 1) launch_points_t<float>(b, p, sigma);
+    else launch_points_t<double>(b, p, sigma);
+}
+
+static void launch_bcs(fi_batch_s* b, int p, int group)
+{
+    const BatchTables& t = b->t;
+    for (int d = 0; d < 3; ++d)
+        launch_bc_packed(t.bc[p][group][d], t.blk2bc[p][group][d], t.nblk_bc[p][group][d], b->stream);
 }
 
 // one PT iteration of every running instance at table parity p
 static void launch_iteration(fi_batch_s* b, int p)
 {
-    const BatchTables& t = b->t;
     launch_fused_all(b, p);
-    const int kib[4] = {0, 1, 0, 1}, knd[4] = {3, 3, 2, 2};
-    for (int k = 0; k < 4; ++k)
-        launch_point_packed<double>(t.inst[p], t.box[k], t.blk2box[k], t.nblk_box[k], kib[k] != 0, knd[k], true,
-                                    b->stream);
-    for (int d = 0; d < 3; ++d) launch_bc_packed(t.bc[p][0][d], t.blk2bc[p][0][d], t.nblk_bc[p][0][d], b->stream);
-    for (int k = 0; k < 4; ++k)
-        launch_point_packed<double>(t.inst[p], t.box[k], t.blk2box[k], t.nblk_box[k], kib[k] != 0, knd[k], false,
-                                    b->stream);
-    for (int d = 0; d < 3; ++d) launch_bc_packed(t.bc[p][1][d], t.blk2bc[p][1][d], t.nblk_bc[p][1][d], b->stream);
+    launch_points(b, p, true);
+    launch_bcs(b, p, 0);
+    launch_points(b, p, false);
+    launch_bcs(b, p, 1);
 }
 
 static int capture_graphs(fi_batch_s* b)
@@ -384,18 +456,10 @@ static int autotune(fi_batch_s* b, int budget, int* used)
             if (k == reps - 1) HIPCHK(hipEventRecord(b->ev[0], b->stream));
             launch_fused_all(b, p);
             if (k == reps - 1) HIPCHK(hipEventRecord(b->ev[1], b->stream));
-            const BatchTables& t = b->t;
-            const int kib[4] = {0, 1, 0, 1}, knd[4] = {3, 3, 2, 2};
-            for (int q = 0; q < 4; ++q)
-                launch_point_packed<double>(t.inst[p], t.box[q], t.blk2box[q], t.nblk_box[q], kib[q] != 0, knd[q],
-                                            true, b->stream);
-            for (int d = 0; d < 3; ++d)
-                launch_bc_packed(t.bc[p][0][d], t.blk2bc[p][0][d], t.nblk_bc[p][0][d], b->stream);
-            for (int q = 0; q < 4; ++q)
-                launch_point_packed<double>(t.inst[p], t.box[q], t.blk2box[q], t.nblk_box[q], kib[q] != 0, knd[q],
-                                            false, b->stream);
-            for (int d = 0; d < 3; ++d)
-                launch_bc_packed(t.bc[p][1][d], t.blk2bc[p][1][d], t.nblk_bc[p][1][d], b->stream);
+            launch_points(b, p, true);
+            launch_bcs(b, p, 0);
+            launch_points(b, p, false);
+            launch_bcs(b, p, 1);
             // the tables were rebuilt for this candidate: keep the models' parity
             // in step with every iteration run here
             for (int r : b->run) {
@@ -462,9 +526,56 @@ static int finish_check(BatchInst& bi)
     return 0;
 }
 
+// A refined batch advances every running job's FP32 sweeps together up to
+// the nearest job's check, then runs the checks (and any FP64 outer steps)
+// of the jobs whose chunk is complete.  Tables follow the jobs' buffers and
+// modes, so they are rebuilt every step.
+static int step_refined(fi_batch_s* b, int* n_finished)
+{
+    b->run.clear();
+    for (size_t q = 0; q < b->inst.size(); ++q)
+        if (b->inst[q].state == FI_BATCH_RUNNING) b->run.push_back((int)q);
+    if (b->run.empty()) return FI_OK;
+    int delta = 1 << 30;
+    for (int r : b->run) delta = std::min(delta, b->inst[r].left);
+    if (delta < 1) return fail(FI_ERR_STATE, "refined batch job without pending sweeps");
+    for (int r : b->run) {
+        int rc = prepare(b->inst[r].m);
+        if (rc) return rc;
+    }
+    int rc = build_tables(b, b->ty, b->kch);
+    if (rc) return rc;
+    for (int it = 0; it < delta; ++it) launch_iteration(b, it & 1);
+    HIPCHK(hipGetLastError());
+    using S = SweepOf<float>;
+    for (int r : b->run) {
+        BatchInst& bi = b->inst[r];
+        S::set(bi.m, S::flipped(bi.m, S::get(bi.m), delta));
+        bi.left -= delta;
+        if (bi.left > 0) continue;
+        rc = job_after_chunk(bi.m, bi.job);
+        bi.left = bi.job.chunk;
+        bi.iters = bi.job.inner_total;
+        for (int q = 0; q < 4; ++q) bi.norms[q] = bi.job.norms[q];
+        if (rc == FI_ERR_NONFINITE) {
+            bi.state = FI_BATCH_FAILED;
+            (*n_finished)++;
+            continue;
+        }
+        if (rc) return rc;
+        if (bi.job.state == JOB_DONE) {
+            bi.state = bi.job.converged ? FI_BATCH_CONVERGED : FI_BATCH_MAX_ITER;
+            (*n_finished)++;
+        }
+    }
+    b->dirty = true;
+    return FI_OK;
+}
+
 static int step(fi_batch_s* b, int* n_finished)
 {
     *n_finished = 0;
+    if (b->refined == 1) return step_refined(b, n_finished);
     if (b->dirty) {
         int rc = rebuild(b);
         if (rc) return rc;
@@ -565,6 +676,8 @@ int fi_batch_add(fi_batch_t b, fi_model_t m, int max_iter, int ncheck, double to
     if (!b || !m) return fail(FI_ERR_ARG, "null argument");
     if (max_iter < 0 || ncheck < 1) return fail(FI_ERR_ARG, "bad max_iter/ncheck");
     if (m->batch) return fail(FI_ERR_ARG, "model already belongs to a batch");
+    if (b->refined == 1) return fail(FI_ERR_STATE, "this batch holds refined solves");
+    b->refined = 0;
     // the model's pending work completes before the batch stream takes over
     HIPCHK(hipStreamSynchronize(m->stream));
     BatchInst bi;
@@ -578,6 +691,59 @@ int fi_batch_add(fi_batch_t b, fi_model_t m, int max_iter, int ncheck, double to
     m->batch = b;
     b->inst.push_back(bi);
     b->dirty = true;
+    return FI_OK;
+}
+
+int fi_batch_add_refined(fi_batch_t b, fi_model_t m, int max_outer, int max_inner, int ncheck, double inner_rtol,
+                         double tol)
+{
+    if (!b || !m) return fail(FI_ERR_ARG, "null argument");
+    if (m->batch) return fail(FI_ERR_ARG, "model already belongs to a batch");
+    if (b->refined == 0) return fail(FI_ERR_STATE, "this batch holds FP64 solves");
+    if (b->refined < 0) {
+        // the FP32 sweep variant of the packed launches
+        b->refined = 1;
+        b->tx = 64;
+        b->ty = 16;
+        b->minw = 1;
+        b->kch = 32;
+        b->autotune = 0;
+    }
+    HIPCHK(hipStreamSynchronize(m->stream));
+    BatchInst bi;
+    bi.m = m;
+    bi.home = m->stream;
+    bi.refined = true;
+    m->stream = b->stream;
+    m->batch = b;
+    int rc = job_begin(m, bi.job, max_outer, max_inner, ncheck, inner_rtol, tol);
+    if (rc && rc != FI_ERR_NONFINITE) {
+        (void)hipStreamSynchronize(b->stream);
+        m->stream = bi.home;
+        m->batch = nullptr;
+        return rc;
+    }
+    for (int q = 0; q < 4; ++q) bi.norms[q] = bi.job.norms[q];
+    bi.iters = bi.job.inner_total;
+    bi.left = bi.job.chunk;
+    if (rc == FI_ERR_NONFINITE) bi.state = FI_BATCH_FAILED;
+    else if (bi.job.state == JOB_DONE) bi.state = bi.job.converged ? FI_BATCH_CONVERGED : FI_BATCH_MAX_ITER;
+    else bi.state = FI_BATCH_RUNNING;
+    b->inst.push_back(bi);
+    b->dirty = true;
+    return FI_OK;
+}
+
+int fi_batch_refined_counts(fi_batch_t b, fi_model_t m, int counts_out[3])
+{
+    if (!b || !m || !counts_out) return fail(FI_ERR_ARG, "null argument");
+    int q = find_inst(b, m);
+    if (q < 0) return fail(FI_ERR_ARG, "model is not in this batch");
+    const BatchInst& bi = b->inst[q];
+    if (!bi.refined) return fail(FI_ERR_ARG, "model is not a refined solve");
+    counts_out[0] = bi.job.outer;
+    counts_out[1] = bi.job.inner_total;
+    counts_out[2] = bi.job.phase1;
     return FI_OK;
 }
 
@@ -630,7 +796,7 @@ int fi_batch_set_option(fi_batch_t b, int key, int value)
     if (!b) return fail(FI_ERR_ARG, "null batch");
     if (key == FI_BOPT_VARIANT) {
         int cfg[16][3];
-        const int n = fused_configs(false, cfg, 16);
+        const int n = fused_configs(b->refined == 1, cfg, 16);
         if (value < 0 || value >= n) return fail(FI_ERR_ARG, "fused variant must be in 0..%d", n - 1);
         b->tx = cfg[value][0];
         b->ty = cfg[value][1];

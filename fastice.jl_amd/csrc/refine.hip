@@ -29,7 +29,7 @@
 #include <cstring>
 #include <vector>
 
-#include "model.h"
+#include "refine.h"
 
 namespace fi {
 
@@ -326,22 +326,56 @@ static StokesArgs<float> make_args32(fi_model_s* m, bool in_place)
     return a;
 }
 
-static void bc32(fi_model_s* m, int group)
+BcBatch refine_bc_batch(fi_model_s* m, int group, int d)
 {
     Refine* r = m->rf;
-    for (int d = 0; d < m->g.nd; ++d) {
-        BcBatch b;
-        const bool homog = !r->full;
-        if (group == FI_BCGROUP_STRESS)
-            b = stress_batch_g(m->desc, d, [r](int id) { return (void*)r->cur(id); }, homog, 1);
-        else if (group == FI_BCGROUP_VELOCITY) {
-            b = velocity_batch_g(m->desc, d, [r](int id) { return (void*)r->cur(id); }, homog, 1);
-            if (r->full) add_op(b, r->eta_cur ? r->eta : r->eta2, L_CCC, 1, 0.0, 1); // eta_next Neumann
-        } else
-            b = residual_batch_g(m->desc, d, [r](int id) { return (void*)r->r[id - FI_F_RP]; }, 1);
-        launch_bc_dim(m->g, b, m->stream);
-    }
+    BcBatch b;
+    const bool homog = !r->full;
+    if (group == FI_BCGROUP_STRESS)
+        b = stress_batch_g(m->desc, d, [r](int id) { return (void*)r->cur(id); }, homog, 1);
+    else if (group == FI_BCGROUP_VELOCITY) {
+        b = velocity_batch_g(m->desc, d, [r](int id) { return (void*)r->cur(id); }, homog, 1);
+        if (r->full) add_op(b, r->eta_cur ? r->eta : r->eta2, L_CCC, 1, 0.0, 1); // eta_next Neumann
+    } else
+        b = residual_batch_g(m->desc, d, [r](int id) { return (void*)r->r[id - FI_F_RP]; }, 1);
+    return b;
 }
+
+static void bc32(fi_model_s* m, int group)
+{
+    for (int d = 0; d < m->g.nd; ++d) launch_bc_dim(m->g, refine_bc_batch(m, group, d), m->stream);
+}
+
+StokesArgs<float> refine_args(fi_model_s* m, bool in_place) { return make_args32(m, in_place); }
+
+bool refine_fused(fi_model_s* m, FusedGeom& fg, BoxList& shell)
+{
+    Refine* r = m->rf;
+    const bool fused = m->use_fused && r->has_interior;
+    fg = r->fg;
+    fg.order = m->forder;
+    shell = fused ? r->shell : full_box(m->g);
+    return fused;
+}
+
+void refine_flip(fi_model_s* m)
+{
+    Refine* r = m->rf;
+    r->sig_cur ^= 1;
+    r->v_cur ^= 1;
+    if (r->full) r->eta_cur ^= 1;
+}
+
+RefineParity refine_parity(fi_model_s* m) { return RefineParity{m->rf->sig_cur, m->rf->v_cur, m->rf->eta_cur}; }
+
+void refine_set_parity(fi_model_s* m, RefineParity p)
+{
+    m->rf->sig_cur = p.sig;
+    m->rf->v_cur = p.v;
+    m->rf->eta_cur = p.eta;
+}
+
+bool refine_full(fi_model_s* m) { return m->rf->full; }
 
 // one FP32 PT iteration of the correction problem
 static void iterate32(fi_model_s* m, hipEvent_t* ev)
@@ -419,12 +453,7 @@ static void widen(fi_model_s* m, double* dst, const float* src)
     hipLaunchKernelGGL(k_widen, ew_grid(m->g.elems), dim3(256), 0, m->stream, dst - m->g.off, src - m->g.off, m->g.elems);
 }
 
-struct AaState {
-    int depth = 0;     // columns kept (0: plain defect correction)
-    int cols = 0;      // columns in use
-    int have_prev = 0; // a previous correction f_{k-1} is stored
-    int prev_slot = 0; // ring slot of the last applied update u_{k-1}
-};
+
 
 static int ensure_anderson(fi_model_s* m, int depth)
 {
@@ -555,11 +584,12 @@ static int anderson_apply(fi_model_s* m, AaState& st, bool restart)
     return FI_OK;
 }
 
-// Phase 1 of a nonlinear refined solve: the full nonlinear PT iteration (eta
-// updated every sweep, real boundary values and forcing) on the whole state in
-// FP32, from the FP64 state, until the FP32 residual has dropped by rtol or
-// stops decreasing; the result becomes the FP64 state.
-static int full_phase(fi_model_s* m, int max_iter, int ncheck, double rtol, int* iters)
+// Start of phase 1 of a nonlinear refined solve: the full nonlinear PT
+// iteration (eta updated every sweep, real boundary values and forcing) on
+// the whole state in FP32, from the FP64 state.  It runs until the FP32
+// residual has dropped by rtol or stops decreasing; the result then becomes
+// the FP64 state (phase1_end).
+static void phase1_begin(fi_model_s* m)
 {
     Refine* r = m->rf;
     r->full = true;
@@ -575,45 +605,209 @@ static int full_phase(fi_model_s* m, int max_iter, int ncheck, double rtol, int*
     if (r->rh) cast(m, r->rh, m->f[FI_F_RHEO], 1.0);
     for (int q = 0; q < 3; ++q)
         if (r->gf[q]) cast(m, r->gf[q], m->f[FI_F_GX + q], 1.0);
-    // PT residuals are not monotone: stop when the best residual has not halved
-    // over four consecutive checks (the FP32 floor), or after a drop by rtol
-    double first = -1.0, best = -1.0;
-    int stall = 0;
-    int done = 0, rc = FI_OK;
-    while (done < max_iter) {
-        int chunk = ncheck < max_iter - done ? ncheck : max_iter - done;
-        for (int it = 0; it < chunk; ++it) iterate32(m, nullptr);
-        done += chunk;
-        double rin = 0.0;
-        rc = inner_norm(m, &rin);
-        if (rc) break;
-        if (!(rin == rin) || rin > 3.4e38) {
-            rc = fail(FI_ERR_NONFINITE, "FP32 nonlinear phase diverged after %d iterations", done);
-            break;
-        }
-        if (first < 0.0) first = best = rin;
-        if (rin <= rtol * first) break;
-        if (rin < 0.5 * best) {
-            best = rin;
-            stall = 0;
-        } else if (++stall >= 4) {
-            break;
-        }
-    }
-    if (!rc) {
-        for (int id = FI_F_P; id <= FI_F_VZ; ++id)
-            if (m->f[id]) widen(m, m->buf(id), r->cur(id));
-        widen(m, m->eta(), r->eta_cur ? r->eta2 : r->eta);
-        m->shadow_dirty = true;
-    }
+}
+
+static void phase1_end(fi_model_s* m)
+{
+    Refine* r = m->rf;
+    for (int id = FI_F_P; id <= FI_F_VZ; ++id)
+        if (m->f[id]) widen(m, m->buf(id), r->cur(id));
+    widen(m, m->eta(), r->eta_cur ? r->eta2 : r->eta);
+    m->shadow_dirty = true;
     r->full = false;
-    *iters = done;
-    return rc;
 }
 
 static void fill32(fi_model_s* m, float* p, float v)
 {
     hipLaunchKernelGGL(k_fill32, ew_grid(m->g.elems), dim3(256), 0, m->stream, p - m->g.off, m->g.elems, v);
+}
+
+// The refined solve as a resumable job: its FP32 sweeps come in chunks of
+// `chunk` sweeps (run by fi_refined_solve or packed with other models by a
+// batch), and after each chunk job_after_chunk checks the FP32 residual and
+// moves on (more sweeps, end of phase 1, an FP64 outer step, a new
+// correction problem, or done).
+
+
+
+static void next_chunk(RefineJob& j)
+{
+    j.chunk = j.ncheck < j.max_inner - j.done ? j.ncheck : j.max_inner - j.done;
+}
+
+// top of an outer step: eta (Picard, relaxed), tau, residuals; then either
+// done or the next correction problem
+static int job_outer(fi_model_s* m, RefineJob& j)
+{
+    Refine* r = m->rf;
+    const bool ib = m->desc.immersed_boundary != 0;
+    const int nd = m->g.nd;
+    if (j.nonlinear && (j.outer > 0 || m->refine_full_phase)) {
+        // Picard update of the frozen viscosity, relaxed in log space
+        StokesArgs<double> a = make_args(m, true);
+        a.rh_relax = FI_RELAX_LOG;
+        a.rh_rate = j.omega;
+        launch_eta_update(m->g, a, m->stream);
+        m->eta_cur ^= 1;
+        for (int d = 0; d < nd; ++d) launch_bc_dim(m->g, neumann_batch(m->eta(), L_CCC, d), m->stream);
+    }
+    {
+        // tau <- 2 eta edot'(V), then the stress boundary conditions the PT
+        // iteration applies after every stress update (boundary_conditions.jl:26-100)
+        StokesArgs<double> a = make_args(m, true);
+        launch_tau(m->g, a, ib, m->stream);
+        run_bc(m, FI_BCGROUP_STRESS);
+        m->shadow_dirty = true;
+    }
+    int rc = residual_norms(m, j.norms);
+    if (rc) return rc;
+    double r0 = 0.0;
+    for (int q = 0; q < 4; ++q) {
+        if (!(j.norms[q] == j.norms[q]) || j.norms[q] > 1.79e308) {
+            j.state = JOB_DONE;
+            return fail(FI_ERR_NONFINITE, "simulation failed at outer step %d: non-finite residual", j.outer);
+        }
+        r0 = j.norms[q] > r0 ? j.norms[q] : r0;
+    }
+    if (j.trace)
+        std::fprintf(stderr, "[fastice] refined: outer %d residual %.3e eta rate %.3f FP32 iterations so far %d\n",
+                     j.outer, r0, j.omega, j.inner_total);
+    // outer viscosity relaxation: halved whenever an outer step fails to
+    // reduce the residual by 10 % (with Anderson mixing the rate stays >= 1/2:
+    // a frozen viscosity update would stall the mixed iteration)
+    if (j.r_prev > 0.0 && r0 > 0.9 * j.r_prev && j.omega > (j.aa.depth > 0 ? 0.5 : 1.0 / 64)) j.omega *= 0.5;
+    j.aa_restart = j.r_prev > 0.0 && r0 > j.r_prev; // the last mixed update did not help
+    j.r_prev = r0;
+    j.r0 = r0;
+    if (r0 < j.tol || j.outer >= j.max_outer) {
+        j.converged = r0 < j.tol;
+        j.state = JOB_DONE;
+        j.chunk = 0;
+        return FI_OK;
+    }
+    // correction problem: frozen eta, forcing from the FP64 defect, zero state
+    cast(m, r->eta, m->eta(), 1.0);
+    cast(m, r->fP, m->f[FI_F_RP], 1.0);
+    const int idR[3] = {FI_F_RVX, FI_F_RVY, FI_F_RVZ};
+    for (int q = 0; q < nd; ++q) cast(m, r->g[q], m->f[idR[q]], -1.0);
+    for (int id = FI_F_P; id <= FI_F_VZ; ++id)
+        if (m->f[id]) {
+            fill32(m, r->d[id][0], 0.0f);
+            fill32(m, r->d[id][1], 0.0f);
+        }
+    r->sig_cur = r->v_cur = 0;
+    j.state = JOB_INNER;
+    j.done = 0;
+    next_chunk(j);
+    return FI_OK;
+}
+
+int job_begin(fi_model_s* m, RefineJob& j, int max_outer, int max_inner, int ncheck, double inner_rtol,
+                     double tol)
+{
+    if (max_outer < 1 || max_inner < 1 || ncheck < 1 || !(inner_rtol > 0.0) || !(tol > 0.0))
+        return fail(FI_ERR_ARG, "bad refinement parameters");
+    int rc = ensure_refine(m);
+    if (rc) return rc;
+    rc = prepare(m);
+    if (rc) return rc;
+    Refine* r = m->rf;
+    if (m->desc.immersed_boundary && r->mask_gen != m->mask_gen) {
+        for (int q = 0; q < 8; ++q) {
+            if (r->wna[q]) cast(m, r->wna[q], m->f[FI_F_OM_NA0 + q], 1.0);
+            if (r->wns[q]) cast(m, r->wns[q], m->f[FI_F_OM_NS0 + q], 1.0);
+        }
+        r->mask_gen = m->mask_gen;
+    }
+    j = RefineJob();
+    j.max_outer = max_outer;
+    j.max_inner = max_inner;
+    j.ncheck = ncheck;
+    j.inner_rtol = inner_rtol;
+    j.tol = tol;
+    j.trace = std::getenv("FASTICE_TRACE") != nullptr;
+    const fi_rheology& rh = m->desc.rheology;
+    j.nonlinear = rh.kind == FI_RHEO_POWERLAW || rh.relax != FI_RELAX_NONE;
+    j.omega = m->refine_eta_rate;
+    j.aa.depth = j.nonlinear ? m->refine_anderson : 0;
+    if (j.aa.depth > 0) {
+        rc = ensure_anderson(m, j.aa.depth);
+        if (rc) return rc;
+    }
+    if (j.nonlinear && m->refine_full_phase) {
+        phase1_begin(m);
+        // PT residuals are not monotone: phase 1 stops when the best residual
+        // has not halved over four consecutive checks (the FP32 floor), or
+        // after a drop by 10^-digits
+        j.p1_rtol = std::pow(10.0, -m->refine_phase1_digits);
+        j.state = JOB_PHASE1;
+        j.done = 0;
+        next_chunk(j);
+        return FI_OK;
+    }
+    return job_outer(m, j);
+}
+
+// after the pending chunk of sweeps has run
+int job_after_chunk(fi_model_s* m, RefineJob& j)
+{
+    j.done += j.chunk;
+    j.chunk = 0;
+    double rin = 0.0;
+    int rc = inner_norm(m, &rin);
+    if (rc) return rc;
+    if (j.state == JOB_PHASE1) {
+        if (!(rin == rin) || rin > 3.4e38) {
+            j.inner_total += j.done;
+            j.phase1 = j.done;
+            m->rf->full = false;
+            j.state = JOB_DONE;
+            return fail(FI_ERR_NONFINITE, "FP32 nonlinear phase diverged after %d iterations", j.done);
+        }
+        bool stop = j.done >= j.max_inner;
+        if (j.first < 0.0) j.first = j.best = rin;
+        if (rin <= j.p1_rtol * j.first) {
+            stop = true;
+        } else if (rin < 0.5 * j.best) {
+            j.best = rin;
+            j.stall = 0;
+        } else if (++j.stall >= 4) {
+            stop = true;
+        }
+        if (!stop) {
+            next_chunk(j);
+            return FI_OK;
+        }
+        phase1_end(m);
+        j.phase1 = j.done;
+        j.inner_total += j.done;
+        if (j.trace) std::fprintf(stderr, "[fastice] refined: FP32 nonlinear phase %d iterations\n", j.done);
+        return job_outer(m, j);
+    }
+    // correction solve
+    if (!(rin == rin) || rin > 3.4e38) {
+        j.inner_total += j.done;
+        j.state = JOB_DONE;
+        return fail(FI_ERR_NONFINITE, "FP32 correction diverged at outer step %d", j.outer);
+    }
+    if (!(rin <= j.inner_rtol * j.r0) && j.done < j.max_inner) {
+        next_chunk(j);
+        return FI_OK;
+    }
+    j.inner_total += j.done;
+    Refine* r = m->rf;
+    if (j.aa.depth > 0) {
+        rc = anderson_apply(m, j.aa, j.aa_restart);
+        if (rc) return rc;
+    } else {
+        for (int id = FI_F_P; id <= FI_F_VZ; ++id)
+            if (m->f[id])
+                hipLaunchKernelGGL(k_add_correction, ew_grid(m->g.elems), dim3(256), 0, m->stream, m->buf(id) - m->g.off,
+                                   r->cur(id) - m->g.off, m->g.elems);
+    }
+    m->shadow_dirty = true;
+    ++j.outer;
+    return job_outer(m, j);
 }
 
 } // namespace fi
@@ -625,127 +819,21 @@ extern "C" {
 int fi_refined_solve(fi_model_t m, int max_outer, int max_inner, int ncheck, double inner_rtol, double tol,
                      int counts_out[3], double norms_inf[4])
 {
-    const bool trace = std::getenv("FASTICE_TRACE") != nullptr;
     if (!m || !norms_inf) return fail(FI_ERR_ARG, "null argument");
-    if (counts_out) counts_out[2] = 0;
-    if (max_outer < 1 || max_inner < 1 || ncheck < 1 || !(inner_rtol > 0.0) || !(tol > 0.0))
-        return fail(FI_ERR_ARG, "bad refinement parameters");
-    int rc = ensure_refine(m);
-    if (rc) return rc;
-    rc = prepare(m);
-    if (rc) return rc;
-    Refine* r = m->rf;
-    const bool ib = m->desc.immersed_boundary != 0;
-    const int nd = m->g.nd;
-    if (ib && r->mask_gen != m->mask_gen) {
-        for (int q = 0; q < 8; ++q) {
-            if (r->wna[q]) cast(m, r->wna[q], m->f[FI_F_OM_NA0 + q], 1.0);
-            if (r->wns[q]) cast(m, r->wns[q], m->f[FI_F_OM_NS0 + q], 1.0);
-        }
-        r->mask_gen = m->mask_gen;
-    }
-    const fi_rheology& rh = m->desc.rheology;
-    const bool nonlinear = rh.kind == FI_RHEO_POWERLAW || rh.relax != FI_RELAX_NONE;
-    int outer = 0, inner_total = 0;
-    // outer viscosity relaxation: the configured rate, halved whenever an outer
-    // step fails to reduce the residual by 10 %
-    double omega = m->refine_eta_rate, r_prev = -1.0;
-    AaState aa;
-    aa.depth = nonlinear ? m->refine_anderson : 0;
-    if (aa.depth > 0) {
-        rc = ensure_anderson(m, aa.depth);
-        if (rc) return rc;
-    }
-    bool aa_restart = false;
-    if (nonlinear && m->refine_full_phase) {
-        int it = 0;
-        rc = full_phase(m, max_inner, ncheck, std::pow(10.0, -m->refine_phase1_digits), &it);
-        inner_total += it;
-        if (counts_out) counts_out[2] = it;
-        if (trace) std::fprintf(stderr, "[fastice] refined: FP32 nonlinear phase %d iterations\n", it);
-        if (rc) {
-            if (counts_out) { counts_out[0] = 0; counts_out[1] = inner_total; }
-            return rc;
-        }
-    }
-    for (;; ++outer) {
-        if (nonlinear && (outer > 0 || m->refine_full_phase)) {
-            // Picard update of the frozen viscosity, relaxed in log space
-            StokesArgs<double> a = make_args(m, true);
-            a.rh_relax = FI_RELAX_LOG;
-            a.rh_rate = omega;
-            launch_eta_update(m->g, a, m->stream);
-            m->eta_cur ^= 1;
-            for (int d = 0; d < nd; ++d) launch_bc_dim(m->g, neumann_batch(m->eta(), L_CCC, d), m->stream);
-        }
-        {
-            // tau <- 2 eta edot'(V), then the stress boundary conditions the PT
-            // iteration applies after every stress update (boundary_conditions.jl:26-100)
-            StokesArgs<double> a = make_args(m, true);
-            launch_tau(m->g, a, ib, m->stream);
-            run_bc(m, FI_BCGROUP_STRESS);
-            m->shadow_dirty = true;
-        }
-        rc = residual_norms(m, norms_inf);
-        if (rc) return rc;
-        double r0 = 0.0;
-        for (int q = 0; q < 4; ++q) {
-            if (!(norms_inf[q] == norms_inf[q]) || norms_inf[q] > 1.79e308) {
-                if (counts_out) { counts_out[0] = outer; counts_out[1] = inner_total; }
-                return fail(FI_ERR_NONFINITE, "simulation failed at outer step %d: non-finite residual", outer);
-            }
-            r0 = norms_inf[q] > r0 ? norms_inf[q] : r0;
-        }
-        if (trace)
-            std::fprintf(stderr, "[fastice] refined: outer %d residual %.3e eta rate %.3f FP32 iterations so far %d\n",
-                         outer, r0, omega, inner_total);
-        // (with Anderson mixing the rate stays >= 1/2: a frozen viscosity
-        // update would stall the mixed iteration)
-        if (r_prev > 0.0 && r0 > 0.9 * r_prev && omega > (aa.depth > 0 ? 0.5 : 1.0 / 64)) omega *= 0.5;
-        aa_restart = r_prev > 0.0 && r0 > r_prev; // the last mixed update did not help
-        r_prev = r0;
-        if (r0 < tol || outer >= max_outer) break;
-        // correction problem: frozen eta, forcing from the FP64 defect, zero state
-        cast(m, r->eta, m->eta(), 1.0);
-        cast(m, r->fP, m->f[FI_F_RP], 1.0);
-        const int idR[3] = {FI_F_RVX, FI_F_RVY, FI_F_RVZ};
-        for (int q = 0; q < nd; ++q) cast(m, r->g[q], m->f[idR[q]], -1.0);
-        for (int id = FI_F_P; id <= FI_F_VZ; ++id)
-            if (m->f[id]) {
-                fill32(m, r->d[id][0], 0.0f);
-                fill32(m, r->d[id][1], 0.0f);
-            }
-        r->sig_cur = r->v_cur = 0;
-        int done = 0;
-        while (done < max_inner) {
-            int chunk = ncheck < max_inner - done ? ncheck : max_inner - done;
-            for (int it = 0; it < chunk; ++it) iterate32(m, nullptr);
-            done += chunk;
-            double rin = 0.0;
-            rc = inner_norm(m, &rin);
-            if (rc) return rc;
-            if (!(rin == rin) || rin > 3.4e38) {
-                if (counts_out) { counts_out[0] = outer; counts_out[1] = inner_total + done; }
-                return fail(FI_ERR_NONFINITE, "FP32 correction diverged at outer step %d", outer);
-            }
-            if (rin <= inner_rtol * r0) break;
-        }
-        inner_total += done;
-        if (aa.depth > 0) {
-            rc = anderson_apply(m, aa, aa_restart);
-            if (rc) return rc;
-        } else {
-            for (int id = FI_F_P; id <= FI_F_VZ; ++id)
-                if (m->f[id])
-                    hipLaunchKernelGGL(k_add_correction, ew_grid(m->g.elems), dim3(256), 0, m->stream,
-                                       m->buf(id) - m->g.off, r->cur(id) - m->g.off, m->g.elems);
-        }
-        m->shadow_dirty = true;
+    if (counts_out) counts_out[0] = counts_out[1] = counts_out[2] = 0;
+    RefineJob j;
+    int rc = job_begin(m, j, max_outer, max_inner, ncheck, inner_rtol, tol);
+    while (!rc && j.state != JOB_DONE) {
+        for (int it = 0; it < j.chunk; ++it) iterate32(m, nullptr);
+        rc = job_after_chunk(m, j);
     }
     if (counts_out) {
-        counts_out[0] = outer;
-        counts_out[1] = inner_total;
+        counts_out[0] = j.outer;
+        counts_out[1] = j.inner_total;
+        counts_out[2] = j.phase1;
     }
+    for (int q = 0; q < 4; ++q) norms_inf[q] = j.norms[q];
+    if (rc) return rc;
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(m->stream));
     return FI_OK;

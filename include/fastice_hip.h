@@ -249,6 +249,17 @@ int fi_batch_create(fi_batch_t* out);
 int fi_batch_destroy(fi_batch_t b);
 /* attach a model and start its solve (state FI_BATCH_RUNNING) */
 int fi_batch_add(fi_batch_t b, fi_model_t m, int max_iter, int ncheck, double tol);
+/* attach a model and start its FP32-refined solve (the arguments of
+ * fi_refined_solve).  A batch holds either FP64 solves (fi_batch_add) or
+ * refined solves: the FP32 sweeps of every running refined model are packed
+ * into the same launches up to the nearest model's residual check, then each
+ * model whose chunk ended runs its check and FP64 outer step on the batch's
+ * stream.  Results are bitwise those of fi_refined_solve on the model alone;
+ * fi_batch_status reports FP32 sweeps as iterations. */
+int fi_batch_add_refined(fi_batch_t b, fi_model_t m, int max_outer, int max_inner, int ncheck, double inner_rtol,
+                         double tol);
+/* counts_out = {outer steps, FP32 sweeps, FP32 sweeps of the nonlinear phase} */
+int fi_batch_refined_counts(fi_batch_t b, fi_model_t m, int counts_out[3]);
 int fi_batch_remove(fi_batch_t b, fi_model_t m);
 /* advance every running model to the next residual check of the nearest one,
  * check the models due, and return how many finished in *n_finished */
@@ -258,7 +269,8 @@ int fi_batch_solve(fi_batch_t b);
 /* state (FI_BATCH_*), iterations done and last residual norms of a model;
  * a failed model (non-finite residual) reports FI_BATCH_FAILED */
 int fi_batch_status(fi_batch_t b, fi_model_t m, int* state, int* iters, double norms_inf[4]);
-/* FI_BOPT_VARIANT: fused sweep variant (index into fi_fused_variants(0)),
+/* FI_BOPT_VARIANT: fused sweep variant (index into fi_fused_variants(0), or
+ * fi_fused_variants(1) for a refined batch),
  * FI_BOPT_ZCHUNK: z planes per tile, FI_BOPT_AUTOTUNE 1/0: time every FI_BOPT_GRAPH_ITERS: iterations per graph
  * replay (0: no graph), FI_BOPT_TILE_ORDER: as FI_OPT_TILE_ORDER.  Results do
  * not depend on any of them. */

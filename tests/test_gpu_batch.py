@@ -142,3 +142,61 @@ def test_batch_max_iter_and_destroy(pkg):
         m2.close()  # detaches from the batch
         assert batch.info()["attached"] == 1
     m1.close()
+
+
+REFINED_CASES = [
+    ("glacier_glen", (40, 34, 18), 3),
+    ("manufactured", (22, 19, 25), None),
+    ("glacier_2d", (64, 32), 7),
+    ("glacier_linear", (56, 40, 24), 2),
+]
+
+
+def _make_refined(pkg, case):
+    kind, dims, seed = case
+    if kind == "glacier_2d":
+        m, _ = pkg.glacier_2d(dims, seed=seed, glen=True)
+        return m
+    m, _, _ = _make(pkg, case)
+    return m
+
+
+def test_batch_refined_equals_alone(pkg):
+    """FP32-refined solves packed in one batch end bitwise equal to the same
+    refined solves alone: outer steps, FP32 sweeps, residual norms, fields."""
+    _say("case: refined batch of %d mixed models vs alone" % len(REFINED_CASES))
+    args = dict(max_outer=30, max_inner=3000, ncheck=60, inner_rtol=3e-2)
+    alone = []
+    for case in REFINED_CASES:
+        m = _make_refined(pkg, case)
+        outer, sweeps, norms = m.refined_solve(1e-8, **args)
+        _say("alone %s: outer %d sweeps %d residual %.2e" % (case, outer, sweeps, max(norms)))
+        alone.append((outer, sweeps, dict(m.refined_counts), norms.copy(),
+                      {n: m.field(n).parent() for n in state_names(m.grid.ndims) + ["ETA"]}))
+        m.close()
+    models = []
+    with pkg.Batch() as batch:
+        for case in REFINED_CASES:
+            m = _make_refined(pkg, case)
+            batch.add_refined(m, 1e-8, **args)
+            models.append(m)
+        steps = 0
+        while batch.info()["running"] > 0:
+            batch.step()
+            steps += 1
+            if steps % 50 == 0:
+                _say("batch step %d, running %d" % (steps, batch.info()["running"]))
+            assert steps < 3000
+        _say("batch done after %d steps" % steps)
+        for m, case, (outer, sweeps, counts, norms, fields) in zip(models, REFINED_CASES, alone):
+            state, it, bnorms = batch.status(m)
+            bc = batch.refined_counts(m)
+            _say("%s: %s outer %d sweeps %d (alone %d %d)" % (case, state, bc["outer"], bc["fp32_iterations"], outer, sweeps))
+            assert state in ("converged", "max_iter"), (case, state)
+            assert bc["outer"] == outer and bc["fp32_iterations"] == sweeps, (case, bc, outer, sweeps)
+            assert bc["nonlinear_phase"] == counts["nonlinear_phase"], case
+            assert np.array_equal(bnorms, norms), (case, bnorms, norms)
+            for n, f in fields.items():
+                assert np.array_equal(m.field(n).parent(), f), (case, n)
+    for m in models:
+        m.close()
