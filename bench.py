@@ -9,13 +9,16 @@ Workloads
       gravity-driven.  A step is one FP32 PT iteration of the defect-correction
       solver (the production sweep); the FP64 iteration of the same model is
       timed beside it, and a converged-solve comparison (FP32-refined vs plain
-      FP64, both to FP64 residual 1e-8) runs on the same glacier at 127x127x63.
+      FP64, both to FP64 residual 1e-8) runs on the same glacier at 255x255x127.
   stokes3d_fp64_511x511x255: isoviscous free-slip box with manufactured
       forcing, plain FP64 iteration.
-  batch96_glacier: 96 seeded glaciers, each dimension uniform in 48..192,
-      Glen n = 3, solved in FP64 to residual 1e-8 as one batch (packed
-      launches, HIP graphs); value = solves/s; the same models are then reset
-      and a sample of them solved one at a time for the comparison.
+  batch96_glacier: 96 seeded glaciers, each dimension uniform in 48..192
+      (cubic cells, domain aspect 1 to 4), Glen n = 3, solved as one batch
+      with packed launches, FP32 sweeps
+      inside FP64 defect correction to FP64 residual 1e-8 (--batch-precision
+      fp64: plain FP64 PT with HIP graphs); value = solves/s; the same models
+      are then reset and a sample of them solved one at a time for the
+      comparison.
   serving_poisson40: Poisson arrivals at 40/s, half 2D glacier sections
       (n x n/2, n in 64..511), half 3D glaciers (n x n x n/2, n in 32..127),
       FP64 to 1e-8 or niter = 50n, continuously batched; value = completed
@@ -62,6 +65,10 @@ def parse():
     p.add_argument("--solve-dims", default="255,255,127")
     p.add_argument("--alone-sample", type=int, default=16,
                    help="batch96_glacier: models solved one at a time for the comparison")
+    p.add_argument("--batch-precision", default="refined", choices=("refined", "fp64"),
+                   help="batch96_glacier: FP32-refined solves to 1e-8 (default) or plain FP64")
+    p.add_argument("--batch-chunk", type=int, default=32,
+                   help="batch96_glacier refined: FP32 sweeps between FP64 outer steps, all models")
     p.add_argument("--requests", type=int, default=200, help="serving_poisson40: number of requests")
     p.add_argument("--copy-calib", action="store_true",
                    help="run a 1 GiB device copy first (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -139,7 +146,7 @@ def solve_comparison(pkg, dims):
     t64 = time.perf_counter() - t0
     a.synchronize()
     t0 = time.perf_counter()
-    outer, inner, na = a.refined_solve(1e-8, max_outer=300, ncheck=cad["ncheck"] // 2)
+    outer, inner, na = a.refined_solve(1e-8)
     t32 = time.perf_counter() - t0
     # per-field difference relative to the FP64 solution's stress / velocity scale
     names = ("P", "TXX", "TYY", "TZZ", "TXY", "TXZ", "TYZ", "VX", "VY", "VZ")
@@ -152,7 +159,9 @@ def solve_comparison(pkg, dims):
     b.close()
     return {"grid": list(dims), "tol": 1e-8,
             "fp64": {"iterations": it, "seconds": round(t64, 3), "residual": float(max(nb))},
-            "fp32_refined": {"outer_steps": outer, "fp32_iterations": inner,
+            "fp32_refined": {"outer_loop": "lagged PT: an FP64 outer step (residuals, viscosity, stress lag) "
+                                           "every 32 FP32 sweeps, after a full-FP32 nonlinear start",
+                             "outer_steps": outer, "fp32_iterations": inner,
                              "fp32_nonlinear_phase_iterations": counts["nonlinear_phase"],
                              "seconds": round(t32, 3),
                              "residual": float(max(na))},
@@ -176,32 +185,49 @@ def _reset(model):
 
 
 def bench_batch(pkg, args):
-    """96 uneven glaciers solved as one batch vs one at a time."""
+    """96 uneven glaciers solved as one batch vs one at a time.
+
+    Cells are cubes of side 2/192 (a 192-cell axis spans the default 2.0),
+    so the domain aspect ratio is the grid's (1 to 4).  refined (default):
+    FP32 PT sweeps inside FP64 defect correction (lagged PT), every model to
+    FP64 residual 1e-8 or 400n sweeps, all with one chunk length so their
+    outer steps fall on the same batch steps; fp64: plain FP64 PT to 1e-8 or
+    the driver's niter = 50n with ncheck = 5n."""
+    refined = args.batch_precision == "refined"
     rng = np.random.default_rng(2024)
     dims = [tuple(int(x) for x in rng.integers(48, 193, 3)) for _ in range(96)]
+    h = 2.0 / 192
     cells = sum(d[0] * d[1] * d[2] for d in dims)
-    log("setting up 96 glaciers, %.1f M cells in total" % (cells / 1e6))
+    log("setting up 96 glaciers, %.1f M cells in total (%s)" % (cells / 1e6, args.batch_precision))
     t0 = time.perf_counter()
     inst = []
     for i, d in enumerate(dims):
-        m, cad = pkg.glacier_3d(d, seed=100 + i, glen=True)
+        m, cad = pkg.glacier_3d(d, seed=100 + i, glen=True, lx=d[0] * h, ly=d[1] * h, H=d[2] * h)
         inst.append((m, cad["ncheck"], cad["niter"]))
     for m, _, _ in inst:
         m.synchronize()
     setup_s = time.perf_counter() - t0
     log("setup %.1f s" % setup_s)
+
+    def refined_args(d):
+        return dict(max_outer=-(-400 * max(d) // args.batch_chunk), ncheck=args.batch_chunk)
+
     batch = pkg.Batch()
-    for m, nc, mi in inst:
-        batch.add(m, mi, nc, 1e-8)
+    for (m, nc, mi), d in zip(inst, dims):
+        if refined:
+            batch.add_refined(m, 1e-8, **refined_args(d))
+        else:
+            batch.add(m, mi, nc, 1e-8)
     t0 = time.perf_counter()
     last = t0
-    while batch.info()["running"] > 0 or batch.info()["rebuilds"] == 0:
+    while batch.info()["running"] > 0 or (not refined and batch.info()["rebuilds"] == 0):
         batch.step()
         if time.perf_counter() - last > 10.0:
             last = time.perf_counter()
             log("batch: %.1f s, %d running" % (last - t0, batch.info()["running"]))
     t_batch = time.perf_counter() - t0
     st = [batch.status(m) for m, _, _ in inst]
+    rc = [batch.refined_counts(m) for m, _, _ in inst] if refined else None
     info = batch.info()
     batch.close()
     iters = [s[1] for s in st]
@@ -215,8 +241,12 @@ def bench_batch(pkg, args):
     same = True
     for q in sample:
         m, nc, mi = inst[q]
-        it, _ = m.solve(mi, nc, 1e-8)
-        same = same and it == iters[q]
+        if refined:
+            outer, it, _ = m.refined_solve(1e-8, **refined_args(dims[q]))
+            same = same and it == iters[q] and outer == rc[q]["outer"]
+        else:
+            it, _ = m.solve(mi, nc, 1e-8)
+            same = same and it == iters[q]
         log("alone: model %d %s, %d iterations, %.1f s so far" % (q, dims[q], it, time.perf_counter() - t0))
     t_alone = time.perf_counter() - t0
     alone_rate = len(sample) / t_alone
@@ -225,11 +255,16 @@ def bench_batch(pkg, args):
     for m, _, _ in inst:
         m.close()
     states = {k: sum(s[0] == k for s in st) for k in ("converged", "max_iter", "failed")}
-    config = {"workload": "batch96_glacier",
+    if refined:
+        how = ("FP32 PT sweeps inside FP64 defect correction (lagged PT, an FP64 outer step every %d sweeps) "
+               "to FP64 residual 1e-8 or 400n sweeps" % args.batch_chunk)
+    else:
+        how = "FP64 PT to residual 1e-8 or the driver's niter = 50n, ncheck = 5n (test_vavilov.jl)"
+    config = {"workload": "batch96_glacier", "precision": args.batch_precision,
               "instances": 96, "dims_range": [48, 192], "total_cells": cells,
-              "description": "seeded synthetic glaciers (seeds 100..195), dims uniform in 48..192 per axis, Glen n=3 "
-                             "(A=1, eta_reg=30, log relaxation 0.05), FP64 PT to residual 1e-8 or the driver's "
-                             "niter = 50n, ncheck = 5n (test_vavilov.jl)",
+              "description": "seeded synthetic glaciers (seeds 100..195), dims uniform in 48..192 per axis, cubic cells "
+                             "of side 2/192 (domain n*2/192 per axis), Glen n=3 (A=1, eta_reg=30, log relaxation 0.05), "
+                             + how,
               "batch_seconds": round(t_batch, 3), "states": states,
               "iterations": {"min": int(min(iters)), "max": int(max(iters)), "sum": int(sum(iters))},
               "one_at_a_time": {"models": len(sample), "seconds": round(t_alone, 3),
@@ -237,8 +272,15 @@ def bench_batch(pkg, args):
                                 "same_iterations_as_batch": bool(same)},
               "speedup_vs_one_at_a_time": round((96 / t_batch) / alone_rate, 3),
               "batch_info": info, "setup_s": round(setup_s, 2)}
-    line = _line("batched solves/s (96 glaciers 48-192 per axis, FP64, tol 1e-8 / niter 50n)", 96 / t_batch, "solves/s",
-                 t_batch * 1e3, 1, 0, "f64", config)
+    if refined:
+        config["outer_steps"] = {"min": min(r["outer"] for r in rc), "max": max(r["outer"] for r in rc)}
+        config["max_residual"] = float("%.3e" % max(float(np.max(s[2])) for s in st))
+        metric = "batched solves/s (96 glaciers 48-192 per axis, FP32-refined to FP64 residual 1e-8)"
+        dtype = "f32+f64"
+    else:
+        metric = "batched solves/s (96 glaciers 48-192 per axis, FP64, tol 1e-8 / niter 50n)"
+        dtype = "f64"
+    line = _line(metric, 96 / t_batch, "solves/s", t_batch * 1e3, 1, 0, dtype, config)
     line["roofline"] = None
     line["cpu_baseline"] = None
     print(json.dumps(line), flush=True)

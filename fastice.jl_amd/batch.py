@@ -16,7 +16,7 @@ import ctypes as C
 import numpy as np
 
 from . import _abi as A
-from .full_stokes import build_desc
+from .full_stokes import build_desc, refined_defaults
 
 STATES = {A.FI_BATCH_RUNNING: "running", A.FI_BATCH_CONVERGED: "converged", A.FI_BATCH_MAX_ITER: "max_iter",
           A.FI_BATCH_FAILED: "failed"}
@@ -60,12 +60,11 @@ class Batch:
         A.check(A.lib().fi_batch_add(self._h, model._h, int(max_iter), int(ncheck), float(tol)))
         self._models[id(model)] = model
 
-    def add_refined(self, model, tol, max_outer=100, max_inner=None, ncheck=None, inner_rtol=3e-2):
-        """Attach a model and start its FP32-refined solve (the arguments of
-        IsothermalFullStokesModel.refined_solve)."""
+    def add_refined(self, model, tol, max_outer=None, max_inner=None, ncheck=None, inner_rtol=3e-2):
+        """Attach a model and start its FP32-refined solve (the arguments and
+        defaults of IsothermalFullStokesModel.refined_solve)."""
         n = max(model.grid.dims)
-        ncheck = ncheck or n
-        max_inner = max_inner or 100 * n
+        max_outer, max_inner, ncheck = refined_defaults(n, model._refine_mode, max_outer, max_inner, ncheck)
         A.check(A.lib().fi_batch_add_refined(self._h, model._h, int(max_outer), int(max_inner), int(ncheck),
                                              float(inner_rtol), float(tol)))
         self._models[id(model)] = model

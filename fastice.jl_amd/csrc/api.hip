@@ -669,6 +669,9 @@ int fi_set_option(fi_model_t m, int key, int value)
             m->fty = cfg[value][1];
             m->fminw = cfg[value][2];
         }
+    } else if (key == FI_OPT_REFINE_MODE) {
+        if (value < 0 || value > 1) return fail(FI_ERR_ARG, "refinement mode must be 0 or 1");
+        m->refine_mode = value;
     } else if (key == FI_OPT_REFINE_ANDERSON) {
         if (value < 0 || value > 3) return fail(FI_ERR_ARG, "Anderson depth must be in 0..3");
         m->refine_anderson = value;

@@ -136,8 +136,12 @@ void launch_ib_flags(const Geo& g, const double* const na[8], const double* cons
                      hipStream_t s);
 template <typename T>
 void launch_residuals(const Geo& g, const StokesArgs<T>& a, bool ib, hipStream_t s);
+// FP32 outputs of the stress lag (old - new) of a compute_tau (xx yy zz xy xz yz)
+struct TauLag {
+    float* t[6];
+};
 template <typename T>
-void launch_tau(const Geo& g, const StokesArgs<T>& a, bool ib, hipStream_t s);
+void launch_tau(const Geo& g, const StokesArgs<T>& a, bool ib, hipStream_t s, const TauLag* lag = nullptr);
 template <typename T>
 void launch_eta_update(const Geo& g, const StokesArgs<T>& a, hipStream_t s);
 

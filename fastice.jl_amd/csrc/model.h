@@ -42,6 +42,7 @@ struct fi_model_s {
     double refine_eta_rate = 1.0;   // log-space relaxation of the outer viscosity update (initial)
     int refine_phase1_digits = 3;   // the FP32 nonlinear phase stops after this many digits of residual drop
     int refine_anderson = 2;        // Anderson depth of the nonlinear outer loop (0: plain defect correction)
+    int refine_mode = 1;            // outer loop: 1 lagged PT, 0 correction solves
     int kch = 0;                             // z planes per fused workgroup (0: sized to the grid)
     int ftx = 64, fty = 8, fminw = 1;        // fused sweep variant of the FP64 iteration (lanes, rows, min waves)
     int ftx32 = 64, fty32 = 16, fminw32 = 1; // and of the FP32 sweeps of the refined solver

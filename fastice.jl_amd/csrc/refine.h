@@ -17,14 +17,21 @@ enum { JOB_PHASE1 = 0, JOB_INNER = 1, JOB_DONE = 2 };
 
 struct RefineJob {
     int max_outer = 0, max_inner = 0, ncheck = 1;
+    int mode = 1;         // 1 lagged PT, 0 correction solves
+    int since_eta = 0;    // lagged PT: FP32 sweeps since the last viscosity update
+    double lag_best = -1.0; // lagged PT: best outer residual since the chunk length changed
+    int lag_stall = 0;    // lagged PT: sweeps since that residual last halved
+    int lag_window = 0;   // lagged PT: stall limit in sweeps; then the chunk length halves (down to 8)
+    long sweep_cap = 0;   // lagged PT: FP32 sweeps allowed in all (max_outer x the initial ncheck)
     double inner_rtol = 0.0, tol = 0.0;
     bool nonlinear = false, trace = false;
     int state = JOB_DONE;
     int chunk = 0;        // sweeps of the pending chunk
+    int last = 0;         // sweeps of the chunk just run
     int done = 0;         // sweeps of the current phase 1 / correction solve
     int outer = 0, inner_total = 0, phase1 = 0;
-    double p1_rtol = 0.0, first = -1.0, best = -1.0;
-    int stall = 0;
+    double p1_rtol = 0.0, first = -1.0, best = -1.0; // phase 1: drop, peak and best residuals
+    int stall = 0, stall_window = 0; // phase 1: sweeps without halving, and the limit
     double omega = 1.0, r_prev = -1.0, r0 = 0.0;
     bool aa_restart = false;
     AaState aa;

@@ -1,8 +1,11 @@
 // refine.hip -- FP32 pseudo-transient sweeps inside an FP64 defect-correction
 // (iterative-refinement) loop.
 //
-// The FP64 model state x = (P, tau, V, eta) is the master.  Each outer step,
-// all in FP64 on the GPU:
+// The FP64 model state x = (P, tau, V, eta) is the master.  Two outer loops
+// share the FP32 sweeps: lagged PT (default, job_outer_lag below), in which
+// the FP32 sweeps are the plain PT iteration in correction form with the
+// viscosity lagged by one chunk, and correction solves, described here.
+// Each outer step of the correction solves, all in FP64 on the GPU:
 //   1. eta <- rheology(tau_II(tau), eta)          (nonlinear rheologies only)
 //   2. tau <- 2 eta (edot(V) - div V / 3)          (compute_tau!: zero constitutive defect)
 //   3. r_P, r_V <- compute_residuals!(x); Inf-norms; stop when below tol
@@ -632,13 +635,20 @@ static void fill32(fi_model_s* m, float* p, float v)
 
 static void next_chunk(RefineJob& j)
 {
+    if (j.state == JOB_INNER && j.mode == 1) {
+        j.chunk = j.ncheck; // lagged PT: one chunk between outer steps
+        return;
+    }
     j.chunk = j.ncheck < j.max_inner - j.done ? j.ncheck : j.max_inner - j.done;
 }
 
 // top of an outer step: eta (Picard, relaxed), tau, residuals; then either
 // done or the next correction problem
+static int job_outer_lag(fi_model_s* m, RefineJob& j);
+
 static int job_outer(fi_model_s* m, RefineJob& j)
 {
+    if (j.mode == 1) return job_outer_lag(m, j);
     Refine* r = m->rf;
     const bool ib = m->desc.immersed_boundary != 0;
     const int nd = m->g.nd;
@@ -702,6 +712,86 @@ static int job_outer(fi_model_s* m, RefineJob& j)
     return FI_OK;
 }
 
+// Lagged-PT outer step.  The FP64 state (P, tau, V, eta) is the PT iterate
+// itself, tau included.  Write T = 2 eta edot'(V) (compute_tau!) over tau and
+// let the FP32 stress correction start from the lag tau - T: the correction
+// sweeps are then, to rounding, the plain PT sweeps from the FP64 state with
+// eta frozen (the residuals evaluated with T plus the lag give the same
+// updates).  eta itself is updated here, between chunks, by the relaxation
+// the PT iteration applies every sweep, compounded over the chunk:
+// rate_C = 1 - (1 - rate)^C.
+static int job_outer_lag(fi_model_s* m, RefineJob& j)
+{
+    Refine* r = m->rf;
+    const bool ib = m->desc.immersed_boundary != 0;
+    const int nd = m->g.nd;
+    if (j.nonlinear && j.since_eta > 0) {
+        StokesArgs<double> a = make_args(m, true);
+        const fi_rheology& rh = m->desc.rheology;
+        if (rh.relax != FI_RELAX_NONE) a.rh_rate = 1.0 - std::pow(1.0 - rh.rate, (double)j.since_eta);
+        launch_eta_update(m->g, a, m->stream);
+        m->eta_cur ^= 1;
+        for (int d = 0; d < nd; ++d) launch_bc_dim(m->g, neumann_batch(m->eta(), L_CCC, d), m->stream);
+    }
+    j.since_eta = 0;
+    for (int id = FI_F_P; id <= FI_F_VZ; ++id)
+        if (m->f[id]) {
+            fill32(m, r->d[id][0], 0.0f);
+            fill32(m, r->d[id][1], 0.0f);
+        }
+    r->sig_cur = r->v_cur = 0;
+    {
+        StokesArgs<double> a = make_args(m, true);
+        const int tid[6] = {FI_F_TXX, FI_F_TYY, FI_F_TZZ, FI_F_TXY, FI_F_TXZ, FI_F_TYZ};
+        TauLag lg;
+        for (int q = 0; q < 6; ++q) lg.t[q] = m->f[tid[q]] ? r->d[tid[q]][0] : nullptr;
+        launch_tau(m->g, a, ib, m->stream, &lg);
+        run_bc(m, FI_BCGROUP_STRESS);
+        m->shadow_dirty = true;
+    }
+    bc32(m, FI_BCGROUP_STRESS); // homogeneous boundary values of the lag
+    int rc = residual_norms(m, j.norms);
+    if (rc) return rc;
+    double r0 = 0.0;
+    for (int q = 0; q < 4; ++q) {
+        if (!(j.norms[q] == j.norms[q]) || j.norms[q] > 1.79e308) {
+            j.state = JOB_DONE;
+            return fail(FI_ERR_NONFINITE, "simulation failed at outer step %d: non-finite residual", j.outer);
+        }
+        r0 = j.norms[q] > r0 ? j.norms[q] : r0;
+    }
+    if (j.trace)
+        std::fprintf(stderr, "[fastice] refined (lagged PT): outer %d residual %.3e FP32 iterations so far %d\n",
+                     j.outer, r0, j.inner_total);
+    j.r0 = r0;
+    if (r0 < j.tol || j.inner_total >= j.sweep_cap) {
+        j.converged = r0 < j.tol;
+        j.state = JOB_DONE;
+        j.chunk = 0;
+        return FI_OK;
+    }
+    // a viscosity lag of ncheck sweeps can hold a stiff problem in a limit
+    // cycle: when the residual has not halved for lag_window sweeps, halve
+    // the chunk (the lag) and start watching again
+    if (j.lag_best < 0.0 || r0 < 0.5 * j.lag_best) {
+        j.lag_best = r0;
+        j.lag_stall = 0;
+    } else if ((j.lag_stall += j.last) >= j.lag_window && j.ncheck > 8) {
+        j.ncheck = j.ncheck / 2 > 8 ? j.ncheck / 2 : 8;
+        j.lag_best = r0;
+        j.lag_stall = 0;
+        if (j.trace) std::fprintf(stderr, "[fastice] refined (lagged PT): chunk now %d sweeps\n", j.ncheck);
+    }
+    cast(m, r->eta, m->eta(), 1.0);
+    cast(m, r->fP, m->f[FI_F_RP], 1.0);
+    const int idR[3] = {FI_F_RVX, FI_F_RVY, FI_F_RVZ};
+    for (int q = 0; q < nd; ++q) cast(m, r->g[q], m->f[idR[q]], -1.0);
+    j.state = JOB_INNER;
+    j.done = 0;
+    next_chunk(j);
+    return FI_OK;
+}
+
 int job_begin(fi_model_s* m, RefineJob& j, int max_outer, int max_inner, int ncheck, double inner_rtol,
                      double tol)
 {
@@ -728,18 +818,23 @@ int job_begin(fi_model_s* m, RefineJob& j, int max_outer, int max_inner, int nch
     j.trace = std::getenv("FASTICE_TRACE") != nullptr;
     const fi_rheology& rh = m->desc.rheology;
     j.nonlinear = rh.kind == FI_RHEO_POWERLAW || rh.relax != FI_RELAX_NONE;
+    j.mode = m->refine_mode;
+    j.lag_window = 8 * std::max(m->g.n[0], std::max(m->g.n[1], m->g.nd == 3 ? m->g.n[2] : 0));
+    j.sweep_cap = (long)max_outer * ncheck;
     j.omega = m->refine_eta_rate;
-    j.aa.depth = j.nonlinear ? m->refine_anderson : 0;
+    j.aa.depth = j.nonlinear && j.mode == 0 ? m->refine_anderson : 0;
     if (j.aa.depth > 0) {
         rc = ensure_anderson(m, j.aa.depth);
         if (rc) return rc;
     }
     if (j.nonlinear && m->refine_full_phase) {
         phase1_begin(m);
-        // PT residuals are not monotone: phase 1 stops when the best residual
-        // has not halved over four consecutive checks (the FP32 floor), or
-        // after a drop by 10^-digits
+        // PT residuals are not monotone and rise first: phase 1 stops after a
+        // drop by 10^-digits from the peak residual, or when the best residual
+        // has not halved over 4 n_max sweeps since (the FP32 floor; a PT
+        // transient spans about n_max sweeps, whatever the check cadence)
         j.p1_rtol = std::pow(10.0, -m->refine_phase1_digits);
+        j.stall_window = 4 * std::max(m->g.n[0], std::max(m->g.n[1], m->g.nd == 3 ? m->g.n[2] : 0));
         j.state = JOB_PHASE1;
         j.done = 0;
         next_chunk(j);
@@ -752,7 +847,20 @@ int job_begin(fi_model_s* m, RefineJob& j, int max_outer, int max_inner, int nch
 int job_after_chunk(fi_model_s* m, RefineJob& j)
 {
     j.done += j.chunk;
+    j.last = j.chunk;
     j.chunk = 0;
+    if (j.state == JOB_INNER && j.mode == 1) {
+        // lagged PT: the end of every ncheck sweeps is an outer step
+        j.inner_total += j.done;
+        j.since_eta += j.done;
+        for (int id = FI_F_P; id <= FI_F_VZ; ++id)
+            if (m->f[id])
+                hipLaunchKernelGGL(k_add_correction, ew_grid(m->g.elems), dim3(256), 0, m->stream, m->buf(id) - m->g.off,
+                                   m->rf->cur(id) - m->g.off, m->g.elems);
+        m->shadow_dirty = true;
+        ++j.outer;
+        return job_outer_lag(m, j);
+    }
     double rin = 0.0;
     int rc = inner_norm(m, &rin);
     if (rc) return rc;
@@ -765,13 +873,16 @@ int job_after_chunk(fi_model_s* m, RefineJob& j)
             return fail(FI_ERR_NONFINITE, "FP32 nonlinear phase diverged after %d iterations", j.done);
         }
         bool stop = j.done >= j.max_inner;
-        if (j.first < 0.0) j.first = j.best = rin;
-        if (rin <= j.p1_rtol * j.first) {
+        if (rin > j.first) {
+            // still in the start-up transient: measure from the peak
+            j.first = j.best = rin;
+            j.stall = 0;
+        } else if (rin <= j.p1_rtol * j.first) {
             stop = true;
         } else if (rin < 0.5 * j.best) {
             j.best = rin;
             j.stall = 0;
-        } else if (++j.stall >= 4) {
+        } else if ((j.stall += j.last) >= j.stall_window) {
             stop = true;
         }
         if (!stop) {
@@ -785,6 +896,9 @@ int job_after_chunk(fi_model_s* m, RefineJob& j)
         return job_outer(m, j);
     }
     // correction solve
+    if (j.trace)
+        std::fprintf(stderr, "[fastice] refined: outer %d correction sweeps %d inner residual %.3e (%.3e of the defect)\n",
+                     j.outer, j.done, rin, rin / j.r0);
     if (!(rin == rin) || rin > 3.4e38) {
         j.inner_total += j.done;
         j.state = JOB_DONE;

@@ -884,11 +884,20 @@ __global__ __launch_bounds__(256) void k_residuals3(StokesArgs<T> a)
     }
 }
 
+// compute_tau! store; LAG: also the FP32 difference old - new of the stress
+// it replaces (written in place, each location by one thread)
+template <bool LAG, typename T>
+__device__ __forceinline__ void put_tau(T* dst, const T* old, float* lag, long c, T v)
+{
+    if (LAG) lag[c] = (float)(old[c] - v);
+    dst[c] = v;
+}
+
 // compute_tau!: kernels_3d.jl:171-191 (IB variant: kernels_3d.jl:216-254 with
 // its undefined `Pr` location read as the Center location); writes the
 // current stress (txxn == txx)
-template <typename T, bool IB>
-__global__ __launch_bounds__(256) void k_tau3(StokesArgs<T> a)
+template <typename T, bool IB, bool LAG>
+__global__ __launch_bounds__(256) void k_tau3(StokesArgs<T> a, TauLag lg)
 {
     IDX3
     const T ix = a.ix, iy = a.iy, iz = a.iz;
@@ -908,33 +917,35 @@ __global__ __launch_bounds__(256) void k_tau3(StokesArgs<T> a)
             ezz = (Vz[c + sz] - Vz[c]) * iz;
         }
         T divV = exx + eyy + ezz;
-        a.txxn[c] = T(2) * ec * (exx - divV / T(3));
-        a.tyyn[c] = T(2) * ec * (eyy - divV / T(3));
-        a.tzzn[c] = T(2) * ec * (ezz - divV / T(3));
+        put_tau<LAG>(a.txxn, a.txx, lg.t[0], c, T(2) * ec * (exx - divV / T(3)));
+        put_tau<LAG>(a.tyyn, a.tyy, lg.t[1], c, T(2) * ec * (eyy - divV / T(3)));
+        put_tau<LAG>(a.tzzn, a.tzz, lg.t[2], c, T(2) * ec * (ezz - divV / T(3)));
     } else {
-        a.txxn[c] = T(0); a.tyyn[c] = T(0); a.tzzn[c] = T(0);
+        put_tau<LAG>(a.txxn, a.txx, lg.t[0], c, T(0));
+        put_tau<LAG>(a.tyyn, a.tyy, lg.t[1], c, T(0));
+        put_tau<LAG>(a.tzzn, a.tzz, lg.t[2], c, T(0));
     }
     if (fl & F_VVC) {
         T exy;
         if (IB) exy = T(0.5) * ((ns[OM_CVC][c] * Vy[c] - ns[OM_CVC][c - 1] * Vy[c - 1]) * ix +
                                 (ns[OM_VCC][c] * Vx[c] - ns[OM_VCC][c - sy] * Vx[c - sy]) * iy);
         else exy = T(0.5) * ((Vy[c] - Vy[c - 1]) * ix + (Vx[c] - Vx[c - sy]) * iy);
-        a.txyn[c] = T(2) * lerp4(eta[c - 1 - sy], eta[c - sy], eta[c - 1], eta[c]) * exy;
-    } else a.txyn[c] = T(0);
+        put_tau<LAG>(a.txyn, a.txy, lg.t[3], c, T(2) * lerp4(eta[c - 1 - sy], eta[c - sy], eta[c - 1], eta[c]) * exy);
+    } else put_tau<LAG>(a.txyn, a.txy, lg.t[3], c, T(0));
     if (fl & F_VCV) {
         T exz;
         if (IB) exz = T(0.5) * ((ns[OM_CCV][c] * Vz[c] - ns[OM_CCV][c - 1] * Vz[c - 1]) * ix +
                                 (ns[OM_VCC][c] * Vx[c] - ns[OM_VCC][c - sz] * Vx[c - sz]) * iz);
         else exz = T(0.5) * ((Vz[c] - Vz[c - 1]) * ix + (Vx[c] - Vx[c - sz]) * iz);
-        a.txzn[c] = T(2) * lerp4(eta[c - 1 - sz], eta[c - sz], eta[c - 1], eta[c]) * exz;
-    } else a.txzn[c] = T(0);
+        put_tau<LAG>(a.txzn, a.txz, lg.t[4], c, T(2) * lerp4(eta[c - 1 - sz], eta[c - sz], eta[c - 1], eta[c]) * exz);
+    } else put_tau<LAG>(a.txzn, a.txz, lg.t[4], c, T(0));
     if (fl & F_CVV) {
         T eyz;
         if (IB) eyz = T(0.5) * ((ns[OM_CCV][c] * Vz[c] - ns[OM_CCV][c - sy] * Vz[c - sy]) * iy +
                                 (ns[OM_CVC][c] * Vy[c] - ns[OM_CVC][c - sz] * Vy[c - sz]) * iz);
         else eyz = T(0.5) * ((Vz[c] - Vz[c - sy]) * iy + (Vy[c] - Vy[c - sz]) * iz);
-        a.tyzn[c] = T(2) * lerp4(eta[c - sy - sz], eta[c - sz], eta[c - sy], eta[c]) * eyz;
-    } else a.tyzn[c] = T(0);
+        put_tau<LAG>(a.tyzn, a.tyz, lg.t[5], c, T(2) * lerp4(eta[c - sy - sz], eta[c - sz], eta[c - sy], eta[c]) * eyz);
+    } else put_tau<LAG>(a.tyzn, a.tyz, lg.t[5], c, T(0));
 }
 
 static inline dim3 block2();
@@ -1204,8 +1215,8 @@ __global__ __launch_bounds__(256) void k_residuals2(StokesArgs<T> a)
     }
 }
 
-template <typename T, bool IB>
-__global__ __launch_bounds__(256) void k_tau2(StokesArgs<T> a)
+template <typename T, bool IB, bool LAG>
+__global__ __launch_bounds__(256) void k_tau2(StokesArgs<T> a, TauLag lg)
 {
     IDX2
     const T ix = a.ix, iy = a.iy;
@@ -1223,18 +1234,19 @@ __global__ __launch_bounds__(256) void k_tau2(StokesArgs<T> a)
             eyy = (Vy[c + sy] - Vy[c]) * iy;
         }
         T divV = exx + eyy;
-        a.txxn[c] = T(2) * ec * (exx - divV / T(3));
-        a.tyyn[c] = T(2) * ec * (eyy - divV / T(3));
+        put_tau<LAG>(a.txxn, a.txx, lg.t[0], c, T(2) * ec * (exx - divV / T(3)));
+        put_tau<LAG>(a.tyyn, a.tyy, lg.t[1], c, T(2) * ec * (eyy - divV / T(3)));
     } else {
-        a.txxn[c] = T(0); a.tyyn[c] = T(0);
+        put_tau<LAG>(a.txxn, a.txx, lg.t[0], c, T(0));
+        put_tau<LAG>(a.tyyn, a.tyy, lg.t[1], c, T(0));
     }
     if (fl & F_VVC) {
         T exy;
         if (IB) exy = T(0.5) * ((ns[OM2_CV][c] * Vy[c] - ns[OM2_CV][c - 1] * Vy[c - 1]) * ix +
                                 (ns[OM2_VC][c] * Vx[c] - ns[OM2_VC][c - sy] * Vx[c - sy]) * iy);
         else exy = T(0.5) * ((Vy[c] - Vy[c - 1]) * ix + (Vx[c] - Vx[c - sy]) * iy);
-        a.txyn[c] = T(2) * lerp4(eta[c - 1 - sy], eta[c - sy], eta[c - 1], eta[c]) * exy;
-    } else a.txyn[c] = T(0);
+        put_tau<LAG>(a.txyn, a.txy, lg.t[3], c, T(2) * lerp4(eta[c - 1 - sy], eta[c - sy], eta[c - 1], eta[c]) * exy);
+    } else put_tau<LAG>(a.txyn, a.txy, lg.t[3], c, T(0));
 }
 
 // ---------------------------------------------------------------------------
@@ -1408,14 +1420,26 @@ void launch_residuals(const Geo& g, const StokesArgs<T>& a, bool ib, hipStream_t
 }
 
 template <typename T>
-void launch_tau(const Geo& g, const StokesArgs<T>& a, bool ib, hipStream_t s)
+void launch_tau(const Geo& g, const StokesArgs<T>& a, bool ib, hipStream_t s, const TauLag* lag)
 {
+    TauLag lg{};
+    if (lag) lg = *lag;
     if (g.nd == 3) {
-        if (ib) hipLaunchKernelGGL((k_tau3<T, true>), grid3(g), block2(), 0, s, a);
-        else hipLaunchKernelGGL((k_tau3<T, false>), grid3(g), block2(), 0, s, a);
+        if (lag) {
+            if (ib) hipLaunchKernelGGL((k_tau3<T, true, true>), grid3(g), block2(), 0, s, a, lg);
+            else hipLaunchKernelGGL((k_tau3<T, false, true>), grid3(g), block2(), 0, s, a, lg);
+        } else {
+            if (ib) hipLaunchKernelGGL((k_tau3<T, true, false>), grid3(g), block2(), 0, s, a, lg);
+            else hipLaunchKernelGGL((k_tau3<T, false, false>), grid3(g), block2(), 0, s, a, lg);
+        }
     } else {
-        if (ib) hipLaunchKernelGGL((k_tau2<T, true>), grid2(g), block2(), 0, s, a);
-        else hipLaunchKernelGGL((k_tau2<T, false>), grid2(g), block2(), 0, s, a);
+        if (lag) {
+            if (ib) hipLaunchKernelGGL((k_tau2<T, true, true>), grid2(g), block2(), 0, s, a, lg);
+            else hipLaunchKernelGGL((k_tau2<T, false, true>), grid2(g), block2(), 0, s, a, lg);
+        } else {
+            if (ib) hipLaunchKernelGGL((k_tau2<T, true, false>), grid2(g), block2(), 0, s, a, lg);
+            else hipLaunchKernelGGL((k_tau2<T, false, false>), grid2(g), block2(), 0, s, a, lg);
+        }
     }
 }
 
@@ -1460,7 +1484,7 @@ void launch_point_packed(const PackedInst<T>* inst, const PackedBox* boxes, cons
     template void launch_fused<T>(const Geo&, const StokesArgs<T>&, const FusedGeom&, bool, int, int, int,       \
                                   hipStream_t);                                                                 \
     template void launch_residuals<T>(const Geo&, const StokesArgs<T>&, bool, hipStream_t);                      \
-    template void launch_tau<T>(const Geo&, const StokesArgs<T>&, bool, hipStream_t);                      \
+    template void launch_tau<T>(const Geo&, const StokesArgs<T>&, bool, hipStream_t, const TauLag*);                      \
     template void launch_eta_update<T>(const Geo&, const StokesArgs<T>&, hipStream_t);                     \
     template void launch_fused_packed<T>(const PackedInst<T>*, const int4*, int, bool, int, int, int, int,         \
                                          hipStream_t);                                                          \

@@ -228,6 +228,7 @@ class IsothermalFullStokesModel:
         h = C.c_void_p()
         A.check(lib.fi_model_create(C.byref(self.desc), C.byref(h)))
         self._h = h
+        self._refine_mode = 1
         F = A.F
         fld = lambda name: DeviceField(self, F[name])
         nd = grid.ndims
@@ -298,18 +299,22 @@ class IsothermalFullStokesModel:
         self.residual.norms = out
         return it.value, out
 
-    def refined_solve(self, tol, max_outer=100, max_inner=None, ncheck=None, inner_rtol=3e-2):
+    def refined_solve(self, tol, max_outer=None, max_inner=None, ncheck=None, inner_rtol=3e-2):
         """Solve with FP32 PT sweeps inside an FP64 defect-correction loop.
 
         Returns (outer steps, FP32 iterations, Inf-norms).  Same discrete
         solution as solve(); the outer residual is evaluated in FP64.
-        ncheck: FP32 sweeps between inner residual checks (default n; half
-        the driver's check cadence was the fastest on the Glen-law glacier,
-        whose cadence is 5n); inner_rtol: inexact inner solves (3e-2 drop).
+        Lagged PT (refine_mode 1, the default): ncheck = FP32 sweeps per
+        outer step (default 32: longer chunks lag the viscosity enough to
+        destabilise the Glen-law glaciers; the solver halves it, down to 8,
+        when the residual stops halving); max_outer x ncheck caps the FP32
+        sweeps in all (default 400 n), max_inner (default 100 n) the full-FP32
+        nonlinear start.  Correction solves (refine_mode 0): ncheck = sweeps
+        between inner residual checks (default n), max_outer 100, max_inner
+        100 n, inner_rtol the inner solves' residual drop.
         """
         n = max(self.grid.dims)
-        ncheck = ncheck or n
-        max_inner = max_inner or 100 * n
+        max_outer, max_inner, ncheck = refined_defaults(n, self._refine_mode, max_outer, max_inner, ncheck)
         counts = (C.c_int * 3)()
         out = np.zeros(4)
         A.check(A.lib().fi_refined_solve(self._h, int(max_outer), int(max_inner), int(ncheck), float(inner_rtol),
@@ -338,8 +343,20 @@ class IsothermalFullStokesModel:
              "refine_eta_rate_permil": A.FI_OPT_REFINE_ETA_RATE,
              "refine_phase1_digits": A.FI_OPT_REFINE_PHASE1_DIGITS, "fused_variant": A.FI_OPT_FUSED_VARIANT,
              "fused_variant_fp32": A.FI_OPT_FUSED_VARIANT_FP32, "tile_order": A.FI_OPT_TILE_ORDER,
-             "refine_anderson": A.FI_OPT_REFINE_ANDERSON}.get(key, key)
+             "refine_anderson": A.FI_OPT_REFINE_ANDERSON, "refine_mode": A.FI_OPT_REFINE_MODE}.get(key, key)
         A.check(A.lib().fi_set_option(self._h, int(k), int(value)))
+        if k == A.FI_OPT_REFINE_MODE:
+            self._refine_mode = int(value)
+
+
+def refined_defaults(n, mode, max_outer, max_inner, ncheck):
+    """Default (max_outer, max_inner, ncheck) of a refined solve of a grid of
+    n cells along its longest axis."""
+    if mode == 1:
+        # max_inner caps the full-FP32 nonlinear start
+        ncheck = ncheck or 32
+        return max_outer or -(-400 * n // ncheck), max_inner or 100 * n, ncheck
+    return max_outer or 100, max_inner or 100 * n, ncheck or n
 
 
 def advance_iteration(model, t=0.0, dt=1.0):

@@ -174,7 +174,17 @@ int fi_fused_region(fi_model_t m, int lo[3], int hi[3]);
  * relaxation of the outer viscosity update in per mille (default 1000),
  * halved whenever an outer step reduces the residual by less than 10 %;
  * FI_OPT_REFINE_ANDERSON is the depth (0..3, default 2) of the Anderson
- * mixing of the nonlinear outer updates (0: plain defect correction). */
+ * mixing of the nonlinear outer updates (0: plain defect correction).
+ * FI_OPT_REFINE_MODE selects the outer loop: 1 (default) lagged PT, where
+ * every outer step follows exactly ncheck FP32 sweeps, the stress correction
+ * starts from the lag of the PT stress behind 2 eta edot'(V) and the
+ * viscosity is relaxed as the PT iteration would have over those sweeps;
+ * the chunk halves (down to 8 sweeps) whenever the outer residual has not
+ * halved over 8 n_max sweeps; max_outer x ncheck caps the FP32 sweeps in all
+ * (max_inner caps the full-FP32 nonlinear start; inner_rtol, the
+ * Anderson depth and the rate halving are not used); 0 correction solves:
+ * each outer step solves the correction problem to inner_rtol with the
+ * stress lag dropped, the viscosity following a relaxed Picard iteration. */
 enum {
     FI_OPT_FUSED = 1,
     FI_OPT_ZCHUNK = 2,
@@ -184,7 +194,8 @@ enum {
     FI_OPT_FUSED_VARIANT = 6,
     FI_OPT_FUSED_VARIANT_FP32 = 7,
     FI_OPT_TILE_ORDER = 8,
-    FI_OPT_REFINE_ANDERSON = 9
+    FI_OPT_REFINE_ANDERSON = 9,
+    FI_OPT_REFINE_MODE = 10
 };
 int fi_set_option(fi_model_t m, int key, int value);
 /* Compiled variants of the fused sweep (FI_OPT_FUSED_VARIANT* take an index

@@ -161,14 +161,26 @@ def _make_refined(pkg, case):
     return m
 
 
-def test_batch_refined_equals_alone(pkg):
+@pytest.mark.parametrize("mode", [1, 0])
+def test_batch_refined_equals_alone(pkg, mode):
     """FP32-refined solves packed in one batch end bitwise equal to the same
-    refined solves alone: outer steps, FP32 sweeps, residual norms, fields."""
-    _say("case: refined batch of %d mixed models vs alone" % len(REFINED_CASES))
-    args = dict(max_outer=30, max_inner=3000, ncheck=60, inner_rtol=3e-2)
+    refined solves alone: outer steps, FP32 sweeps, residual norms, fields.
+    Lagged PT (mode 1) with a common chunk of 32 sweeps; correction solves
+    (mode 0) with chunks of 60 sweeps between inner checks."""
+    _say("case: refined batch (mode %d) of %d mixed models vs alone" % (mode, len(REFINED_CASES)))
+    if mode == 1:
+        args = dict(max_outer=400, ncheck=32)
+    else:
+        args = dict(max_outer=30, max_inner=3000, ncheck=60, inner_rtol=3e-2)
+
+    def make(case):
+        m = _make_refined(pkg, case)
+        m.set_option("refine_mode", mode)
+        return m
+
     alone = []
     for case in REFINED_CASES:
-        m = _make_refined(pkg, case)
+        m = make(case)
         outer, sweeps, norms = m.refined_solve(1e-8, **args)
         _say("alone %s: outer %d sweeps %d residual %.2e" % (case, outer, sweeps, max(norms)))
         alone.append((outer, sweeps, dict(m.refined_counts), norms.copy(),
@@ -177,7 +189,7 @@ def test_batch_refined_equals_alone(pkg):
     models = []
     with pkg.Batch() as batch:
         for case in REFINED_CASES:
-            m = _make_refined(pkg, case)
+            m = make(case)
             batch.add_refined(m, 1e-8, **args)
             models.append(m)
         steps = 0

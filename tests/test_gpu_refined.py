@@ -69,14 +69,16 @@ def test_refined_manufactured_2d(pkg):
     assert max(errs.values()) < 1e-6, errs
 
 
+@pytest.mark.parametrize("mode", [1, 0])
 @pytest.mark.parametrize("glen", [False, True])
-def test_refined_glacier_matches_fp64(pkg, glen):
-    _say("case: refined vs FP64, immersed-boundary glacier 70x40x20, glen=%s" % glen)
+def test_refined_glacier_matches_fp64(pkg, glen, mode):
+    _say("case: refined (mode %d) vs FP64, immersed-boundary glacier 70x40x20, glen=%s" % (mode, glen))
     a, cad = pkg.glacier_3d((70, 40, 20), seed=2, glen=glen)
     b, _ = pkg.glacier_3d((70, 40, 20), seed=2, glen=glen)
     it, nb = b.solve(400 * 70, cad["ncheck"], 1e-8)
     assert max(nb) < 1e-8, nb
-    outer, inner, na = a.refined_solve(1e-8, max_outer=200)
+    a.set_option("refine_mode", mode)
+    outer, inner, na = a.refined_solve(1e-8, max_outer=200 if mode == 0 else None)
     _say("outer %d inner %d fp64 iters %d" % (outer, inner, it))
     assert max(na) < 1e-8, na
     errs = _field_errors(a, b, ["P", "TXX", "TXY", "VX", "VY", "VZ"])
