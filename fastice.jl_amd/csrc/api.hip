@@ -176,9 +176,9 @@ void run_bc(fi_model_s* m, int group)
     }
 }
 
-int prepare(fi_model_s* m)
+int prepare(fi_model_s* m, bool sync_shadow)
 {
-    if (m->shadow_dirty) {
+    if (sync_shadow && m->shadow_dirty) {
         // positions no kernel or boundary condition writes must hold the same
         // value in both buffers
         const size_t bytes = m->g.elems * sizeof(double);
@@ -247,11 +247,13 @@ double bits_to_double(unsigned long long b)
 }
 
 // enqueue compute_residuals! and the Inf-norms into the pinned h_norm
-int residual_norms_enqueue(fi_model_s* m)
+int residual_norms_enqueue(fi_model_s* m, bool sync_shadow)
 {
     const bool ib = m->desc.immersed_boundary != 0;
-    int rc = prepare(m);
-    if (rc) return rc;
+    if (sync_shadow) {
+        int rc = prepare(m);
+        if (rc) return rc;
+    }
     StokesArgs<double> a = make_args(m, true);
     launch_residuals(m->g, a, ib, m->stream);
     run_bc(m, FI_BCGROUP_RESIDUAL);
@@ -270,9 +272,9 @@ void residual_norms_read(fi_model_s* m, double out[4])
     for (int q = 0; q < 4; ++q) out[q] = (q < nf) ? bits_to_double(m->h_norm[q]) : 0.0;
 }
 
-int residual_norms(fi_model_s* m, double out[4])
+int residual_norms(fi_model_s* m, double out[4], bool sync_shadow)
 {
-    int rc = residual_norms_enqueue(m);
+    int rc = residual_norms_enqueue(m, sync_shadow);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(m->stream));
     residual_norms_read(m, out);

@@ -72,6 +72,7 @@ struct fi_batch_s {
     int tune_runs = 0;
     int rebuilds = 0;
     hipEvent_t ev[2] = {nullptr, nullptr};
+    std::vector<long> sig; // refined batches: what the current tables were built from
 };
 
 namespace fi {
@@ -530,6 +531,25 @@ static int finish_check(BatchInst& bi)
 // the nearest job's check, then runs the checks (and any FP64 outer steps)
 // of the jobs whose chunk is complete.  Tables follow the jobs' buffers and
 // modes, so they are rebuilt every step.
+// what the refined tables depend on: tile options, the running set and each
+// model's buffer parity and phase (full nonlinear or correction)
+static std::vector<long> refined_signature(fi_batch_s* b)
+{
+    std::vector<long> sg = {b->tx, b->ty, b->minw, b->kch, b->order};
+    for (int r : b->run) {
+        fi_model_s* m = b->inst[r].m;
+        const RefineParity p = refine_parity(m);
+        sg.push_back(r);
+        sg.push_back((long)(intptr_t)m);
+        sg.push_back(p.sig | (p.v << 1) | (p.eta << 2) | (refine_full(m) ? 8 : 0));
+    }
+    return sg;
+}
+
+// Refined batches: every running job has a pending chunk of FP32 sweeps; run
+// the smallest of them for all jobs (packed, replayed from graphs while the
+// tables stay valid), then give every job whose chunk ended its check: the
+// checks of all due jobs are enqueued around one synchronisation.
 static int step_refined(fi_batch_s* b, int* n_finished)
 {
     b->run.clear();
@@ -540,20 +560,42 @@ static int step_refined(fi_batch_s* b, int* n_finished)
     for (int r : b->run) delta = std::min(delta, b->inst[r].left);
     if (delta < 1) return fail(FI_ERR_STATE, "refined batch job without pending sweeps");
     for (int r : b->run) {
-        int rc = prepare(b->inst[r].m);
+        int rc = prepare(b->inst[r].m, false);
         if (rc) return rc;
     }
-    int rc = build_tables(b, b->ty, b->kch);
-    if (rc) return rc;
-    for (int it = 0; it < delta; ++it) launch_iteration(b, it & 1);
+    std::vector<long> sg = refined_signature(b);
+    if (b->dirty || sg != b->sig) {
+        int rc = build_tables(b, b->ty, b->kch);
+        if (rc) return rc;
+        rc = capture_graphs(b);
+        if (rc) return rc;
+        b->sig = sg;
+        b->dirty = false;
+    }
+    int done = 0, p = 0;
+    while (b->graph_iters > 0 && b->graph[p] && delta - done >= b->graph_iters) {
+        HIPCHK(hipGraphLaunch(b->graph[p], b->stream));
+        done += b->graph_iters;
+        p ^= (b->graph_iters & 1);
+    }
+    for (; done < delta; ++done, p ^= 1) launch_iteration(b, p);
     HIPCHK(hipGetLastError());
     using S = SweepOf<float>;
+    std::vector<int> due;
     for (int r : b->run) {
         BatchInst& bi = b->inst[r];
         S::set(bi.m, S::flipped(bi.m, S::get(bi.m), delta));
         bi.left -= delta;
-        if (bi.left > 0) continue;
-        rc = job_after_chunk(bi.m, bi.job);
+        if (bi.left <= 0) due.push_back(r);
+    }
+    for (int r : due) {
+        int rc = job_chunk_enqueue(b->inst[r].m, b->inst[r].job);
+        if (rc) return rc;
+    }
+    if (!due.empty()) HIPCHK(hipStreamSynchronize(b->stream));
+    for (int r : due) {
+        BatchInst& bi = b->inst[r];
+        int rc = job_chunk_finish(bi.m, bi.job);
         bi.left = bi.job.chunk;
         bi.iters = bi.job.inner_total;
         for (int q = 0; q < 4; ++q) bi.norms[q] = bi.job.norms[q];
@@ -568,7 +610,6 @@ static int step_refined(fi_batch_s* b, int* n_finished)
             (*n_finished)++;
         }
     }
-    b->dirty = true;
     return FI_OK;
 }
 

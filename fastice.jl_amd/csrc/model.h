@@ -85,11 +85,16 @@ StokesArgs<double> make_args(fi_model_s* m, bool in_place);
 BcBatch stress_batch(fi_model_s* m, int d);
 BcBatch velocity_batch(fi_model_s* m, int d, bool with_eta);
 void run_bc(fi_model_s* m, int group);
-int prepare(fi_model_s* m);
+// before kernels run: the immersed-boundary validity bits and, with
+// sync_shadow, the ping-pong copies the FP64 iteration needs
+int prepare(fi_model_s* m, bool sync_shadow = true);
 void iterate(fi_model_s* m, hipEvent_t* ev);
 int advance(fi_model_s* m, int niter);
-int residual_norms(fi_model_s* m, double out[4]);
-int residual_norms_enqueue(fi_model_s* m);
+// residuals and their Inf-norms of the current state; sync_shadow = false
+// leaves the ping-pong buffers of the FP64 iteration stale (refined solves,
+// which do not iterate in FP64, defer that copy to the next prepare)
+int residual_norms(fi_model_s* m, double out[4], bool sync_shadow = true);
+int residual_norms_enqueue(fi_model_s* m, bool sync_shadow = true);
 void residual_norms_read(fi_model_s* m, double out[4]);
 void batch_touch(fi_model_s* m);  // the model's buffer parity changed outside its batch
 void batch_detach(fi_model_s* m); // remove the model from its batch (model destruction)

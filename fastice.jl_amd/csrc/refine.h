@@ -37,12 +37,18 @@ struct RefineJob {
     AaState aa;
     double norms[4] = {0, 0, 0, 0};
     bool converged = false;
+    bool pending_outer = false; // the enqueued half was a lagged-PT outer step
 };
 
 // job control: begin (may finish at once), sweeps of the pending chunk, and
 // the check after the chunk has run
 int job_begin(fi_model_s* m, RefineJob& j, int max_outer, int max_inner, int ncheck, double inner_rtol, double tol);
 int job_after_chunk(fi_model_s* m, RefineJob& j);
+// job_after_chunk in two halves: enqueue the device work the decision needs,
+// then (after the stream has synchronised) decide; a batch enqueues the
+// halves of all its due jobs around one synchronisation
+int job_chunk_enqueue(fi_model_s* m, RefineJob& j);
+int job_chunk_finish(fi_model_s* m, RefineJob& j);
 
 // pieces of one FP32 sweep for packed launches: the sweep's kernel arguments
 // (current buffers in, other buffers out), the boundary batch of one group

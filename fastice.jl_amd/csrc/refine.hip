@@ -408,7 +408,9 @@ static void iterate32(fi_model_s* m, hipEvent_t* ev)
     if (r->full) r->eta_cur ^= 1;
 }
 
-static int inner_norm(fi_model_s* m, double* out)
+// FP32 residual Inf-norm of the correction state: enqueue, then read once
+// the stream has synchronised
+static int inner_norm_enqueue(fi_model_s* m)
 {
     Refine* r = m->rf;
     const bool ib = m->desc.immersed_boundary != 0;
@@ -429,7 +431,13 @@ static int inner_norm(fi_model_s* m, double* out)
     blocks = blocks > 1024 ? 1024 : (blocks < 1 ? 1 : blocks);
     hipLaunchKernelGGL(k_norms32, dim3(blocks, nf, 1), dim3(256), 0, m->stream, na, m->g.sy, m->g.sz, m->g.nd, r->d_norm);
     HIPCHK(hipMemcpyAsync(r->h_norm, r->d_norm, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, m->stream));
-    HIPCHK(hipStreamSynchronize(m->stream));
+    return FI_OK;
+}
+
+static void inner_norm_read(fi_model_s* m, double* out)
+{
+    Refine* r = m->rf;
+    const int nf = m->g.nd == 3 ? 4 : 3;
     double mx = 0.0;
     for (int q = 0; q < nf; ++q) {
         unsigned b = (unsigned)r->h_norm[q];
@@ -437,11 +445,19 @@ static int inner_norm(fi_model_s* m, double* out)
         std::memcpy(&v, &b, sizeof(v));
         if (!(v == v) || b >= 0x7f800000u) {
             *out = (double)v;
-            return FI_OK;
+            return;
         }
         mx = (double)v > mx ? (double)v : mx;
     }
     *out = mx;
+}
+
+static int inner_norm(fi_model_s* m, double* out)
+{
+    int rc = inner_norm_enqueue(m);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(m->stream));
+    inner_norm_read(m, out);
     return FI_OK;
 }
 
@@ -669,7 +685,7 @@ static int job_outer(fi_model_s* m, RefineJob& j)
         run_bc(m, FI_BCGROUP_STRESS);
         m->shadow_dirty = true;
     }
-    int rc = residual_norms(m, j.norms);
+    int rc = residual_norms(m, j.norms, false);
     if (rc) return rc;
     double r0 = 0.0;
     for (int q = 0; q < 4; ++q) {
@@ -720,7 +736,7 @@ static int job_outer(fi_model_s* m, RefineJob& j)
 // updates).  eta itself is updated here, between chunks, by the relaxation
 // the PT iteration applies every sweep, compounded over the chunk:
 // rate_C = 1 - (1 - rate)^C.
-static int job_outer_lag(fi_model_s* m, RefineJob& j)
+static int job_outer_lag_enqueue(fi_model_s* m, RefineJob& j)
 {
     Refine* r = m->rf;
     const bool ib = m->desc.immersed_boundary != 0;
@@ -750,8 +766,20 @@ static int job_outer_lag(fi_model_s* m, RefineJob& j)
         m->shadow_dirty = true;
     }
     bc32(m, FI_BCGROUP_STRESS); // homogeneous boundary values of the lag
-    int rc = residual_norms(m, j.norms);
+    int rc = residual_norms_enqueue(m, false);
     if (rc) return rc;
+    // forcing and frozen viscosity of the next chunk (unused if the solve ends here)
+    cast(m, r->eta, m->eta(), 1.0);
+    cast(m, r->fP, m->f[FI_F_RP], 1.0);
+    const int idR[3] = {FI_F_RVX, FI_F_RVY, FI_F_RVZ};
+    for (int q = 0; q < nd; ++q) cast(m, r->g[q], m->f[idR[q]], -1.0);
+    return FI_OK;
+}
+
+// after the stream has synchronised: the outer residual decides
+static int job_outer_lag_finish(fi_model_s* m, RefineJob& j)
+{
+    residual_norms_read(m, j.norms);
     double r0 = 0.0;
     for (int q = 0; q < 4; ++q) {
         if (!(j.norms[q] == j.norms[q]) || j.norms[q] > 1.79e308) {
@@ -782,14 +810,18 @@ static int job_outer_lag(fi_model_s* m, RefineJob& j)
         j.lag_stall = 0;
         if (j.trace) std::fprintf(stderr, "[fastice] refined (lagged PT): chunk now %d sweeps\n", j.ncheck);
     }
-    cast(m, r->eta, m->eta(), 1.0);
-    cast(m, r->fP, m->f[FI_F_RP], 1.0);
-    const int idR[3] = {FI_F_RVX, FI_F_RVY, FI_F_RVZ};
-    for (int q = 0; q < nd; ++q) cast(m, r->g[q], m->f[idR[q]], -1.0);
     j.state = JOB_INNER;
     j.done = 0;
     next_chunk(j);
     return FI_OK;
+}
+
+static int job_outer_lag(fi_model_s* m, RefineJob& j)
+{
+    int rc = job_outer_lag_enqueue(m, j);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(m->stream));
+    return job_outer_lag_finish(m, j);
 }
 
 int job_begin(fi_model_s* m, RefineJob& j, int max_outer, int max_inner, int ncheck, double inner_rtol,
@@ -799,7 +831,7 @@ int job_begin(fi_model_s* m, RefineJob& j, int max_outer, int max_inner, int nch
         return fail(FI_ERR_ARG, "bad refinement parameters");
     int rc = ensure_refine(m);
     if (rc) return rc;
-    rc = prepare(m);
+    rc = prepare(m, false);
     if (rc) return rc;
     Refine* r = m->rf;
     if (m->desc.immersed_boundary && r->mask_gen != m->mask_gen) {
@@ -844,7 +876,7 @@ int job_begin(fi_model_s* m, RefineJob& j, int max_outer, int max_inner, int nch
 }
 
 // after the pending chunk of sweeps has run
-int job_after_chunk(fi_model_s* m, RefineJob& j)
+int job_chunk_enqueue(fi_model_s* m, RefineJob& j)
 {
     j.done += j.chunk;
     j.last = j.chunk;
@@ -859,11 +891,19 @@ int job_after_chunk(fi_model_s* m, RefineJob& j)
                                    m->rf->cur(id) - m->g.off, m->g.elems);
         m->shadow_dirty = true;
         ++j.outer;
-        return job_outer_lag(m, j);
+        j.pending_outer = true;
+        return job_outer_lag_enqueue(m, j);
     }
+    j.pending_outer = false;
+    return inner_norm_enqueue(m);
+}
+
+int job_chunk_finish(fi_model_s* m, RefineJob& j)
+{
+    if (j.pending_outer) return job_outer_lag_finish(m, j);
     double rin = 0.0;
-    int rc = inner_norm(m, &rin);
-    if (rc) return rc;
+    inner_norm_read(m, &rin);
+    int rc = FI_OK;
     if (j.state == JOB_PHASE1) {
         if (!(rin == rin) || rin > 3.4e38) {
             j.inner_total += j.done;
@@ -924,6 +964,14 @@ int job_after_chunk(fi_model_s* m, RefineJob& j)
     return job_outer(m, j);
 }
 
+int job_after_chunk(fi_model_s* m, RefineJob& j)
+{
+    int rc = job_chunk_enqueue(m, j);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(m->stream));
+    return job_chunk_finish(m, j);
+}
+
 } // namespace fi
 
 using namespace fi;
@@ -958,7 +1006,7 @@ int fi_refined_time_inner(fi_model_t m, int n_iter, double out_ms[4])
     if (!m || !out_ms || n_iter < 1) return fail(FI_ERR_ARG, "bad arguments");
     int rc = ensure_refine(m);
     if (rc) return rc;
-    rc = prepare(m);
+    rc = prepare(m, false);
     if (rc) return rc;
     hipEvent_t ev[6];
     for (int q = 0; q < 6; ++q) HIPCHK(hipEventCreate(&ev[q]));
@@ -983,7 +1031,7 @@ int fi_refined_advance_inner(fi_model_t m, int n_iter)
     if (!m || n_iter < 0) return fail(FI_ERR_ARG, "bad arguments");
     int rc = ensure_refine(m);
     if (rc) return rc;
-    rc = prepare(m);
+    rc = prepare(m, false);
     if (rc) return rc;
     for (int it = 0; it < n_iter; ++it) iterate32(m, nullptr);
     HIPCHK(hipGetLastError());
