@@ -31,6 +31,24 @@ struct Geo {
     size_t elems; // elements per allocation
 };
 
+// Volume fractions as the sweeps read them: FP64 sweeps read the FP64 masks;
+// FP32 sweeps (the corrections of the refined solver, whose FP64 residuals
+// use the FP64 masks) read them quantised to q / 255, q in 0..255, which is
+// exact for empty and full locations (255 * (1/255) rounds to 1 in FP32)
+// and within 1/510 elsewhere.
+struct U8Mask {
+    const uint8_t* p;
+    __device__ __forceinline__ float operator[](long i) const { return (float)p[i] * (1.0f / 255.0f); }
+};
+template <typename T>
+struct MaskOf {
+    using ptr = const T*;
+};
+template <>
+struct MaskOf<float> {
+    using ptr = U8Mask;
+};
+
 // Arguments of the PT sweep / residual kernels (passed by value).  The
 // stress and velocity state is double buffered: kernels read the previous
 // iteration (P, txx, ..., Vx, ...) and write this iteration (Pn, ..., Vxn,
@@ -46,8 +64,8 @@ struct StokesArgs {
     const T *gx, *gy, *gz;
     const T* rh;
     T *rP, *rVx, *rVy, *rVz;
-    const T* na[8];
-    const T* ns[8];
+    typename MaskOf<T>::ptr na[8];
+    typename MaskOf<T>::ptr ns[8];
     const uint8_t* flags; // immersed-boundary validity bits (F_* below)
     const T* fP;          // forcing of the continuity equation (defect correction), or null
     int frozen_eta;       // 1: eta is fixed, eta_next is not written

@@ -46,8 +46,8 @@ struct Refine {
     bool full = false;      // sweeps solve the full state (true) or the correction (false)
     float* fP = nullptr;    // continuity forcing r_P
     float* g[3] = {};       // momentum forcing -r_V
-    float* wna[8] = {};     // FP32 masks (IB)
-    float* wns[8] = {};
+    uint8_t* wna[8] = {};   // masks of the FP32 sweeps (IB), quantised to q / 255
+    uint8_t* wns[8] = {};
     float* r[4] = {};       // inner residuals
     std::vector<void*> allocs;
     int sig_cur = 0, v_cur = 0;
@@ -77,6 +77,15 @@ __global__ __launch_bounds__(256) void k_cast(float* dst, const double* src, dou
 {
     for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
         dst[t] = (float)(scale * src[t]);
+}
+
+// volume fraction in [0, 1] -> q = round(255 w)
+__global__ __launch_bounds__(256) void k_quantise_mask(uint8_t* dst, const double* src, size_t n)
+{
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+        const double w = src[t] < 0.0 ? 0.0 : (src[t] > 1.0 ? 1.0 : src[t]);
+        dst[t] = (uint8_t)rint(255.0 * w);
+    }
 }
 
 __global__ __launch_bounds__(256) void k_fill32(float* p, size_t n, float v)
@@ -236,6 +245,17 @@ static int alloc32(fi_model_s* m, float** out)
     return FI_OK;
 }
 
+static int alloc_mask8(fi_model_s* m, uint8_t** out)
+{
+    Refine* r = m->rf;
+    uint8_t* base = nullptr;
+    if (hipMalloc(&base, m->g.elems) != hipSuccess) return fail(FI_ERR_OOM, "FP32 mask allocation failed");
+    r->allocs.push_back(base);
+    HIPCHK(hipMemsetAsync(base, 0, m->g.elems, m->stream));
+    *out = base + m->g.off;
+    return FI_OK;
+}
+
 static int ensure_refine(fi_model_s* m)
 {
     if (m->rf) return FI_OK;
@@ -259,8 +279,8 @@ static int ensure_refine(fi_model_s* m)
         const int na3[4] = {OM_CCC, OM_VVC, OM_VCV, OM_CVV}, ns3[3] = {OM_VCC, OM_CVC, OM_CCV};
         const int na2[2] = {OM2_CC, OM2_VV}, ns2[2] = {OM2_VC, OM2_CV};
         const int nna = nd == 3 ? 4 : 2, nns = nd == 3 ? 3 : 2;
-        for (int q = 0; q < nna && !rc; ++q) rc = alloc32(m, &r->wna[nd == 3 ? na3[q] : na2[q]]);
-        for (int q = 0; q < nns && !rc; ++q) rc = alloc32(m, &r->wns[nd == 3 ? ns3[q] : ns2[q]]);
+        for (int q = 0; q < nna && !rc; ++q) rc = alloc_mask8(m, &r->wna[nd == 3 ? na3[q] : na2[q]]);
+        for (int q = 0; q < nns && !rc; ++q) rc = alloc_mask8(m, &r->wns[nd == 3 ? ns3[q] : ns2[q]]);
     }
     if (!rc && (hipMalloc(&r->d_norm, 4 * sizeof(unsigned long long)) != hipSuccess ||
                 hipHostMalloc(&r->h_norm, 4 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess))
@@ -307,8 +327,8 @@ static StokesArgs<float> make_args32(fi_model_s* m, bool in_place)
     }
     a.rP = r->r[0]; a.rVx = r->r[1]; a.rVy = r->r[2]; a.rVz = r->r[3];
     for (int q = 0; q < 8; ++q) {
-        a.na[q] = r->wna[q];
-        a.ns[q] = r->wns[q];
+        a.na[q] = U8Mask{r->wna[q]};
+        a.ns[q] = U8Mask{r->wns[q]};
     }
     a.flags = m->flags;
     a.nx = m->g.n[0]; a.ny = m->g.n[1]; a.nz = m->g.n[2];
@@ -464,6 +484,12 @@ static int inner_norm(fi_model_s* m, double* out)
 static void cast(fi_model_s* m, float* dst, const double* src, double scale)
 {
     hipLaunchKernelGGL(k_cast, ew_grid(m->g.elems), dim3(256), 0, m->stream, dst - m->g.off, src - m->g.off, scale,
+                       m->g.elems);
+}
+
+static void quantise_mask(fi_model_s* m, uint8_t* dst, const double* src)
+{
+    hipLaunchKernelGGL(k_quantise_mask, ew_grid(m->g.elems), dim3(256), 0, m->stream, dst - m->g.off, src - m->g.off,
                        m->g.elems);
 }
 
@@ -836,8 +862,8 @@ int job_begin(fi_model_s* m, RefineJob& j, int max_outer, int max_inner, int nch
     Refine* r = m->rf;
     if (m->desc.immersed_boundary && r->mask_gen != m->mask_gen) {
         for (int q = 0; q < 8; ++q) {
-            if (r->wna[q]) cast(m, r->wna[q], m->f[FI_F_OM_NA0 + q], 1.0);
-            if (r->wns[q]) cast(m, r->wns[q], m->f[FI_F_OM_NS0 + q], 1.0);
+            if (r->wna[q]) quantise_mask(m, r->wna[q], m->f[FI_F_OM_NA0 + q]);
+            if (r->wns[q]) quantise_mask(m, r->wns[q], m->f[FI_F_OM_NS0 + q]);
         }
         r->mask_gen = m->mask_gen;
     }

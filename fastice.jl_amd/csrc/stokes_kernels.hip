@@ -149,7 +149,7 @@ __device__ __forceinline__ void sigma_point3(const StokesArgs<T>& a, long c, lon
         a.txzn[c] = upd_to(a.txz[c], exz_eta, exz, a.dtau[4]);
         a.tyzn[c] = upd_to(a.tyz[c], eyz_eta, eyz, a.dtau[5]);
     } else {
-        const T *wx = a.ns[OM_VCC], *wy = a.ns[OM_CVC], *wz = a.ns[OM_CCV];
+        const auto wx = a.ns[OM_VCC], wy = a.ns[OM_CVC], wz = a.ns[OM_CCV];
         const unsigned fl = a.flags[c];
         if (fl & F_CCC) {
             T exx = dif(wx[c + 1] * Vx[c + 1], wx[c] * Vx[c], ix);
@@ -199,7 +199,7 @@ __device__ __forceinline__ void velocity_point3(const StokesArgs<T>& a, long c, 
         a.Vzn[c] = upd_V(a.Vz[c], dif(P[c], P[c - sz], iz), dif(tzz[c], tzz[c - sz], iz), dif(txz[c + 1], txz[c], ix),
                          dif(tyz[c + sy], tyz[c], iy), gz, ez, a.dV[2]);
     } else {
-        const T *wc = a.na[OM_CCC], *wxy = a.na[OM_VVC], *wxz = a.na[OM_VCV], *wyz = a.na[OM_CVV];
+        const auto wc = a.na[OM_CCC], wxy = a.na[OM_VVC], wxz = a.na[OM_VCV], wyz = a.na[OM_CVV];
         const unsigned fl = a.flags[c];
         a.Vxn[c] = (fl & F_VCC)
                        ? upd_V_ib(a.Vx[c], dif(wc[c] * P[c], wc[c - 1] * P[c - 1], ix), dif(wc[c] * txx[c], wc[c - 1] * txx[c - 1], ix),
@@ -280,6 +280,8 @@ __device__ __forceinline__ T ldo(const T* base, uint32_t bo)
 {
     return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + bo);
 }
+// a mask read at the byte offset of the FP32 fields it weights
+__device__ __forceinline__ float ldo(U8Mask base, uint32_t bo) { return base[bo / sizeof(float)]; }
 template <typename T>
 __device__ __forceinline__ void sto(T* base, uint32_t bo, T v)
 {
@@ -410,7 +412,7 @@ __device__ __forceinline__ void fused3_tile(const StokesArgs<T>& a, const FusedG
                 yzc = upd_to(oyz, el, eyz, a.dtau[5]);
             }
         } else {
-            const T *wx = a.ns[OM_VCC], *wy = a.ns[OM_CVC], *wz = a.ns[OM_CCV];
+            const auto wx = a.ns[OM_VCC], wy = a.ns[OM_CVC], wz = a.ns[OM_CCV];
             const unsigned fl = ldo(a.flags, c / E);
             const T wx0 = ldo(wx, c), wy0 = ldo(wy, c), wz0 = ldo(wz, c);
             if (can_c && (fl & F_CCC)) {
@@ -481,7 +483,7 @@ __device__ __forceinline__ void fused3_tile(const StokesArgs<T>& a, const FusedG
                 sto(a.Vyn, cv, upd_V(vym, dif(P_, P_jm, iy), dif(yyk, yy_jm, iy), dif(xy_ip, xy, ix), dif(yz_kp, yz, iz), g1, ey, a.dV[1]));
                 sto(a.Vzn, cv, upd_V(vzm, dif(P_, Pkm, iz), dif(zzk, zzkm, iz), dif(xz_ip, xz, ix), dif(yz_jp, yz, iy), g2, ez, a.dV[2]));
             } else {
-                const T *wc = a.na[OM_CCC], *wxy = a.na[OM_VVC], *wxz = a.na[OM_VCV], *wyz = a.na[OM_CVV];
+                const auto wc = a.na[OM_CCC], wxy = a.na[OM_VVC], wxz = a.na[OM_VCV], wyz = a.na[OM_CVV];
                 const unsigned fl = ldo(a.flags, cv / E);
                 const T wc0 = ldo(wc, cv), wxy0 = ldo(wxy, cv), wxz0 = ldo(wxz, cv), wyz0 = ldo(wyz, cv);
                 sto(a.Vxn, cv, (fl & F_VCC) ? upd_V_ib(vxm, dif(wc0 * P_, ldo(wc, cv - E) * P_im, ix), dif(wc0 * xxk, ldo(wc, cv - E) * xx_im, ix),
@@ -565,8 +567,8 @@ __device__ __forceinline__ void fused3_ib_tile(const StokesArgs<T>& a, const Fus
     const int ic = min(i, xe + 1), jc = min(j, fg.hi[1] + 1);
     const uint32_t col = ((uint32_t)ic + (uint32_t)a.sy * (uint32_t)jc) * E;
     const T ix = a.ix, iy = a.iy, iz = a.iz;
-    const T *wvx = a.ns[OM_VCC], *wvy = a.ns[OM_CVC], *wvz = a.ns[OM_CCV];
-    const T *wc = a.na[OM_CCC], *wxy = a.na[OM_VVC], *wxz = a.na[OM_VCV], *wyz = a.na[OM_CVV];
+    const auto wvx = a.ns[OM_VCC], wvy = a.ns[OM_CVC], wvz = a.ns[OM_CCV];
+    const auto wc = a.na[OM_CCC], wxy = a.na[OM_VVC], wxz = a.na[OM_VCV], wyz = a.na[OM_CVV];
     // old-stress components this thread's row needs (as in fused3_tile)
     const unsigned rneed = ty == 0 ? 0x05u : ty == TY - 1 ? 0x50u : 0x7fu;
 
@@ -860,10 +862,10 @@ __global__ __launch_bounds__(256) void k_residuals3(StokesArgs<T> a)
         a.rVz[c] = -((P[c] - P[c - sz]) * iz) + (tzz[c] - tzz[c - sz]) * iz + (txz[c + 1] - txz[c]) * ix +
                    (tyz[c + sy] - tyz[c]) * iy - gz;
     } else {
-        const T* const* na = a.na;
-        const T* const* ns = a.ns;
-        const T *wx = ns[OM_VCC], *wy = ns[OM_CVC], *wz = ns[OM_CCV];
-        const T *wc = na[OM_CCC], *wxy = na[OM_VVC], *wxz = na[OM_VCV], *wyz = na[OM_CVV];
+        const auto* na = a.na;
+        const auto* ns = a.ns;
+        const auto wx = ns[OM_VCC], wy = ns[OM_CVC], wz = ns[OM_CCV];
+        const auto wc = na[OM_CCC], wxy = na[OM_VVC], wxz = na[OM_VCV], wyz = na[OM_CVV];
         const unsigned fl = a.flags[c];
         if (fl & F_CCC)
             a.rP[c] = (wx[c + 1] * Vx[c + 1] - wx[c] * Vx[c]) * ix + (wy[c + sy] * Vy[c + sy] - wy[c] * Vy[c]) * iy +
@@ -902,7 +904,7 @@ __global__ __launch_bounds__(256) void k_tau3(StokesArgs<T> a, TauLag lg)
     IDX3
     const T ix = a.ix, iy = a.iy, iz = a.iz;
     const T *Vx = a.Vx, *Vy = a.Vy, *Vz = a.Vz, *eta = a.eta;
-    const T* const* ns = a.ns;
+    const auto* ns = a.ns;
     const unsigned fl = IB ? a.flags[c] : 0xffu;
     T ec = eta[c];
     if (fl & F_CCC) {
@@ -1090,7 +1092,7 @@ __device__ __forceinline__ void sigma_point2(const StokesArgs<T>& a, long c, lon
         a.tyyn[c] = upd_td(a.tyy[c], ec, eyy, divV, a.dtau[1]);
         a.txyn[c] = upd_to(a.txy[c], exy_eta, exy, a.dtau[2]);
     } else {
-        const T *wx = a.ns[OM2_VC], *wy = a.ns[OM2_CV];
+        const auto wx = a.ns[OM2_VC], wy = a.ns[OM2_CV];
         const unsigned fl = a.flags[c];
         if (fl & F_CCC) {
             T exx = dif(wx[c + 1] * Vx[c + 1], wx[c] * Vx[c], ix);
@@ -1125,7 +1127,7 @@ __device__ __forceinline__ void velocity_point2(const StokesArgs<T>& a, long c, 
         a.Vxn[c] = upd_V(a.Vx[c], dif(P[c], P[c - 1], ix), dif(txx[c], txx[c - 1], ix), dif(txy[c + sy], txy[c], iy), zero, gx, ex, a.dV[0]);
         a.Vyn[c] = upd_V(a.Vy[c], dif(P[c], P[c - sy], iy), dif(tyy[c], tyy[c - sy], iy), dif(txy[c + 1], txy[c], ix), zero, gy, ey, a.dV[1]);
     } else {
-        const T *wc = a.na[OM2_CC], *wv = a.na[OM2_VV];
+        const auto wc = a.na[OM2_CC], wv = a.na[OM2_VV];
         const unsigned fl = a.flags[c];
         a.Vxn[c] = (fl & F_VCC) ? upd_V_ib(a.Vx[c], dif(wc[c] * P[c], wc[c - 1] * P[c - 1], ix), dif(wc[c] * txx[c], wc[c - 1] * txx[c - 1], ix),
                                            dif(wv[c + sy] * txy[c + sy], wv[c] * txy[c], iy), zero, gx, ex, a.ns[OM2_VC][c], a.dV[0])
@@ -1198,7 +1200,7 @@ __global__ __launch_bounds__(256) void k_residuals2(StokesArgs<T> a)
         a.rVx[c] = -((P[c] - P[c - 1]) * ix) + (txx[c] - txx[c - 1]) * ix + (txy[c + sy] - txy[c]) * iy - gx;
         a.rVy[c] = -((P[c] - P[c - sy]) * iy) + (tyy[c] - tyy[c - sy]) * iy + (txy[c + 1] - txy[c]) * ix - gy;
     } else {
-        const T *wx = a.ns[OM2_VC], *wy = a.ns[OM2_CV], *wc = a.na[OM2_CC], *wv = a.na[OM2_VV];
+        const auto wx = a.ns[OM2_VC], wy = a.ns[OM2_CV], wc = a.na[OM2_CC], wv = a.na[OM2_VV];
         const unsigned fl = a.flags[c];
         if (fl & F_CCC)
             a.rP[c] = (wx[c + 1] * Vx[c + 1] - wx[c] * Vx[c]) * ix + (wy[c + sy] * Vy[c + sy] - wy[c] * Vy[c]) * iy +
@@ -1221,7 +1223,7 @@ __global__ __launch_bounds__(256) void k_tau2(StokesArgs<T> a, TauLag lg)
     IDX2
     const T ix = a.ix, iy = a.iy;
     const T *Vx = a.Vx, *Vy = a.Vy, *eta = a.eta;
-    const T* const* ns = a.ns;
+    const auto* ns = a.ns;
     const unsigned fl = IB ? a.flags[c] : 0xffu;
     T ec = eta[c];
     if (fl & F_CCC) {
