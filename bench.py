@@ -19,6 +19,8 @@ Workloads
       fp64: plain FP64 PT with HIP graphs); value = solves/s; the same models
       are then reset and a sample of them solved one at a time for the
       comparison.
+  h5_glacier_255x255x127: HDF5 + XDMF3 output (src/Writers.jl) of the fields
+      the glacier driver saves, from device memory; value = GB/s of data.
   serving_poisson40: Poisson arrivals at 40/s, half 2D glacier sections
       (n x n/2, n in 64..511), half 3D glaciers (n x n x n/2, n in 32..127),
       FP64 to 1e-8 or niter = 50n, continuously batched; value = completed
@@ -50,6 +52,7 @@ WORKLOADS = {
     "stokes3d_fp64_511x511x255": (511, 511, 255),
     "batch96_glacier": None,
     "serving_poisson40": None,
+    "h5_glacier_255x255x127": None,
 }
 
 
@@ -286,6 +289,61 @@ def bench_batch(pkg, args):
     print(json.dumps(line), flush=True)
 
 
+def bench_h5(pkg, args):
+    """Output of a solved glacier: write_h5 + write_xdmf of the fields the
+    glacier driver saves (test_vavilov.jl:206, 237-247) at 255x255x127."""
+    import shutil
+    import tempfile
+    dims = (255, 255, 127)
+    m, cad = pkg.glacier_3d(dims, seed=1, glen=True)
+    m.advance(200)
+    m.synchronize()
+    out = tempfile.mkdtemp(prefix="fastice_h5_")
+    cells = dims[0] * dims[1] * dims[2]
+    nbytes = 4 * 8 * cells
+    reps = max(1, min(args.steps, 5))
+    try:
+        h5 = os.path.join(out, "results.h5")
+        xdmf = os.path.join(out, "results.xdmf3")
+
+        def once():
+            fields = pkg.writers.glacier_output_fields(m)
+            pkg.writers.write_h5(m.grid, h5, fields)
+            pkg.writers.write_xdmf(m.grid, xdmf, fields, ("results.h5",))
+            return fields
+
+        once()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            once()
+        t = (time.perf_counter() - t0) / reps
+        # the same arrays from host memory: the HDF5 write alone
+        host = {k: (v.interior() if hasattr(v, "interior") else v) for k, v in pkg.writers.glacier_output_fields(m).items()}
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            pkg.writers.write_h5(m.grid, h5, host)
+        th = (time.perf_counter() - t0) / reps
+        ok = all(np.array_equal(pkg.writers.read_h5(h5, k), v) for k, v in host.items())
+        size = os.path.getsize(h5)
+    finally:
+        shutil.rmtree(out, ignore_errors=True)
+        m.close()
+    log("h5: %.3f s per write (%.2f GB/s), host arrays %.3f s, file %d B, round trip %s" % (t, nbytes / t / 1e9, th,
+                                                                                            size, ok))
+    config = {"workload": "h5_glacier_255x255x127", "grid": list(dims),
+              "fields": ["Pr", "Vm", "wt_ns_c", "wt_na_c"],
+              "description": "device fields of a seeded glacier after 200 PT iterations -> interior download, "
+                             "velocity magnitude, HDF5 datasets and XDMF3 (Writers.jl) on the box's local disk",
+              "bytes_per_write": nbytes, "file_bytes": size, "round_trip_equal": bool(ok),
+              "host_arrays_seconds": round(th, 4)}
+    line = _line("HDF5+XDMF3 output GB/s (4 glacier fields, 255x255x127)", nbytes / t / 1e9, "GB/s", t * 1e3, reps, 1,
+                 "f64", config)
+    line["roofline"] = None
+    line["cpu_baseline"] = {"value": round(nbytes / th / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
+                            "sample": "the same four arrays written from host memory (HDF5 write only)"}
+    print(json.dumps(line), flush=True)
+
+
 def bench_serving(pkg, args):
     """Continuous-batching service under Poisson arrivals at 40/s."""
     reqs = pkg.poisson_requests(args.requests, 40.0, seed=7)
@@ -346,6 +404,8 @@ def main():
         return bench_batch(pkg, args)
     if args.workload == "serving_poisson40":
         return bench_serving(pkg, args)
+    if args.workload == "h5_glacier_255x255x127":
+        return bench_h5(pkg, args)
     dims = WORKLOADS[args.workload]
     cells = dims[0] * dims[1] * dims[2]
     glacier = args.workload.startswith("glacier")

@@ -313,6 +313,7 @@ void free_model(fi_model_s* m)
     if (m->d_norm) (void)hipFree(m->d_norm);
     if (m->h_norm) (void)hipHostFree(m->h_norm);
     if (m->d_dem) (void)hipFree(m->d_dem);
+    if (m->d_stage) (void)hipFree(m->d_stage);
     if (m->stream) (void)hipStreamDestroy(m->stream);
     delete m;
 }
@@ -469,6 +470,19 @@ static void mark_dirty(fi_model_t m, int id)
     if (m->batch) batch_touch(m); // the batch re-prepares the model before its next step
 }
 
+// interior of a field <-> contiguous x-fastest array: stage[t], t = i + mx (j + my k),
+// is the field at Chmy index (i+1, j+1, k+1) (sz = 0 in 2D)
+__global__ __launch_bounds__(256) void k_interior_stage(double* stage, double* field, int mx, int my, long n, long sy,
+                                                        long sz, int unpack)
+{
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        const long i = t % mx, r = t / mx, j = r % my, k = r / my;
+        const long c = 1 + i + sy * (1 + j) + sz * (1 + k);
+        if (unpack) field[c] = stage[t];
+        else stage[t] = field[c];
+    }
+}
+
 static int field_copy(fi_model_t m, int id, double* host, size_t count, int layout, bool up)
 {
     if (!m) return fail(FI_ERR_ARG, "null model");
@@ -490,13 +504,29 @@ static int field_copy(fi_model_t m, int id, double* host, size_t count, int layo
         const int L = field_loc(g.nd, id);
         const int mx = g.n[0] + (L & 1), my = g.n[1] + ((L >> 1) & 1);
         const int mz = (g.nd == 3) ? g.n[2] + ((L >> 2) & 1) : 1;
-        const size_t w = (size_t)mx * sizeof(double);
-        for (int k = 0; k < mz; ++k) {
-            double* d0 = dev + 1 + g.sy + ((g.nd == 3) ? g.sz * (k + 1) : 0);
-            double* h0 = host + (size_t)k * mx * my;
-            if (up) HIPCHK(hipMemcpy2D(d0, g.sy * sizeof(double), h0, w, w, my, hipMemcpyHostToDevice));
-            else HIPCHK(hipMemcpy2D(h0, w, d0, g.sy * sizeof(double), w, my, hipMemcpyDeviceToHost));
+        const size_t n = (size_t)mx * my * mz;
+        if (n > m->stage_elems) {
+            if (m->d_stage) HIPCHK(hipFree(m->d_stage));
+            m->d_stage = nullptr;
+            m->stage_elems = 0;
+            if (hipMalloc(&m->d_stage, n * sizeof(double)) != hipSuccess)
+                return fail(FI_ERR_OOM, "transfer staging (%zu bytes)", n * sizeof(double));
+            m->stage_elems = n;
         }
+        const long sz = g.nd == 3 ? g.sz : 0;
+        size_t blocks = (n + 255) / 256;
+        if (blocks > 8192) blocks = 8192;
+        if (up) {
+            HIPCHK(hipMemcpyAsync(m->d_stage, host, n * sizeof(double), hipMemcpyHostToDevice, m->stream));
+            hipLaunchKernelGGL(k_interior_stage, dim3((unsigned)blocks), dim3(256), 0, m->stream, m->d_stage, dev, mx, my,
+                               (long)n, g.sy, sz, 1);
+        } else {
+            hipLaunchKernelGGL(k_interior_stage, dim3((unsigned)blocks), dim3(256), 0, m->stream, m->d_stage, dev, mx, my,
+                               (long)n, g.sy, sz, 0);
+            HIPCHK(hipMemcpyAsync(host, m->d_stage, n * sizeof(double), hipMemcpyDeviceToHost, m->stream));
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(m->stream));
     }
     return FI_OK;
 }

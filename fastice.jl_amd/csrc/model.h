@@ -55,6 +55,8 @@ struct fi_model_s {
     unsigned long long* h_norm = nullptr;
     double* d_dem = nullptr;
     size_t dem_elems = 0;
+    double* d_stage = nullptr; // contiguous interior of one field, for host transfers
+    size_t stage_elems = 0;
     fi_batch_s* batch = nullptr; // the batch this model is attached to (batch.hip)
 
     static bool is_sig(int id) { return id >= FI_F_P && id <= FI_F_TYZ; }
