@@ -16,8 +16,8 @@ from .batch import Batch, model_bytes
 from .full_stokes import (BoundaryCondition, IsothermalFullStokesModel, IsothermalImmersedBoundaryFullStokesModel,
                           SBC, Slip, TBC, Traction, VBC, Velocity, advance_iteration, build_desc, compute_residuals,
                           norm_inf)
-from .geometries import (AABB, GaussianGlacier, SyntheticElevation, center, dilate, extents, make_synthetic,
-                         splitmix64, union)
+from .geometries import (AABB, DataElevation, GaussianGlacier, SyntheticElevation, center, dilate, extents, extract_dem,
+                         filtered, make_synthetic, rotate, rotate_minmax, splitmix64, union)
 from .grids import Center, UniformGrid, Vertex, flip
 from .level_sets import compute_levelset_from_dem, compute_levelset_from_polygon, invert_levelset, omega_from_psi
 from .physics import LinearViscousRheology, PowerLawRheology, Relaxation, ViscoplasticRegularisation

@@ -1,6 +1,7 @@
 """Geometries: synthetic elevation models (src/Geometries/*.jl).
 
-make_synthetic / AABB / extents / dilate mirror the reference; the seeded
+make_synthetic / AABB / extents / dilate / DataElevation / rotate mirror the
+reference, extract_dem the DEM preparation of the glacier driver; the seeded
 eight-bump glacier generator builds the benchmark geometry (a parabolic ice
 surface over a bed of Gaussian bumps).
 """
@@ -50,6 +51,75 @@ class SyntheticElevation:
         self.z_bed = z_bed
         self.z_surf = z_surf
         self.domain = AABB(-lx / 2, lx / 2, -ly / 2, ly / 2, zmin, zmax)
+        self.rotated_domain = self.domain
+        self.rotation = np.eye(3)
+
+
+def filtered(X):
+    """The values of X that are not NaN (geometry_helpers.jl:21)."""
+    X = np.asarray(X, dtype=np.float64).ravel()
+    return X[~np.isnan(X)]
+
+
+def rotate(X, Y, Z, R):
+    """(R (x, y, z)) for every point of the arrays X, Y, Z (geometry_helpers.jl:31-36)."""
+    R = np.asarray(R, dtype=np.float64).reshape(3, 3)
+    X, Y, Z = (np.asarray(a, dtype=np.float64) for a in (X, Y, Z))
+    return (R[0, 0] * X + R[0, 1] * Y + R[0, 2] * Z, R[1, 0] * X + R[1, 1] * Y + R[1, 2] * Z,
+            R[2, 0] * X + R[2, 1] * Y + R[2, 2] * Z)
+
+
+def rotate_minmax(X, Y, Z, R):
+    """rotate of the two points (min x, min y, min z) and (max x, max y, max z),
+    z without NaNs (geometry_helpers.jl:39).  Note that this rotates two
+    corners of the box, not all eight, as the reference does."""
+    zf = filtered(Z)
+    return rotate(np.array([np.min(X), np.max(X)]), np.array([np.min(Y), np.max(Y)]),
+                  np.array([np.min(zf), np.max(zf)]), R)
+
+
+class DataElevation:
+    """DataElevation(x, y, offsets, z_bed, z_surf, R) (elevation_data.jl:19-33).
+
+    x, y: vertex coordinates of the DEM; z_bed, z_surf: elevations (NaN
+    where there is no data); R: rotation applied to the points before the
+    distance to the DEM is taken (compute_levelset_from_dem's R).  domain is
+    the box of x, y and the NaN-free elevations; rotated_domain the union of
+    the boxes of the rotated bed and surface corner points."""
+
+    def __init__(self, x, y, offsets, z_bed, z_surf, R):
+        self.x = np.asarray(x, dtype=np.float64)
+        self.y = np.asarray(y, dtype=np.float64)
+        self.offsets = offsets
+        self.z_bed = np.asarray(z_bed, dtype=np.float64)
+        self.z_surf = np.asarray(z_surf, dtype=np.float64)
+        self.rotation = np.asarray(R, dtype=np.float64).reshape(3, 3)
+        zb, zs = filtered(self.z_bed), filtered(self.z_surf)
+        self.domain = AABB(np.min(self.x), np.max(self.x), np.min(self.y), np.max(self.y),
+                           min(np.min(zb), np.min(zs)), max(np.max(zb), np.max(zs)))
+        bed = AABB.from_coords(*rotate_minmax(self.x, self.y, self.z_bed, self.rotation))
+        surf = AABB.from_coords(*rotate_minmax(self.x, self.y, self.z_surf, self.rotation))
+        self.rotated_domain = union(bed, surf)
+
+
+def extract_dem(dem, dilation=0.05):
+    """The DEM preparation of scripts_future_API/test_vavilov.jl:65-95 from a
+    DataElevation (the reference loads it from a JLD2 file first): dilate the
+    domain by 5 %, put z = 0 at its bottom and scale z by its height, set
+    the horizontal extents to 10 and the height to 1.  Returns a dict with
+    bed, ice (elevations at the DEM vertices, Julia index order), the 2D
+    vertex grid of the DEM (dem_grid) and lx, ly, lz."""
+    from .grids import UniformGrid
+    dm = dilate(dem.domain, dilation)
+    lx, ly, lz = extents(dm)
+    nx, ny = (n - 1 for n in dem.z_surf.shape)
+    ice = (dem.z_surf - dm.zmin) / lz
+    bed = (dem.z_bed - dm.zmin) / lz
+    lx /= lx / 10
+    ly /= ly / 10
+    lz /= lz
+    grid_2d = UniformGrid(origin=(-lx / 2, -ly / 2), extent=(lx, ly), dims=(nx, ny))
+    return {"bed": bed, "ice": ice, "dem_grid": grid_2d, "lx": lx, "ly": ly, "lz": lz}
 
 
 def _linrange(a, b, n):

@@ -116,6 +116,25 @@ def _glacier_ib_pair(pkg, orc, dims, seed=5):
     return gl, grid, dgrid, model, ref
 
 
+def test_levelset_rotated_dem(pkg, orc):
+    """compute_levelset_from_dem! with a rotation R (compute_levelsets.jl:1-8,
+    the R of DataElevation): the GPU level sets of a glacier seen through a
+    rotation about x and z match the oracle's."""
+    _say("case: rotated level sets from DEM, 24x20x12 glacier")
+    gl, grid, dgrid, model, ref = _glacier_ib_pair(pkg, orc, (24, 20, 12))
+    from fastice_jl_amd.level_sets import dem_parent
+    a, b = 0.2, 0.35
+    Rx = np.array([[1.0, 0.0, 0.0], [0.0, math.cos(a), -math.sin(a)], [0.0, math.sin(a), math.cos(a)]])
+    Rz = np.array([[math.cos(b), -math.sin(b), 0.0], [math.sin(b), math.cos(b), 0.0], [0.0, 0.0, 1.0]])
+    R = Rz @ Rx
+    for phase, fn, psi in ((0, gl.surface, model.level_sets.na), (1, gl.bed, model.level_sets.ns)):
+        pkg.compute_levelset_from_dem(psi, fn, dgrid, grid, R=R)
+        ref.levelset_from_dem(phase, dem_parent(fn, dgrid), dgrid.dims, dgrid.origin, dgrid.extent_, R=R)
+    for name in ("PSI_NA", "PSI_NS"):
+        assert rel_err(model.field(name).parent(), ref.get(name)) <= TOL, name
+    model.close()
+
+
 def test_levelset_and_volume_fractions_3d(pkg, orc):
     _say("case: level sets from DEM + volume fractions, 24x20x12 glacier")
     gl, grid, dgrid, model, ref = _glacier_ib_pair(pkg, orc, (24, 20, 12))

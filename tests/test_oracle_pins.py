@@ -367,6 +367,29 @@ def test_levelset_flat_and_inclined_dem(pkg, orc):
     assert np.allclose(ref.get("PSI_NS")[inner][ok[inner]], -dist[inner][ok[inner]], atol=1e-12)
 
 
+def test_levelset_rotated_flat_dem(pkg, orc):
+    """init_levelset! takes the distance of R (x, y, z) (compute_levelsets.jl:1-8):
+    a flat DEM z = 0.4 seen through a rotation by 0.3 about x gives
+    psi = (R X)_z - 0.4 where the foot point lies inside the DEM."""
+    from fastice_jl_amd.level_sets import dem_parent
+    grid, dgrid, ref = _ls_oracle(pkg, orc)
+    c, s = math.cos(0.3), math.sin(0.3)
+    R = np.array([[1.0, 0.0, 0.0], [0.0, c, -s], [0.0, s, c]])
+    ref.levelset_from_dem(0, dem_parent(lambda x, y: 0.4 + 0 * x, dgrid), dgrid.dims, dgrid.origin, dgrid.extent_,
+                          R=R)
+    X, Y, Z = grid.parent_coords(0b111)
+    px, py, pz = X, c * Y - s * Z, s * Y + c * Z
+    want = pz - 0.4
+    cut = 4 * max(grid.spacing_)
+    inner = (slice(1, 12), slice(1, 11), slice(1, 10))
+    ok = (np.abs(want) < 0.5 * cut) & (np.abs(px) < 0.7) & (np.abs(py) < 0.7)
+    assert ok[inner].sum() > 20
+    psi = ref.get("PSI_NA")
+    assert np.allclose(psi[inner][ok[inner]], want[inner][ok[inner]], atol=1e-13)
+    # a transposed rotation (the plausible mistake) is far off
+    assert np.max(np.abs(psi[inner][ok[inner]] - (c * Z - s * Y - 0.4)[inner][ok[inner]])) > 1e-2
+
+
 def _closest_point_triangle(p, a, b, c):
     """Closest point on a triangle (region classification; C. Ericson, Real-Time Collision Detection, 5.1.5)."""
     ab, ac, ap = b - a, c - a, p - a
