@@ -19,6 +19,8 @@ Workloads
       fp64: plain FP64 PT with HIP graphs); value = solves/s; the same models
       are then reset and a sample of them solved one at a time for the
       comparison.
+  heat_diffusion: PT heat diffusion (Models/Diffusion/Heat) at 1023^3, FP64;
+      value = T_eff by benchmark_diffusion_3D.jl's formula.
   h5_glacier_255x255x127: HDF5 + XDMF3 output (src/Writers.jl) of the fields
       the glacier driver saves, from device memory; value = GB/s of data.
   serving_poisson40: Poisson arrivals at 40/s, half 2D glacier sections
@@ -53,6 +55,7 @@ WORKLOADS = {
     "batch96_glacier": None,
     "serving_poisson40": None,
     "h5_glacier_255x255x127": None,
+    "heat_diffusion": None,
 }
 
 
@@ -72,6 +75,7 @@ def parse():
                    help="batch96_glacier: FP32-refined solves to 1e-8 (default) or plain FP64")
     p.add_argument("--batch-chunk", type=int, default=32,
                    help="batch96_glacier refined: FP32 sweeps between FP64 outer steps, all models")
+    p.add_argument("--heat-n", type=int, default=1023, help="heat_diffusion: cells per axis")
     p.add_argument("--requests", type=int, default=200, help="serving_poisson40: number of requests")
     p.add_argument("--copy-calib", action="store_true",
                    help="run a 1 GiB device copy first (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -344,6 +348,78 @@ def bench_h5(pkg, args):
     print(json.dumps(line), flush=True)
 
 
+def bench_heat(pkg, args):
+    """PT heat diffusion at 1023^3 in FP64 (the size of
+    scripts_future_API/benchmark_diffusion_3D.jl:65): T_eff by that script's
+    formula, 8 * cells * 8 B per iteration (:119-123), and the fused
+    kernel's roofline with its 9 arrays per cell (T, T_o, q read; T, q
+    written)."""
+    H = pkg.heat
+    n = args.heat_n
+    dims = (n, n, n)
+    L = 10.0
+    grid = pkg.UniformGrid(origin=(-0.5 * L,) * 3, extent=(L,) * 3, dims=dims)
+    dx = L / n
+    lam = 1.0
+    bcs = {side: H.BoundaryCondition(H.Flux, 0.0) for pair in H.SIDES for side in pair}
+    m = H.HeatDiffusionModel(grid, bcs, physics={"λ_ρCp": lam},
+                             iter_params={"Δτ": {"q": 1.0, "T": dx * dx / (lam * 6.3)}})
+    t0 = time.perf_counter()
+    xc = grid.coords(pkg.Center(), 0, 0, n + 2)
+    g = np.exp(-xc * xc)
+    T0 = np.multiply.outer(np.multiply.outer(g, g), g)  # exp(-x^2 - y^2 - z^2) on the parent box
+    m.temperature.set(T0.T)  # symmetric: the transposed view is the Julia-order array, without a copy
+    m.temperature_old.set(T0.T)
+    del T0
+    m.apply_bc()
+    log("heat %s: set-up %.1f s" % (dims, time.perf_counter() - t0))
+    dt = 0.1
+    m.advance(args.warmup, dt)
+    ms = m.time_iterations(dt, args.steps)
+    cells = n ** 3
+    teff = 8 * cells * 8 / (ms[0] * 1e-3) / 1e9
+    alg = 9 * cells * 8
+    achieved = alg / (ms[1] * 1e-3) / 1e9
+    norms = m.residual_norms(dt)
+    m.close()
+    traffic, src = profile_traffic("heat_diffusion_%d" % n, "fused")
+    config = {"workload": "heat_diffusion_%d" % n, "grid": list(dims), "dtype": "f64",
+              "description": "PT heat diffusion (Models/Diffusion/Heat), Gaussian initial temperature, zero flux on "
+                             "every side, one fused kernel per iteration + boundary kernels",
+              "t_eff_formula": "8 * cells * sizeof(Float64) / t_iteration (benchmark_diffusion_3D.jl:119-123)",
+              "iteration_ms": round(ms[0], 4), "kernel_ms": round(ms[1], 4),
+              "residual_after": [float("%.3e" % x) for x in norms]}
+    line = _line("PT heat diffusion T_eff GB/s at %d^3" % n, teff, "GB/s", ms[0], args.steps, args.warmup, "f64", config)
+    line["roofline"] = {"bound": "hbm", "kernel": "k_heat_iter<3>", "achieved": round(achieved, 2),
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                        "traffic": traffic, "traffic_source": src, "algorithmic_bytes_per_launch": alg}
+    if args.no_cpu:
+        line["cpu_baseline"] = None
+    else:
+        line["cpu_baseline"] = heat_cpu_baseline(args.cpu_seconds)
+    print(json.dumps(line), flush=True)
+
+
+def heat_cpu_baseline(seconds):
+    """The oracle's heat iteration on a 255^3 sample, T_eff by the same formula."""
+    orc = ge.load_oracle()
+    n = 255
+    h = orc.OracleHeat(3, [n, n, n], [10.0] * 3, [[2, 2]] * 3, [[0.0, 0.0]] * 3, 1.0, 1.0, (10.0 / n) ** 2 / 6.3)
+    x = (np.arange(n + 3) - 0.5) * (10.0 / n) - 5.0
+    g = np.exp(-x * x)
+    T0 = np.multiply.outer(np.multiply.outer(g, g), g)
+    h.set("T", T0)
+    h.set("TO", T0)
+    h.advance(0.1, 1)
+    it, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < seconds and it < 200:
+        h.advance(0.1, 1)
+        it += 1
+    el = time.perf_counter() - t0
+    return {"value": round(8 * n ** 3 * 8 * it / el / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
+            "sample": "%d iterations of the CPU oracle's PT heat iteration at 255^3 (%.1f s)" % (it, el)}
+
+
 def bench_serving(pkg, args):
     """Continuous-batching service under Poisson arrivals at 40/s."""
     reqs = pkg.poisson_requests(args.requests, 40.0, seed=7)
@@ -406,6 +482,8 @@ def main():
         return bench_serving(pkg, args)
     if args.workload == "h5_glacier_255x255x127":
         return bench_h5(pkg, args)
+    if args.workload == "heat_diffusion":
+        return bench_heat(pkg, args)
     dims = WORKLOADS[args.workload]
     cells = dims[0] * dims[1] * dims[2]
     glacier = args.workload.startswith("glacier")

@@ -24,7 +24,8 @@ from .physics import LinearViscousRheology, PowerLawRheology, Relaxation, Viscop
 from .problems import (glacier_2d, glacier_2d_spec, glacier_3d, glacier_3d_spec, manufactured_2d, manufactured_3d,
                        pt_params)
 from .serving import Request, Server, percentiles, poisson_requests
-from . import _hdf5, writers
+from . import _hdf5, heat, writers
+from .heat import HeatDiffusionModel
 from .writers import read_h5, write_h5, write_xdmf
 
 

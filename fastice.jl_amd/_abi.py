@@ -117,6 +117,15 @@ class NonFiniteError(FastIceError):
     """simulation failed: non-finite residual (tm_stokes_chmy.jl:104-106)."""
 
 
+class fi_heat_desc(C.Structure):
+    _fields_ = [("ndims", C.c_int), ("n", C.c_int * 3), ("origin", C.c_double * 3), ("extent", C.c_double * 3),
+                ("bc_kind", (C.c_int * 2) * 3), ("bc_value", (C.c_double * 2) * 3), ("lambda_rho_cp", C.c_double),
+                ("dtau_q", C.c_double), ("dtau_T", C.c_double)]
+
+
+FI_HEAT_BC_NONE, FI_HEAT_BC_VALUE, FI_HEAT_BC_FLUX = 0, 1, 2
+FI_HF_T, FI_HF_T_OLD, FI_HF_QX, FI_HF_QY, FI_HF_QZ = 0, 1, 2, 3, 4
+
 _lib = None
 
 _DP = C.POINTER(C.c_double)
@@ -163,6 +172,15 @@ def _declare(lib):
         "fi_batch_status": (C.c_int, [C.c_void_p, H, C.POINTER(C.c_int), C.POINTER(C.c_int), _DP]),
         "fi_batch_set_option": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
         "fi_batch_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+        "fi_heat_create": (C.c_int, [C.POINTER(fi_heat_desc), C.POINTER(H)]),
+        "fi_heat_destroy": (C.c_int, [H]),
+        "fi_heat_field_upload": (C.c_int, [H, C.c_int, _DP, C.c_size_t, C.c_int]),
+        "fi_heat_field_download": (C.c_int, [H, C.c_int, _DP, C.c_size_t, C.c_int]),
+        "fi_heat_apply_bc": (C.c_int, [H]),
+        "fi_heat_advance_iterations": (C.c_int, [H, C.c_double, C.c_int]),
+        "fi_heat_residual_norms": (C.c_int, [H, C.c_double, _DP]),
+        "fi_heat_step": (C.c_int, [H, C.c_double, C.c_int, C.c_int, C.c_double, C.POINTER(C.c_int), _DP]),
+        "fi_heat_time_iterations": (C.c_int, [H, C.c_double, C.c_int, _DP]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -332,23 +332,14 @@ size_t parent_count(const Geo& g)
     return c;
 }
 
-} // namespace fi
-
-extern "C" {
-
-int fi_abi_version(void) { return FI_ABI_VERSION; }
-
-const char* fi_last_error(void) { return g_err; }
-
-// device layout: padded rows, Chmy index 1 of every row 128-byte aligned
-static Geo make_geo(const fi_model_desc& desc)
+Geo geo_for(int nd, const int n[3], const double extent[3])
 {
     Geo g;
-    g.nd = desc.ndims;
+    g.nd = nd;
     for (int d = 0; d < 3; ++d) {
         if (d < g.nd) {
-            g.n[d] = desc.n[d];
-            g.dx[d] = desc.extent[d] / (double)desc.n[d];
+            g.n[d] = n[d];
+            g.dx[d] = extent[d] / (double)n[d];
             g.idx[d] = 1.0 / g.dx[d];
         } else {
             g.n[d] = 0;
@@ -366,6 +357,71 @@ static Geo make_geo(const fi_model_desc& desc)
     g.elems = (size_t)(g.off + g.sy * rows + align);
     return g;
 }
+
+// interior of a field <-> contiguous x-fastest array: stage[t], t = i + mx (j + my k),
+// is the field at Chmy index (i+1, j+1, k+1) (sz = 0 in 2D)
+__global__ __launch_bounds__(256) void k_interior_stage(double* stage, double* field, int mx, int my, long n, long sy,
+                                                        long sz, int unpack)
+{
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        const long i = t % mx, r = t / mx, j = r % my, k = r / my;
+        const long c = 1 + i + sy * (1 + j) + sz * (1 + k);
+        if (unpack) field[c] = stage[t];
+        else stage[t] = field[c];
+    }
+}
+
+int field_transfer(const Geo& g, double* dev, int L, double* host, size_t count, int layout, bool up, hipStream_t stream,
+                   double** stage, size_t* stage_elems)
+{
+    HIPCHK(hipStreamSynchronize(stream));
+    if (layout == FI_LAYOUT_PARENT) {
+        const size_t w = (size_t)(g.n[0] + 3) * sizeof(double);
+        const size_t rows = (g.nd == 3) ? (size_t)(g.n[1] + 3) * (g.n[2] + 3) : (size_t)(g.n[1] + 3);
+        if (up) HIPCHK(hipMemcpy2D(dev, g.sy * sizeof(double), host, w, w, rows, hipMemcpyHostToDevice));
+        else HIPCHK(hipMemcpy2D(host, w, dev, g.sy * sizeof(double), w, rows, hipMemcpyDeviceToHost));
+        return FI_OK;
+    }
+    const int mx = g.n[0] + (L & 1), my = g.n[1] + ((L >> 1) & 1);
+    const int mz = (g.nd == 3) ? g.n[2] + ((L >> 2) & 1) : 1;
+    const size_t n = (size_t)mx * my * mz;
+    if (n != count) return fail(FI_ERR_ARG, "interior transfer of %zu doubles, got %zu", n, count);
+    if (n > *stage_elems) {
+        if (*stage) HIPCHK(hipFree(*stage));
+        *stage = nullptr;
+        *stage_elems = 0;
+        if (hipMalloc(stage, n * sizeof(double)) != hipSuccess)
+            return fail(FI_ERR_OOM, "transfer staging (%zu bytes)", n * sizeof(double));
+        *stage_elems = n;
+    }
+    const long sz = g.nd == 3 ? g.sz : 0;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    if (up) {
+        HIPCHK(hipMemcpyAsync(*stage, host, n * sizeof(double), hipMemcpyHostToDevice, stream));
+        hipLaunchKernelGGL(k_interior_stage, dim3((unsigned)blocks), dim3(256), 0, stream, *stage, dev, mx, my, (long)n,
+                           g.sy, sz, 1);
+    } else {
+        hipLaunchKernelGGL(k_interior_stage, dim3((unsigned)blocks), dim3(256), 0, stream, *stage, dev, mx, my, (long)n,
+                           g.sy, sz, 0);
+        HIPCHK(hipMemcpyAsync(host, *stage, n * sizeof(double), hipMemcpyDeviceToHost, stream));
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(stream));
+    return FI_OK;
+}
+
+} // namespace fi
+
+extern "C" {
+
+int fi_abi_version(void) { return FI_ABI_VERSION; }
+
+const char* fi_last_error(void) { return g_err; }
+
+// device layout: padded rows, Chmy index 1 of every row 128-byte aligned
+static Geo make_geo(const fi_model_desc& desc) { return geo_for(desc.ndims, desc.n, desc.extent); }
+
 
 size_t fi_model_bytes(const fi_model_desc* desc)
 {
@@ -470,19 +526,6 @@ static void mark_dirty(fi_model_t m, int id)
     if (m->batch) batch_touch(m); // the batch re-prepares the model before its next step
 }
 
-// interior of a field <-> contiguous x-fastest array: stage[t], t = i + mx (j + my k),
-// is the field at Chmy index (i+1, j+1, k+1) (sz = 0 in 2D)
-__global__ __launch_bounds__(256) void k_interior_stage(double* stage, double* field, int mx, int my, long n, long sy,
-                                                        long sz, int unpack)
-{
-    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
-        const long i = t % mx, r = t / mx, j = r % my, k = r / my;
-        const long c = 1 + i + sy * (1 + j) + sz * (1 + k);
-        if (unpack) field[c] = stage[t];
-        else stage[t] = field[c];
-    }
-}
-
 static int field_copy(fi_model_t m, int id, double* host, size_t count, int layout, bool up)
 {
     if (!m) return fail(FI_ERR_ARG, "null model");
@@ -491,44 +534,9 @@ static int field_copy(fi_model_t m, int id, double* host, size_t count, int layo
     size_t need = fi_field_size(m, id, layout);
     if (layout != FI_LAYOUT_PARENT && layout != FI_LAYOUT_INTERIOR) return fail(FI_ERR_ARG, "bad layout %d", layout);
     if (count != need) return fail(FI_ERR_ARG, "field %d needs %zu doubles in this layout, got %zu", id, need, count);
-    const Geo& g = m->g;
-    double* dev = m->buf(id);
     if (up) mark_dirty(m, id);
-    HIPCHK(hipStreamSynchronize(m->stream));
-    if (layout == FI_LAYOUT_PARENT) {
-        const size_t w = (size_t)(g.n[0] + 3) * sizeof(double);
-        const size_t rows = (g.nd == 3) ? (size_t)(g.n[1] + 3) * (g.n[2] + 3) : (size_t)(g.n[1] + 3);
-        if (up) HIPCHK(hipMemcpy2D(dev, g.sy * sizeof(double), host, w, w, rows, hipMemcpyHostToDevice));
-        else HIPCHK(hipMemcpy2D(host, w, dev, g.sy * sizeof(double), w, rows, hipMemcpyDeviceToHost));
-    } else {
-        const int L = field_loc(g.nd, id);
-        const int mx = g.n[0] + (L & 1), my = g.n[1] + ((L >> 1) & 1);
-        const int mz = (g.nd == 3) ? g.n[2] + ((L >> 2) & 1) : 1;
-        const size_t n = (size_t)mx * my * mz;
-        if (n > m->stage_elems) {
-            if (m->d_stage) HIPCHK(hipFree(m->d_stage));
-            m->d_stage = nullptr;
-            m->stage_elems = 0;
-            if (hipMalloc(&m->d_stage, n * sizeof(double)) != hipSuccess)
-                return fail(FI_ERR_OOM, "transfer staging (%zu bytes)", n * sizeof(double));
-            m->stage_elems = n;
-        }
-        const long sz = g.nd == 3 ? g.sz : 0;
-        size_t blocks = (n + 255) / 256;
-        if (blocks > 8192) blocks = 8192;
-        if (up) {
-            HIPCHK(hipMemcpyAsync(m->d_stage, host, n * sizeof(double), hipMemcpyHostToDevice, m->stream));
-            hipLaunchKernelGGL(k_interior_stage, dim3((unsigned)blocks), dim3(256), 0, m->stream, m->d_stage, dev, mx, my,
-                               (long)n, g.sy, sz, 1);
-        } else {
-            hipLaunchKernelGGL(k_interior_stage, dim3((unsigned)blocks), dim3(256), 0, m->stream, m->d_stage, dev, mx, my,
-                               (long)n, g.sy, sz, 0);
-            HIPCHK(hipMemcpyAsync(host, m->d_stage, n * sizeof(double), hipMemcpyDeviceToHost, m->stream));
-        }
-        HIPCHK(hipGetLastError());
-        HIPCHK(hipStreamSynchronize(m->stream));
-    }
-    return FI_OK;
+    return field_transfer(m->g, m->buf(id), field_loc(m->g.nd, id), host, count, layout, up, m->stream, &m->d_stage,
+                          &m->stage_elems);
 }
 
 int fi_field_upload(fi_model_t m, int id, const double* host, size_t count, int layout)

@@ -101,6 +101,13 @@ void residual_norms_read(fi_model_s* m, double out[4]);
 void batch_touch(fi_model_s* m);  // the model's buffer parity changed outside its batch
 void batch_detach(fi_model_s* m); // remove the model from its batch (model destruction)
 double bits_to_double(unsigned long long b);
+// padded layout of a grid (rows of 32 elements, index 1 on a 128-byte line)
+Geo geo_for(int nd, const int n[3], const double extent[3]);
+// host <-> device copy of one field at location L in FI_LAYOUT_PARENT or
+// FI_LAYOUT_INTERIOR (the interior through a contiguous staging buffer,
+// grown on demand); synchronises the stream
+int field_transfer(const Geo& g, double* dev, int L, double* host, size_t count, int layout, bool up, hipStream_t stream,
+                   double** stage, size_t* stage_elems);
 BcBatch neumann_batch(double* p, int L, int d);
 void free_refine(fi_model_s* m);
 

@@ -292,6 +292,52 @@ int fi_batch_set_option(fi_batch_t b, int key, int value);
  * runs, table rebuilds} */
 int fi_batch_info(fi_batch_t b, int out[8]);
 
+/* ---------------------------------------------------------------------------
+ * PT heat diffusion: HeatDiffusionModel (src/Models/Diffusion/Heat/Heat.jl:16-40),
+ * advance_iteration! (:42-58) with update_q! / update_T! (kernels.jl:3-29)
+ * and the Value / Flux boundary conditions (boundary_conditions.jl:1-69).
+ * The reference model does not run as shipped (it reads a missing
+ * model.fields and calls KernelLaunch with the wrong arguments); this is the
+ * model its kernels and boundary conditions define.  Fields: T and T_o at
+ * cell centres, q = (qx, qy[, qz]) at the faces of its own dimension; same
+ * layouts as the Stokes fields.  One PT iteration:
+ *   q   -= (q + lambda_rho_cp * grad T) * dtau_q,  then q = v on Flux faces
+ *   T   -= ((T - T_o) / dt + div q) * dtau_T,       then T ghost = 2 v - T on Value sides
+ * --------------------------------------------------------------------------- */
+typedef struct fi_heat_s* fi_heat_t;
+enum { FI_HEAT_BC_NONE = 0, FI_HEAT_BC_VALUE = 1, FI_HEAT_BC_FLUX = 2 };
+enum { FI_HF_T = 0, FI_HF_T_OLD = 1, FI_HF_QX = 2, FI_HF_QY = 3, FI_HF_QZ = 4, FI_HF_COUNT = 5 };
+typedef struct {
+    int ndims;               /* 2 or 3 */
+    int n[3];                /* cells */
+    double origin[3];
+    double extent[3];
+    int bc_kind[3][2];       /* FI_HEAT_BC_* per dimension and side (west/east, south/north, bottom/top) */
+    double bc_value[3][2];
+    double lambda_rho_cp;    /* physics.λ_ρCp */
+    double dtau_q, dtau_T;   /* iter_params.Δτ.q, Δτ.T */
+} fi_heat_desc;
+/* HeatDiffusionModel(; arch, grid, boundary_conditions, physics, iter_params) */
+int fi_heat_create(const fi_heat_desc* desc, fi_heat_t* out);
+int fi_heat_destroy(fi_heat_t h);
+/* set!(field, values) / interior(field): layouts as fi_field_upload */
+int fi_heat_field_upload(fi_heat_t h, int id, const double* host, size_t count, int layout);
+int fi_heat_field_download(fi_heat_t h, int id, double* host, size_t count, int layout);
+/* the Value boundary conditions on T (after set!) */
+int fi_heat_apply_bc(fi_heat_t h);
+/* advance_iteration!(model, t, Δt), n_iter times */
+int fi_heat_advance_iterations(fi_heat_t h, double dt, int n_iter);
+/* Inf-norms of (T - T_o)/dt + div q (cells) and q + lambda grad T (faces not
+ * fixed by a Flux condition) */
+int fi_heat_residual_norms(fi_heat_t h, double dt, double norms_inf[2]);
+/* advance_timestep! (exported by Heat.jl, not defined there): PT iterations
+ * with a residual check every ncheck until both norms are below tol or
+ * max_iter, then T_o <- T.  FI_ERR_NONFINITE on a non-finite residual. */
+int fi_heat_step(fi_heat_t h, double dt, int max_iter, int ncheck, double tol, int* iters_out, double norms_inf[2]);
+/* measurement hook: n_iter iterations timed with HIP events; out_ms =
+ * {iteration, fused kernel} averages */
+int fi_heat_time_iterations(fi_heat_t h, double dt, int n_iter, double out_ms[2]);
+
 #ifdef __cplusplus
 }
 #endif

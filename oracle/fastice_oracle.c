@@ -27,6 +27,8 @@
 
 #include <math.h>
 #include <stddef.h>
+#include <stdlib.h>
+#include <string.h>
 
 #define TOL_EMPTY 1e-6 /* null_spaces.jl:16,41 */
 
@@ -1174,4 +1176,120 @@ void or_invert_levelset(const or_model* m, double* psi)
         for (int j = 1; j <= m->n[1] + 1; ++j)
             for (int i = 1; i <= m->n[0] + 1; ++i) psi[i + sy * j + sz * k] = -psi[i + sy * j + sz * k];
     or_bc_neumann_vertex(m, psi);
+}
+
+/* ------------------------------------------------------------------------
+ * PT heat diffusion (src/Models/Diffusion/Heat/kernels.jl:3-29, with the
+ * Value / Flux boundary conditions of Heat/boundary_conditions.jl:1-69:
+ * Value -> DirichletBC{HalfCell} on T, Flux -> DirichletBC{FullCell} on q)
+ * ---------------------------------------------------------------------- */
+static long heat_stride(const or_heat* h, int d) { return d == 0 ? 1 : (d == 1 ? h->sy : h->sz); }
+
+/* q.x[I] -= (q.x[I] + lam * dvx(T, I) / dx) * dtq  (kernels.jl:5-9) */
+static double heat_flux_new(const or_heat* h, const or_heat_fields* f, int d, long c)
+{
+    const double dT = h->lam * (f->T[c] - f->T[c - heat_stride(h, d)]) / h->dx[d];
+    return f->q[d][c] - (f->q[d][c] + dT) * h->dtq;
+}
+
+static long heat_index(const or_heat* h, int i, int j, int k) { return i + h->sy * j + h->sz * k; }
+
+void or_heat_bc(const or_heat* h, or_heat_fields* f)
+{
+    /* x, then y, then z; every tangential index of the padded box */
+    const int nd = h->ndims;
+    for (int d = 0; d < nd; ++d) {
+        const int e1 = d == 0 ? 1 : 0, e2 = nd == 3 ? (d == 2 ? 1 : 2) : -1;
+        const int m1 = h->n[e1] + 2, m2 = e2 >= 0 ? h->n[e2] + 2 : 1;
+        const long st = heat_stride(h, d), s1 = heat_stride(h, e1), s2 = e2 >= 0 ? heat_stride(h, e2) : 0;
+        for (int s = 0; s < 2; ++s) {
+            if (h->bc_kind[d][s] != OR_HEAT_VALUE) continue;
+            const double v = h->bc_val[d][s];
+            const long g = s == 0 ? 0 : h->n[d] + 1, src = s == 0 ? 1 : h->n[d];
+            for (int b = 0; b < m2; ++b)
+                for (int a = 0; a < m1; ++a) {
+                    const long base = a * s1 + b * s2;
+                    f->T[base + g * st] = 2.0 * v - f->T[base + src * st];
+                }
+        }
+    }
+}
+
+void or_heat_advance(const or_heat* h, or_heat_fields* f, double dt, int niter)
+{
+    const int nd = h->ndims;
+    const int nz = nd == 3 ? h->n[2] : 1, kz0 = nd == 3 ? 1 : 0;
+    const long total = h->sz ? h->sz * (h->n[2] + 3) : h->sy * (h->n[1] + 3);
+    double* qn[3] = {NULL, NULL, NULL};
+    for (int d = 0; d < nd; ++d) qn[d] = (double*)malloc(sizeof(double) * total);
+    for (int it = 0; it < niter; ++it) {
+        /* update_q! over every face of each component, then the flux BCs */
+        for (int d = 0; d < nd; ++d) {
+            memcpy(qn[d], f->q[d], sizeof(double) * total);
+            const int hx = h->n[0] + (d == 0), hy = h->n[1] + (d == 1), hz = nd == 3 ? h->n[2] + (d == 2) : 1;
+            for (int k = kz0; k < kz0 + hz; ++k)
+                for (int j = 1; j <= hy; ++j)
+                    for (int i = 1; i <= hx; ++i) {
+                        const long c = heat_index(h, i, j, k);
+                        qn[d][c] = heat_flux_new(h, f, d, c);
+                    }
+            for (int s = 0; s < 2; ++s) {
+                if (h->bc_kind[d][s] != OR_HEAT_FLUX) continue;
+                const int face = s == 0 ? 1 : h->n[d] + 1;
+                for (int k = kz0; k < kz0 + hz; ++k)
+                    for (int j = 1; j <= hy; ++j)
+                        for (int i = 1; i <= hx; ++i) {
+                            const int idx = d == 0 ? i : (d == 1 ? j : k);
+                            if (idx == face) qn[d][heat_index(h, i, j, k)] = h->bc_val[d][s];
+                        }
+            }
+        }
+        for (int d = 0; d < nd; ++d) memcpy(f->q[d], qn[d], sizeof(double) * total);
+        /* update_T! (kernels.jl:19-29), then the value BCs */
+        for (int k = kz0; k < kz0 + nz; ++k)
+            for (int j = 1; j <= h->n[1]; ++j)
+                for (int i = 1; i <= h->n[0]; ++i) {
+                    const long c = heat_index(h, i, j, k);
+                    double divq = (f->q[0][c + 1] - f->q[0][c]) / h->dx[0];
+                    divq = divq + (f->q[1][c + h->sy] - f->q[1][c]) / h->dx[1];
+                    if (nd == 3) divq = divq + (f->q[2][c + h->sz] - f->q[2][c]) / h->dx[2];
+                    const double dTdt = (f->T[c] - f->To[c]) / dt;
+                    f->T[c] = f->T[c] - (dTdt + divq) * h->dtT;
+                }
+        or_heat_bc(h, f);
+    }
+    for (int d = 0; d < nd; ++d) free(qn[d]);
+}
+
+static double nanmax(double a, double b) { return (a != a || b != b) ? NAN : (a > b ? a : b); }
+
+void or_heat_residual_norms(const or_heat* h, const or_heat_fields* f, double dt, double out[2])
+{
+    const int nd = h->ndims;
+    const int kz0 = nd == 3 ? 1 : 0;
+    double rT = 0.0, rq = 0.0;
+    for (int k = kz0; k < kz0 + (nd == 3 ? h->n[2] : 1); ++k)
+        for (int j = 1; j <= h->n[1]; ++j)
+            for (int i = 1; i <= h->n[0]; ++i) {
+                const long c = heat_index(h, i, j, k);
+                double divq = (f->q[0][c + 1] - f->q[0][c]) / h->dx[0];
+                divq = divq + (f->q[1][c + h->sy] - f->q[1][c]) / h->dx[1];
+                if (nd == 3) divq = divq + (f->q[2][c + h->sz] - f->q[2][c]) / h->dx[2];
+                rT = nanmax(rT, fabs((f->T[c] - f->To[c]) / dt + divq));
+            }
+    for (int d = 0; d < nd; ++d) {
+        const int hx = h->n[0] + (d == 0), hy = h->n[1] + (d == 1), hz = nd == 3 ? h->n[2] + (d == 2) : 1;
+        for (int k = kz0; k < kz0 + hz; ++k)
+            for (int j = 1; j <= hy; ++j)
+                for (int i = 1; i <= hx; ++i) {
+                    const int idx = d == 0 ? i : (d == 1 ? j : k);
+                    if ((idx == 1 && h->bc_kind[d][0] == OR_HEAT_FLUX) ||
+                        (idx == h->n[d] + 1 && h->bc_kind[d][1] == OR_HEAT_FLUX))
+                        continue;
+                    const long c = heat_index(h, i, j, k);
+                    rq = nanmax(rq, fabs(f->q[d][c] + h->lam * (f->T[c] - f->T[c - heat_stride(h, d)]) / h->dx[d]));
+                }
+    }
+    out[0] = rT;
+    out[1] = rq;
 }

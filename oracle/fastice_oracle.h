@@ -35,6 +35,9 @@
  *   or_sd_poly,             distances to axis-aligned squares in closed
  *   or_levelset_from_polygon form, brute-force point-segment distances and a
  *                           winding-number inside test on random polygons
+ *   or_heat_advance,        steady linear profile between two values, heat
+ *   or_heat_bc,             conservation under zero flux, backward-Euler
+ *   or_heat_residual_norms  decay of a discrete cosine mode
  * Parity unpinned (choices this library specifies where the reference
  * delegates to unvendored Chmy.jl and no check can reach the reference's own
  * bits): the ghost values at edges and corners that the x-then-y-then-z
@@ -130,6 +133,32 @@ void or_levelset_from_dem(const or_model* m, double* psi, const double* dem,
 void or_invert_levelset(const or_model* m, double* psi);
 double or_sd_poly(double px, double py, const double* xy, int npts);
 void or_levelset_from_polygon(const or_model* m, double* psi, const double* xy, int npts);
+
+/* PT heat diffusion (src/Models/Diffusion/Heat: Heat.jl, kernels.jl), run
+ * as that model's advance_iteration! intends.  T, To: Center; q[d]: Vertex
+ * in dimension d.
+ * Same dense parent layout and strides as or_model. */
+enum { OR_HEAT_NONE = 0, OR_HEAT_VALUE = 1, OR_HEAT_FLUX = 2 };
+typedef struct {
+    int ndims;
+    int n[3];
+    double dx[3];
+    long sy, sz;
+    int bc_kind[3][2];
+    double bc_val[3][2];
+    double lam;        /* physics.λ_ρCp */
+    double dtq, dtT;   /* iter_params.Δτ.q, Δτ.T */
+} or_heat;
+typedef struct {
+    double *T, *To, *q[3];
+} or_heat_fields;
+/* update_q! + flux BCs, update_T! + value BCs (Heat.jl:42-58), niter times */
+void or_heat_advance(const or_heat* h, or_heat_fields* f, double dt, int niter);
+/* the value BCs on T alone (the bc! a caller applies after set!) */
+void or_heat_bc(const or_heat* h, or_heat_fields* f);
+/* Inf-norms of (T - To)/dt + div q over cells and of q + lam grad T over
+ * the faces a flux BC does not fix; NaN propagates */
+void or_heat_residual_norms(const or_heat* h, const or_heat_fields* f, double dt, double out[2]);
 
 #ifdef __cplusplus
 }

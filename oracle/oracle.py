@@ -105,6 +105,9 @@ def lib():
             "or_levelset_from_polygon": (None, [M, _DP, _DP, C.c_int]),
             "or_sd_poly": (C.c_double, [C.c_double, C.c_double, _DP, C.c_int]),
             "or_invert_levelset": (None, [M, _DP]),
+            "or_heat_advance": (None, [C.POINTER(or_heat), C.POINTER(or_heat_fields), C.c_double, C.c_int]),
+            "or_heat_bc": (None, [C.POINTER(or_heat), C.POINTER(or_heat_fields)]),
+            "or_heat_residual_norms": (None, [C.POINTER(or_heat), C.POINTER(or_heat_fields), C.c_double, _DP]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(L, name)
@@ -313,3 +316,56 @@ class OracleModel:
         for q in range(nq):
             om[q] = _p(self.arrays["OM_%s%d" % (ph, q)])
         lib().or_omega_from_psi(C.byref(self.m), _p(psi), om)
+
+
+class or_heat(C.Structure):
+    _fields_ = [("ndims", C.c_int), ("n", C.c_int * 3), ("dx", C.c_double * 3), ("sy", C.c_long), ("sz", C.c_long),
+                ("bc_kind", (C.c_int * 2) * 3), ("bc_val", (C.c_double * 2) * 3), ("lam", C.c_double),
+                ("dtq", C.c_double), ("dtT", C.c_double)]
+
+
+class or_heat_fields(C.Structure):
+    _fields_ = [("T", _DP), ("To", _DP), ("q", _DP * 3)]
+
+
+class OracleHeat:
+    """CPU restatement of the PT heat-diffusion model (or_heat_*): fields T,
+    To, QX, QY[, QZ] as parent-layout arrays in Julia index order."""
+
+    def __init__(self, ndims, n, extent, bc_kind, bc_val, lam, dtq, dtT):
+        h = self.h = or_heat()
+        h.ndims = ndims
+        for d in range(3):
+            h.n[d] = n[d] if d < ndims else 0
+            h.dx[d] = extent[d] / n[d] if d < ndims else 1.0
+            for s in range(2):
+                h.bc_kind[d][s] = bc_kind[d][s] if d < ndims else 0
+                h.bc_val[d][s] = bc_val[d][s] if d < ndims else 0.0
+        h.sy = n[0] + 3
+        h.sz = (n[0] + 3) * (n[1] + 3) if ndims == 3 else 0
+        h.lam, h.dtq, h.dtT = lam, dtq, dtT
+        self.shape = tuple(n[d] + 3 for d in range(ndims))
+        names = ["T", "TO", "QX", "QY"] + (["QZ"] if ndims == 3 else [])
+        self.arrays = {k: np.zeros(self.shape, order="F") for k in names}
+        self.f = or_heat_fields()
+        self.f.T = self.arrays["T"].ctypes.data_as(_DP)
+        self.f.To = self.arrays["TO"].ctypes.data_as(_DP)
+        for d, k in enumerate(("QX", "QY", "QZ")[:ndims]):
+            self.f.q[d] = self.arrays[k].ctypes.data_as(_DP)
+
+    def set(self, name, a):
+        self.arrays[name][...] = np.asarray(a, dtype=np.float64)
+
+    def get(self, name):
+        return self.arrays[name].copy(order="F")
+
+    def apply_bc(self):
+        lib().or_heat_bc(C.byref(self.h), C.byref(self.f))
+
+    def advance(self, dt, niter):
+        lib().or_heat_advance(C.byref(self.h), C.byref(self.f), C.c_double(dt), int(niter))
+
+    def residual_norms(self, dt):
+        out = (C.c_double * 2)()
+        lib().or_heat_residual_norms(C.byref(self.h), C.byref(self.f), C.c_double(dt), out)
+        return np.array(out[:])

@@ -524,3 +524,71 @@ def test_levelset_from_polygon_half_plane(pkg, orc):
     # Neumann vertex ghosts mirror the node next to the boundary node
     assert np.array_equal(psi[0, 1:11], psi[2, 1:11]) and np.array_equal(psi[14, 1:11], psi[12, 1:11])
     assert np.array_equal(psi[1:14, 0], psi[1:14, 2]) and np.array_equal(psi[1:14, 11], psi[1:14, 9])
+
+
+# --- PT heat diffusion (src/Models/Diffusion/Heat/kernels.jl:3-29) ---------
+
+def _heat(orc, n, extent, kinds, vals, lam=1.0, dtq=1.0, dtT=None):
+    nd = len(n)
+    dx = [e / m for e, m in zip(extent, n)]
+    if dtT is None:
+        dtT = min(dx) ** 2 / (lam * 2.1 * nd)
+    kinds = list(kinds) + [[0, 0]] * (3 - nd)
+    vals = list(vals) + [[0.0, 0.0]] * (3 - nd)
+    return orc.OracleHeat(nd, list(n) + [1] * (3 - nd), list(extent) + [1.0] * (3 - nd), kinds, vals, lam, dtq, dtT)
+
+
+def test_heat_steady_linear_profile(orc):
+    """Value conditions T = 1 (west) and 3 (east), no flux elsewhere, dt -> inf:
+    the discrete steady state is T = 1 + 2 x / L at the cell centres."""
+    n, ext = (16, 6, 5), (1.0, 0.5, 0.4)
+    h = _heat(orc, n, ext, [[1, 1], [2, 2], [2, 2]], [[1.0, 3.0], [0.0, 0.0], [0.0, 0.0]])
+    h.apply_bc()
+    h.advance(1e30, 8000)
+    x = (np.arange(n[0] + 3) - 0.5) * ext[0] / n[0]
+    want = 1.0 + 2.0 * x
+    T = h.get("T")
+    assert np.max(np.abs(T[1:-2, 1:-2, 1:-2] - want[1:-2, None, None])) < 1e-10
+    r = h.residual_norms(1e30)
+    assert r[0] < 1e-9 and r[1] < 1e-9
+
+
+def test_heat_time_term_decays_geometrically(orc):
+    """No flux on any side: div q sums to zero over the cells, so each
+    iteration scales sum(T - T_o) by exactly (1 - dtau_T / dt)."""
+    n, ext = (9, 7, 6), (1.0, 0.8, 0.6)
+    h = _heat(orc, n, ext, [[2, 2]] * 3, [[0.0, 0.0]] * 3)
+    rng = np.random.default_rng(4)
+    T = rng.standard_normal(h.shape)
+    To = rng.standard_normal(h.shape)
+    h.set("T", T)
+    h.set("TO", To)
+    inner = (slice(1, n[0] + 1), slice(1, n[1] + 1), slice(1, n[2] + 1))
+    s0 = np.sum(T[inner] - To[inner])
+    dt, it = 0.05, 25
+    h.advance(dt, it)
+    s = np.sum(h.get("T")[inner] - To[inner])
+    want = s0 * (1.0 - h.h.dtT / dt) ** it
+    assert abs(s - want) < 1e-11 * max(1.0, abs(s0))
+    # a plausible mistake, the time term with the wrong sign, does not decay
+    assert abs(s0 * (1.0 + h.h.dtT / dt) ** it - want) > 1e-3
+
+
+def test_heat_backward_euler_cosine_mode(orc):
+    """T_o = cos(pi x / L) with no flux on any side: one converged time step
+    gives T = T_o / (1 + dt lam kappa^2), kappa^2 = (2/dx)^2 sin^2(pi dx / 2L),
+    the discrete eigenvalue of the cell-centred Laplacian."""
+    n, ext, lam, dt = (24, 4), (2.0, 0.5), 0.7, 0.01
+    h = _heat(orc, n, ext, [[2, 2], [2, 2]], [[0.0, 0.0], [0.0, 0.0]], lam=lam)
+    dx = ext[0] / n[0]
+    x = (np.arange(n[0] + 3) - 0.5) * dx
+    To = np.repeat(np.cos(math.pi * x / ext[0])[:, None], n[1] + 3, axis=1)
+    h.set("TO", To)
+    h.set("T", To)
+    h.advance(dt, 6000)
+    k2 = (2.0 / dx) ** 2 * math.sin(math.pi * dx / (2.0 * ext[0])) ** 2
+    want = To / (1.0 + dt * lam * k2)
+    T = h.get("T")
+    assert np.max(np.abs(T[1:-2, 1:-2] - want[1:-2, 1:-2])) < 1e-11
+    # the continuous eigenvalue (pi/L)^2 differs visibly at this resolution
+    assert np.max(np.abs(To / (1.0 + dt * lam * (math.pi / ext[0]) ** 2) - want)[1:-2, 1:-2]) > 1e-7
