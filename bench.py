@@ -390,7 +390,7 @@ def bench_heat(pkg, args):
               "iteration_ms": round(ms[0], 4), "kernel_ms": round(ms[1], 4),
               "residual_after": [float("%.3e" % x) for x in norms]}
     line = _line("PT heat diffusion T_eff GB/s at %d^3" % n, teff, "GB/s", ms[0], args.steps, args.warmup, "f64", config)
-    line["roofline"] = {"bound": "hbm", "kernel": "k_heat_iter<3>", "achieved": round(achieved, 2),
+    line["roofline"] = {"bound": "hbm", "kernel": "k_heat_iter_z", "achieved": round(achieved, 2),
                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                         "traffic": traffic, "traffic_source": src, "algorithmic_bytes_per_launch": alg}
     if args.no_cpu:

@@ -4,9 +4,10 @@
 // One iteration of the reference is two launches with boundary conditions
 // after each (Heat.jl:42-58): update_q! then the Flux conditions (the face
 // value of q), update_T! then the Value conditions (the ghost of T).  Here
-// one kernel does both updates.  Each thread owns a cell and the three
-// faces below it (x, y, z); it recomputes the new flux of its three upper
-// faces from the same inputs their owners use, so every flux is evaluated
+// one kernel does both updates (in 3D marching up z, k_heat_iter_z).  Each
+// thread owns a cell and the three faces below it (x, y, z); it recomputes
+// the new flux of its three upper faces from the same inputs their owners
+// use, so every flux is evaluated
 // by the same expression wherever it is needed, and writes the new flux of
 // its own faces and the new temperature.  Flux conditions are applied in the
 // kernel (the face takes the prescribed value before the divergence uses
@@ -17,6 +18,7 @@
 #include <cstdio>
 #include <cstring>
 #include <new>
+#include <vector>
 
 #include "model.h"
 
@@ -95,6 +97,61 @@ __global__ __launch_bounds__(256) void k_heat_iter(HeatArgs a)
     a.Tn[c] = a.T[c] - (dTdt + divq) * a.dtT;
 }
 
+// 3D: the same iteration marching up a chunk of HEAT_KC z planes per thread
+// column.  T of the planes below, at and above k and the new flux of the z
+// face below k stay in registers, so each plane of T and q_z is read once per
+// column; every flux is still computed from the same values by the same
+// expression as in k_heat_iter.
+constexpr int HEAT_KC = 32;
+
+__device__ __forceinline__ double heat_zflux(const HeatArgs& a, long c, int idx, double Tup, double Tlo)
+{
+    if (idx == 1 && a.flux_lo[2]) return a.v_lo[2];
+    if (idx == a.n[2] + 1 && a.flux_hi[2]) return a.v_hi[2];
+    const double q = a.q[2][c];
+    const double dT = a.lam * (Tup - Tlo) / a.dx[2];
+    return q - (q + dT) * a.dtq;
+}
+
+__global__ __launch_bounds__(256) void k_heat_iter_z(HeatArgs a)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    if (i > a.n[0] + 1 || j > a.n[1] + 1) return;
+    const int k0 = (int)blockIdx.z * HEAT_KC + 1;
+    const int k1 = min(k0 + HEAT_KC - 1, a.n[2] + 1);
+    const long sy = a.st[1], sz = a.st[2];
+    long c = (long)i + sy * j + sz * k0;
+    const bool xin = i <= a.n[0], yin = j <= a.n[1];
+    if (!(xin && yin)) {
+        // the x faces at i = nx + 1 or the y faces at j = ny + 1 of this column
+        for (int k = k0; k <= k1 && k <= a.n[2]; ++k, c += sz) {
+            if (yin) a.qn[0][c] = heat_flux(a, 0, c, i);
+            if (xin) a.qn[1][c] = heat_flux(a, 1, c, j);
+        }
+        return;
+    }
+    double Tc = a.T[c];
+    double qz_lo = heat_zflux(a, c, k0, Tc, a.T[c - sz]);
+    for (int k = k0; k <= k1; ++k, c += sz) {
+        a.qn[2][c] = qz_lo;
+        if (k > a.n[2]) break; // the top z face of the column
+        const double qx_lo = heat_flux(a, 0, c, i);
+        const double qy_lo = heat_flux(a, 1, c, j);
+        a.qn[0][c] = qx_lo;
+        a.qn[1][c] = qy_lo;
+        const double Tp = a.T[c + sz];
+        const double qz_hi = heat_zflux(a, c + sz, k + 1, Tp, Tc);
+        double divq = (heat_flux(a, 0, c + 1, i + 1) - qx_lo) / a.dx[0];
+        divq = divq + (heat_flux(a, 1, c + sy, j + 1) - qy_lo) / a.dx[1];
+        divq = divq + (qz_hi - qz_lo) / a.dx[2];
+        const double dTdt = (Tc - a.To[c]) / a.dt;
+        a.Tn[c] = Tc - (dTdt + divq) * a.dtT;
+        Tc = Tp;
+        qz_lo = qz_hi;
+    }
+}
+
 // Inf-norms of the T and q residuals as |x| bit patterns (NaN above +Inf)
 template <int ND>
 __global__ __launch_bounds__(256) void k_heat_residual(HeatArgs a, unsigned long long* out)
@@ -151,6 +208,13 @@ __global__ void k_heat_zero2(unsigned long long* p)
 }
 
 static dim3 heat_grid(const Geo& g) { return dim3((g.n[0] + 1 + 63) / 64, (g.n[1] + 1 + 3) / 4, g.nd == 3 ? g.n[2] + 1 : 1); }
+static dim3 heat_grid_z(const Geo& g) { return dim3((g.n[0] + 1 + 63) / 64, (g.n[1] + 1 + 3) / 4, (g.n[2] + HEAT_KC) / HEAT_KC); }
+
+static void launch_heat_iter(fi_heat_s* h, const HeatArgs& a)
+{
+    if (h->g.nd == 3) hipLaunchKernelGGL(k_heat_iter_z, heat_grid_z(h->g), dim3(64, 4, 1), 0, h->stream, a);
+    else hipLaunchKernelGGL(k_heat_iter<2>, heat_grid(h->g), dim3(64, 4, 1), 0, h->stream, a);
+}
 
 static int heat_loc(int nd, int id)
 {
@@ -215,9 +279,7 @@ static int heat_prepare(fi_heat_s* h)
 
 static int heat_iterate(fi_heat_s* h, double dt)
 {
-    HeatArgs a = heat_args(h, dt);
-    if (h->g.nd == 3) hipLaunchKernelGGL(k_heat_iter<3>, heat_grid(h->g), dim3(64, 4, 1), 0, h->stream, a);
-    else hipLaunchKernelGGL(k_heat_iter<2>, heat_grid(h->g), dim3(64, 4, 1), 0, h->stream, a);
+    launch_heat_iter(h, heat_args(h, dt));
     h->cur ^= 1;
     heat_value_bc(h, h->cur);
     return FI_OK;
@@ -383,28 +445,31 @@ int fi_heat_time_iterations(fi_heat_t h, double dt, int n_iter, double out_ms[2]
     if (!h || !out_ms || n_iter < 1 || !(dt > 0.0)) return fail(FI_ERR_ARG, "bad arguments");
     int rc = heat_prepare(h);
     if (rc) return rc;
-    hipEvent_t ev[3];
-    for (int q = 0; q < 3; ++q) HIPCHK(hipEventCreate(&ev[q]));
-    double acc[2] = {0.0, 0.0};
+    // back to back, as a solve runs them: events around every kernel, one
+    // synchronisation at the end
+    std::vector<hipEvent_t> ev(2 * (size_t)n_iter + 1);
+    for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+    HIPCHK(hipEventRecord(ev[0], h->stream));
     for (int it = 0; it < n_iter; ++it) {
-        HeatArgs a = heat_args(h, dt);
-        HIPCHK(hipEventRecord(ev[0], h->stream));
-        if (h->g.nd == 3) hipLaunchKernelGGL(k_heat_iter<3>, heat_grid(h->g), dim3(64, 4, 1), 0, h->stream, a);
-        else hipLaunchKernelGGL(k_heat_iter<2>, heat_grid(h->g), dim3(64, 4, 1), 0, h->stream, a);
-        HIPCHK(hipEventRecord(ev[1], h->stream));
+        if (it > 0) HIPCHK(hipEventRecord(ev[2 * it], h->stream));
+        launch_heat_iter(h, heat_args(h, dt));
+        HIPCHK(hipEventRecord(ev[2 * it + 1], h->stream));
         h->cur ^= 1;
         heat_value_bc(h, h->cur);
-        HIPCHK(hipEventRecord(ev[2], h->stream));
-        HIPCHK(hipEventSynchronize(ev[2]));
-        float t01 = 0.0f, t02 = 0.0f;
-        HIPCHK(hipEventElapsedTime(&t01, ev[0], ev[1]));
-        HIPCHK(hipEventElapsedTime(&t02, ev[0], ev[2]));
-        acc[0] += t02;
-        acc[1] += t01;
     }
-    for (int q = 0; q < 3; ++q) (void)hipEventDestroy(ev[q]);
-    out_ms[0] = acc[0] / n_iter;
-    out_ms[1] = acc[1] / n_iter;
+    HIPCHK(hipEventRecord(ev[2 * (size_t)n_iter], h->stream));
+    HIPCHK(hipEventSynchronize(ev[2 * (size_t)n_iter]));
+    double kernel = 0.0;
+    for (int it = 0; it < n_iter; ++it) {
+        float t = 0.0f;
+        HIPCHK(hipEventElapsedTime(&t, ev[2 * it], ev[2 * it + 1]));
+        kernel += t;
+    }
+    float total = 0.0f;
+    HIPCHK(hipEventElapsedTime(&total, ev[0], ev[2 * (size_t)n_iter]));
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    out_ms[0] = total / n_iter;
+    out_ms[1] = kernel / n_iter;
     return FI_OK;
 }
 

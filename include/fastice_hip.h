@@ -334,8 +334,8 @@ int fi_heat_residual_norms(fi_heat_t h, double dt, double norms_inf[2]);
  * with a residual check every ncheck until both norms are below tol or
  * max_iter, then T_o <- T.  FI_ERR_NONFINITE on a non-finite residual. */
 int fi_heat_step(fi_heat_t h, double dt, int max_iter, int ncheck, double tol, int* iters_out, double norms_inf[2]);
-/* measurement hook: n_iter iterations timed with HIP events; out_ms =
- * {iteration, fused kernel} averages */
+/* measurement hook: n_iter iterations back to back with HIP events around
+ * each kernel; out_ms = {iteration, fused kernel} averages */
 int fi_heat_time_iterations(fi_heat_t h, double dt, int n_iter, double out_ms[2]);
 
 #ifdef __cplusplus

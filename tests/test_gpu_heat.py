@@ -2,9 +2,10 @@
 
 Random temperatures, previous temperatures and fluxes, boundary sides of
 every kind (Value, Flux, none), 2D and 3D, grid sizes that do not fill the
-64 x 4 thread blocks: after a few PT iterations the fused kernel's T and q
-equal the oracle's (update_q! + flux conditions, update_T! + value
-conditions, in separate passes) bit for bit, and so do the residual norms.
+64 x 4 thread blocks or the 32-plane z chunks of the 3D kernel: after a few
+PT iterations the fused kernel's T and q equal the oracle's (update_q! +
+flux conditions, update_T! + value conditions, in separate passes) bit for
+bit, and so do the residual norms.
 """
 import numpy as np
 import pytest
@@ -19,6 +20,7 @@ def _say(msg):
 CASES = [
     ((21, 13, 9), [[1, 2], [0, 1], [2, 0]]),
     ((70, 6, 5), [[2, 2], [2, 2], [2, 2]]),
+    ((17, 9, 70), [[0, 2], [1, 2], [1, 2]]),  # z chunks of 32 planes: seams at 32 and 64
     ((33, 27), [[1, 1], [2, 0]]),
 ]
 
