@@ -76,6 +76,8 @@ def parse():
     p.add_argument("--batch-chunk", type=int, default=32,
                    help="batch96_glacier refined: FP32 sweeps between FP64 outer steps, all models")
     p.add_argument("--heat-n", type=int, default=1023, help="heat_diffusion: cells per axis")
+    p.add_argument("--serving-precision", default="fp64", choices=("fp64", "refined"),
+                   help="serving_poisson40: plain FP64 with the driver's cadence (default) or FP32-refined solves")
     p.add_argument("--requests", type=int, default=200, help="serving_poisson40: number of requests")
     p.add_argument("--copy-calib", action="store_true",
                    help="run a 1 GiB device copy first (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -425,7 +427,7 @@ def bench_serving(pkg, args):
     reqs = pkg.poisson_requests(args.requests, 40.0, seed=7)
     log("serving %d requests (%d 3D), arrivals over %.1f s" % (len(reqs), sum(r.ndims == 3 for r in reqs),
                                                                 reqs[-1].t_arrival))
-    srv = pkg.Server(pkg, max_slots=96)
+    srv = pkg.Server(pkg, max_slots=96, precision=args.serving_precision)
     try:
         s = srv.run(reqs, realtime=True, log=log)
         recs = list(srv.results.values())
@@ -433,7 +435,7 @@ def bench_serving(pkg, args):
         srv.close()
     # offline batched throughput of the same mix: every request present at t = 0
     log("offline batch of the same %d requests" % len(reqs))
-    srv = pkg.Server(pkg, max_slots=96)
+    srv = pkg.Server(pkg, max_slots=96, precision=args.serving_precision)
     try:
         off = srv.run(reqs, offline=True, log=log)
     finally:
@@ -443,8 +445,11 @@ def bench_serving(pkg, args):
     config = {"workload": "serving_poisson40", "requests": len(reqs), "arrival_rate_per_s": 40.0,
               "offered_window_s": round(reqs[-1].t_arrival, 3),
               "mix": "p=0.5 3D glacier (n log-uniform 32..127, n x n x ceil(n/2)) / 2D glacier section "
-                     "(n log-uniform 64..511, n x ceil(n/2)); Glen n=3, FP64 to residual 1e-8 or niter 50n, "
-                     "ncheck 5n",
+                     "(n log-uniform 64..511, n x ceil(n/2)); Glen n=3, "
+                     + ("FP32 sweeps in FP64 defect correction (lagged PT, 32 sweeps per outer step) to residual "
+                        "1e-8 or 400n sweeps" if args.serving_precision == "refined" else
+                        "FP64 to residual 1e-8 or niter 50n, ncheck 5n"),
+              "precision": args.serving_precision,
               "latency_s": {k: (round(v, 4) if v is not None else None) for k, v in s["latency_s"].items()},
               "latency_s_3d": {k: round(v, 4) for k, v in pkg.percentiles(lat3).items()} if lat3 else None,
               "latency_s_2d": {k: round(v, 4) for k, v in pkg.percentiles(lat2).items()} if lat2 else None,
@@ -454,7 +459,7 @@ def bench_serving(pkg, args):
               "makespan_s": round(s["makespan_s"], 3), "converged": s["converged"],
               "batch_steps": s["batch_steps"], "setup_s_mean": round(s["setup_s_mean"], 4)}
     line = _line("serving solves/s (p99 latency in config), Poisson 40/s", s["solves_per_s"], "solves/s",
-                 s["makespan_s"] * 1e3, 1, 0, "f64", config)
+                 s["makespan_s"] * 1e3, 1, 0, "f32+f64" if args.serving_precision == "refined" else "f64", config)
     line["roofline"] = None
     line["cpu_baseline"] = None
     print(json.dumps(line), flush=True)

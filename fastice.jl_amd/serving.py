@@ -3,9 +3,10 @@
 No counterpart in the reference (its drivers solve one model per run,
 scripts_future_API/tm_stokes_chmy.jl:96-108).  Requests arrive as a Poisson
 process; each is a seeded synthetic glacier, 3D (test_vavilov.jl set-up) or
-a 2D section of one (polygon level sets), solved in FP64 to a residual
-tolerance with the reference's glacier cadence (niter = 50n, ncheck = 5n).
-The server
+a 2D section of one (polygon level sets), solved to a residual tolerance:
+in FP64 with the reference's glacier cadence (niter = 50n, ncheck = 5n), or
+(precision="refined") by FP32 sweeps inside FP64 defect correction (lagged
+PT, an outer step every 32 sweeps, at most 400n sweeps).  The server
 
   * admits a waiting request when a batch slot is free and the device has the
     model's bytes (fi_model_bytes) plus a reserve free (hipMemGetInfo),
@@ -80,10 +81,16 @@ class Server:
 
     max_slots: models attached to the batch at once; hbm_reserve: bytes kept
     free beyond the admitted models; tol: residual tolerance of every solve;
-    keep_models: keep finished models open (tests read their fields).
+    keep_models: keep finished models open (tests read their fields);
+    precision: "fp64" or "refined" (a refined model also holds its FP32
+    workspace, counted here as half the FP64 model's bytes on top).
     """
 
-    def __init__(self, pkg, max_slots=64, hbm_reserve=4 << 30, tol=1e-8, keep_models=False, batch_options=None):
+    def __init__(self, pkg, max_slots=64, hbm_reserve=4 << 30, tol=1e-8, keep_models=False, batch_options=None,
+                 precision="fp64"):
+        if precision not in ("fp64", "refined"):
+            raise ValueError("precision must be 'fp64' or 'refined'")
+        self.precision = precision
         self.pkg = pkg
         self.max_slots = max_slots
         self.hbm_reserve = hbm_reserve
@@ -96,6 +103,8 @@ class Server:
     def _fits(self, req):
         kw, ib, _, _ = request_spec(req)
         need = model_bytes(immersed_boundary=ib, **kw)
+        if self.precision == "refined":
+            need += need // 2
         free, _ = self.pkg.mem_info()
         return free >= need + self.hbm_reserve
 
@@ -103,7 +112,10 @@ class Server:
         ta = time.perf_counter() - t0
         _, _, ncheck, max_iter = request_spec(req)
         m = build_model(self.pkg, req)
-        self.batch.add(m, max_iter, ncheck, self.tol)
+        if self.precision == "refined":
+            self.batch.add_refined(m, self.tol)
+        else:
+            self.batch.add(m, max_iter, ncheck, self.tol)
         self.models[req.rid] = (req, m)
         self.results[req.rid] = dict(rid=req.rid, ndims=req.ndims, dims=list(req.dims), t_arrival=req.t_arrival,
                                      t_admit=ta, setup_s=time.perf_counter() - t0 - ta)

@@ -63,3 +63,27 @@ def test_serving_realtime_and_admission(pkg):
             srv.run(reqs[:1], realtime=False)
     finally:
         srv.close()
+
+
+def test_serving_refined_mix_equals_alone(pkg):
+    """Refined serving: every request converges to the tolerance, and each
+    result equals the same refined solve alone (outer steps, sweeps, fields)."""
+    reqs = pkg.poisson_requests(6, 200.0, seed=13, range2d=(24, 64), range3d=(28, 40))
+    _say("case: refined serving of %d requests %s" % (len(reqs), reqs))
+    srv = pkg.Server(pkg, max_slots=4, keep_models=True, precision="refined")
+    try:
+        summary = srv.run(reqs, realtime=False)
+        _say("summary %s" % summary)
+        assert summary["converged"] == len(reqs)
+        for rid, rec in srv.results.items():
+            req = reqs[rid]
+            m = rec["model"]
+            alone = pkg.serving.build_model(pkg, req)
+            outer, sweeps, norms = alone.refined_solve(srv.tol)
+            assert sweeps == rec["iterations"], (req, sweeps, rec["iterations"])
+            assert float(np.max(norms)) == rec["residual"]
+            for n in state_names(req.ndims) + ["ETA"]:
+                assert np.array_equal(m.field(n).parent(), alone.field(n).parent()), (req, n)
+            alone.close()
+    finally:
+        srv.close()
