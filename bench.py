@@ -518,6 +518,9 @@ def main():
     teff64 = 23 * 8 * cells / (ms64 * 1e-3) / 1e9
     ev64 = event_times(A, A.lib().fi_time_iterations, model, max(10, min(50, args.steps // 4)))
     icells = fused_cells(A, model)
+    # stress points [1, n+1]^3 outside the fused interior: the fused launch
+    # also updates their stress (the shell's stress update rides in it)
+    shell_pts = (dims[0] + 1) * (dims[1] + 1) * (dims[2] + 1) - icells if icells else 0
     gfields = 3 if model.desc.gravity_is_field else 0
     log("fp64 %.3f ms/iter, T_eff %.1f GB/s" % (ms64, teff64))
 
@@ -533,15 +536,17 @@ def main():
         ev32 = event_times(A, A.lib().fi_refined_time_inner, model, max(10, min(50, args.steps // 4)))
         log("fp32 %.3f ms/iter, T_eff %.1f GB/s" % (ms32, teff32))
         value, ms_step, dtype = teff32, ms32, "f32 sweeps in f64 defect correction"
-        # fused FP32 correction sweep: R P tau(6) V(3) eta fP g(3), W P tau(6) V(3)
-        kname, kms, kbytes = "k_fused3<float, IB>", ev32[1], 25 * 4 * icells
+        # fused FP32 correction sweep: R P tau(6) V(3) eta fP g(3), W P tau(6) V(3) per interior cell;
+        # shell stress: R P tau(6) V(3) eta fP, W P tau(6) per shell point
+        kname, kms, kbytes = "k_fused3<float, IB>", ev32[1], 25 * 4 * icells + 19 * 4 * shell_pts
         role = "fp32_fused"
         events = {"fp32_iteration": ev32[0], "fp32_fused": ev32[1], "fp32_shell": ev32[2], "fp32_bc": ev32[3],
                   "fp64_iteration": ev64[0], "fp64_fused": ev64[1], "fp64_shell": ev64[2], "fp64_bc": ev64[3]}
     else:
         value, ms_step, dtype = teff64, ms64, "f64"
-        # fused FP64 sweep: R P tau(6) V(3) eta [g(3)], W P tau(6) V(3) eta_next
-        kname, kms, kbytes = "k_fused3<double>", ev64[1], (22 + gfields) * 8 * icells
+        # fused FP64 sweep: R P tau(6) V(3) eta [g(3)], W P tau(6) V(3) eta_next per interior cell;
+        # shell stress: R P tau(6) V(3) eta, W P tau(6) per shell point
+        kname, kms, kbytes = "k_fused3<double>", ev64[1], (22 + gfields) * 8 * icells + 18 * 8 * shell_pts
         role = "fp64_fused"
         events = {"fp64_iteration": ev64[0], "fp64_fused": ev64[1], "fp64_shell": ev64[2], "fp64_bc": ev64[3]}
     achieved = kbytes / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
@@ -588,6 +593,7 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "traffic_source": tsrc,
                      "kernel_ms": round(float(kms), 4), "algorithmic_bytes_per_launch": int(kbytes),
+                     "interior_cells": int(icells), "shell_stress_points": int(shell_pts),
                      "event_ms": {k: round(float(v), 4) for k, v in events.items()}},
         "cpu_baseline": cpu,
     }

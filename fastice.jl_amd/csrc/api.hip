@@ -214,10 +214,10 @@ void iterate(fi_model_s* m, hipEvent_t* ev)
     if (fused) {
         FusedGeom fg = m->fg;
         fg.order = m->forder;
-        launch_fused(m->g, a, fg, ib, m->ftx, m->fty, m->fminw, m->stream);
+        launch_fused(m->g, a, fg, ib, m->ftx, m->fty, m->fminw, m->stream, m->fused_shell ? &boxes : nullptr);
     }
     if (ev) (void)hipEventRecord(ev[1], m->stream);
-    launch_sigma_boxes(m->g, a, boxes, ib, m->stream);
+    if (!(fused && m->fused_shell)) launch_sigma_boxes(m->g, a, boxes, ib, m->stream);
     m->sig_cur ^= 1;
     if (ev) (void)hipEventRecord(ev[2], m->stream);
     run_bc(m, FI_BCGROUP_STRESS);
@@ -709,6 +709,8 @@ int fi_set_option(fi_model_t m, int key, int value)
             m->fty = cfg[value][1];
             m->fminw = cfg[value][2];
         }
+    } else if (key == FI_OPT_FUSED_SHELL) {
+        m->fused_shell = value != 0;
     } else if (key == FI_OPT_REFINE_MODE) {
         if (value < 0 || value > 1) return fail(FI_ERR_ARG, "refinement mode must be 0 or 1");
         m->refine_mode = value;

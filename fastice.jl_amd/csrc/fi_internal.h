@@ -138,9 +138,10 @@ void launch_sigma_boxes(const Geo& g, const StokesArgs<T>& a, const BoxList& bl,
 template <typename T>
 void launch_velocity_boxes(const Geo& g, const StokesArgs<T>& a, const BoxList& bl, bool ib, hipStream_t s);
 // fused stress + velocity sweep over the interior region (3D)
+// (shell_sigma: also the pointwise stress update of those shell boxes, in the same launch)
 template <typename T>
 void launch_fused(const Geo& g, const StokesArgs<T>& a, const FusedGeom& fg, bool ib, int tx, int ty, int minw,
-                  hipStream_t s);
+                  hipStream_t s, const BoxList* shell_sigma = nullptr);
 // compiled (tile lanes in x, tile rows, min waves per SIMD) variants of the fused sweep
 int fused_configs(bool fp32, int cfg[][3], int max);
 // x tiling of a fused region for tiles of tx lanes (sets xa, nx0, nbx)

@@ -412,10 +412,10 @@ static void iterate32(fi_model_s* m, hipEvent_t* ev)
     if (fused) {
         FusedGeom fg = r->fg;
         fg.order = m->forder;
-        launch_fused(m->g, a, fg, ib, m->ftx32, m->fty32, m->fminw32, m->stream);
+        launch_fused(m->g, a, fg, ib, m->ftx32, m->fty32, m->fminw32, m->stream, m->fused_shell ? &boxes : nullptr);
     }
     if (ev) (void)hipEventRecord(ev[1], m->stream);
-    launch_sigma_boxes(m->g, a, boxes, ib, m->stream);
+    if (!(fused && m->fused_shell)) launch_sigma_boxes(m->g, a, boxes, ib, m->stream);
     r->sig_cur ^= 1;
     if (ev) (void)hipEventRecord(ev[2], m->stream);
     bc32(m, FI_BCGROUP_STRESS);

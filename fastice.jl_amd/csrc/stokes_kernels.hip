@@ -790,10 +790,29 @@ __device__ __forceinline__ bool fused_tile_of(int b, int order, int nbx, int nby
     return true;
 }
 
-// MINW: minimum waves per SIMD the register allocation must allow (1: none)
+// MINW: minimum waves per SIMD the register allocation must allow (1: none).
+// Workgroups from nfused on update the stress of the boundary shell (the
+// pointwise sigma_point3, one point per thread, over the boxes of shell
+// whose starts count workgroups of TX * TY threads): the shell's stress
+// reads only the previous iteration, like the fused tiles, and the two write
+// disjoint points, so they share the launch and the shell fills its tail.
 template <typename T, bool IB, int TX, int TY, int MINW>
-__global__ __launch_bounds__(TX * TY, MINW) void k_fused3(StokesArgs<T> a, FusedGeom fg)
+__global__ __launch_bounds__(TX * TY, MINW) void k_fused3(StokesArgs<T> a, FusedGeom fg, BoxList shell, int nfused)
 {
+    if ((int)blockIdx.x >= nfused) {
+        const int bid = (int)blockIdx.x - nfused;
+        int q = 0;
+        while (q + 1 < shell.nb && bid >= shell.start[q + 1]) ++q;
+        const Box b = shell.b[q];
+        const long nx = b.i1 - b.i0 + 1, ny = b.j1 - b.j0 + 1, nz = b.k1 - b.k0 + 1;
+        const long t = (long)(bid - shell.start[q]) * (TX * TY) + threadIdx.x + TX * threadIdx.y;
+        if (t >= nx * ny * nz) return;
+        const int i = b.i0 + (int)(t % nx);
+        const long r = t / nx;
+        const int j = b.j0 + (int)(r % ny), k = b.k0 + (int)(r / ny);
+        sigma_point3<T, IB>(a, (long)i + a.sy * j + a.sz * k, a.sy, a.sz);
+        return;
+    }
     const int nby = (fg.hi[1] - fg.lo[1] + 1 + TY - 3) / (TY - 2);
     const int nbz = (fg.hi[2] - fg.lo[2] + fg.kch) / fg.kch;
     int bx, by, bz;
@@ -1389,7 +1408,7 @@ int fused_auto_kch(const FusedGeom& fg, int tiles_per_plane)
 
 template <typename T>
 void launch_fused(const Geo& g, const StokesArgs<T>& a, const FusedGeom& fg0, bool ib, int tx, int ty, int minw,
-                  hipStream_t s)
+                  hipStream_t s, const BoxList* shell_sigma)
 {
     FusedGeom fg = fg0;
     fused_xtiles(fg, tx, (int)sizeof(T), g.off);
@@ -1397,12 +1416,25 @@ void launch_fused(const Geo& g, const StokesArgs<T>& a, const FusedGeom& fg0, bo
     if (fg.kch <= 0) fg.kch = fused_auto_kch(fg, fg.nbx * nby);
     const int nbz = (fg.hi[2] - fg.lo[2] + fg.kch) / fg.kch;
     const int ntiles = fg.nbx * nby * nbz;
-    dim3 grd((ntiles + 8 * NXCD - 1) / (8 * NXCD) * (8 * NXCD)), blk(tx, ty, 1);
-#define FI_LAUNCH_(TX_, TY_, W_)                                                                \
-    if (tx == TX_ && ty == TY_ && minw == W_) {                                                 \
-        if (ib) hipLaunchKernelGGL((k_fused3<T, true, TX_, TY_, W_>), grd, blk, 0, s, a, fg);  \
-        else hipLaunchKernelGGL((k_fused3<T, false, TX_, TY_, W_>), grd, blk, 0, s, a, fg);    \
-        return;                                                                                 \
+    const int nfused = (ntiles + 8 * NXCD - 1) / (8 * NXCD) * (8 * NXCD);
+    BoxList sh;
+    sh.nb = 0;
+    sh.start[0] = 0;
+    if (shell_sigma) {
+        // box starts in workgroups of tx * ty threads
+        sh = *shell_sigma;
+        for (int q = 0; q < sh.nb; ++q) {
+            const Box& b = sh.b[q];
+            const long n = (long)(b.i1 - b.i0 + 1) * (b.j1 - b.j0 + 1) * (b.k1 - b.k0 + 1);
+            sh.start[q + 1] = sh.start[q] + (int)((n + tx * ty - 1) / (tx * ty));
+        }
+    }
+    dim3 grd(nfused + (sh.nb ? sh.start[sh.nb] : 0)), blk(tx, ty, 1);
+#define FI_LAUNCH_(TX_, TY_, W_)                                                                            \
+    if (tx == TX_ && ty == TY_ && minw == W_) {                                                             \
+        if (ib) hipLaunchKernelGGL((k_fused3<T, true, TX_, TY_, W_>), grd, blk, 0, s, a, fg, sh, nfused);  \
+        else hipLaunchKernelGGL((k_fused3<T, false, TX_, TY_, W_>), grd, blk, 0, s, a, fg, sh, nfused);    \
+        return;                                                                                             \
     }
     if constexpr (sizeof(T) == 8) { FI_FUSED_CFG_D(FI_LAUNCH_) }
     else { FI_FUSED_CFG_F(FI_LAUNCH_) }
@@ -1484,7 +1516,7 @@ void launch_point_packed(const PackedInst<T>* inst, const PackedBox* boxes, cons
     template void launch_sigma_boxes<T>(const Geo&, const StokesArgs<T>&, const BoxList&, bool, hipStream_t);    \
     template void launch_velocity_boxes<T>(const Geo&, const StokesArgs<T>&, const BoxList&, bool, hipStream_t); \
     template void launch_fused<T>(const Geo&, const StokesArgs<T>&, const FusedGeom&, bool, int, int, int,       \
-                                  hipStream_t);                                                                 \
+                                  hipStream_t, const BoxList*);                                                 \
     template void launch_residuals<T>(const Geo&, const StokesArgs<T>&, bool, hipStream_t);                      \
     template void launch_tau<T>(const Geo&, const StokesArgs<T>&, bool, hipStream_t, const TauLag*);                      \
     template void launch_eta_update<T>(const Geo&, const StokesArgs<T>&, hipStream_t);                     \

@@ -163,7 +163,9 @@ int fi_fused_region(fi_model_t m, int lo[3], int hi[3]);
 
 /* Options (no counterpart in the reference).  Kernel path, results do not
  * depend on them: FI_OPT_FUSED 1/0 selects the fused interior sweep or the
- * pointwise sweep everywhere; FI_OPT_ZCHUNK sets the z planes per workgroup of
+ * pointwise sweep everywhere; FI_OPT_FUSED_SHELL 1/0 runs the stress update of
+ * the boundary shell inside the fused launch (default 1) or in a launch of its
+ * own; FI_OPT_ZCHUNK sets the z planes per workgroup of
  * the fused sweep; FI_OPT_FUSED_VARIANT(_FP32) selects a compiled variant of
  * the FP64 (FP32) fused sweep; FI_OPT_TILE_ORDER maps workgroups to tiles:
  * 0 x-fastest, 1 one contiguous run per XCD, 2 runs of 8 tiles per XCD
@@ -195,7 +197,8 @@ enum {
     FI_OPT_FUSED_VARIANT_FP32 = 7,
     FI_OPT_TILE_ORDER = 8,
     FI_OPT_REFINE_ANDERSON = 9,
-    FI_OPT_REFINE_MODE = 10
+    FI_OPT_REFINE_MODE = 10,
+    FI_OPT_FUSED_SHELL = 11
 };
 int fi_set_option(fi_model_t m, int key, int value);
 /* Compiled variants of the fused sweep (FI_OPT_FUSED_VARIANT* take an index

@@ -325,3 +325,21 @@ def test_glacier_2d_levelsets_fractions_iterations(pkg, orc):
     assert not bad, worst
     np.testing.assert_allclose(model.compute_residuals(), ref.compute_residuals(), rtol=TOL, atol=0)
     model.close()
+
+
+def test_shell_in_fused_launch_bitwise(pkg):
+    """The boundary shell's stress update inside the fused launch gives the
+    same fields, bit for bit, as its own launch (FP64 glacier iterations and
+    FP32 correction sweeps)."""
+    _say("case: shell stress in the fused launch vs separate, 70x40x20 glacier")
+    out = []
+    for fs in (0, 1):
+        m, _ = pkg.glacier_3d((70, 40, 20), seed=6, glen=True)
+        m.set_option("fused_shell", fs)
+        m.advance(9)
+        pkg._abi.check(pkg._abi.lib().fi_refined_advance_inner(m._h, 7))
+        m.synchronize()
+        out.append({n: m.field(n).parent() for n in state_names(3) + ["ETA"]})
+        m.close()
+    for n in out[0]:
+        assert np.array_equal(out[0][n], out[1][n]), n
