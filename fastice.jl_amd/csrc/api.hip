@@ -8,6 +8,9 @@
 #include <cstring>
 #include <new>
 
+#include <algorithm>
+#include <vector>
+
 #include "model.h"
 
 using namespace fi;
@@ -110,6 +113,8 @@ StokesArgs<double> make_args(fi_model_s* m, bool in_place)
         a.ns[q] = m->f[FI_F_OM_NS0 + q];
     }
     a.flags = m->flags;
+    a.crange = m->crange;
+    a.erange = m->erange;
     a.nx = m->g.n[0];
     a.ny = m->g.n[1];
     a.nz = m->g.n[2];
@@ -185,6 +190,8 @@ int prepare(fi_model_s* m, bool sync_shadow)
         for (int id = FI_F_P; id <= FI_F_VZ; ++id)
             if (m->f[id]) HIPCHK(hipMemcpyAsync(m->alt(id) - m->g.off, m->buf(id) - m->g.off, bytes,
                                                 hipMemcpyDeviceToDevice, m->stream));
+        m->zs[1 - m->sig_cur] = m->zs[m->sig_cur];
+        m->zv[1 - m->v_cur] = m->zv[m->v_cur];
         m->shadow_dirty = false;
     }
     if (m->desc.immersed_boundary && m->flags_dirty) {
@@ -195,6 +202,10 @@ int prepare(fi_model_s* m, bool sync_shadow)
             ns[q] = m->f[FI_F_OM_NS0 + q];
         }
         launch_ib_flags(m->g, na, ns, m->flags, m->stream);
+        if (m->crange) launch_col_ranges(m->g, m->flags, m->crange, m->erange, m->stream);
+        // locations may have changed validity: no buffer is known to be clean
+        m->zs[0] = m->zs[1] = m->zv[0] = m->zv[1] = false;
+        m->zs32[0] = m->zs32[1] = m->zv32[0] = m->zv32[1] = false;
         m->flags_dirty = false;
     }
     return FI_OK;
@@ -208,6 +219,10 @@ void iterate(fi_model_s* m, hipEvent_t* ev)
 {
     const bool ib = m->desc.immersed_boundary != 0;
     StokesArgs<double> a = make_args(m, false);
+    const int so = 1 - m->sig_cur, vo = 1 - m->v_cur; // the buffers this iteration writes
+    a.z_read = m->zv[m->v_cur];
+    a.z_wsig = m->zs[so];
+    a.z_wvel = m->zv[vo];
     const bool fused = m->use_fused && m->has_interior;
     const BoxList boxes = fused ? m->shell : full_box(m->g);
     if (ev) (void)hipEventRecord(ev[0], m->stream);
@@ -228,6 +243,9 @@ void iterate(fi_model_s* m, hipEvent_t* ev)
     run_bc(m, FI_BCGROUP_VELOCITY);
     if (ev) (void)hipEventRecord(ev[5], m->stream);
     m->eta_cur ^= 1;
+    // every invalid location of the written buffers now holds +0
+    m->zs[so] = true;
+    m->zv[vo] = true;
 }
 
 int advance(fi_model_s* m, int niter)
@@ -310,6 +328,8 @@ void free_model(fi_model_s* m)
     for (int q = 0; q < 10; ++q)
         if (m->sbase[q]) (void)hipFree(m->sbase[q]);
     if (m->flags_base) (void)hipFree(m->flags_base);
+    if (m->crange) (void)hipFree(m->crange);
+    if (m->erange) (void)hipFree(m->erange);
     if (m->d_norm) (void)hipFree(m->d_norm);
     if (m->h_norm) (void)hipHostFree(m->h_norm);
     if (m->d_dem) (void)hipFree(m->d_dem);
@@ -432,6 +452,7 @@ size_t fi_model_bytes(const fi_model_desc* desc)
         if (field_used(*desc, id)) fields += (id >= FI_F_P && id <= FI_F_VZ) ? 2 : 1;
     size_t bytes = fields * g.elems * sizeof(double) + 4 * sizeof(unsigned long long);
     if (desc->immersed_boundary) bytes += g.elems;
+    if (desc->immersed_boundary && g.nd == 3) bytes += 2 * col_ranges_count(g) * sizeof(int2);
     return bytes;
 }
 
@@ -482,6 +503,14 @@ int fi_model_create(const fi_model_desc* desc, fi_model_t* out)
             free_model(m);
             return fail(FI_ERR_HIP, "hipMemsetAsync failed");
         }
+        if (g.nd == 3) {
+            const size_t nc = col_ranges_count(g);
+            if (hipMalloc(&m->crange, nc * sizeof(int2)) != hipSuccess ||
+                hipMalloc(&m->erange, nc * sizeof(int2)) != hipSuccess) {
+                free_model(m);
+                return fail(FI_ERR_OOM, "device allocation of the column ranges failed");
+            }
+        }
     }
     // the fused sweep addresses fields with 32-bit byte offsets
     if (g.nd == 3 && g.elems * sizeof(double) < (1ull << 32))
@@ -518,7 +547,7 @@ size_t fi_field_size(fi_model_t m, int id, int layout)
 
 static void mark_dirty(fi_model_t m, int id)
 {
-    if (id >= FI_F_P && id <= FI_F_VZ) m->shadow_dirty = true;
+    if (id >= FI_F_P && id <= FI_F_VZ) m->state_written();
     if (id >= FI_F_OM_NA0 && id < FI_F_COUNT) {
         m->flags_dirty = true;
         ++m->mask_gen;
@@ -575,10 +604,10 @@ int fi_apply_bc(fi_model_t m, int group)
         }
     } else if (group == FI_BCGROUP_VELOCITY) {
         for (int d = 0; d < m->g.nd; ++d) launch_bc_dim(m->g, velocity_batch(m, d, false), m->stream);
-        m->shadow_dirty = true;
+        m->state_written();
     } else {
         run_bc(m, group);
-        if (group == FI_BCGROUP_STRESS) m->shadow_dirty = true;
+        if (group == FI_BCGROUP_STRESS) m->state_written();
     }
     HIPCHK(hipGetLastError());
     return FI_OK;
@@ -610,7 +639,7 @@ int fi_compute_tau(fi_model_t m)
     if (rc) return rc;
     StokesArgs<double> a = make_args(m, true);
     launch_tau(m->g, a, m->desc.immersed_boundary != 0, m->stream);
-    m->shadow_dirty = true;
+    m->state_written();
     HIPCHK(hipGetLastError());
     return FI_OK;
 }
@@ -678,6 +707,32 @@ int fi_fused_region(fi_model_t m, int lo[3], int hi[3])
         hi[d] = on ? m->fg.hi[d] : -1;
     }
     return on ? 1 : 0;
+}
+
+int fi_fused_active_cells(fi_model_t m, long long out[2])
+{
+    if (!m || !out) return fail(FI_ERR_ARG, "null argument");
+    out[0] = out[1] = 0;
+    if (!(m->use_fused && m->has_interior)) return FI_OK;
+    const FusedGeom& fg = m->fg;
+    const long long nz = fg.hi[2] - fg.lo[2] + 1;
+    out[0] = (long long)(fg.hi[0] - fg.lo[0] + 1) * (fg.hi[1] - fg.lo[1] + 1) * nz;
+    if (!m->crange) {
+        out[1] = out[0];
+        return FI_OK;
+    }
+    int rc = prepare(m, false);
+    if (rc) return rc;
+    std::vector<int2> r(col_ranges_count(m->g));
+    HIPCHK(hipMemcpyAsync(r.data(), m->crange, r.size() * sizeof(int2), hipMemcpyDeviceToHost, m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));
+    for (int j = fg.lo[1]; j <= fg.hi[1]; ++j)
+        for (int i = fg.lo[0]; i <= fg.hi[0]; ++i) {
+            const int2 c = r[(size_t)i + (size_t)m->g.sy * j];
+            const long long lo = std::max(c.x, fg.lo[2]), hi = std::min(c.y, fg.hi[2]);
+            if (hi >= lo) out[1] += hi - lo + 1;
+        }
+    return FI_OK;
 }
 
 int fi_set_option(fi_model_t m, int key, int value)

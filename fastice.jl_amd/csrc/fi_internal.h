@@ -67,6 +67,17 @@ struct StokesArgs {
     typename MaskOf<T>::ptr na[8];
     typename MaskOf<T>::ptr ns[8];
     const uint8_t* flags; // immersed-boundary validity bits (F_* below)
+    // 3D immersed boundary, per column i + sy j: the planes k holding a valid
+    // location (crange) and the planes whose viscosity a valid location may
+    // read (erange); empty columns have x > y
+    const int2* crange;
+    const int2* erange;
+    // 1: the current velocity is +0 at every invalid location of the fused
+    // sweep's footprint, so the sweep may take it as 0 without reading it
+    int z_read;
+    // 1: the output stress / velocity buffers already hold +0 at every
+    // invalid location of the fused interior, so the sweep need not store it
+    int z_wsig, z_wvel;
     const T* fP;          // forcing of the continuity equation (defect correction), or null
     int frozen_eta;       // 1: eta is fixed, eta_next is not written
     int nx, ny, nz;
@@ -153,6 +164,10 @@ bool fused_split(const Geo& g, int ty, int kch, FusedGeom& fg, BoxList& shell);
 BoxList full_box(const Geo& g);
 void launch_ib_flags(const Geo& g, const double* const na[8], const double* const ns[8], uint8_t* flags,
                      hipStream_t s);
+// 3D: the per-column plane ranges of the flags (StokesArgs::crange / erange),
+// arrays of sy * (n[1] + 3) entries; own and eta must not overlap
+void launch_col_ranges(const Geo& g, const uint8_t* flags, int2* own, int2* eta, hipStream_t s);
+inline size_t col_ranges_count(const Geo& g) { return (size_t)g.sy * (size_t)(g.n[1] + 3); }
 template <typename T>
 void launch_residuals(const Geo& g, const StokesArgs<T>& a, bool ib, hipStream_t s);
 // FP32 outputs of the stress lag (old - new) of a compute_tau (xx yy zz xy xz yz)

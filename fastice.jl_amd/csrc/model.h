@@ -36,6 +36,18 @@ struct fi_model_s {
     uint8_t* flags_base = nullptr;
     uint8_t* flags = nullptr;
     bool flags_dirty = true;
+    // 3D immersed boundary: per-column plane ranges of the flags (launch_col_ranges)
+    int2* crange = nullptr;
+    int2* erange = nullptr;
+    // zs[b] / zv[b]: physical buffer b (0: f, 1: sf) of the stress / velocity
+    // is known to hold +0 at every invalid location of the fused footprint
+    // (a sweep's output is; an upload, a state change outside the sweeps or
+    // a mask change makes it unknown)
+    bool zs[2] = {false, false};
+    bool zv[2] = {false, false};
+    // the same for the two FP32 state buffers of the refined solver
+    bool zs32[2] = {false, false};
+    bool zv32[2] = {false, false};
     int mask_gen = 0; // bumped whenever a mask component changes
     int use_fused = 1;
     int refine_full_phase = 1;      // nonlinear refined solves start with a full FP32 PT phase
@@ -46,7 +58,7 @@ struct fi_model_s {
     int fused_shell = 1;            // the shell's stress update rides in the fused launch
     int kch = 0;                             // z planes per fused workgroup (0: sized to the grid)
     int ftx = 64, fty = 8, fminw = 1;        // fused sweep variant of the FP64 iteration (lanes, rows, min waves)
-    int ftx32 = 64, fty32 = 16, fminw32 = 1; // and of the FP32 sweeps of the refined solver
+    int ftx32 = 64, fty32 = 8, fminw32 = 1;  // and of the FP32 sweeps of the refined solver
     int forder = 2;                          // workgroup-to-tile order of the fused sweeps
     FusedGeom fg;
     BoxList shell;
@@ -60,6 +72,13 @@ struct fi_model_s {
     size_t stage_elems = 0;
     fi_batch_s* batch = nullptr; // the batch this model is attached to (batch.hip)
 
+    // the state changed outside the sweeps: both buffers must be made equal
+    // again before the next FP64 sweep, and neither is known to be clean
+    void state_written()
+    {
+        shadow_dirty = true;
+        zs[0] = zs[1] = zv[0] = zv[1] = false;
+    }
     static bool is_sig(int id) { return id >= FI_F_P && id <= FI_F_TYZ; }
     static bool is_vel(int id) { return id >= FI_F_VX && id <= FI_F_VZ; }
     double* eta() { return f[eta_cur ? FI_F_ETA_NEXT : FI_F_ETA]; }

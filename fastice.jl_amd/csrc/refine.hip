@@ -291,6 +291,8 @@ static int ensure_refine(fi_model_s* m)
         return rc;
     }
     if (nd == 3 && m->g.elems * sizeof(float) < (1ull << 32)) r->has_interior = fused_split(m->g, 16, m->kch, r->fg, r->shell);
+    // the allocations are cleared on the model's stream
+    m->zs32[0] = m->zs32[1] = m->zv32[0] = m->zv32[1] = true;
     return FI_OK;
 }
 
@@ -331,6 +333,8 @@ static StokesArgs<float> make_args32(fi_model_s* m, bool in_place)
         a.ns[q] = U8Mask{r->wns[q]};
     }
     a.flags = m->flags;
+    a.crange = m->crange;
+    a.erange = m->erange;
     a.nx = m->g.n[0]; a.ny = m->g.n[1]; a.nz = m->g.n[2];
     a.sy = m->g.sy; a.sz = m->g.sz;
     a.ix = (float)m->g.idx[0]; a.iy = (float)m->g.idx[1]; a.iz = (float)m->g.idx[2];
@@ -406,6 +410,10 @@ static void iterate32(fi_model_s* m, hipEvent_t* ev)
     Refine* r = m->rf;
     const bool ib = m->desc.immersed_boundary != 0;
     StokesArgs<float> a = make_args32(m, false);
+    const int so = 1 - r->sig_cur, vo = 1 - r->v_cur; // the buffers this sweep writes
+    a.z_read = m->zv32[r->v_cur];
+    a.z_wsig = m->zs32[so];
+    a.z_wvel = m->zv32[vo];
     const bool fused = m->use_fused && r->has_interior;
     const BoxList boxes = fused ? r->shell : full_box(m->g);
     if (ev) (void)hipEventRecord(ev[0], m->stream);
@@ -426,6 +434,9 @@ static void iterate32(fi_model_s* m, hipEvent_t* ev)
     bc32(m, FI_BCGROUP_VELOCITY);
     if (ev) (void)hipEventRecord(ev[5], m->stream);
     if (r->full) r->eta_cur ^= 1;
+    // every invalid location of the written buffers now holds +0
+    m->zs32[so] = true;
+    m->zv32[vo] = true;
 }
 
 // FP32 residual Inf-norm of the correction state: enqueue, then read once
@@ -644,6 +655,7 @@ static void phase1_begin(fi_model_s* m)
             cast(m, r->d[id][1], m->buf(id), 1.0);
         }
     r->sig_cur = r->v_cur = 0;
+    m->zs32[0] = m->zs32[1] = m->zv32[0] = m->zv32[1] = false;
     cast(m, r->eta, m->eta(), 1.0);
     cast(m, r->eta2, m->eta_next(), 1.0);
     r->eta_cur = 0;
@@ -658,7 +670,7 @@ static void phase1_end(fi_model_s* m)
     for (int id = FI_F_P; id <= FI_F_VZ; ++id)
         if (m->f[id]) widen(m, m->buf(id), r->cur(id));
     widen(m, m->eta(), r->eta_cur ? r->eta2 : r->eta);
-    m->shadow_dirty = true;
+    m->state_written();
     r->full = false;
 }
 
@@ -709,7 +721,7 @@ static int job_outer(fi_model_s* m, RefineJob& j)
         StokesArgs<double> a = make_args(m, true);
         launch_tau(m->g, a, ib, m->stream);
         run_bc(m, FI_BCGROUP_STRESS);
-        m->shadow_dirty = true;
+        m->state_written();
     }
     int rc = residual_norms(m, j.norms, false);
     if (rc) return rc;
@@ -748,6 +760,7 @@ static int job_outer(fi_model_s* m, RefineJob& j)
             fill32(m, r->d[id][1], 0.0f);
         }
     r->sig_cur = r->v_cur = 0;
+    m->zs32[0] = m->zs32[1] = m->zv32[0] = m->zv32[1] = true;
     j.state = JOB_INNER;
     j.done = 0;
     next_chunk(j);
@@ -782,6 +795,8 @@ static int job_outer_lag_enqueue(fi_model_s* m, RefineJob& j)
             fill32(m, r->d[id][1], 0.0f);
         }
     r->sig_cur = r->v_cur = 0;
+    // the lag written below is +0 at every invalid location, like the fill
+    m->zs32[0] = m->zs32[1] = m->zv32[0] = m->zv32[1] = true;
     {
         StokesArgs<double> a = make_args(m, true);
         const int tid[6] = {FI_F_TXX, FI_F_TYY, FI_F_TZZ, FI_F_TXY, FI_F_TXZ, FI_F_TYZ};
@@ -789,7 +804,7 @@ static int job_outer_lag_enqueue(fi_model_s* m, RefineJob& j)
         for (int q = 0; q < 6; ++q) lg.t[q] = m->f[tid[q]] ? r->d[tid[q]][0] : nullptr;
         launch_tau(m->g, a, ib, m->stream, &lg);
         run_bc(m, FI_BCGROUP_STRESS);
-        m->shadow_dirty = true;
+        m->state_written();
     }
     bc32(m, FI_BCGROUP_STRESS); // homogeneous boundary values of the lag
     int rc = residual_norms_enqueue(m, false);
@@ -915,7 +930,7 @@ int job_chunk_enqueue(fi_model_s* m, RefineJob& j)
             if (m->f[id])
                 hipLaunchKernelGGL(k_add_correction, ew_grid(m->g.elems), dim3(256), 0, m->stream, m->buf(id) - m->g.off,
                                    m->rf->cur(id) - m->g.off, m->g.elems);
-        m->shadow_dirty = true;
+        m->state_written();
         ++j.outer;
         j.pending_outer = true;
         return job_outer_lag_enqueue(m, j);
@@ -985,7 +1000,7 @@ int job_chunk_finish(fi_model_s* m, RefineJob& j)
                 hipLaunchKernelGGL(k_add_correction, ew_grid(m->g.elems), dim3(256), 0, m->stream, m->buf(id) - m->g.off,
                                    r->cur(id) - m->g.off, m->g.elems);
     }
-    m->shadow_dirty = true;
+    m->state_written();
     ++j.outer;
     return job_outer(m, j);
 }

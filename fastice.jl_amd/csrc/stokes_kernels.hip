@@ -13,9 +13,14 @@
 //   lerp Center->Vertex 0.5*(f[i-1]+f[i]); Vertex->Center 0.5*(f[i]+f[i+1]),
 //   nested x innermost, then y, then z.
 //   masked d/dx(f, w): (w[r]*f[r] - w[l]*f[l]) * (1/dx), w at f's location.
+#include <algorithm>
+
 #include "fi_internal.h"
 
 namespace fi {
+
+// empty plane range of a column (lo > hi), see k_col_range
+constexpr int RANGE_NONE = 1 << 20;
 
 // ---------------------------------------------------------------------------
 // point arithmetic
@@ -99,8 +104,9 @@ __device__ __forceinline__ T rheology(const StokesArgs<T>& a, long c, T tII)
     return eta;
 }
 
+// the same at byte offset bo, with the lagged viscosity at bo already in a register
 template <typename T>
-__device__ __forceinline__ T rheology_o(const StokesArgs<T>& a, uint32_t bo, T tII)
+__device__ __forceinline__ T rheology_o(const StokesArgs<T>& a, uint32_t bo, T tII, T old)
 {
     T eta;
     if (a.rh_kind == FI_RHEO_LINEAR_CONST) eta = a.rh_eta;
@@ -111,7 +117,6 @@ __device__ __forceinline__ T rheology_o(const StokesArgs<T>& a, uint32_t bo, T t
     }
     if (a.rh_has_reg) eta = T(1) / (T(1) / eta + a.rh_inv_eta_reg);
     if (a.rh_relax != FI_RELAX_NONE) {
-        T old = *reinterpret_cast<const T*>(reinterpret_cast<const char*>(a.eta) + bo);
         T w = T(1) - a.rh_rate;
         if (a.rh_relax == FI_RELAX_LIN) eta = old * w + eta * a.rh_rate;
         else if (a.rh_relax == FI_RELAX_LOG) eta = exp(log(old) * w + log(eta) * a.rh_rate);
@@ -502,7 +507,7 @@ __device__ __forceinline__ void fused3_tile(const StokesArgs<T>& a, const FusedG
             if (!a.frozen_eta) {
                 const T t = tauII(xxk, yyk, zzk, lerp4(xy, xy_ip, xy_jp, xy_ijp), lerp4(xz, xz_ip, xz_kp, xz_ikp),
                                   lerp4(yz, yz_jp, yz_kp, yz_jkp));
-                sto(a.eta_next, cv, rheology_o(a, cv, t));
+                sto(a.eta_next, cv, rheology_o(a, cv, t, em));
             }
         }
 
@@ -542,6 +547,26 @@ __device__ __forceinline__ void fused3_tile(const StokesArgs<T>& a, const FusedG
 // rates and na-weighted stress for the masked momentum fluxes (the products
 // are the operands the pointwise kernels form, so results are identical).
 // Raw off-diagonal stress is kept beside it for tau_II when eta is updated.
+//
+// Empty space is not read.  A thread reads the old stress of a location only
+// where the location is valid (elsewhere the update is 0 whatever it holds),
+// and the flags and na weights of a cell only on the planes of its column
+// that hold a valid location (crange: elsewhere both are unused, every new
+// stress there being 0).  With frozen eta (FP32 correction sweeps) the
+// viscosity is read only on the planes a valid location may need (erange).
+// Velocity and its ns weight are skipped outside crange only when z_read
+// says that the current velocity is +0 at every invalid location (then the
+// weighted value is +0 either way), and the new state at invalid locations is
+// stored only when the output buffers are not known to hold +0 there already
+// (z_wsig, z_wvel).  Every value the sweep produces is therefore the one of
+// the full sweep, bit for bit.
+template <typename P>
+__device__ __forceinline__ auto ldif(bool p, P base, uint32_t bo) -> decltype(ldo(base, bo))
+{
+    using R = decltype(ldo(base, bo));
+    return p ? ldo(base, bo) : R(0);
+}
+
 template <typename T, int TX, int TY>
 __device__ __forceinline__ void fused3_ib_tile(const StokesArgs<T>& a, const FusedGeom& fg, int bx, int by, int bz)
 {
@@ -554,8 +579,8 @@ __device__ __forceinline__ void fused3_ib_tile(const StokesArgs<T>& a, const Fus
     fused_xrange(fg, TX, bx, xs, xe);
     const int i = xs - 1 + tx;
     const int j = fg.lo[1] - 1 + by * (TY - 2) + ty;
-    const int ks = fg.lo[2] + bz * fg.kch;
-    const int ke = min(ks + fg.kch - 1, fg.hi[2]);
+    const int ks0 = fg.lo[2] + bz * fg.kch;
+    const int ke0 = min(ks0 + fg.kch - 1, fg.hi[2]);
     const bool inner = tx >= 1 && tx <= TX - 2 && ty >= 1 && ty <= TY - 2 && i <= xe && j <= fg.hi[1];
     const bool can_c = tx < TX - 1 && ty < TY - 1;
     const bool can_xy = tx > 0 && ty > 0;
@@ -569,39 +594,94 @@ __device__ __forceinline__ void fused3_ib_tile(const StokesArgs<T>& a, const Fus
     const T ix = a.ix, iy = a.iy, iz = a.iz;
     const auto wvx = a.ns[OM_VCC], wvy = a.ns[OM_CVC], wvz = a.ns[OM_CCV];
     const auto wc = a.na[OM_CCC], wxy = a.na[OM_VVC], wxz = a.na[OM_VCV], wyz = a.na[OM_CVV];
-    // old-stress components this thread's row needs (as in fused3_tile)
+    // old-stress components this thread's row needs (as in fused3_tile), and
+    // those it computes at all
     const unsigned rneed = ty == 0 ? 0x05u : ty == TY - 1 ? 0x50u : 0x7fu;
+    const unsigned rcan = (can_c ? 0x0fu : 0u) | (can_xy ? 0x10u : 0u) | (can_xz ? 0x20u : 0u) | (can_yz ? 0x40u : 0u);
+    // planes of this column with a valid location, and with a needed viscosity
+    const int2 rg = a.crange[(long)ic + a.sy * (long)jc];
+    const int2 re = a.erange[(long)ic + a.sy * (long)jc];
+    const bool zr = a.z_read != 0, ws = a.z_wsig == 0, wv = a.z_wvel == 0;
+    auto inr = [&](int p) { return p >= rg.x && p <= rg.y; };
+    auto inv = [&](int p) { return !zr || inr(p); };
+    auto ine = [&](int p) { return need_raw || (p >= re.x && p <= re.y); };
+
+    // Planes of the chunk whose new state can differ from 0: the tile's
+    // footprint holds valid locations only on [tlo, thi].  Once the output
+    // buffers are known to be +0 at invalid locations, the march covers
+    // [max(ks0, tlo - 1), min(ke0, thi)] only; the planes of the chunk outside
+    // it keep their stress and velocity (0), and with a viscosity update their
+    // viscosity goes through the rheology at tau_II = 0 (every stress around
+    // such a cell, on its plane and the next, is at an invalid location).
+    __shared__ int s_lo, s_hi;
+    if (tx == 0 && ty == 0) {
+        s_lo = RANGE_NONE;
+        s_hi = -RANGE_NONE;
+    }
+    __syncthreads();
+    if (rg.x <= rg.y) {
+        atomicMin(&s_lo, rg.x);
+        atomicMax(&s_hi, rg.y);
+    }
+    __syncthreads();
+    const bool narrow = !ws && !wv;
+    const int ks = narrow ? max(ks0, s_lo - 1) : ks0;
+    const int ke = narrow ? min(ke0, s_hi) : ke0;
+    if (need_raw && inner) {
+        // viscosity of the planes outside [ks, ke]: four planes per pass
+        auto eta_only = [&](int k0, int k1) {
+            for (int k = k0; k <= k1; k += 4) {
+                T o[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    o[u] = k + u <= k1 ? ldo(a.eta, col + bsz * (uint32_t)(k + u)) : T(1);
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    if (k + u <= k1) {
+                        const uint32_t cu = col + bsz * (uint32_t)(k + u);
+                        sto(a.eta_next, cu, rheology_o(a, cu, T(0), o[u]));
+                    }
+            }
+        };
+        if (ks > ke) {
+            eta_only(ks0, ke0);
+        } else {
+            eta_only(ks0, ks - 1);
+            eta_only(ke + 1, ke0);
+        }
+    }
+    if (ks > ke) return;
 
     const uint32_t c2 = col + bsz * (uint32_t)(ks - 2), c1 = c2 + bsz, c0 = c1 + bsz, c3 = c2 - bsz;
     // raw velocity and ns weights of the own column
-    T vxm = ldo(a.Vx, c2), vym = ldo(a.Vy, c2), vzm = ldo(a.Vz, c2);
-    T mxm = ldo(wvx, c2), mym = ldo(wvy, c2), mzm = ldo(wvz, c2);
-    unsigned flm = ldo(a.flags, c2 / E);
+    T vxm = ldif(inv(ks - 2), a.Vx, c2), vym = ldif(inv(ks - 2), a.Vy, c2), vzm = ldif(inv(ks - 2), a.Vz, c2);
+    T mxm = ldif(inv(ks - 2), wvx, c2), mym = ldif(inv(ks - 2), wvy, c2), mzm = ldif(inv(ks - 2), wvz, c2);
+    unsigned flm = ldif(inr(ks - 2), a.flags, c2 / E);
     T qxm = mxm * vxm, qym = mym * vym;
-    T vx0 = ldo(a.Vx, c1), vy0 = ldo(a.Vy, c1), vz0 = ldo(a.Vz, c1);
-    T mx0 = ldo(wvx, c1), my0 = ldo(wvy, c1), mz0 = ldo(wvz, c1);
+    T vx0 = ldif(inv(ks - 1), a.Vx, c1), vy0 = ldif(inv(ks - 1), a.Vy, c1), vz0 = ldif(inv(ks - 1), a.Vz, c1);
+    T mx0 = ldif(inv(ks - 1), wvx, c1), my0 = ldif(inv(ks - 1), wvy, c1), mz0 = ldif(inv(ks - 1), wvz, c1);
     T qx0 = mx0 * vx0, qy0 = my0 * vy0, qz0 = mz0 * vz0;
-    T vzp = ldo(a.Vz, c0), mzp = ldo(wvz, c0);
+    T vzp = ldif(inv(ks), a.Vz, c0), mzp = ldif(inv(ks), wvz, c0);
     T qzp = mzp * vzp;
-    T mc0 = ldo(wc, c1), mxy0 = ldo(wxy, c1), mxz0 = ldo(wxz, c1), myz0 = ldo(wyz, c1);
-    unsigned fl0 = ldo(a.flags, c1 / E);
-    T emm = ldo(a.eta, c3), em = ldo(a.eta, c2), e0 = ldo(a.eta, c1);
+    T mc0 = ldif(inr(ks - 1), wc, c1), mxy0 = ldif(inr(ks - 1), wxy, c1), mxz0 = ldif(inr(ks - 1), wxz, c1),
+      myz0 = ldif(inr(ks - 1), wyz, c1);
+    unsigned fl0 = ldif(inr(ks - 1), a.flags, c1 / E);
+    T emm = ldif(ine(ks - 3), a.eta, c3), em = ldif(ine(ks - 2), a.eta, c2), e0 = ldif(ine(ks - 1), a.eta, c1);
     // own new stress: weighted at k and k-1, raw diagonal at k
     T Pw = T(0), Pwm = T(0), xxw = T(0), yyw = T(0), zzw = T(0), zzwm = T(0), xxr = T(0), yyr = T(0), zzr = T(0);
     sE[(ks - 2) % 3][ty][tx] = em;
 
     for (int kk = ks - 1; kk <= ke + 1; ++kk) {
         const uint32_t c = col + bsz * (uint32_t)kk;
-        // previous-iteration stress at plane kk (used after the barrier); the
-        // halo planes ks-1 and ke+1 only feed P, tzz and txz, tyz
-        // (an unneeded component is read from the line of a needed one
-        // instead: no branch, no extra HBM traffic, its value is never used)
-        const unsigned nd = rneed & (kk == ks - 1 ? 0x09u : kk == ke + 1 ? 0x60u : 0x7fu);
-        const T* fb = (nd & 1u) ? a.P : (nd & 16u) ? a.txy : (nd & 32u) ? a.txz : (nd & 64u) ? a.tyz : a.P;
-        const T oP = ldo((nd & 1u) ? a.P : fb, c), oxx = ldo((nd & 2u) ? a.txx : fb, c),
-                oyy = ldo((nd & 4u) ? a.tyy : fb, c), ozz = ldo((nd & 8u) ? a.tzz : fb, c),
-                oxy = ldo((nd & 16u) ? a.txy : fb, c), oxz = ldo((nd & 32u) ? a.txz : fb, c),
-                oyz = ldo((nd & 64u) ? a.tyz : fb, c);
+        // previous-iteration stress at plane kk (used after the barrier): the
+        // components this thread computes at a valid location that the tile
+        // needs (the halo planes ks-1 and ke+1 only feed P, tzz and txz, tyz)
+        const unsigned valid = (fl0 & F_CCC ? 0x0fu : 0u) | (fl0 & F_VVC ? 0x10u : 0u) | (fl0 & F_VCV ? 0x20u : 0u) |
+                               (fl0 & F_CVV ? 0x40u : 0u);
+        const unsigned nd = rneed & rcan & valid & (kk == ks - 1 ? 0x09u : kk == ke + 1 ? 0x60u : 0x7fu);
+        const T oP = ldif(nd & 1u, a.P, c), oxx = ldif(nd & 2u, a.txx, c), oyy = ldif(nd & 4u, a.tyy, c),
+                ozz = ldif(nd & 8u, a.tzz, c), oxy = ldif(nd & 16u, a.txy, c), oxz = ldif(nd & 32u, a.txz, c),
+                oyz = ldif(nd & 64u, a.tyz, c);
         sVx[ty][tx] = qx0;
         sVy[ty][tx] = qy0;
         sVz[ty][tx] = qz0;
@@ -611,18 +691,19 @@ __device__ __forceinline__ void fused3_ib_tile(const StokesArgs<T>& a, const Fus
         unsigned nfl = 0;
         if (kk <= ke) {
             const uint32_t cn = c + bsz;
-            nvx = ldo(a.Vx, cn);
-            nvy = ldo(a.Vy, cn);
-            nvz2 = ldo(a.Vz, cn + bsz);
-            nmx = ldo(wvx, cn);
-            nmy = ldo(wvy, cn);
-            nmz2 = ldo(wvz, cn + bsz);
-            nmc = ldo(wc, cn);
-            nmxy = ldo(wxy, cn);
-            nmxz = ldo(wxz, cn);
-            nmyz = ldo(wyz, cn);
-            nfl = ldo(a.flags, cn / E);
-            ne = ldo(a.eta, cn);
+            const bool v1 = inv(kk + 1), v2 = inv(kk + 2), r1 = inr(kk + 1);
+            nvx = ldif(v1, a.Vx, cn);
+            nvy = ldif(v1, a.Vy, cn);
+            nvz2 = ldif(v2, a.Vz, cn + bsz);
+            nmx = ldif(v1, wvx, cn);
+            nmy = ldif(v1, wvy, cn);
+            nmz2 = ldif(v2, wvz, cn + bsz);
+            nmc = ldif(r1, wc, cn);
+            nmxy = ldif(r1, wxy, cn);
+            nmxz = ldif(r1, wxz, cn);
+            nmyz = ldif(r1, wyz, cn);
+            nfl = ldif(r1, a.flags, cn / E);
+            ne = ldif(ine(kk + 1), a.eta, cn);
         }
         __syncthreads();
 
@@ -668,13 +749,15 @@ __device__ __forceinline__ void fused3_ib_tile(const StokesArgs<T>& a, const Fus
             sS[b][8][ty][tx] = yzc;
         }
         if (inner && kk >= ks && kk <= ke) {
-            sto(a.Pn, c, Pc);
-            sto(a.txxn, c, xxc);
-            sto(a.tyyn, c, yyc);
-            sto(a.tzzn, c, zzc);
-            sto(a.txyn, c, xyc);
-            sto(a.txzn, c, xzc);
-            sto(a.tyzn, c, yzc);
+            if (ws || (fl0 & F_CCC)) {
+                sto(a.Pn, c, Pc);
+                sto(a.txxn, c, xxc);
+                sto(a.tyyn, c, yyc);
+                sto(a.tzzn, c, zzc);
+            }
+            if (ws || (fl0 & F_VVC)) sto(a.txyn, c, xyc);
+            if (ws || (fl0 & F_VCV)) sto(a.txzn, c, xzc);
+            if (ws || (fl0 & F_CVV)) sto(a.tyzn, c, yzc);
         }
         __syncthreads();
 
@@ -686,28 +769,36 @@ __device__ __forceinline__ void fused3_ib_tile(const StokesArgs<T>& a, const Fus
             const T ex = T(0.5) * (sE[ek][ty][tx - 1] + em);
             const T ey = T(0.5) * (sE[ek][ty - 1][tx] + em);
             const T ez = T(0.5) * (emm + em);
-            const T g0 = a.grav_field ? ldo(a.gx, cv) : a.grav[0];
-            const T g1 = a.grav_field ? ldo(a.gy, cv) : a.grav[1];
-            const T g2 = a.grav_field ? ldo(a.gz, cv) : a.grav[2];
             const T xyw = sS[bo][3][ty][tx], xzw = sS[bo][4][ty][tx], yzw = sS[bo][5][ty][tx];
-            sto(a.Vxn, cv, (flm & F_VCC) ? upd_V_ib(vxm, dif(Pw, sS[bo][0][ty][tx - 1], ix), dif(xxw, sS[bo][1][ty][tx - 1], ix),
-                                                    dif(sS[bo][3][ty + 1][tx], xyw, iy), dif(sS[bn][4][ty][tx], xzw, iz),
-                                                    g0, ex, mxm, a.dV[0])
-                                         : T(0));
-            sto(a.Vyn, cv, (flm & F_CVC) ? upd_V_ib(vym, dif(Pw, sS[bo][0][ty - 1][tx], iy), dif(yyw, sS[bo][2][ty - 1][tx], iy),
-                                                    dif(sS[bo][3][ty][tx + 1], xyw, ix), dif(sS[bn][5][ty][tx], yzw, iz),
-                                                    g1, ey, mym, a.dV[1])
-                                         : T(0));
-            sto(a.Vzn, cv, (flm & F_CCV) ? upd_V_ib(vzm, dif(Pw, Pwm, iz), dif(zzw, zzwm, iz),
-                                                    dif(sS[bo][4][ty][tx + 1], xzw, ix), dif(sS[bo][5][ty + 1][tx], yzw, iy),
-                                                    g2, ez, mzm, a.dV[2])
-                                         : T(0));
+            if (flm & F_VCC) {
+                const T g0 = a.grav_field ? ldo(a.gx, cv) : a.grav[0];
+                sto(a.Vxn, cv, upd_V_ib(vxm, dif(Pw, sS[bo][0][ty][tx - 1], ix), dif(xxw, sS[bo][1][ty][tx - 1], ix),
+                                        dif(sS[bo][3][ty + 1][tx], xyw, iy), dif(sS[bn][4][ty][tx], xzw, iz), g0, ex,
+                                        mxm, a.dV[0]));
+            } else if (wv) {
+                sto(a.Vxn, cv, T(0));
+            }
+            if (flm & F_CVC) {
+                const T g1 = a.grav_field ? ldo(a.gy, cv) : a.grav[1];
+                sto(a.Vyn, cv, upd_V_ib(vym, dif(Pw, sS[bo][0][ty - 1][tx], iy), dif(yyw, sS[bo][2][ty - 1][tx], iy),
+                                        dif(sS[bo][3][ty][tx + 1], xyw, ix), dif(sS[bn][5][ty][tx], yzw, iz), g1, ey,
+                                        mym, a.dV[1]));
+            } else if (wv) {
+                sto(a.Vyn, cv, T(0));
+            }
+            if (flm & F_CCV) {
+                const T g2 = a.grav_field ? ldo(a.gz, cv) : a.grav[2];
+                sto(a.Vzn, cv, upd_V_ib(vzm, dif(Pw, Pwm, iz), dif(zzw, zzwm, iz), dif(sS[bo][4][ty][tx + 1], xzw, ix),
+                                        dif(sS[bo][5][ty + 1][tx], yzw, iy), g2, ez, mzm, a.dV[2]));
+            } else if (wv) {
+                sto(a.Vzn, cv, T(0));
+            }
             if (need_raw) {
                 const T t = tauII(xxr, yyr, zzr,
                                   lerp4(sS[bo][6][ty][tx], sS[bo][6][ty][tx + 1], sS[bo][6][ty + 1][tx], sS[bo][6][ty + 1][tx + 1]),
                                   lerp4(sS[bo][7][ty][tx], sS[bo][7][ty][tx + 1], sS[bn][7][ty][tx], sS[bn][7][ty][tx + 1]),
                                   lerp4(sS[bo][8][ty][tx], sS[bo][8][ty + 1][tx], sS[bn][8][ty][tx], sS[bn][8][ty + 1][tx]));
-                sto(a.eta_next, cv, rheology_o(a, cv, t));
+                sto(a.eta_next, cv, rheology_o(a, cv, t, em));
             }
         }
 
@@ -913,6 +1004,15 @@ __device__ __forceinline__ void put_tau(T* dst, const T* old, float* lag, long c
     if (LAG) lag[c] = (float)(old[c] - v);
     dst[c] = v;
 }
+// the same at an invalid immersed-boundary location: the stress is 0, and so
+// is the lag (the sweeps never read the old stress of an invalid location,
+// and their first sweep overwrites it with 0)
+template <bool LAG, typename T>
+__device__ __forceinline__ void put_tau0(T* dst, float* lag, long c)
+{
+    if (LAG) lag[c] = 0.0f;
+    dst[c] = T(0);
+}
 
 // compute_tau!: kernels_3d.jl:171-191 (IB variant: kernels_3d.jl:216-254 with
 // its undefined `Pr` location read as the Center location); writes the
@@ -942,9 +1042,9 @@ __global__ __launch_bounds__(256) void k_tau3(StokesArgs<T> a, TauLag lg)
         put_tau<LAG>(a.tyyn, a.tyy, lg.t[1], c, T(2) * ec * (eyy - divV / T(3)));
         put_tau<LAG>(a.tzzn, a.tzz, lg.t[2], c, T(2) * ec * (ezz - divV / T(3)));
     } else {
-        put_tau<LAG>(a.txxn, a.txx, lg.t[0], c, T(0));
-        put_tau<LAG>(a.tyyn, a.tyy, lg.t[1], c, T(0));
-        put_tau<LAG>(a.tzzn, a.tzz, lg.t[2], c, T(0));
+        put_tau0<LAG>(a.txxn, lg.t[0], c);
+        put_tau0<LAG>(a.tyyn, lg.t[1], c);
+        put_tau0<LAG>(a.tzzn, lg.t[2], c);
     }
     if (fl & F_VVC) {
         T exy;
@@ -952,21 +1052,21 @@ __global__ __launch_bounds__(256) void k_tau3(StokesArgs<T> a, TauLag lg)
                                 (ns[OM_VCC][c] * Vx[c] - ns[OM_VCC][c - sy] * Vx[c - sy]) * iy);
         else exy = T(0.5) * ((Vy[c] - Vy[c - 1]) * ix + (Vx[c] - Vx[c - sy]) * iy);
         put_tau<LAG>(a.txyn, a.txy, lg.t[3], c, T(2) * lerp4(eta[c - 1 - sy], eta[c - sy], eta[c - 1], eta[c]) * exy);
-    } else put_tau<LAG>(a.txyn, a.txy, lg.t[3], c, T(0));
+    } else put_tau0<LAG>(a.txyn, lg.t[3], c);
     if (fl & F_VCV) {
         T exz;
         if (IB) exz = T(0.5) * ((ns[OM_CCV][c] * Vz[c] - ns[OM_CCV][c - 1] * Vz[c - 1]) * ix +
                                 (ns[OM_VCC][c] * Vx[c] - ns[OM_VCC][c - sz] * Vx[c - sz]) * iz);
         else exz = T(0.5) * ((Vz[c] - Vz[c - 1]) * ix + (Vx[c] - Vx[c - sz]) * iz);
         put_tau<LAG>(a.txzn, a.txz, lg.t[4], c, T(2) * lerp4(eta[c - 1 - sz], eta[c - sz], eta[c - 1], eta[c]) * exz);
-    } else put_tau<LAG>(a.txzn, a.txz, lg.t[4], c, T(0));
+    } else put_tau0<LAG>(a.txzn, lg.t[4], c);
     if (fl & F_CVV) {
         T eyz;
         if (IB) eyz = T(0.5) * ((ns[OM_CCV][c] * Vz[c] - ns[OM_CCV][c - sy] * Vz[c - sy]) * iy +
                                 (ns[OM_CVC][c] * Vy[c] - ns[OM_CVC][c - sz] * Vy[c - sz]) * iz);
         else eyz = T(0.5) * ((Vz[c] - Vz[c - sy]) * iy + (Vy[c] - Vy[c - sz]) * iz);
         put_tau<LAG>(a.tyzn, a.tyz, lg.t[5], c, T(2) * lerp4(eta[c - sy - sz], eta[c - sz], eta[c - sy], eta[c]) * eyz);
-    } else put_tau<LAG>(a.tyzn, a.tyz, lg.t[5], c, T(0));
+    } else put_tau0<LAG>(a.tyzn, lg.t[5], c);
 }
 
 static inline dim3 block2();
@@ -1081,6 +1181,54 @@ void launch_ib_flags(const Geo& g, const double* const na[8], const double* cons
     }
     dim3 blk(64, 4, 1), grd((g.n[0] + 1 + 63) / 64, (g.n[1] + 1 + 3) / 4, g.nd == 3 ? g.n[2] + 1 : 1);
     hipLaunchKernelGGL(k_flags, grd, blk, 0, s, m, flags, g.nd, g.n[0], g.n[1], g.sy, g.nd == 3 ? g.sz : 0);
+}
+
+// Plane ranges per column (i, j), i in [0, nx+2], j in [0, ny+2], once per
+// geometry.  own: the first and last plane k whose flags are non-zero (some
+// location of the cell is valid); every plane outside it has no valid
+// location.  eta: the own ranges of the 3 x 3 columns around (i, j), each
+// widened by one plane on both sides: a valid location at (i', j', k') reads
+// the viscosity of the cells (i'-1..i', j'-1..j', k'-1..k') only, so outside
+// this range no valid location reads the viscosity of the column.
+__global__ __launch_bounds__(256) void k_col_range(const uint8_t* __restrict__ flags, int2* __restrict__ own, int nx,
+                                                   int ny, int nz, long sy, long sz)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i > nx + 2 || j > ny + 2) return;
+    const long col = i + sy * j;
+    int lo = RANGE_NONE, hi = -RANGE_NONE;
+    for (int k = 0; k <= nz + 2; ++k)
+        if (flags[col + sz * k]) {
+            lo = min(lo, k);
+            hi = k;
+        }
+    own[col] = make_int2(lo, hi);
+}
+
+__global__ __launch_bounds__(256) void k_col_widen(const int2* __restrict__ own, int2* __restrict__ eta, int nx,
+                                                   int ny, long sy)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i > nx + 2 || j > ny + 2) return;
+    int lo = RANGE_NONE, hi = -RANGE_NONE;
+    for (int jj = max(j - 1, 0); jj <= min(j + 1, ny + 2); ++jj)
+        for (int ii = max(i - 1, 0); ii <= min(i + 1, nx + 2); ++ii) {
+            const int2 r = own[ii + sy * jj];
+            if (r.x <= r.y) {
+                lo = min(lo, r.x - 1);
+                hi = max(hi, r.y + 1);
+            }
+        }
+    eta[i + sy * j] = make_int2(lo, hi);
+}
+
+void launch_col_ranges(const Geo& g, const uint8_t* flags, int2* own, int2* eta, hipStream_t s)
+{
+    dim3 grd((g.n[0] + 3 + 255) / 256, g.n[1] + 3, 1);
+    hipLaunchKernelGGL(k_col_range, grd, dim3(256), 0, s, flags, own, g.n[0], g.n[1], g.n[2], g.sy, g.sz);
+    hipLaunchKernelGGL(k_col_widen, grd, dim3(256), 0, s, own, eta, g.n[0], g.n[1], g.sy);
 }
 
 // ---------------------------------------------------------------------------
@@ -1258,8 +1406,8 @@ __global__ __launch_bounds__(256) void k_tau2(StokesArgs<T> a, TauLag lg)
         put_tau<LAG>(a.txxn, a.txx, lg.t[0], c, T(2) * ec * (exx - divV / T(3)));
         put_tau<LAG>(a.tyyn, a.tyy, lg.t[1], c, T(2) * ec * (eyy - divV / T(3)));
     } else {
-        put_tau<LAG>(a.txxn, a.txx, lg.t[0], c, T(0));
-        put_tau<LAG>(a.tyyn, a.tyy, lg.t[1], c, T(0));
+        put_tau0<LAG>(a.txxn, lg.t[0], c);
+        put_tau0<LAG>(a.tyyn, lg.t[1], c);
     }
     if (fl & F_VVC) {
         T exy;
@@ -1267,7 +1415,7 @@ __global__ __launch_bounds__(256) void k_tau2(StokesArgs<T> a, TauLag lg)
                                 (ns[OM2_VC][c] * Vx[c] - ns[OM2_VC][c - sy] * Vx[c - sy]) * iy);
         else exy = T(0.5) * ((Vy[c] - Vy[c - 1]) * ix + (Vx[c] - Vx[c - sy]) * iy);
         put_tau<LAG>(a.txyn, a.txy, lg.t[3], c, T(2) * lerp4(eta[c - 1 - sy], eta[c - sy], eta[c - 1], eta[c]) * exy);
-    } else put_tau<LAG>(a.txyn, a.txy, lg.t[3], c, T(0));
+    } else put_tau0<LAG>(a.txyn, lg.t[3], c);
 }
 
 // ---------------------------------------------------------------------------
@@ -1365,7 +1513,7 @@ void launch_velocity_boxes(const Geo& g, const StokesArgs<T>& a, const BoxList& 
 // sweep; the first entry is the default.  64-lane tiles sweep 62 columns at
 // any alignment; 66-lane tiles sweep 64 line-aligned columns.
 #define FI_FUSED_CFG_D(X) X(64, 8, 1) X(64, 8, 4) X(64, 4, 1) X(64, 4, 3) X(66, 7, 1) X(66, 8, 3)
-#define FI_FUSED_CFG_F(X) X(64, 16, 1) X(64, 8, 1) X(66, 15, 1) X(66, 15, 4) X(66, 7, 1)
+#define FI_FUSED_CFG_F(X) X(64, 8, 1) X(64, 16, 1) X(66, 15, 1) X(66, 15, 4) X(66, 7, 1)
 
 int fused_configs(bool fp32, int cfg[][3], int max)
 {
@@ -1413,7 +1561,12 @@ void launch_fused(const Geo& g, const StokesArgs<T>& a, const FusedGeom& fg0, bo
     FusedGeom fg = fg0;
     fused_xtiles(fg, tx, (int)sizeof(T), g.off);
     const int nby = (fg.hi[1] - fg.lo[1] + 1 + ty - 3) / (ty - 2);
-    if (fg.kch <= 0) fg.kch = fused_auto_kch(fg, fg.nbx * nby);
+    if (fg.kch <= 0) {
+        fg.kch = fused_auto_kch(fg, fg.nbx * nby);
+        // FP32 immersed-boundary sweeps skip empty space, so the work per
+        // chunk is uneven: shorter chunks balance it better
+        if (ib && sizeof(T) == 4) fg.kch = std::min(fg.kch, 32);
+    }
     const int nbz = (fg.hi[2] - fg.lo[2] + fg.kch) / fg.kch;
     const int ntiles = fg.nbx * nby * nbz;
     const int nfused = (ntiles + 8 * NXCD - 1) / (8 * NXCD) * (8 * NXCD);

@@ -160,6 +160,12 @@ int fi_time_iterations(fi_model_t m, int n_iter, double out_ms[4]);
 /* The interior box [lo, hi] (1-based Chmy indices) swept by the fused kernel;
  * returns 1 when the fused sweep is active, 0 otherwise. */
 int fi_fused_region(fi_model_t m, int lo[3], int hi[3]);
+/* Measurement hook (no counterpart in the reference): out[0] = cells of the
+ * fused interior, out[1] = those the immersed-boundary sweep processes, i.e.
+ * on planes of their column that hold a valid location (the sweep reads and
+ * writes nothing on the other planes but the viscosity it updates; out[1] =
+ * out[0] without an immersed boundary, both 0 without a fused interior). */
+int fi_fused_active_cells(fi_model_t m, long long out[2]);
 
 /* Options (no counterpart in the reference).  Kernel path, results do not
  * depend on them: FI_OPT_FUSED 1/0 selects the fused interior sweep or the

@@ -343,3 +343,70 @@ def test_shell_in_fused_launch_bitwise(pkg):
         m.close()
     for n in out[0]:
         assert np.array_equal(out[0][n], out[1][n]), n
+
+
+def test_ib_sweep_skips_empty_space_bitwise(pkg, orc):
+    """The immersed-boundary sweep reads and writes nothing on the planes of a
+    column that hold no valid location, reads no old stress at an invalid
+    location, and skips velocity reads and stores of invalid locations only
+    once the buffers are known to hold +0 there.  From a random state that is
+    non-zero at invalid locations as well, the first iteration (which reads
+    and writes everything), the second (which may skip reads) and the later
+    ones (which skip both) equal the pointwise sweep bit for bit and the
+    oracle; an upload of one field starts the sequence again; and a refined
+    solve (FP32 sweeps after the full FP32 phase and after zero fills and
+    stress lags) gives the same FP64 state with and without the fused sweep."""
+    _say("case: empty-space skipping, 70x40x20 glacier, random state, fused vs pointwise vs oracle")
+    import ctypes as C
+    rng = np.random.default_rng(21)
+    ms = []
+    for fused in (1, 0):
+        m, _ = pkg.glacier_3d((70, 40, 20), seed=6, glen=True)
+        m.set_option("fused", fused)
+        m.set_option("zchunk", 7)
+        ms.append(m)
+    a, b = ms
+    ref = orc.OracleModel(a.desc)
+    for ph, base in (("NA", pkg._abi.FI_F_OM_NA0), ("NS", pkg._abi.FI_F_OM_NS0)):
+        for q in range(8):
+            ref.set("OM_%s%d" % (ph, q), pkg.full_stokes.DeviceField(a, base + q).parent())
+    shape = a.grid.parent_shape()
+    for n in state_names(3) + ["ETA", "ETA_NEXT"]:
+        x = np.asfortranarray((1.0 + 0.1 * rng.random(shape)) if n.startswith("ETA") else 1e-2 * rng.standard_normal(shape))
+        for m in (a, b):
+            m.field(n).set(x)
+        ref.set(n, x)
+    cells = (C.c_longlong * 2)()
+    pkg._abi.check(pkg._abi.lib().fi_fused_active_cells(a._h, cells))
+    _say("fused interior %d cells, %d on planes with a valid location" % (cells[0], cells[1]))
+    assert 0 < cells[1] < cells[0]
+
+    def same(tag):
+        for name in state_names(3) + ["ETA"]:
+            fa = a.field(name).parent()
+            assert np.all(np.isfinite(fa)), (tag, name)
+            assert np.array_equal(fa, b.field(name).parent()), (tag, name)
+        worst, bad = compare_state(a, ref, state_names(3) + ["ETA"], TOL)
+        assert not bad, (tag, worst)
+
+    for step, n_it in enumerate((1, 1, 1, 4)):
+        for m in (a, b, ref):
+            m.advance(n_it)
+        same("iteration block %d" % step)
+    v = np.asfortranarray(1e-2 * rng.standard_normal(shape))
+    for m in (a, b):
+        m.field("VX").set(v)
+    ref.set("VX", v)
+    for step in range(3):
+        for m in (a, b, ref):
+            m.advance(1)
+        same("after an upload, iteration %d" % step)
+    oa = a.refined_solve(1e-12, max_outer=3, max_inner=140, ncheck=16)
+    ob = b.refined_solve(1e-12, max_outer=3, max_inner=140, ncheck=16)
+    assert oa[0] == ob[0] and oa[1] == ob[1]
+    for name in state_names(3) + ["ETA"]:
+        fa = a.field(name).parent()
+        assert np.all(np.isfinite(fa)), name
+        assert np.array_equal(fa, b.field(name).parent()), ("refined", name)
+    a.close()
+    b.close()

@@ -44,6 +44,8 @@ def _expected_bytes(dims, nfields_single, immersed):
     b = (nfields_single + 10) * elems * 8 + 32
     if immersed:
         b += elems
+        if len(n) == 3:
+            b += 2 * sy * (n[1] + 3) * 8  # per-column plane ranges (two int2 per column)
     return b
 
 
