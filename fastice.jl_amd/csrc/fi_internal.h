@@ -38,8 +38,20 @@ struct Geo {
 // and within 1/510 elsewhere.
 struct U8Mask {
     const uint8_t* p;
-    __device__ __forceinline__ float operator[](long i) const { return (float)p[i] * (1.0f / 255.0f); }
+    int s; // bytes from one element to the next: 1, or 8 in the packed words below
+    __device__ __forceinline__ float operator[](long i) const { return (float)p[i * s] * (1.0f / 255.0f); }
 };
+// 3D FP32 sweeps with an immersed boundary keep the seven u8 masks they read
+// and the validity byte of a cell in one 8-byte word (one load per cell and
+// plane): bytes 0-3 na at CCC VVC VCV CVV, bytes 4-6 ns at VCC CVC CCV, byte
+// 7 the validity bits
+enum { MW_NA_CCC = 0, MW_NA_VVC, MW_NA_VCV, MW_NA_CVV, MW_NS_VCC, MW_NS_CVC, MW_NS_CCV, MW_FLAGS };
+__device__ __forceinline__ float mw_mask(uint2 w, int byte)
+{
+    const uint32_t d = byte < 4 ? w.x : w.y;
+    return (float)((d >> (8 * (byte & 3))) & 0xffu) * (1.0f / 255.0f);
+}
+__device__ __forceinline__ unsigned mw_flags(uint2 w) { return w.y >> 24; }
 template <typename T>
 struct MaskOf {
     using ptr = const T*;
@@ -67,6 +79,7 @@ struct StokesArgs {
     typename MaskOf<T>::ptr na[8];
     typename MaskOf<T>::ptr ns[8];
     const uint8_t* flags; // immersed-boundary validity bits (F_* below)
+    const uint2* mw;      // 3D FP32 immersed boundary: the packed mask words (MW_*), else null
     // 3D immersed boundary, per column i + sy j: the planes k holding a valid
     // location (crange) and the planes whose viscosity a valid location may
     // read (erange); empty columns have x > y

@@ -46,8 +46,9 @@ struct Refine {
     bool full = false;      // sweeps solve the full state (true) or the correction (false)
     float* fP = nullptr;    // continuity forcing r_P
     float* g[3] = {};       // momentum forcing -r_V
-    uint8_t* wna[8] = {};   // masks of the FP32 sweeps (IB), quantised to q / 255
+    uint8_t* wna[8] = {};   // 2D masks of the FP32 sweeps (IB), quantised to q / 255
     uint8_t* wns[8] = {};
+    uint8_t* wpk = nullptr; // 3D: the packed mask words (MW_*), 8 bytes per element
     float* r[4] = {};       // inner residuals
     std::vector<void*> allocs;
     int sig_cur = 0, v_cur = 0;
@@ -85,6 +86,25 @@ __global__ __launch_bounds__(256) void k_quantise_mask(uint8_t* dst, const doubl
     for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
         const double w = src[t] < 0.0 ? 0.0 : (src[t] > 1.0 ? 1.0 : src[t]);
         dst[t] = (uint8_t)rint(255.0 * w);
+    }
+}
+
+// 3D: the seven masks the FP32 sweeps read, quantised as k_quantise_mask
+// does, and the validity byte, one 8-byte word per element (MW_*)
+struct PackSrc {
+    const double* w[7];
+};
+__global__ __launch_bounds__(256) void k_pack_masks(uint2* dst, PackSrc src, const uint8_t* flags, size_t n)
+{
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+        uint32_t b[8];
+        for (int q = 0; q < 7; ++q) {
+            const double v = src.w[q][t];
+            const double w = v < 0.0 ? 0.0 : (v > 1.0 ? 1.0 : v);
+            b[q] = (uint32_t)(uint8_t)rint(255.0 * w);
+        }
+        b[7] = flags[t];
+        dst[t] = make_uint2(b[0] | b[1] << 8 | b[2] << 16 | b[3] << 24, b[4] | b[5] << 8 | b[6] << 16 | b[7] << 24);
     }
 }
 
@@ -278,9 +298,21 @@ static int ensure_refine(fi_model_s* m)
         // the masks the sweeps read: na at stress locations, ns at velocity locations
         const int na3[4] = {OM_CCC, OM_VVC, OM_VCV, OM_CVV}, ns3[3] = {OM_VCC, OM_CVC, OM_CCV};
         const int na2[2] = {OM2_CC, OM2_VV}, ns2[2] = {OM2_VC, OM2_CV};
-        const int nna = nd == 3 ? 4 : 2, nns = nd == 3 ? 3 : 2;
-        for (int q = 0; q < nna && !rc; ++q) rc = alloc_mask8(m, &r->wna[nd == 3 ? na3[q] : na2[q]]);
-        for (int q = 0; q < nns && !rc; ++q) rc = alloc_mask8(m, &r->wns[nd == 3 ? ns3[q] : ns2[q]]);
+        (void)na3;
+        (void)ns3;
+        if (nd == 3) {
+            uint8_t* base = nullptr;
+            if (hipMalloc(&base, m->g.elems * 8) != hipSuccess) {
+                rc = fail(FI_ERR_OOM, "FP32 mask allocation failed");
+            } else {
+                r->allocs.push_back(base);
+                if (hipMemsetAsync(base, 0, m->g.elems * 8, m->stream) != hipSuccess) rc = fail(FI_ERR_HIP, "hipMemsetAsync failed");
+                r->wpk = base + 8 * m->g.off;
+            }
+        } else {
+            for (int q = 0; q < 2 && !rc; ++q) rc = alloc_mask8(m, &r->wna[na2[q]]);
+            for (int q = 0; q < 2 && !rc; ++q) rc = alloc_mask8(m, &r->wns[ns2[q]]);
+        }
     }
     if (!rc && (hipMalloc(&r->d_norm, 4 * sizeof(unsigned long long)) != hipSuccess ||
                 hipHostMalloc(&r->h_norm, 4 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess))
@@ -290,7 +322,8 @@ static int ensure_refine(fi_model_s* m)
         free_refine(m);
         return rc;
     }
-    if (nd == 3 && m->g.elems * sizeof(float) < (1ull << 32)) r->has_interior = fused_split(m->g, 16, m->kch, r->fg, r->shell);
+    // the fused sweep addresses FP32 fields and the 8-byte mask words with 32-bit byte offsets
+    if (nd == 3 && m->g.elems * 8 < (1ull << 32)) r->has_interior = fused_split(m->g, 16, m->kch, r->fg, r->shell);
     // the allocations are cleared on the model's stream
     m->zs32[0] = m->zs32[1] = m->zv32[0] = m->zv32[1] = true;
     return FI_OK;
@@ -329,8 +362,18 @@ static StokesArgs<float> make_args32(fi_model_s* m, bool in_place)
     }
     a.rP = r->r[0]; a.rVx = r->r[1]; a.rVy = r->r[2]; a.rVz = r->r[3];
     for (int q = 0; q < 8; ++q) {
-        a.na[q] = U8Mask{r->wna[q]};
-        a.ns[q] = U8Mask{r->wns[q]};
+        a.na[q] = U8Mask{r->wna[q], 1};
+        a.ns[q] = U8Mask{r->wns[q], 1};
+    }
+    if (r->wpk) {
+        a.na[OM_CCC] = U8Mask{r->wpk + MW_NA_CCC, 8};
+        a.na[OM_VVC] = U8Mask{r->wpk + MW_NA_VVC, 8};
+        a.na[OM_VCV] = U8Mask{r->wpk + MW_NA_VCV, 8};
+        a.na[OM_CVV] = U8Mask{r->wpk + MW_NA_CVV, 8};
+        a.ns[OM_VCC] = U8Mask{r->wpk + MW_NS_VCC, 8};
+        a.ns[OM_CVC] = U8Mask{r->wpk + MW_NS_CVC, 8};
+        a.ns[OM_CCV] = U8Mask{r->wpk + MW_NS_CCV, 8};
+        a.mw = reinterpret_cast<const uint2*>(r->wpk);
     }
     a.flags = m->flags;
     a.crange = m->crange;
@@ -502,6 +545,27 @@ static void quantise_mask(fi_model_s* m, uint8_t* dst, const double* src)
 {
     hipLaunchKernelGGL(k_quantise_mask, ew_grid(m->g.elems), dim3(256), 0, m->stream, dst - m->g.off, src - m->g.off,
                        m->g.elems);
+}
+
+// the FP32 masks (and, in 3D, the validity bytes packed with them) from the
+// model's FP64 masks, whenever those changed; after prepare()
+static void sync_masks(fi_model_s* m)
+{
+    Refine* r = m->rf;
+    if (!m->desc.immersed_boundary || r->mask_gen == m->mask_gen) return;
+    if (r->wpk) {
+        PackSrc src;
+        const int na[4] = {OM_CCC, OM_VVC, OM_VCV, OM_CVV}, ns[3] = {OM_VCC, OM_CVC, OM_CCV};
+        for (int q = 0; q < 4; ++q) src.w[q] = m->f[FI_F_OM_NA0 + na[q]] - m->g.off;
+        for (int q = 0; q < 3; ++q) src.w[4 + q] = m->f[FI_F_OM_NS0 + ns[q]] - m->g.off;
+        hipLaunchKernelGGL(k_pack_masks, ew_grid(m->g.elems), dim3(256), 0, m->stream,
+                           reinterpret_cast<uint2*>(r->wpk - 8 * m->g.off), src, m->flags - m->g.off, m->g.elems);
+    }
+    for (int q = 0; q < 8; ++q) {
+        if (r->wna[q]) quantise_mask(m, r->wna[q], m->f[FI_F_OM_NA0 + q]);
+        if (r->wns[q]) quantise_mask(m, r->wns[q], m->f[FI_F_OM_NS0 + q]);
+    }
+    r->mask_gen = m->mask_gen;
 }
 
 static void widen(fi_model_s* m, double* dst, const float* src)
@@ -874,14 +938,7 @@ int job_begin(fi_model_s* m, RefineJob& j, int max_outer, int max_inner, int nch
     if (rc) return rc;
     rc = prepare(m, false);
     if (rc) return rc;
-    Refine* r = m->rf;
-    if (m->desc.immersed_boundary && r->mask_gen != m->mask_gen) {
-        for (int q = 0; q < 8; ++q) {
-            if (r->wna[q]) quantise_mask(m, r->wna[q], m->f[FI_F_OM_NA0 + q]);
-            if (r->wns[q]) quantise_mask(m, r->wns[q], m->f[FI_F_OM_NS0 + q]);
-        }
-        r->mask_gen = m->mask_gen;
-    }
+    sync_masks(m);
     j = RefineJob();
     j.max_outer = max_outer;
     j.max_inner = max_inner;
@@ -1049,6 +1106,7 @@ int fi_refined_time_inner(fi_model_t m, int n_iter, double out_ms[4])
     if (rc) return rc;
     rc = prepare(m, false);
     if (rc) return rc;
+    sync_masks(m);
     hipEvent_t ev[6];
     for (int q = 0; q < 6; ++q) HIPCHK(hipEventCreate(&ev[q]));
     double acc[4] = {0, 0, 0, 0};
@@ -1074,6 +1132,7 @@ int fi_refined_advance_inner(fi_model_t m, int n_iter)
     if (rc) return rc;
     rc = prepare(m, false);
     if (rc) return rc;
+    sync_masks(m);
     for (int it = 0; it < n_iter; ++it) iterate32(m, nullptr);
     HIPCHK(hipGetLastError());
     return FI_OK;

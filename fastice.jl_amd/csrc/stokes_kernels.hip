@@ -129,9 +129,13 @@ __device__ __forceinline__ T rheology_o(const StokesArgs<T>& a, uint32_t bo, T t
 // 3D pointwise sweeps (used on the boundary shell and for small grids)
 // ---------------------------------------------------------------------------
 // update_sigma!: FullStokes kernels_3d.jl:130-152, IB kernels_3d.jl:141-182
+// With an immersed boundary, a point whose stress locations are all invalid
+// is left alone once the output buffers are known to hold 0 there (z_wsig,
+// z_wvel: see fused3_ib_tile); so are the zero stores of invalid locations.
 template <typename T, bool IB>
 __device__ __forceinline__ void sigma_point3(const StokesArgs<T>& a, long c, long sy, long sz)
 {
+    if (IB && a.z_wsig && !(a.flags[c] & (F_CCC | F_VVC | F_VCV | F_CVV))) return;
     const T ix = a.ix, iy = a.iy, iz = a.iz;
     const T *Vx = a.Vx, *Vy = a.Vy, *Vz = a.Vz, *eta = a.eta;
     const T ec = eta[c];
@@ -165,21 +169,21 @@ __device__ __forceinline__ void sigma_point3(const StokesArgs<T>& a, long c, lon
             a.txxn[c] = upd_td(a.txx[c], ec, exx, divV, a.dtau[0]);
             a.tyyn[c] = upd_td(a.tyy[c], ec, eyy, divV, a.dtau[1]);
             a.tzzn[c] = upd_td(a.tzz[c], ec, ezz, divV, a.dtau[2]);
-        } else {
+        } else if (!a.z_wsig) {
             a.Pn[c] = T(0); a.txxn[c] = T(0); a.tyyn[c] = T(0); a.tzzn[c] = T(0);
         }
         if (fl & F_VVC) {
             T exy = T(0.5) * (dif(wy[c] * Vy[c], wy[c - 1] * Vy[c - 1], ix) + dif(wx[c] * Vx[c], wx[c - sy] * Vx[c - sy], iy));
             a.txyn[c] = upd_to(a.txy[c], exy_eta, exy, a.dtau[3]);
-        } else a.txyn[c] = T(0);
+        } else if (!a.z_wsig) a.txyn[c] = T(0);
         if (fl & F_VCV) {
             T exz = T(0.5) * (dif(wz[c] * Vz[c], wz[c - 1] * Vz[c - 1], ix) + dif(wx[c] * Vx[c], wx[c - sz] * Vx[c - sz], iz));
             a.txzn[c] = upd_to(a.txz[c], exz_eta, exz, a.dtau[4]);
-        } else a.txzn[c] = T(0);
+        } else if (!a.z_wsig) a.txzn[c] = T(0);
         if (fl & F_CVV) {
             T eyz = T(0.5) * (dif(wz[c] * Vz[c], wz[c - sy] * Vz[c - sy], iy) + dif(wy[c] * Vy[c], wy[c - sz] * Vy[c - sz], iz));
             a.tyzn[c] = upd_to(a.tyz[c], eyz_eta, eyz, a.dtau[5]);
-        } else a.tyzn[c] = T(0);
+        } else if (!a.z_wsig) a.tyzn[c] = T(0);
     }
 }
 
@@ -187,6 +191,7 @@ __device__ __forceinline__ void sigma_point3(const StokesArgs<T>& a, long c, lon
 template <typename T, bool IB>
 __device__ __forceinline__ void velocity_point3(const StokesArgs<T>& a, long c, long sy, long sz)
 {
+    if (IB && a.z_wvel && a.frozen_eta && !(a.flags[c] & (F_VCC | F_CVC | F_CCV))) return;
     const T ix = a.ix, iy = a.iy, iz = a.iz;
     const T *P = a.Pn, *txx = a.txxn, *tyy = a.tyyn, *tzz = a.tzzn, *txy = a.txyn, *txz = a.txzn, *tyz = a.tyzn;
     const T* eta = a.eta;
@@ -206,21 +211,24 @@ __device__ __forceinline__ void velocity_point3(const StokesArgs<T>& a, long c, 
     } else {
         const auto wc = a.na[OM_CCC], wxy = a.na[OM_VVC], wxz = a.na[OM_VCV], wyz = a.na[OM_CVV];
         const unsigned fl = a.flags[c];
-        a.Vxn[c] = (fl & F_VCC)
-                       ? upd_V_ib(a.Vx[c], dif(wc[c] * P[c], wc[c - 1] * P[c - 1], ix), dif(wc[c] * txx[c], wc[c - 1] * txx[c - 1], ix),
-                                  dif(wxy[c + sy] * txy[c + sy], wxy[c] * txy[c], iy), dif(wxz[c + sz] * txz[c + sz], wxz[c] * txz[c], iz),
-                                  gx, ex, a.ns[OM_VCC][c], a.dV[0])
-                       : T(0);
-        a.Vyn[c] = (fl & F_CVC)
-                       ? upd_V_ib(a.Vy[c], dif(wc[c] * P[c], wc[c - sy] * P[c - sy], iy), dif(wc[c] * tyy[c], wc[c - sy] * tyy[c - sy], iy),
-                                  dif(wxy[c + 1] * txy[c + 1], wxy[c] * txy[c], ix), dif(wyz[c + sz] * tyz[c + sz], wyz[c] * tyz[c], iz),
-                                  gy, ey, a.ns[OM_CVC][c], a.dV[1])
-                       : T(0);
-        a.Vzn[c] = (fl & F_CCV)
-                       ? upd_V_ib(a.Vz[c], dif(wc[c] * P[c], wc[c - sz] * P[c - sz], iz), dif(wc[c] * tzz[c], wc[c - sz] * tzz[c - sz], iz),
-                                  dif(wxz[c + 1] * txz[c + 1], wxz[c] * txz[c], ix), dif(wyz[c + sy] * tyz[c + sy], wyz[c] * tyz[c], iy),
-                                  gz, ez, a.ns[OM_CCV][c], a.dV[2])
-                       : T(0);
+        if (fl & F_VCC)
+            a.Vxn[c] = upd_V_ib(a.Vx[c], dif(wc[c] * P[c], wc[c - 1] * P[c - 1], ix), dif(wc[c] * txx[c], wc[c - 1] * txx[c - 1], ix),
+                                dif(wxy[c + sy] * txy[c + sy], wxy[c] * txy[c], iy), dif(wxz[c + sz] * txz[c + sz], wxz[c] * txz[c], iz),
+                                gx, ex, a.ns[OM_VCC][c], a.dV[0]);
+        else if (!a.z_wvel)
+            a.Vxn[c] = T(0);
+        if (fl & F_CVC)
+            a.Vyn[c] = upd_V_ib(a.Vy[c], dif(wc[c] * P[c], wc[c - sy] * P[c - sy], iy), dif(wc[c] * tyy[c], wc[c - sy] * tyy[c - sy], iy),
+                                dif(wxy[c + 1] * txy[c + 1], wxy[c] * txy[c], ix), dif(wyz[c + sz] * tyz[c + sz], wyz[c] * tyz[c], iz),
+                                gy, ey, a.ns[OM_CVC][c], a.dV[1]);
+        else if (!a.z_wvel)
+            a.Vyn[c] = T(0);
+        if (fl & F_CCV)
+            a.Vzn[c] = upd_V_ib(a.Vz[c], dif(wc[c] * P[c], wc[c - sz] * P[c - sz], iz), dif(wc[c] * tzz[c], wc[c - sz] * tzz[c - sz], iz),
+                                dif(wxz[c + 1] * txz[c + 1], wxz[c] * txz[c], ix), dif(wyz[c + sy] * tyz[c + sy], wyz[c] * tyz[c], iy),
+                                gz, ez, a.ns[OM_CCV][c], a.dV[2]);
+        else if (!a.z_wvel)
+            a.Vzn[c] = T(0);
     }
     if (a.frozen_eta) return;
     const T t = tauII(txx[c], tyy[c], tzz[c], lerp4(txy[c], txy[c + 1], txy[c + sy], txy[c + 1 + sy]),
@@ -287,6 +295,10 @@ __device__ __forceinline__ T ldo(const T* base, uint32_t bo)
 }
 // a mask read at the byte offset of the FP32 fields it weights
 __device__ __forceinline__ float ldo(U8Mask base, uint32_t bo) { return base[bo / sizeof(float)]; }
+__device__ __forceinline__ uint2 ldo(const uint2* base, uint32_t bo)
+{
+    return *reinterpret_cast<const uint2*>(reinterpret_cast<const char*>(base) + bo);
+}
 template <typename T>
 __device__ __forceinline__ void sto(T* base, uint32_t bo, T v)
 {
@@ -567,9 +579,16 @@ __device__ __forceinline__ auto ldif(bool p, P base, uint32_t bo) -> decltype(ld
     return p ? ldo(base, bo) : R(0);
 }
 
+__device__ __forceinline__ uint2 ldw(bool p, const uint2* base, uint32_t bo)
+{
+    return p ? *reinterpret_cast<const uint2*>(reinterpret_cast<const char*>(base) + bo) : make_uint2(0u, 0u);
+}
+
 template <typename T, int TX, int TY>
 __device__ __forceinline__ void fused3_ib_tile(const StokesArgs<T>& a, const FusedGeom& fg, int bx, int by, int bz)
 {
+    // FP32 sweeps read masks and flags from the packed words (a.mw)
+    constexpr bool PACKED = sizeof(T) == 4;
     __shared__ T sVx[TY][TX], sVy[TY][TX], sVz[TY][TX];
     __shared__ T sE[3][TY][TX];
     __shared__ T sS[2][9][TY][TX]; // na-weighted P txx tyy txy txz tyz; raw txy txz tyz
@@ -628,15 +647,17 @@ __device__ __forceinline__ void fused3_ib_tile(const StokesArgs<T>& a, const Fus
     const int ks = narrow ? max(ks0, s_lo - 1) : ks0;
     const int ke = narrow ? min(ke0, s_hi) : ke0;
     if (need_raw && inner) {
-        // viscosity of the planes outside [ks, ke]: four planes per pass
+        // viscosity of the planes outside [ks, ke]: EU planes per pass (more
+        // loads in flight made the register allocation of the whole kernel worse)
+        constexpr int EU = 4;
         auto eta_only = [&](int k0, int k1) {
-            for (int k = k0; k <= k1; k += 4) {
-                T o[4];
+            for (int k = k0; k <= k1; k += EU) {
+                T o[EU];
 #pragma unroll
-                for (int u = 0; u < 4; ++u)
+                for (int u = 0; u < EU; ++u)
                     o[u] = k + u <= k1 ? ldo(a.eta, col + bsz * (uint32_t)(k + u)) : T(1);
 #pragma unroll
-                for (int u = 0; u < 4; ++u)
+                for (int u = 0; u < EU; ++u)
                     if (k + u <= k1) {
                         const uint32_t cu = col + bsz * (uint32_t)(k + u);
                         sto(a.eta_next, cu, rheology_o(a, cu, T(0), o[u]));
@@ -655,17 +676,36 @@ __device__ __forceinline__ void fused3_ib_tile(const StokesArgs<T>& a, const Fus
     const uint32_t c2 = col + bsz * (uint32_t)(ks - 2), c1 = c2 + bsz, c0 = c1 + bsz, c3 = c2 - bsz;
     // raw velocity and ns weights of the own column
     T vxm = ldif(inv(ks - 2), a.Vx, c2), vym = ldif(inv(ks - 2), a.Vy, c2), vzm = ldif(inv(ks - 2), a.Vz, c2);
-    T mxm = ldif(inv(ks - 2), wvx, c2), mym = ldif(inv(ks - 2), wvy, c2), mzm = ldif(inv(ks - 2), wvz, c2);
-    unsigned flm = ldif(inr(ks - 2), a.flags, c2 / E);
-    T qxm = mxm * vxm, qym = mym * vym;
     T vx0 = ldif(inv(ks - 1), a.Vx, c1), vy0 = ldif(inv(ks - 1), a.Vy, c1), vz0 = ldif(inv(ks - 1), a.Vz, c1);
-    T mx0 = ldif(inv(ks - 1), wvx, c1), my0 = ldif(inv(ks - 1), wvy, c1), mz0 = ldif(inv(ks - 1), wvz, c1);
+    T vzp = ldif(inv(ks), a.Vz, c0);
+    T mxm, mym, mzm, mx0, my0, mz0, mzp, mc0, mxy0, mxz0, myz0;
+    unsigned flm, fl0;
+    // FP32: the mask word of the plane after the current one (read a plane ahead)
+    uint2 wn1 = make_uint2(0u, 0u);
+    if constexpr (PACKED) {
+        // a word read where the velocity is (a superset of crange): outside
+        // crange its flags are 0 and its na weights multiply 0, as above
+        const uint2 w2 = ldw(inv(ks - 2), a.mw, 2 * c2), w1 = ldw(inv(ks - 1), a.mw, 2 * c1);
+        wn1 = ldw(inv(ks), a.mw, 2 * c0);
+        mxm = mw_mask(w2, MW_NS_VCC), mym = mw_mask(w2, MW_NS_CVC), mzm = mw_mask(w2, MW_NS_CCV);
+        flm = mw_flags(w2);
+        mx0 = mw_mask(w1, MW_NS_VCC), my0 = mw_mask(w1, MW_NS_CVC), mz0 = mw_mask(w1, MW_NS_CCV);
+        mzp = mw_mask(wn1, MW_NS_CCV);
+        mc0 = mw_mask(w1, MW_NA_CCC), mxy0 = mw_mask(w1, MW_NA_VVC), mxz0 = mw_mask(w1, MW_NA_VCV),
+        myz0 = mw_mask(w1, MW_NA_CVV);
+        fl0 = mw_flags(w1);
+    } else {
+        mxm = ldif(inv(ks - 2), wvx, c2), mym = ldif(inv(ks - 2), wvy, c2), mzm = ldif(inv(ks - 2), wvz, c2);
+        flm = ldif(inr(ks - 2), a.flags, c2 / E);
+        mx0 = ldif(inv(ks - 1), wvx, c1), my0 = ldif(inv(ks - 1), wvy, c1), mz0 = ldif(inv(ks - 1), wvz, c1);
+        mzp = ldif(inv(ks), wvz, c0);
+        mc0 = ldif(inr(ks - 1), wc, c1), mxy0 = ldif(inr(ks - 1), wxy, c1), mxz0 = ldif(inr(ks - 1), wxz, c1),
+        myz0 = ldif(inr(ks - 1), wyz, c1);
+        fl0 = ldif(inr(ks - 1), a.flags, c1 / E);
+    }
+    T qxm = mxm * vxm, qym = mym * vym;
     T qx0 = mx0 * vx0, qy0 = my0 * vy0, qz0 = mz0 * vz0;
-    T vzp = ldif(inv(ks), a.Vz, c0), mzp = ldif(inv(ks), wvz, c0);
     T qzp = mzp * vzp;
-    T mc0 = ldif(inr(ks - 1), wc, c1), mxy0 = ldif(inr(ks - 1), wxy, c1), mxz0 = ldif(inr(ks - 1), wxz, c1),
-      myz0 = ldif(inr(ks - 1), wyz, c1);
-    unsigned fl0 = ldif(inr(ks - 1), a.flags, c1 / E);
     T emm = ldif(ine(ks - 3), a.eta, c3), em = ldif(ine(ks - 2), a.eta, c2), e0 = ldif(ine(ks - 1), a.eta, c1);
     // own new stress: weighted at k and k-1, raw diagonal at k
     T Pw = T(0), Pwm = T(0), xxw = T(0), yyw = T(0), zzw = T(0), zzwm = T(0), xxr = T(0), yyr = T(0), zzr = T(0);
@@ -691,18 +731,44 @@ __device__ __forceinline__ void fused3_ib_tile(const StokesArgs<T>& a, const Fus
         unsigned nfl = 0;
         if (kk <= ke) {
             const uint32_t cn = c + bsz;
-            const bool v1 = inv(kk + 1), v2 = inv(kk + 2), r1 = inr(kk + 1);
-            nvx = ldif(v1, a.Vx, cn);
-            nvy = ldif(v1, a.Vy, cn);
-            nvz2 = ldif(v2, a.Vz, cn + bsz);
-            nmx = ldif(v1, wvx, cn);
-            nmy = ldif(v1, wvy, cn);
-            nmz2 = ldif(v2, wvz, cn + bsz);
-            nmc = ldif(r1, wc, cn);
-            nmxy = ldif(r1, wxy, cn);
-            nmxz = ldif(r1, wxz, cn);
-            nmyz = ldif(r1, wyz, cn);
-            nfl = ldif(r1, a.flags, cn / E);
+            const bool v1 = inv(kk + 1), v2 = inv(kk + 2);
+            if (v1) {
+                nvx = ldo(a.Vx, cn);
+                nvy = ldo(a.Vy, cn);
+            }
+            if constexpr (PACKED) {
+                uint2 wn2 = make_uint2(0u, 0u);
+                if (v2) {
+                    nvz2 = ldo(a.Vz, cn + bsz);
+                    wn2 = ldo(a.mw, 2 * (cn + bsz));
+                }
+                nmx = mw_mask(wn1, MW_NS_VCC);
+                nmy = mw_mask(wn1, MW_NS_CVC);
+                nmc = mw_mask(wn1, MW_NA_CCC);
+                nmxy = mw_mask(wn1, MW_NA_VVC);
+                nmxz = mw_mask(wn1, MW_NA_VCV);
+                nmyz = mw_mask(wn1, MW_NA_CVV);
+                nfl = mw_flags(wn1);
+                nmz2 = mw_mask(wn2, MW_NS_CCV);
+                wn1 = wn2;
+            } else {
+                const bool r1 = inr(kk + 1);
+                if (v1) {
+                    nmx = ldo(wvx, cn);
+                    nmy = ldo(wvy, cn);
+                }
+                if (v2) {
+                    nvz2 = ldo(a.Vz, cn + bsz);
+                    nmz2 = ldo(wvz, cn + bsz);
+                }
+                if (r1) {
+                    nmc = ldo(wc, cn);
+                    nmxy = ldo(wxy, cn);
+                    nmxz = ldo(wxz, cn);
+                    nmyz = ldo(wyz, cn);
+                    nfl = ldo(a.flags, cn / E);
+                }
+            }
             ne = ldif(ine(kk + 1), a.eta, cn);
         }
         __syncthreads();
