@@ -115,6 +115,7 @@ StokesArgs<double> make_args(fi_model_s* m, bool in_place)
     a.flags = m->flags;
     a.crange = m->crange;
     a.erange = m->erange;
+    a.tmarch = m->tmarch;
     a.nx = m->g.n[0];
     a.ny = m->g.n[1];
     a.nz = m->g.n[2];
@@ -330,6 +331,7 @@ void free_model(fi_model_s* m)
     if (m->flags_base) (void)hipFree(m->flags_base);
     if (m->crange) (void)hipFree(m->crange);
     if (m->erange) (void)hipFree(m->erange);
+    if (m->tmarch) (void)hipFree(m->tmarch);
     if (m->d_norm) (void)hipFree(m->d_norm);
     if (m->h_norm) (void)hipHostFree(m->h_norm);
     if (m->d_dem) (void)hipFree(m->d_dem);
@@ -452,7 +454,8 @@ size_t fi_model_bytes(const fi_model_desc* desc)
         if (field_used(*desc, id)) fields += (id >= FI_F_P && id <= FI_F_VZ) ? 2 : 1;
     size_t bytes = fields * g.elems * sizeof(double) + 4 * sizeof(unsigned long long);
     if (desc->immersed_boundary) bytes += g.elems;
-    if (desc->immersed_boundary && g.nd == 3) bytes += 2 * col_ranges_count(g) * sizeof(int2);
+    if (desc->immersed_boundary && g.nd == 3)
+        bytes += (2 * col_ranges_count(g) + tile_march_count(g)) * sizeof(int2);
     return bytes;
 }
 
@@ -506,7 +509,8 @@ int fi_model_create(const fi_model_desc* desc, fi_model_t* out)
         if (g.nd == 3) {
             const size_t nc = col_ranges_count(g);
             if (hipMalloc(&m->crange, nc * sizeof(int2)) != hipSuccess ||
-                hipMalloc(&m->erange, nc * sizeof(int2)) != hipSuccess) {
+                hipMalloc(&m->erange, nc * sizeof(int2)) != hipSuccess ||
+                hipMalloc(&m->tmarch, tile_march_count(g) * sizeof(int2)) != hipSuccess) {
                 free_model(m);
                 return fail(FI_ERR_OOM, "device allocation of the column ranges failed");
             }

@@ -35,7 +35,8 @@ struct Geo {
 // FP32 sweeps (the corrections of the refined solver, whose FP64 residuals
 // use the FP64 masks) read them quantised to q / 255, q in 0..255, which is
 // exact for empty and full locations (255 * (1/255) rounds to 1 in FP32)
-// and within 1/510 elsewhere.
+// and within 1/510 elsewhere, except that a fraction in (0, 1/510) becomes
+// 1/255 rather than 0 (the velocity update divides by it).
 struct U8Mask {
     const uint8_t* p;
     int s; // bytes from one element to the next: 1, or 8 in the packed words below
@@ -85,6 +86,9 @@ struct StokesArgs {
     // read (erange); empty columns have x > y
     const int2* crange;
     const int2* erange;
+    // FP64 immersed-boundary sweeps: one entry per tile of the launch, the
+    // z range each tile marched (see k_eta_rest); null where not used
+    int2* tmarch;
     // 1: the current velocity is +0 at every invalid location of the fused
     // sweep's footprint, so the sweep may take it as 0 without reading it
     int z_read;
@@ -181,6 +185,12 @@ void launch_ib_flags(const Geo& g, const double* const na[8], const double* cons
 // arrays of sy * (n[1] + 3) entries; own and eta must not overlap
 void launch_col_ranges(const Geo& g, const uint8_t* flags, int2* own, int2* eta, hipStream_t s);
 inline size_t col_ranges_count(const Geo& g) { return (size_t)g.sy * (size_t)(g.n[1] + 3); }
+// tiles of any fused launch over g (64- or 66-lane tiles, at least 2 rows
+// and 1 plane per tile): the capacity of StokesArgs::tmarch
+inline size_t tile_march_count(const Geo& g)
+{
+    return (size_t)((g.n[0] + 61) / 62 + 2) * (size_t)((g.n[1] + 1) / 2 + 1) * (size_t)(g.n[2] + 1);
+}
 template <typename T>
 void launch_residuals(const Geo& g, const StokesArgs<T>& a, bool ib, hipStream_t s);
 // FP32 outputs of the stress lag (old - new) of a compute_tau (xx yy zz xy xz yz)

@@ -39,6 +39,7 @@ struct fi_model_s {
     // 3D immersed boundary: per-column plane ranges of the flags (launch_col_ranges)
     int2* crange = nullptr;
     int2* erange = nullptr;
+    int2* tmarch = nullptr; // the z range each tile of the last FP64 fused sweep marched
     // zs[b] / zv[b]: physical buffer b (0: f, 1: sf) of the stress / velocity
     // is known to hold +0 at every invalid location of the fused footprint
     // (a sweep's output is; an upload, a state change outside the sweeps or

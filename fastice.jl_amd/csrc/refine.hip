@@ -81,11 +81,20 @@ __global__ __launch_bounds__(256) void k_cast(float* dst, const double* src, dou
 }
 
 // volume fraction in [0, 1] -> q = round(255 w)
+// q = rint(255 w) of the fraction clamped to [0, 1], and at least 1 for any
+// w > 0: a location that is not empty keeps a non-zero weight (the velocity
+// update divides by it)
+__device__ __forceinline__ uint8_t quantise_u8(double v)
+{
+    const double w = v < 0.0 ? 0.0 : (v > 1.0 ? 1.0 : v);
+    const double q = rint(255.0 * w);
+    return (uint8_t)(w > 0.0 && q < 1.0 ? 1.0 : q);
+}
+
 __global__ __launch_bounds__(256) void k_quantise_mask(uint8_t* dst, const double* src, size_t n)
 {
     for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
-        const double w = src[t] < 0.0 ? 0.0 : (src[t] > 1.0 ? 1.0 : src[t]);
-        dst[t] = (uint8_t)rint(255.0 * w);
+        dst[t] = quantise_u8(src[t]);
     }
 }
 
@@ -99,9 +108,7 @@ __global__ __launch_bounds__(256) void k_pack_masks(uint2* dst, PackSrc src, con
     for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
         uint32_t b[8];
         for (int q = 0; q < 7; ++q) {
-            const double v = src.w[q][t];
-            const double w = v < 0.0 ? 0.0 : (v > 1.0 ? 1.0 : v);
-            b[q] = (uint32_t)(uint8_t)rint(255.0 * w);
+            b[q] = quantise_u8(src.w[q][t]);
         }
         b[7] = flags[t];
         dst[t] = make_uint2(b[0] | b[1] << 8 | b[2] << 16 | b[3] << 24, b[4] | b[5] << 8 | b[6] << 16 | b[7] << 24);

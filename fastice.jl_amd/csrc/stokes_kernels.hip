@@ -65,6 +65,29 @@ __device__ __forceinline__ T upd_V_ib(T v, T dP, T da, T db, T dc, T g, T ev, T 
 {
     return v + (-dP + da + db + dc - g) / ev / w * dv;
 }
+// The FP32 sweeps (corrections of the refined solver, whose accuracy the FP64
+// outer steps set) divide with the hardware reciprocal (v_rcp_f32, within 1
+// ulp) instead of the correctly rounded division sequence: the three
+// divisions by 3 become products with 1/3 and the velocity update one
+// reciprocal of eta_V * omega_ns.  The FP64 iteration keeps the reference's
+// correctly rounded divisions.
+template <>
+__device__ __forceinline__ float upd_td<float>(float t, float eta, float e, float divV, float dt)
+{
+    return t - (t - 2.0f * eta * (e - divV * (1.0f / 3.0f))) * dt;
+}
+template <>
+__device__ __forceinline__ float upd_V<float>(float v, float dP, float da, float db, float dc, float g, float ev, float dv)
+{
+    return v + (-dP + da + db + dc - g) * __builtin_amdgcn_rcpf(ev) * dv;
+}
+template <>
+__device__ __forceinline__ float upd_V_ib<float>(float v, float dP, float da, float db, float dc, float g, float ev,
+                                                 float w, float dv)
+{
+    return v + (-dP + da + db + dc - g) * __builtin_amdgcn_rcpf(ev * w) * dv;
+}
+
 // tau_II (kernels_3d.jl:162-165)
 template <typename T>
 __device__ __forceinline__ T tauII(T xx, T yy, T zz, T lxy, T lxz, T lyz)
@@ -643,33 +666,14 @@ __device__ __forceinline__ void fused3_ib_tile(const StokesArgs<T>& a, const Fus
         atomicMax(&s_hi, rg.y);
     }
     __syncthreads();
-    const bool narrow = !ws && !wv;
+    // with a viscosity update, the planes outside the march are left to
+    // k_eta_rest, which reads the march of every tile from a.tmarch
+    const bool narrow = !ws && !wv && (!need_raw || a.tmarch != nullptr);
     const int ks = narrow ? max(ks0, s_lo - 1) : ks0;
     const int ke = narrow ? min(ke0, s_hi) : ke0;
-    if (need_raw && inner) {
-        // viscosity of the planes outside [ks, ke]: EU planes per pass (more
-        // loads in flight made the register allocation of the whole kernel worse)
-        constexpr int EU = 4;
-        auto eta_only = [&](int k0, int k1) {
-            for (int k = k0; k <= k1; k += EU) {
-                T o[EU];
-#pragma unroll
-                for (int u = 0; u < EU; ++u)
-                    o[u] = k + u <= k1 ? ldo(a.eta, col + bsz * (uint32_t)(k + u)) : T(1);
-#pragma unroll
-                for (int u = 0; u < EU; ++u)
-                    if (k + u <= k1) {
-                        const uint32_t cu = col + bsz * (uint32_t)(k + u);
-                        sto(a.eta_next, cu, rheology_o(a, cu, T(0), o[u]));
-                    }
-            }
-        };
-        if (ks > ke) {
-            eta_only(ks0, ke0);
-        } else {
-            eta_only(ks0, ks - 1);
-            eta_only(ke + 1, ke0);
-        }
+    if (narrow && need_raw && tx == 0 && ty == 0) {
+        const int nby = (fg.hi[1] - fg.lo[1] + 1 + TY - 3) / (TY - 2);
+        a.tmarch[bx + fg.nbx * (by + nby * bz)] = make_int2(ks, ke);
     }
     if (ks > ke) return;
 
@@ -908,6 +912,27 @@ __device__ __forceinline__ void fused3_ib_tile(const StokesArgs<T>& a, const Fus
         em = e0;
         e0 = ne;
     }
+}
+
+// The viscosity update of the interior cells a narrowed immersed-boundary
+// sweep did not march: each tile of the launch (lanes tx, rows ty, z chunks
+// fg.kch) recorded its march [ks, ke] in a.tmarch; a cell of its chunk
+// outside it gets eta_next = rheology(tau_II = 0, eta), as the sweep would
+// have given it (every stress its tau_II averages is at an invalid location).
+template <typename T>
+__global__ __launch_bounds__(256) void k_eta_rest(StokesArgs<T> a, FusedGeom fg, int tx, int ty)
+{
+    const int i = fg.lo[0] + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int j = fg.lo[1] + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+    const int k = fg.lo[2] + (int)blockIdx.z;
+    if (i > fg.hi[0] || j > fg.hi[1]) return;
+    const int bx = (fg.nx0 && i < fg.xa) ? 0 : fg.nx0 + (i - fg.xa) / (tx - 2);
+    const int by = (j - fg.lo[1]) / (ty - 2), bz = (k - fg.lo[2]) / fg.kch;
+    const int nby = (fg.hi[1] - fg.lo[1] + 1 + ty - 3) / (ty - 2);
+    const int2 r = a.tmarch[bx + fg.nbx * (by + nby * bz)];
+    if (k >= r.x && k <= r.y) return;
+    const uint32_t bo = (uint32_t)((i + a.sy * j + a.sz * k) * (long)sizeof(T));
+    sto(a.eta_next, bo, rheology_o(a, bo, T(0), ldo(a.eta, bo)));
 }
 
 // Tile order.  Workgroups are dispatched round-robin over the 8 XCDs
@@ -1620,11 +1645,12 @@ int fused_auto_kch(const FusedGeom& fg, int tiles_per_plane)
     return (int)k;
 }
 
+// the fused launch; fg receives the geometry it used (x tiles, z chunk)
 template <typename T>
-void launch_fused(const Geo& g, const StokesArgs<T>& a, const FusedGeom& fg0, bool ib, int tx, int ty, int minw,
-                  hipStream_t s, const BoxList* shell_sigma)
+static void launch_fused_sweep(const Geo& g, const StokesArgs<T>& a, const FusedGeom& fg0, bool ib, int tx, int ty,
+                               int minw, hipStream_t s, const BoxList* shell_sigma, FusedGeom& fg)
 {
-    FusedGeom fg = fg0;
+    fg = fg0;
     fused_xtiles(fg, tx, (int)sizeof(T), g.off);
     const int nby = (fg.hi[1] - fg.lo[1] + 1 + ty - 3) / (ty - 2);
     if (fg.kch <= 0) {
@@ -1658,6 +1684,20 @@ void launch_fused(const Geo& g, const StokesArgs<T>& a, const FusedGeom& fg0, bo
     if constexpr (sizeof(T) == 8) { FI_FUSED_CFG_D(FI_LAUNCH_) }
     else { FI_FUSED_CFG_F(FI_LAUNCH_) }
 #undef FI_LAUNCH_
+}
+
+template <typename T>
+void launch_fused(const Geo& g, const StokesArgs<T>& a, const FusedGeom& fg0, bool ib, int tx, int ty, int minw,
+                  hipStream_t s, const BoxList* shell_sigma)
+{
+    FusedGeom fg;
+    launch_fused_sweep(g, a, fg0, ib, tx, ty, minw, s, shell_sigma, fg);
+    // a narrowed immersed-boundary sweep with a viscosity update leaves the
+    // viscosity of the planes it did not march to k_eta_rest
+    if (ib && !a.frozen_eta && a.z_wsig && a.z_wvel && a.tmarch) {
+        dim3 grd((fg.hi[0] - fg.lo[0] + 64) / 64, (fg.hi[1] - fg.lo[1] + 4) / 4, fg.hi[2] - fg.lo[2] + 1);
+        hipLaunchKernelGGL((k_eta_rest<T>), grd, dim3(64, 4, 1), 0, s, a, fg, tx, ty);
+    }
 }
 
 template <typename T>
