@@ -73,6 +73,7 @@ struct fi_batch_s {
     int rebuilds = 0;
     hipEvent_t ev[2] = {nullptr, nullptr};
     std::vector<long> sig; // refined batches: what the current tables were built from
+    std::vector<unsigned> zbuilt; // FP64 batches: the running models' clean-buffer bits the tables hold
 };
 
 namespace fi {
@@ -189,6 +190,18 @@ struct SweepOf<double> {
         return Parity{p.sig ^ (n & 1), p.v ^ (n & 1), p.eta ^ (n & 1)};
     }
     static StokesArgs<double> args(fi_model_s* m) { return make_args(m, false); }
+    static void zflags(fi_model_s* m, Parity p, StokesArgs<double>& a)
+    {
+        a.z_read = m->zv[p.v];
+        a.z_wsig = m->zs[1 - p.sig];
+        a.z_wvel = m->zv[1 - p.v];
+    }
+    static unsigned zbits(fi_model_s* m) { return m->zs[0] | m->zs[1] << 1 | m->zv[0] << 2 | m->zv[1] << 3; }
+    static void swept(fi_model_s* m, Parity p0, int n)
+    {
+        if (n >= 1) m->zs[1 - p0.sig] = m->zv[1 - p0.v] = true;
+        if (n >= 2) m->zs[p0.sig] = m->zv[p0.v] = true;
+    }
     static BcBatch bc(fi_model_s* m, int group, int d)
     {
         return group == 0 ? stress_batch(m, d) : velocity_batch(m, d, true);
@@ -209,6 +222,18 @@ struct SweepOf<float> {
         return Parity{p.sig ^ (n & 1), p.v ^ (n & 1), p.eta ^ (refine_full(m) ? (n & 1) : 0)};
     }
     static StokesArgs<float> args(fi_model_s* m) { return refine_args(m, false); }
+    static void zflags(fi_model_s* m, Parity p, StokesArgs<float>& a)
+    {
+        a.z_read = m->zv32[p.v];
+        a.z_wsig = m->zs32[1 - p.sig];
+        a.z_wvel = m->zv32[1 - p.v];
+    }
+    static unsigned zbits(fi_model_s* m) { return m->zs32[0] | m->zs32[1] << 1 | m->zv32[0] << 2 | m->zv32[1] << 3; }
+    static void swept(fi_model_s* m, Parity p0, int n)
+    {
+        if (n >= 1) m->zs32[1 - p0.sig] = m->zv32[1 - p0.v] = true;
+        if (n >= 2) m->zs32[p0.sig] = m->zv32[p0.v] = true;
+    }
     static BcBatch bc(fi_model_s* m, int group, int d)
     {
         return refine_bc_batch(m, group == 0 ? FI_BCGROUP_STRESS : FI_BCGROUP_VELOCITY, d);
@@ -253,6 +278,14 @@ static int build_tables_t(fi_batch_s* b, int ty, int kch)
             PackedInst<T> pi;
             pi.a = S::args(m);
             pi.fg = fg;
+            // which buffers of this parity hold +0 at every invalid location,
+            // as the model tracks it when the tables are built (the packed
+            // sweeps keep a clean buffer clean, so the flags stay true while
+            // the tables live; a change of them rebuilds the tables); the
+            // viscosity of unmarched planes needs k_eta_rest, which packed
+            // launches do not run, so FP64 sweeps march every plane
+            pi.a.tmarch = nullptr;
+            S::zflags(m, pp, pi.a);
             pinst[p].push_back(pi);
             // stress BCs see the new stress buffers, velocity BCs the new velocity
             pp.sig ^= 1;
@@ -465,6 +498,7 @@ static int autotune(fi_batch_s* b, int budget, int* used)
             // in step with every iteration run here
             for (int r : b->run) {
                 fi_model_s* m = b->inst[r].m;
+                SweepOf<double>::swept(m, SweepOf<double>::get(m), 1);
                 m->sig_cur ^= 1;
                 m->v_cur ^= 1;
                 m->eta_cur ^= 1;
@@ -492,6 +526,13 @@ static int autotune(fi_batch_s* b, int budget, int* used)
     return capture_graphs(b);
 }
 
+static std::vector<unsigned> fp64_zbits(fi_batch_s* b)
+{
+    std::vector<unsigned> z;
+    for (int r : b->run) z.push_back(SweepOf<double>::zbits(b->inst[r].m));
+    return z;
+}
+
 static int rebuild(fi_batch_s* b)
 {
     b->run.clear();
@@ -505,6 +546,7 @@ static int rebuild(fi_batch_s* b)
     if (rc) return rc;
     rc = capture_graphs(b);
     if (rc) return rc;
+    b->zbuilt = fp64_zbits(b);
     b->dirty = false;
     return FI_OK;
 }
@@ -541,7 +583,7 @@ static std::vector<long> refined_signature(fi_batch_s* b)
         const RefineParity p = refine_parity(m);
         sg.push_back(r);
         sg.push_back((long)(intptr_t)m);
-        sg.push_back(p.sig | (p.v << 1) | (p.eta << 2) | (refine_full(m) ? 8 : 0));
+        sg.push_back(p.sig | (p.v << 1) | (p.eta << 2) | (refine_full(m) ? 8 : 0) | SweepOf<float>::zbits(m) << 4);
     }
     return sg;
 }
@@ -584,6 +626,7 @@ static int step_refined(fi_batch_s* b, int* n_finished)
     std::vector<int> due;
     for (int r : b->run) {
         BatchInst& bi = b->inst[r];
+        S::swept(bi.m, S::get(bi.m), delta);
         S::set(bi.m, S::flipped(bi.m, S::get(bi.m), delta));
         bi.left -= delta;
         if (bi.left <= 0) due.push_back(r);
@@ -617,7 +660,8 @@ static int step(fi_batch_s* b, int* n_finished)
 {
     *n_finished = 0;
     if (b->refined == 1) return step_refined(b, n_finished);
-    if (b->dirty) {
+    // the tables carry the models' clean-buffer bits: rebuild when they changed
+    if (b->dirty || fp64_zbits(b) != b->zbuilt) {
         int rc = rebuild(b);
         if (rc) return rc;
     }
@@ -652,6 +696,7 @@ static int step(fi_batch_s* b, int* n_finished)
     for (int r : b->run) {
         BatchInst& bi = b->inst[r];
         fi_model_s* m = bi.m;
+        SweepOf<double>::swept(m, SweepOf<double>::get(m), adv);
         m->sig_cur ^= (adv & 1);
         m->v_cur ^= (adv & 1);
         m->eta_cur ^= (adv & 1);
