@@ -186,7 +186,8 @@ void launch_ib_flags(const Geo& g, const double* const na[8], const double* cons
 void launch_col_ranges(const Geo& g, const uint8_t* flags, int2* own, int2* eta, hipStream_t s);
 inline size_t col_ranges_count(const Geo& g) { return (size_t)g.sy * (size_t)(g.n[1] + 3); }
 // tiles of any fused launch over g (64- or 66-lane tiles, at least 2 rows
-// and 1 plane per tile): the capacity of StokesArgs::tmarch
+// and 1 plane per tile): the capacity of StokesArgs::tmarch (whose
+// allocation holds one more 8-byte slot, for k_eta_target's scalar)
 inline size_t tile_march_count(const Geo& g)
 {
     return (size_t)((g.n[0] + 61) / 62 + 2) * (size_t)((g.n[1] + 1) / 2 + 1) * (size_t)(g.n[2] + 1);

@@ -385,6 +385,7 @@ static StokesArgs<float> make_args32(fi_model_s* m, bool in_place)
     a.flags = m->flags;
     a.crange = m->crange;
     a.erange = m->erange;
+    a.tmarch = m->tmarch; // the full FP32 phase updates eta: k_eta_rest covers the unmarched planes
     a.nx = m->g.n[0]; a.ny = m->g.n[1]; a.nz = m->g.n[2];
     a.sy = m->g.sy; a.sz = m->g.sz;
     a.ix = (float)m->g.idx[0]; a.iy = (float)m->g.idx[1]; a.iz = (float)m->g.idx[2];

@@ -127,7 +127,40 @@ __device__ __forceinline__ T rheology(const StokesArgs<T>& a, long c, T tII)
     return eta;
 }
 
-// the same at byte offset bo, with the lagged viscosity at bo already in a register
+// The same at byte offset bo with the lagged viscosity at bo in a register,
+// split into the target viscosity, the target in the relaxation's scale (its
+// log or log10, for a constant-coefficient rheology the same at every cell
+// when tII is) and the relaxation towards it.
+template <typename T>
+__device__ __forceinline__ T rheo_target(const StokesArgs<T>& a, uint32_t bo, T tII)
+{
+    T eta;
+    if (a.rh_kind == FI_RHEO_LINEAR_CONST) eta = a.rh_eta;
+    else if (a.rh_kind == FI_RHEO_LINEAR_FIELD) eta = *reinterpret_cast<const T*>(reinterpret_cast<const char*>(a.rh) + bo);
+    else {
+        T A = a.rh ? *reinterpret_cast<const T*>(reinterpret_cast<const char*>(a.rh) + bo) : a.rh_A;
+        eta = T(0.5) / (A * ipow_(tII, a.rh_n - T(1)));
+    }
+    if (a.rh_has_reg) eta = T(1) / (T(1) / eta + a.rh_inv_eta_reg);
+    return eta;
+}
+template <typename T>
+__device__ __forceinline__ T rheo_scaled(const StokesArgs<T>& a, T eta)
+{
+    if (a.rh_relax == FI_RELAX_LOG) return log(eta);
+    if (a.rh_relax == FI_RELAX_LOG10) return log10(eta);
+    return eta;
+}
+template <typename T>
+__device__ __forceinline__ T rheo_relax(const StokesArgs<T>& a, T st, T old)
+{
+    if (a.rh_relax == FI_RELAX_NONE) return st;
+    const T w = T(1) - a.rh_rate;
+    if (a.rh_relax == FI_RELAX_LIN) return old * w + st * a.rh_rate;
+    if (a.rh_relax == FI_RELAX_LOG) return exp(log(old) * w + st * a.rh_rate);
+    return pow(T(10), log10(old) * w + st * a.rh_rate);
+}
+
 template <typename T>
 __device__ __forceinline__ T rheology_o(const StokesArgs<T>& a, uint32_t bo, T tII, T old)
 {
@@ -344,7 +377,9 @@ __host__ __device__ __forceinline__ void fused_xrange(const FusedGeom& fg, int t
     }
 }
 
-template <typename T, bool IB, int TX, int TY>
+// VISC false: a sweep with frozen viscosity (the FP32 correction sweeps),
+// compiled without the viscosity update and its registers
+template <typename T, bool IB, int TX, int TY, bool VISC>
 __device__ __forceinline__ void fused3_tile(const StokesArgs<T>& a, const FusedGeom& fg, int bx, int by, int bz)
 {
     __shared__ T sVx[TY][TX], sVy[TY][TX], sVz[TY][TX];
@@ -539,7 +574,7 @@ __device__ __forceinline__ void fused3_tile(const StokesArgs<T>& a, const FusedG
                                                       g2, ez, ldo(a.ns[OM_CCV], cv), a.dV[2])
                                            : T(0));
             }
-            if (!a.frozen_eta) {
+            if (VISC && !a.frozen_eta) {
                 const T t = tauII(xxk, yyk, zzk, lerp4(xy, xy_ip, xy_jp, xy_ijp), lerp4(xz, xz_ip, xz_kp, xz_ikp),
                                   lerp4(yz, yz_jp, yz_kp, yz_jkp));
                 sto(a.eta_next, cv, rheology_o(a, cv, t, em));
@@ -607,14 +642,14 @@ __device__ __forceinline__ uint2 ldw(bool p, const uint2* base, uint32_t bo)
     return p ? *reinterpret_cast<const uint2*>(reinterpret_cast<const char*>(base) + bo) : make_uint2(0u, 0u);
 }
 
-template <typename T, int TX, int TY>
+template <typename T, int TX, int TY, bool VISC>
 __device__ __forceinline__ void fused3_ib_tile(const StokesArgs<T>& a, const FusedGeom& fg, int bx, int by, int bz)
 {
     // FP32 sweeps read masks and flags from the packed words (a.mw)
     constexpr bool PACKED = sizeof(T) == 4;
     __shared__ T sVx[TY][TX], sVy[TY][TX], sVz[TY][TX];
     __shared__ T sE[3][TY][TX];
-    __shared__ T sS[2][9][TY][TX]; // na-weighted P txx tyy txy txz tyz; raw txy txz tyz
+    __shared__ T sS[2][VISC ? 9 : 6][TY][TX]; // na-weighted P txx tyy txy txz tyz; raw txy txz tyz
 
     const int tx = threadIdx.x, ty = threadIdx.y;
     int xs, xe; // inner x range of this tile
@@ -628,7 +663,7 @@ __device__ __forceinline__ void fused3_ib_tile(const StokesArgs<T>& a, const Fus
     const bool can_xy = tx > 0 && ty > 0;
     const bool can_xz = tx > 0;
     const bool can_yz = ty > 0;
-    const bool need_raw = !a.frozen_eta;
+    const bool need_raw = VISC && !a.frozen_eta;
     constexpr uint32_t E = sizeof(T);
     const uint32_t bsz = (uint32_t)a.sz * E;
     const int ic = min(i, xe + 1), jc = min(j, fg.hi[1] + 1);
@@ -813,10 +848,12 @@ __device__ __forceinline__ void fused3_ib_tile(const StokesArgs<T>& a, const Fus
         sS[b][3][ty][tx] = mxy0 * xyc;
         sS[b][4][ty][tx] = mxz0 * xzc;
         sS[b][5][ty][tx] = myz0 * yzc;
-        if (need_raw) {
-            sS[b][6][ty][tx] = xyc;
-            sS[b][7][ty][tx] = xzc;
-            sS[b][8][ty][tx] = yzc;
+        if constexpr (VISC) {
+            if (need_raw) {
+                sS[b][6][ty][tx] = xyc;
+                sS[b][7][ty][tx] = xzc;
+                sS[b][8][ty][tx] = yzc;
+            }
         }
         if (inner && kk >= ks && kk <= ke) {
             if (ws || (fl0 & F_CCC)) {
@@ -863,12 +900,14 @@ __device__ __forceinline__ void fused3_ib_tile(const StokesArgs<T>& a, const Fus
             } else if (wv) {
                 sto(a.Vzn, cv, T(0));
             }
-            if (need_raw) {
-                const T t = tauII(xxr, yyr, zzr,
-                                  lerp4(sS[bo][6][ty][tx], sS[bo][6][ty][tx + 1], sS[bo][6][ty + 1][tx], sS[bo][6][ty + 1][tx + 1]),
-                                  lerp4(sS[bo][7][ty][tx], sS[bo][7][ty][tx + 1], sS[bn][7][ty][tx], sS[bn][7][ty][tx + 1]),
-                                  lerp4(sS[bo][8][ty][tx], sS[bo][8][ty + 1][tx], sS[bn][8][ty][tx], sS[bn][8][ty + 1][tx]));
-                sto(a.eta_next, cv, rheology_o(a, cv, t, em));
+            if constexpr (VISC) {
+                if (need_raw) {
+                    const T t = tauII(xxr, yyr, zzr,
+                                      lerp4(sS[bo][6][ty][tx], sS[bo][6][ty][tx + 1], sS[bo][6][ty + 1][tx], sS[bo][6][ty + 1][tx + 1]),
+                                      lerp4(sS[bo][7][ty][tx], sS[bo][7][ty][tx + 1], sS[bn][7][ty][tx], sS[bn][7][ty][tx + 1]),
+                                      lerp4(sS[bo][8][ty][tx], sS[bo][8][ty + 1][tx], sS[bn][8][ty][tx], sS[bn][8][ty + 1][tx]));
+                    sto(a.eta_next, cv, rheology_o(a, cv, t, em));
+                }
             }
         }
 
@@ -920,7 +959,7 @@ __device__ __forceinline__ void fused3_ib_tile(const StokesArgs<T>& a, const Fus
 // outside it gets eta_next = rheology(tau_II = 0, eta), as the sweep would
 // have given it (every stress its tau_II averages is at an invalid location).
 template <typename T>
-__global__ __launch_bounds__(256) void k_eta_rest(StokesArgs<T> a, FusedGeom fg, int tx, int ty)
+__global__ __launch_bounds__(256) void k_eta_rest(StokesArgs<T> a, FusedGeom fg, int tx, int ty, const T* st0)
 {
     const int i = fg.lo[0] + (int)(blockIdx.x * blockDim.x + threadIdx.x);
     const int j = fg.lo[1] + (int)(blockIdx.y * blockDim.y + threadIdx.y);
@@ -932,7 +971,17 @@ __global__ __launch_bounds__(256) void k_eta_rest(StokesArgs<T> a, FusedGeom fg,
     const int2 r = a.tmarch[bx + fg.nbx * (by + nby * bz)];
     if (k >= r.x && k <= r.y) return;
     const uint32_t bo = (uint32_t)((i + a.sy * j + a.sz * k) * (long)sizeof(T));
-    sto(a.eta_next, bo, rheology_o(a, bo, T(0), ldo(a.eta, bo)));
+    // a constant-coefficient rheology has one target at tau_II = 0: its scaled
+    // value comes from k_eta_target (the same operations, once)
+    const T st = a.rh ? rheo_scaled(a, rheo_target(a, bo, T(0))) : *st0;
+    sto(a.eta_next, bo, rheo_relax(a, st, ldo(a.eta, bo)));
+}
+
+// the scaled target viscosity at tau_II = 0 of a constant-coefficient rheology
+template <typename T>
+__global__ void k_eta_target(StokesArgs<T> a, T* st0)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *st0 = rheo_scaled(a, rheo_target(a, 0u, T(0)));
 }
 
 // Tile order.  Workgroups are dispatched round-robin over the 8 XCDs
@@ -978,7 +1027,7 @@ __device__ __forceinline__ bool fused_tile_of(int b, int order, int nbx, int nby
 // whose starts count workgroups of TX * TY threads): the shell's stress
 // reads only the previous iteration, like the fused tiles, and the two write
 // disjoint points, so they share the launch and the shell fills its tail.
-template <typename T, bool IB, int TX, int TY, int MINW>
+template <typename T, bool IB, int TX, int TY, int MINW, bool VISC>
 __global__ __launch_bounds__(TX * TY, MINW) void k_fused3(StokesArgs<T> a, FusedGeom fg, BoxList shell, int nfused)
 {
     if ((int)blockIdx.x >= nfused) {
@@ -999,8 +1048,8 @@ __global__ __launch_bounds__(TX * TY, MINW) void k_fused3(StokesArgs<T> a, Fused
     const int nbz = (fg.hi[2] - fg.lo[2] + fg.kch) / fg.kch;
     int bx, by, bz;
     if (!fused_tile_of(blockIdx.x, fg.order, fg.nbx, nby, nbz, bx, by, bz)) return;
-    if constexpr (IB) fused3_ib_tile<T, TX, TY>(a, fg, bx, by, bz);
-    else fused3_tile<T, false, TX, TY>(a, fg, bx, by, bz);
+    if constexpr (IB) fused3_ib_tile<T, TX, TY, VISC>(a, fg, bx, by, bz);
+    else fused3_tile<T, false, TX, TY, VISC>(a, fg, bx, by, bz);
 }
 
 // Packed fused sweep over many instances: a persistent grid walks the
@@ -1026,8 +1075,8 @@ __global__ __launch_bounds__(TX * TY, MINW) void k_fused3_packed(const PackedIns
         const StokesArgs<T> a = inst[w.x].a;
         const FusedGeom fg = inst[w.x].fg;
         __syncthreads(); // the previous item's LDS reads are done
-        if constexpr (IB) fused3_ib_tile<T, TX, TY>(a, fg, w.y, w.z, w.w);
-        else fused3_tile<T, false, TX, TY>(a, fg, w.y, w.z, w.w);
+        if constexpr (IB) fused3_ib_tile<T, TX, TY, true>(a, fg, w.y, w.z, w.w);
+        else fused3_tile<T, false, TX, TY, true>(a, fg, w.y, w.z, w.w);
     }
 }
 
@@ -1675,11 +1724,20 @@ static void launch_fused_sweep(const Geo& g, const StokesArgs<T>& a, const Fused
         }
     }
     dim3 grd(nfused + (sh.nb ? sh.start[sh.nb] : 0)), blk(tx, ty, 1);
-#define FI_LAUNCH_(TX_, TY_, W_)                                                                            \
-    if (tx == TX_ && ty == TY_ && minw == W_) {                                                             \
-        if (ib) hipLaunchKernelGGL((k_fused3<T, true, TX_, TY_, W_>), grd, blk, 0, s, a, fg, sh, nfused);  \
-        else hipLaunchKernelGGL((k_fused3<T, false, TX_, TY_, W_>), grd, blk, 0, s, a, fg, sh, nfused);    \
-        return;                                                                                             \
+    // FP32 sweeps with frozen viscosity run the variant compiled without the viscosity update
+    const bool visc = sizeof(T) == 8 || !a.frozen_eta;
+#define FI_LAUNCH_(TX_, TY_, W_)                                                                                    \
+    if (tx == TX_ && ty == TY_ && minw == W_) {                                                                     \
+        if constexpr (sizeof(T) == 4) {                                                                             \
+            if (!visc) {                                                                                            \
+                if (ib) hipLaunchKernelGGL((k_fused3<T, true, TX_, TY_, W_, false>), grd, blk, 0, s, a, fg, sh, nfused); \
+                else hipLaunchKernelGGL((k_fused3<T, false, TX_, TY_, W_, false>), grd, blk, 0, s, a, fg, sh, nfused);   \
+                return;                                                                                             \
+            }                                                                                                       \
+        }                                                                                                           \
+        if (ib) hipLaunchKernelGGL((k_fused3<T, true, TX_, TY_, W_, true>), grd, blk, 0, s, a, fg, sh, nfused);    \
+        else hipLaunchKernelGGL((k_fused3<T, false, TX_, TY_, W_, true>), grd, blk, 0, s, a, fg, sh, nfused);      \
+        return;                                                                                                     \
     }
     if constexpr (sizeof(T) == 8) { FI_FUSED_CFG_D(FI_LAUNCH_) }
     else { FI_FUSED_CFG_F(FI_LAUNCH_) }
@@ -1695,8 +1753,11 @@ void launch_fused(const Geo& g, const StokesArgs<T>& a, const FusedGeom& fg0, bo
     // a narrowed immersed-boundary sweep with a viscosity update leaves the
     // viscosity of the planes it did not march to k_eta_rest
     if (ib && !a.frozen_eta && a.z_wsig && a.z_wvel && a.tmarch) {
+        // one scalar after the tile entries of tmarch holds the scaled target
+        T* st0 = reinterpret_cast<T*>(a.tmarch + tile_march_count(g));
+        if (!a.rh) hipLaunchKernelGGL((k_eta_target<T>), dim3(1), dim3(64), 0, s, a, st0);
         dim3 grd((fg.hi[0] - fg.lo[0] + 64) / 64, (fg.hi[1] - fg.lo[1] + 4) / 4, fg.hi[2] - fg.lo[2] + 1);
-        hipLaunchKernelGGL((k_eta_rest<T>), grd, dim3(64, 4, 1), 0, s, a, fg, tx, ty);
+        hipLaunchKernelGGL((k_eta_rest<T>), grd, dim3(64, 4, 1), 0, s, a, fg, tx, ty, st0);
     }
 }
 
