@@ -41,8 +41,8 @@ struct BatchTables {
     char* dev = nullptr;
     size_t cap = 0;
     void* inst[2] = {}; // PackedInst<double> (FP64 batches) or PackedInst<float> (refined batches)
-    int4* items[2] = {}; // plain, IB
-    int nitems[2] = {};
+    int4* items[4] = {}; // plain, IB; then the same for frozen-viscosity (FP32 correction) instances
+    int nitems[4] = {};
     PackedBox* box[4] = {};
     int* blk2box[4] = {};
     int nblk_box[4] = {};
@@ -254,7 +254,7 @@ static int build_tables_t(fi_batch_s* b, int ty, int kch)
     using S = SweepOf<T>;
     const int nrun = (int)b->run.size();
     std::vector<PackedInst<T>> pinst[2];
-    std::vector<int4> items[2];
+    std::vector<int4> items[4];
     std::vector<PackedBox> boxes[4];
     std::vector<int> blk2box[4];
     std::vector<PackedBcOp> bcops[2][2][3];
@@ -297,7 +297,8 @@ static int build_tables_t(fi_batch_s* b, int ty, int kch)
             S::set(m, p0);
         }
         if (interior) {
-            const int ib = m->desc.immersed_boundary ? 1 : 0;
+            // frozen-viscosity sweeps go to the packed kernel compiled without the update
+            const int ib = (m->desc.immersed_boundary ? 1 : 0) | (pinst[0].back().a.frozen_eta ? 2 : 0);
             const int nbx = fg.nbx;
             const int nby = (fg.hi[1] - fg.lo[1] + 1 + ty - 3) / (ty - 2);
             const int nbz = (fg.hi[2] - fg.lo[2] + fg.kch) / fg.kch;
@@ -324,9 +325,9 @@ static int build_tables_t(fi_batch_s* b, int ty, int kch)
         }
     }
     std::vector<char> h;
-    size_t o_inst[2], o_items[2], o_box[4], o_blk[4], o_bc[2][2][3], o_bcblk[2][2][3];
+    size_t o_inst[2], o_items[4], o_box[4], o_blk[4], o_bc[2][2][3], o_bcblk[2][2][3];
     for (int p = 0; p < 2; ++p) o_inst[p] = put(h, pinst[p]);
-    for (int q = 0; q < 2; ++q) o_items[q] = put(h, items[q]);
+    for (int q = 0; q < 4; ++q) o_items[q] = put(h, items[q]);
     for (int k = 0; k < 4; ++k) {
         o_box[k] = put(h, boxes[k]);
         o_blk[k] = put(h, blk2box[k]);
@@ -352,7 +353,7 @@ static int build_tables_t(fi_batch_s* b, int ty, int kch)
     }
     if (!h.empty()) HIPCHK(hipMemcpy(t.dev, h.data(), h.size(), hipMemcpyHostToDevice));
     for (int p = 0; p < 2; ++p) t.inst[p] = t.dev + o_inst[p];
-    for (int q = 0; q < 2; ++q) {
+    for (int q = 0; q < 4; ++q) {
         t.items[q] = reinterpret_cast<int4*>(t.dev + o_items[q]);
         t.nitems[q] = (int)items[q].size();
     }
@@ -387,9 +388,9 @@ template <typename T>
 static void launch_fused_all_t(fi_batch_s* b, int p)
 {
     const BatchTables& t = b->t;
-    for (int q = 0; q < 2; ++q)
-        launch_fused_packed<T>(static_cast<const PackedInst<T>*>(t.inst[p]), t.items[q], t.nitems[q], q == 1, b->tx,
-                               b->ty, b->minw, fused_grid(t.nitems[q]), b->stream);
+    for (int q = 0; q < 4; ++q)
+        launch_fused_packed<T>(static_cast<const PackedInst<T>*>(t.inst[p]), t.items[q], t.nitems[q], (q & 1) != 0,
+                               (q & 2) == 0, b->tx, b->ty, b->minw, fused_grid(t.nitems[q]), b->stream);
 }
 
 template <typename T>
@@ -912,7 +913,7 @@ int fi_batch_info(fi_batch_t b, int out[8])
     for (const BatchInst& bi : b->inst) running += bi.state == FI_BATCH_RUNNING;
     out[0] = (int)b->inst.size();
     out[1] = running;
-    out[2] = b->t.nitems[0] + b->t.nitems[1];
+    out[2] = b->t.nitems[0] + b->t.nitems[1] + b->t.nitems[2] + b->t.nitems[3];
     out[3] = (b->tx * 32 + b->ty) * 16 + b->minw;
     out[4] = b->kch * 4 + b->order;
     out[5] = b->graph_iters;

@@ -207,9 +207,10 @@ void launch_eta_update(const Geo& g, const StokesArgs<T>& a, hipStream_t s);
 void launch_bc_dim(const Geo& g, const BcBatch& b, hipStream_t s);
 
 // packed sweeps over many instances (batch.hip builds the tables)
+// visc false: every listed instance sweeps with frozen viscosity (FP32 only)
 template <typename T>
-void launch_fused_packed(const PackedInst<T>* inst, const int4* items, int nitems, bool ib, int tx, int ty, int minw,
-                         int grid, hipStream_t s);
+void launch_fused_packed(const PackedInst<T>* inst, const int4* items, int nitems, bool ib, bool visc, int tx, int ty,
+                         int minw, int grid, hipStream_t s);
 template <typename T>
 void launch_point_packed(const PackedInst<T>* inst, const PackedBox* boxes, const int* blk2box, int nblocks, bool ib,
                          int nd, bool sigma, hipStream_t s);

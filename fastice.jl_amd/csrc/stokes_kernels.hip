@@ -1055,7 +1055,7 @@ __global__ __launch_bounds__(TX * TY, MINW) void k_fused3(StokesArgs<T> a, Fused
 // Packed fused sweep over many instances: a persistent grid walks the
 // (instance, tile) work list; each item is one workgroup tile of one
 // instance, swept exactly as the single-instance kernel sweeps it.
-template <typename T, bool IB, int TX, int TY, int MINW>
+template <typename T, bool IB, int TX, int TY, int MINW, bool VISC>
 __global__ __launch_bounds__(TX * TY, MINW) void k_fused3_packed(const PackedInst<T>* __restrict__ inst,
                                                                  const int4* __restrict__ items, int nitems)
 {
@@ -1075,8 +1075,8 @@ __global__ __launch_bounds__(TX * TY, MINW) void k_fused3_packed(const PackedIns
         const StokesArgs<T> a = inst[w.x].a;
         const FusedGeom fg = inst[w.x].fg;
         __syncthreads(); // the previous item's LDS reads are done
-        if constexpr (IB) fused3_ib_tile<T, TX, TY, true>(a, fg, w.y, w.z, w.w);
-        else fused3_tile<T, false, TX, TY, true>(a, fg, w.y, w.z, w.w);
+        if constexpr (IB) fused3_ib_tile<T, TX, TY, VISC>(a, fg, w.y, w.z, w.w);
+        else fused3_tile<T, false, TX, TY, VISC>(a, fg, w.y, w.z, w.w);
     }
 }
 
@@ -1798,17 +1798,24 @@ void launch_tau(const Geo& g, const StokesArgs<T>& a, bool ib, hipStream_t s, co
 }
 
 template <typename T>
-void launch_fused_packed(const PackedInst<T>* inst, const int4* items, int nitems, bool ib, int tx, int ty, int minw,
-                         int grid, hipStream_t s)
+void launch_fused_packed(const PackedInst<T>* inst, const int4* items, int nitems, bool ib, bool visc, int tx, int ty,
+                         int minw, int grid, hipStream_t s)
 {
     if (nitems <= 0) return;
     int gsz = nitems < grid ? nitems : grid;
     dim3 grd((gsz + 8 * NXCD - 1) / (8 * NXCD) * (8 * NXCD)), blk(tx, ty);
-#define FI_LAUNCH_(TX_, TY_, W_)                                                                                 \
-    if (tx == TX_ && ty == TY_ && minw == W_) {                                                                  \
-        if (ib) hipLaunchKernelGGL((k_fused3_packed<T, true, TX_, TY_, W_>), grd, blk, 0, s, inst, items, nitems);  \
-        else hipLaunchKernelGGL((k_fused3_packed<T, false, TX_, TY_, W_>), grd, blk, 0, s, inst, items, nitems);    \
-        return;                                                                                                  \
+#define FI_LAUNCH_(TX_, TY_, W_)                                                                                        \
+    if (tx == TX_ && ty == TY_ && minw == W_) {                                                                         \
+        if constexpr (sizeof(T) == 4) {                                                                                 \
+            if (!visc) {                                                                                                \
+                if (ib) hipLaunchKernelGGL((k_fused3_packed<T, true, TX_, TY_, W_, false>), grd, blk, 0, s, inst, items, nitems); \
+                else hipLaunchKernelGGL((k_fused3_packed<T, false, TX_, TY_, W_, false>), grd, blk, 0, s, inst, items, nitems);   \
+                return;                                                                                                 \
+            }                                                                                                           \
+        }                                                                                                               \
+        if (ib) hipLaunchKernelGGL((k_fused3_packed<T, true, TX_, TY_, W_, true>), grd, blk, 0, s, inst, items, nitems);  \
+        else hipLaunchKernelGGL((k_fused3_packed<T, false, TX_, TY_, W_, true>), grd, blk, 0, s, inst, items, nitems);    \
+        return;                                                                                                         \
     }
     if constexpr (sizeof(T) == 8) { FI_FUSED_CFG_D(FI_LAUNCH_) }
     else { FI_FUSED_CFG_F(FI_LAUNCH_) }
@@ -1840,7 +1847,7 @@ void launch_point_packed(const PackedInst<T>* inst, const PackedBox* boxes, cons
     template void launch_residuals<T>(const Geo&, const StokesArgs<T>&, bool, hipStream_t);                      \
     template void launch_tau<T>(const Geo&, const StokesArgs<T>&, bool, hipStream_t, const TauLag*);                      \
     template void launch_eta_update<T>(const Geo&, const StokesArgs<T>&, hipStream_t);                     \
-    template void launch_fused_packed<T>(const PackedInst<T>*, const int4*, int, bool, int, int, int, int,         \
+    template void launch_fused_packed<T>(const PackedInst<T>*, const int4*, int, bool, bool, int, int, int, int,   \
                                          hipStream_t);                                                          \
     template void launch_point_packed<T>(const PackedInst<T>*, const PackedBox*, const int*, int, bool, int, bool, \
                                          hipStream_t);
