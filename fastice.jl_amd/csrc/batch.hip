@@ -791,7 +791,7 @@ int fi_batch_add_refined(fi_batch_t b, fi_model_t m, int max_outer, int max_inne
         // the FP32 sweep variant of the packed launches
         b->refined = 1;
         b->tx = 64;
-        b->ty = 16;
+        b->ty = 8;
         b->minw = 1;
         b->kch = 32;
         b->autotune = 0;
