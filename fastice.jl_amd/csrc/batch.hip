@@ -281,9 +281,9 @@ static int build_tables_t(fi_batch_s* b, int ty, int kch)
             // which buffers of this parity hold +0 at every invalid location,
             // as the model tracks it when the tables are built (the packed
             // sweeps keep a clean buffer clean, so the flags stay true while
-            // the tables live; a change of them rebuilds the tables); the
-            // viscosity of unmarched planes needs k_eta_rest, which packed
-            // launches do not run, so FP64 sweeps march every plane
+            // the tables live; a change of them rebuilds the tables); without
+            // a march table the packed sweep updates the viscosity of the
+            // planes it does not march itself
             pi.a.tmarch = nullptr;
             S::zflags(m, pp, pi.a);
             pinst[p].push_back(pi);

@@ -642,7 +642,9 @@ __device__ __forceinline__ uint2 ldw(bool p, const uint2* base, uint32_t bo)
     return p ? *reinterpret_cast<const uint2*>(reinterpret_cast<const char*>(base) + bo) : make_uint2(0u, 0u);
 }
 
-template <typename T, int TX, int TY, bool VISC>
+// ETA_HERE: packed launches, which keep no march table, update the viscosity
+// of the planes a narrowed sweep does not march inside the sweep
+template <typename T, int TX, int TY, bool VISC, bool ETA_HERE>
 __device__ __forceinline__ void fused3_ib_tile(const StokesArgs<T>& a, const FusedGeom& fg, int bx, int by, int bz)
 {
     // FP32 sweeps read masks and flags from the packed words (a.mw)
@@ -701,14 +703,43 @@ __device__ __forceinline__ void fused3_ib_tile(const StokesArgs<T>& a, const Fus
         atomicMax(&s_hi, rg.y);
     }
     __syncthreads();
-    // with a viscosity update, the planes outside the march are left to
-    // k_eta_rest, which reads the march of every tile from a.tmarch
-    const bool narrow = !ws && !wv && (!need_raw || a.tmarch != nullptr);
+    // With a viscosity update, the planes outside the march are left to
+    // k_eta_rest, which reads the march of every tile from a.tmarch (single
+    // models); packed launches (ETA_HERE) update them here.
+    const bool narrow = !ws && !wv && (!need_raw || ETA_HERE || a.tmarch != nullptr);
     const int ks = narrow ? max(ks0, s_lo - 1) : ks0;
     const int ke = narrow ? min(ke0, s_hi) : ke0;
-    if (narrow && need_raw && tx == 0 && ty == 0) {
-        const int nby = (fg.hi[1] - fg.lo[1] + 1 + TY - 3) / (TY - 2);
-        a.tmarch[bx + fg.nbx * (by + nby * bz)] = make_int2(ks, ke);
+    if constexpr (VISC && !ETA_HERE) {
+        if (narrow && need_raw && tx == 0 && ty == 0) {
+            const int nby = (fg.hi[1] - fg.lo[1] + 1 + TY - 3) / (TY - 2);
+            a.tmarch[bx + fg.nbx * (by + nby * bz)] = make_int2(ks, ke);
+        }
+    }
+    if constexpr (VISC && ETA_HERE) {
+        if (narrow && need_raw) {
+            if (inner) {
+                auto eta_only = [&](int k0, int k1) {
+                    for (int k = k0; k <= k1; k += 4) {
+                        T o[4];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u)
+                            o[u] = k + u <= k1 ? ldo(a.eta, col + bsz * (uint32_t)(k + u)) : T(1);
+#pragma unroll
+                        for (int u = 0; u < 4; ++u)
+                            if (k + u <= k1) {
+                                const uint32_t cu = col + bsz * (uint32_t)(k + u);
+                                sto(a.eta_next, cu, rheology_o(a, cu, T(0), o[u]));
+                            }
+                    }
+                };
+                if (ks > ke) {
+                    eta_only(ks0, ke0);
+                } else {
+                    eta_only(ks0, ks - 1);
+                    eta_only(ke + 1, ke0);
+                }
+            }
+        }
     }
     if (ks > ke) return;
 
@@ -1048,7 +1079,7 @@ __global__ __launch_bounds__(TX * TY, MINW) void k_fused3(StokesArgs<T> a, Fused
     const int nbz = (fg.hi[2] - fg.lo[2] + fg.kch) / fg.kch;
     int bx, by, bz;
     if (!fused_tile_of(blockIdx.x, fg.order, fg.nbx, nby, nbz, bx, by, bz)) return;
-    if constexpr (IB) fused3_ib_tile<T, TX, TY, VISC>(a, fg, bx, by, bz);
+    if constexpr (IB) fused3_ib_tile<T, TX, TY, VISC, false>(a, fg, bx, by, bz);
     else fused3_tile<T, false, TX, TY, VISC>(a, fg, bx, by, bz);
 }
 
@@ -1075,7 +1106,7 @@ __global__ __launch_bounds__(TX * TY, MINW) void k_fused3_packed(const PackedIns
         const StokesArgs<T> a = inst[w.x].a;
         const FusedGeom fg = inst[w.x].fg;
         __syncthreads(); // the previous item's LDS reads are done
-        if constexpr (IB) fused3_ib_tile<T, TX, TY, VISC>(a, fg, w.y, w.z, w.w);
+        if constexpr (IB) fused3_ib_tile<T, TX, TY, VISC, true>(a, fg, w.y, w.z, w.w);
         else fused3_tile<T, false, TX, TY, VISC>(a, fg, w.y, w.z, w.w);
     }
 }
