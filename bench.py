@@ -23,6 +23,11 @@ Workloads
       value = T_eff by benchmark_diffusion_3D.jl's formula.
   h5_glacier_255x255x127: HDF5 + XDMF3 output (src/Writers.jl) of the fields
       the glacier driver saves, from device memory; value = GB/s of data.
+  manufactured3d_255: 3D isoviscous free-slip manufactured solution, 255^3,
+      FP64 (scripts_future_API/stokes_manufactured_solution_free_slip_3D.jl):
+      2000 PT iterations timed, then a converged solve to residual 1e-8
+      (ncheck 255) with its error norms against the closed form; value =
+      T_eff of the iteration; the CPU oracle timed on a sample of iterations.
   serving_poisson40: Poisson arrivals at 40/s, half 2D glacier sections
       (n x n/2, n in 64..511), half 3D glaciers (n x n x n/2, n in 32..127),
       FP64 to 1e-8 or niter = 50n, continuously batched; value = completed
@@ -58,6 +63,7 @@ WORKLOADS = {
     "serving_poisson40": None,
     "h5_glacier_255x255x127": None,
     "heat_diffusion": None,
+    "manufactured3d_255": None,
 }
 
 
@@ -355,6 +361,85 @@ def bench_h5(pkg, args):
     print(json.dumps(line), flush=True)
 
 
+def bench_manufactured(pkg, args):
+    """The 3D manufactured free-slip solution at 255^3 in FP64: 2000 PT
+    iterations (T_eff), then a solve to residual 1e-8 checked every 255
+    iterations, with the relative error norms of |V| and tau_II against the
+    closed form (..._3D.jl:140-167)."""
+    A = pkg._abi
+    n = 255
+    kw, cad = pkg.manufactured_3d(n)
+    model = pkg.IsothermalFullStokesModel(**kw)
+    model.viscosity.eta.set(1.0)
+    model.viscosity.eta_next.set(1.0)
+    model.apply_bc("eta")
+    model.synchronize()
+    cells = n ** 3
+    n_it = 2000
+    log("manufactured 255^3: %d FP64 iterations" % n_it)
+    t0 = time.perf_counter()
+    model.advance(n_it)
+    model.synchronize()
+    ms = (time.perf_counter() - t0) / n_it * 1e3
+    teff = 23 * 8 * cells / (ms * 1e-3) / 1e9
+    ev = event_times(A, A.lib().fi_time_iterations, model, 20)
+    icells, _ = fused_cells(A, model)
+    shell_pts = (n + 1) ** 3 - icells if icells else 0
+    kbytes = 25 * 8 * icells + 18 * 8 * shell_pts
+    log("iteration %.3f ms, T_eff %.1f GB/s" % (ms, teff))
+    # converged solve from the zero state
+    for f in ("P", "TXX", "TYY", "TZZ", "TXY", "TXZ", "TYZ", "VX", "VY", "VZ"):
+        model.field(f).set(0.0)
+    model.viscosity.eta.set(1.0)
+    model.viscosity.eta_next.set(1.0)
+    model.apply_bc("eta")
+    model.synchronize()
+    t0 = time.perf_counter()
+    it, norms = model.solve(200 * n, n, 1e-8)
+    model.synchronize()
+    solve_s = time.perf_counter() - t0
+    V = {k: model.field("V" + k.upper()).interior() for k in ("x", "y", "z")}
+    tau = {k: model.field("T" + k.upper()).interior() for k in ("xx", "yy", "zz", "xy", "xz", "yz")}
+    err = pkg.problems.manufactured_errors(model.grid, V, tau)
+    log("solve: %d iterations, %.2f s, residual %.2e, errors %s" % (it, solve_s, max(norms), err))
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline")
+        orc = ge.load_oracle()
+        ref = orc.OracleModel(model.desc)
+        for k in ("GX", "GY", "GZ"):
+            ref.set(k, model.field(k).parent())
+        ref.set("ETA", 1.0)
+        ref.set("ETA_NEXT", 1.0)
+        ref.advance(2)
+        k_cpu, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < args.cpu_seconds:
+            ref.advance(4)
+            k_cpu += 4
+        el = time.perf_counter() - t0
+        cpu = {"value": round(23 * 8 * cells * k_cpu / el / 1e9, 3), "unit": "GB/s",
+               "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), "kind": "port",
+               "sample": "%d FP64 PT iterations of the same 255^3 manufactured problem (CPU oracle, OpenMP, %.1f s)"
+                         % (k_cpu, el)}
+    config = {"workload": "manufactured3d_255", "grid": [n, n, n], "box": "[-1, 1]^3, free slip on every face",
+              "iterations_timed": n_it, "iteration_ms": round(ms, 4),
+              "converged": {"tol": 1e-8, "ncheck": n, "iterations": int(it), "seconds": round(solve_s, 3),
+                            "residual": float(max(norms)),
+                            "error_vs_closed_form": {"Linf_V": err[0], "Linf_tauII": err[1], "L1_V": err[2],
+                                                     "L1_tauII": err[3], "L2_V": err[4], "L2_tauII": err[5]}}}
+    line = _line("PT-Stokes T_eff GB/s at 255^3 (manufactured free slip, FP64)", teff, "GB/s", ms, n_it, 0, "f64",
+                 config)
+    achieved = kbytes / (ev[1] * 1e-3) / 1e9
+    line["roofline"] = {"bound": "hbm", "kernel": "k_fused3<double>", "achieved": round(achieved, 2),
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                        "traffic": None, "kernel_ms": round(float(ev[1]), 4),
+                        "algorithmic_bytes_per_launch": int(kbytes)}
+    line["cpu_baseline"] = cpu
+    model.close()
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def bench_heat(pkg, args):
     """PT heat diffusion at 1023^3 in FP64 (the size of
     scripts_future_API/benchmark_diffusion_3D.jl:65): T_eff by that script's
@@ -494,6 +579,8 @@ def main():
         return bench_h5(pkg, args)
     if args.workload == "heat_diffusion":
         return bench_heat(pkg, args)
+    if args.workload == "manufactured3d_255":
+        return bench_manufactured(pkg, args)
     dims = WORKLOADS[args.workload]
     cells = dims[0] * dims[1] * dims[2]
     glacier = args.workload.startswith("glacier")
