@@ -672,6 +672,12 @@ def main():
         config["description"] = ("synthetic glacier, 8 Gaussian bumps (seed 1), parabolic surface, volume-fraction "
                                  "immersed boundary, Glen n=3 (A=1, eta_reg=30, log relaxation 0.05), traction-free box")
         config["converged_solve"] = solve
+        # the reference's T_eff counts every cell of the box; the sweep reads
+        # and writes only the cells on planes of their column that hold a valid
+        # location (the rest holds no valid location and stays 0)
+        config["empty_space"] = {"interior_cells": int(icells), "processed_cells": int(acells),
+                                 "processed_fraction": round(acells / icells, 4) if icells else None,
+                                 "teff_counts_skipped_cells": True}
     line = {
         "metric": "PT-Stokes T_eff GB/s (% HBM peak) at 511x511x255",
         "value": round(value, 3),
