@@ -172,14 +172,14 @@ BcBatch neumann_batch(double* p, int L, int d)
 
 void run_bc(fi_model_s* m, int group)
 {
+    BcBatch b[3];
     for (int d = 0; d < m->g.nd; ++d) {
-        BcBatch b;
-        if (group == FI_BCGROUP_STRESS) b = stress_batch(m, d);
-        else if (group == FI_BCGROUP_VELOCITY) b = velocity_batch(m, d, true);
-        else if (group == FI_BCGROUP_ETA) b = neumann_batch(m->eta(), L_CCC, d);
-        else b = residual_batch(m, d);
-        launch_bc_dim(m->g, b, m->stream);
+        if (group == FI_BCGROUP_STRESS) b[d] = stress_batch(m, d);
+        else if (group == FI_BCGROUP_VELOCITY) b[d] = velocity_batch(m, d, true);
+        else if (group == FI_BCGROUP_ETA) b[d] = neumann_batch(m->eta(), L_CCC, d);
+        else b[d] = residual_batch(m, d);
     }
+    launch_bc_dims(m->g, b, m->stream);
 }
 
 int prepare(fi_model_s* m, bool sync_shadow)
@@ -600,14 +600,16 @@ int fi_apply_bc(fi_model_t m, int group)
     if (group < FI_BCGROUP_STRESS || group > FI_BCGROUP_RESIDUAL) return fail(FI_ERR_ARG, "bad bc group %d", group);
     if (group == FI_BCGROUP_ETA) {
         // bc!(eta => Neumann()) and bc!(eta_next => Neumann()) (tm_stokes_chmy.jl:93-94)
+        BcBatch b[3];
         for (int d = 0; d < m->g.nd; ++d) {
-            launch_bc_dim(m->g, neumann_batch(m->eta(), L_CCC, d), m->stream);
+            b[d] = neumann_batch(m->eta(), L_CCC, d);
+            add_op(b[d], m->eta_next(), L_CCC, 1, 0.0);
         }
-        for (int d = 0; d < m->g.nd; ++d) {
-            launch_bc_dim(m->g, neumann_batch(m->eta_next(), L_CCC, d), m->stream);
-        }
+        launch_bc_dims(m->g, b, m->stream);
     } else if (group == FI_BCGROUP_VELOCITY) {
-        for (int d = 0; d < m->g.nd; ++d) launch_bc_dim(m->g, velocity_batch(m, d, false), m->stream);
+        BcBatch b[3];
+        for (int d = 0; d < m->g.nd; ++d) b[d] = velocity_batch(m, d, false);
+        launch_bc_dims(m->g, b, m->stream);
         m->state_written();
     } else {
         run_bc(m, group);

@@ -11,6 +11,8 @@
 //   Neumann(),   Vertex: ghost = mirror of the node next to the boundary node
 #include "fi_internal.h"
 
+#include <algorithm>
+
 namespace fi {
 
 template <typename T>
@@ -72,6 +74,162 @@ void launch_bc_dim(const Geo& g, const BcBatch& b, hipStream_t s)
     dim3 blk(64, 4, 1);
     dim3 grd((m1 + 63) / 64, (m2 + 3) / 4, 2);
     hipLaunchKernelGGL(k_bc_dim, grd, blk, 0, s, b, g.nd, g.n[0], g.n[1], g.n[2], g.sy, g.sz);
+}
+
+// Every dimension of a boundary-condition group in one launch, with the
+// result of the launches x, then y, then z above, bit for bit.
+//
+// Per field, pass d writes the points whose index in d is a target of one of
+// its faces and reads each one's source (an index in d that pass d never
+// writes).  The value a point ends with is therefore: the last pass e that
+// writes it applies its face rule to the value its source held after passes
+// 0..e-1, and so on down.  The thread of a face point of pass d writes the
+// point only if no later pass writes it, and evaluates that chain itself:
+// it follows the sources from pass d down to pass 0, loads the one value the
+// chain starts from (a point that no pass writes, so no thread of this
+// launch changes it), or starts from v at a Dirichlet vertex, and applies
+// the face rules of the chain in pass order, in the same arithmetic.
+struct BcField {
+    void* ptr;
+    int L, f32;
+    int kind[3][2]; // -1: pass d does not write side s; else 0 Dirichlet, 1 Neumann
+    double v[3][2];
+};
+struct BcAll {
+    BcField f[8];
+    int nf, nd;
+};
+
+__device__ __forceinline__ int bc_target(int kind, int vert, int s, int nn)
+{
+    if (kind == 0 && vert) return s == 0 ? 1 : nn + 1;
+    if (kind == 1 && vert) return s == 0 ? 0 : nn + 2;
+    return s == 0 ? 0 : nn + 1;
+}
+__device__ __forceinline__ int bc_source(int vert, int s, int nn)
+{
+    if (vert) return s == 0 ? 2 : nn; // Neumann at a vertex (a Dirichlet vertex reads nothing)
+    return s == 0 ? 1 : nn;
+}
+
+template <typename T>
+__device__ __forceinline__ void bc_chain(const BcField& F, int nd, int d, const int n[3], const long st[3], int p[3])
+{
+    // the face rules met along the chain, outermost pass first
+    int cd[3], cs[3], nc = 0;
+    bool from_v = false;
+    double v0 = 0.0;
+    int q[3] = {p[0], p[1], p[2]};
+    for (int e = d; e >= 0 && !from_v; --e) {
+        const int vert = (F.L >> e) & 1;
+        for (int s = 0; s < 2; ++s) {
+            const int k = F.kind[e][s];
+            if (k < 0 || q[e] != bc_target(k, vert, s, n[e])) continue;
+            if (k == 0 && vert) {
+                from_v = true;
+                v0 = F.v[e][s];
+            } else {
+                cd[nc] = e;
+                cs[nc] = s;
+                ++nc;
+                q[e] = bc_source(vert, s, n[e]);
+            }
+            break;
+        }
+    }
+    T* f = static_cast<T*>(F.ptr);
+    T x;
+    if (from_v) x = T(v0);
+    else x = f[q[0] * st[0] + q[1] * st[1] + (nd == 3 ? q[2] * st[2] : 0)];
+    for (int c = nc - 1; c >= 0; --c)
+        if (F.kind[cd[c]][cs[c]] == 0) x = T(2) * T(F.v[cd[c]][cs[c]]) - x;
+    f[p[0] * st[0] + p[1] * st[1] + (nd == 3 ? p[2] * st[2] : 0)] = x;
+}
+
+__global__ __launch_bounds__(256) void k_bc_all(BcAll b, int n0, int n1, int n2, long sy, long sz)
+{
+    const int z = blockIdx.z;
+    const int fi = z / (2 * b.nd), d = (z >> 1) % b.nd, s = z & 1;
+    const BcField& F = b.f[fi];
+    const int k = F.kind[d][s];
+    if (k < 0) return;
+    const int n[3] = {n0, n1, n2};
+    const long st[3] = {1, sy, sz};
+    int e1 = -1, e2 = -1;
+    for (int e = 0; e < b.nd; ++e) {
+        if (e == d) continue;
+        if (e1 < 0) e1 = e; else e2 = e;
+    }
+    const int q1 = blockIdx.x * blockDim.x + threadIdx.x;
+    const int q2 = blockIdx.y * blockDim.y + threadIdx.y;
+    const int ext1 = n[e1] + ((F.L >> e1) & 1) + 2;
+    const int ext2 = (e2 >= 0) ? n[e2] + ((F.L >> e2) & 1) + 2 : 1;
+    if (q1 >= ext1 || q2 >= ext2) return;
+    int p[3] = {0, 0, 0};
+    p[d] = bc_target(k, (F.L >> d) & 1, s, n[d]);
+    p[e1] = q1;
+    if (e2 >= 0) p[e2] = q2;
+    // a later pass that writes p owns it
+    for (int e = d + 1; e < b.nd; ++e)
+        for (int t = 0; t < 2; ++t) {
+            const int ke = F.kind[e][t];
+            if (ke >= 0 && p[e] == bc_target(ke, (F.L >> e) & 1, t, n[e])) return;
+        }
+    if (F.f32) bc_chain<float>(F, b.nd, d, n, st, p);
+    else bc_chain<double>(F, b.nd, d, n, st, p);
+}
+
+void launch_bc_dims(const Geo& g, const BcBatch* b, hipStream_t s)
+{
+    BcAll a;
+    a.nf = 0;
+    a.nd = g.nd;
+    bool ok = true;
+    for (int d = 0; d < g.nd && ok; ++d)
+        for (int o = 0; o < b[d].nop && ok; ++o) {
+            const BcOp& op = b[d].op[o];
+            int fi = 0;
+            while (fi < a.nf && a.f[fi].ptr != op.ptr) ++fi;
+            if (fi == a.nf) {
+                if (a.nf == 8) { ok = false; break; }
+                BcField& F = a.f[a.nf++];
+                F.ptr = op.ptr;
+                F.L = op.L;
+                F.f32 = op.f32;
+                for (int e = 0; e < 3; ++e)
+                    for (int t = 0; t < 2; ++t) {
+                        F.kind[e][t] = -1;
+                        F.v[e][t] = 0.0;
+                    }
+            }
+            BcField& F = a.f[fi];
+            if (F.L != op.L || F.f32 != op.f32) { ok = false; break; }
+            const int sides = op.kind >> 1;
+            for (int t = 0; t < 2; ++t) {
+                if ((sides == 1 && t != 0) || (sides == 2 && t != 1)) continue;
+                if (F.kind[d][t] >= 0) { ok = false; break; } // two rules on one face: keep the launch order
+                F.kind[d][t] = op.kind & 1;
+                F.v[d][t] = op.v;
+            }
+        }
+    if (!ok) {
+        for (int d = 0; d < g.nd; ++d) launch_bc_dim(g, b[d], s);
+        return;
+    }
+    if (a.nf == 0) return;
+    int m1 = 1, m2 = 1;
+    for (int d = 0; d < g.nd; ++d) {
+        int e1 = -1, e2 = -1;
+        for (int e = 0; e < g.nd; ++e) {
+            if (e == d) continue;
+            if (e1 < 0) e1 = e; else e2 = e;
+        }
+        m1 = std::max(m1, g.n[e1] + 3);
+        m2 = std::max(m2, (e2 >= 0) ? g.n[e2] + 3 : 1);
+    }
+    dim3 blk(64, 4, 1);
+    dim3 grd((m1 + 63) / 64, (m2 + 3) / 4, 2 * g.nd * a.nf);
+    hipLaunchKernelGGL(k_bc_all, grd, blk, 0, s, a, g.n[0], g.n[1], g.n[2], g.sy, g.sz);
 }
 
 // Packed boundary conditions of many instances in one dimension: each op

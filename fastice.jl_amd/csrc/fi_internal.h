@@ -205,6 +205,8 @@ void launch_eta_update(const Geo& g, const StokesArgs<T>& a, hipStream_t s);
 
 // boundary conditions, norms, fills (bc_norm_kernels.hip)
 void launch_bc_dim(const Geo& g, const BcBatch& b, hipStream_t s);
+// the batches of every dimension (b[0..nd-1]) in one launch, same result
+void launch_bc_dims(const Geo& g, const BcBatch* b, hipStream_t s);
 
 // packed sweeps over many instances (batch.hip builds the tables)
 // visc false: every listed instance sweeps with frozen viscosity (FP32 only)

@@ -257,14 +257,15 @@ static HeatArgs heat_args(fi_heat_s* h, double dt)
 static void heat_value_bc(fi_heat_s* h, int p)
 {
     const Geo& g = h->g;
+    BcBatch b[3];
     for (int d = 0; d < g.nd; ++d) {
-        BcBatch b;
-        b.nop = 0;
-        b.d = d;
+        b[d].nop = 0;
+        b[d].d = d;
         for (int s = 0; s < 2; ++s)
-            if (h->desc.bc_kind[d][s] == FI_HEAT_BC_VALUE) add_side_op(b, h->f[p][0], 0, 0, h->desc.bc_value[d][s], s);
-        launch_bc_dim(g, b, h->stream);
+            if (h->desc.bc_kind[d][s] == FI_HEAT_BC_VALUE)
+                add_side_op(b[d], h->f[p][0], 0, 0, h->desc.bc_value[d][s], s);
     }
+    launch_bc_dims(g, b, h->stream);
 }
 
 static int heat_prepare(fi_heat_s* h)

@@ -421,7 +421,9 @@ BcBatch refine_bc_batch(fi_model_s* m, int group, int d)
 
 static void bc32(fi_model_s* m, int group)
 {
-    for (int d = 0; d < m->g.nd; ++d) launch_bc_dim(m->g, refine_bc_batch(m, group, d), m->stream);
+    BcBatch b[3];
+    for (int d = 0; d < m->g.nd; ++d) b[d] = refine_bc_batch(m, group, d);
+    launch_bc_dims(m->g, b, m->stream);
 }
 
 StokesArgs<float> refine_args(fi_model_s* m, bool in_place) { return make_args32(m, in_place); }
@@ -785,7 +787,9 @@ static int job_outer(fi_model_s* m, RefineJob& j)
         a.rh_rate = j.omega;
         launch_eta_update(m->g, a, m->stream);
         m->eta_cur ^= 1;
-        for (int d = 0; d < nd; ++d) launch_bc_dim(m->g, neumann_batch(m->eta(), L_CCC, d), m->stream);
+        BcBatch b[3];
+        for (int d = 0; d < nd; ++d) b[d] = neumann_batch(m->eta(), L_CCC, d);
+        launch_bc_dims(m->g, b, m->stream);
     }
     {
         // tau <- 2 eta edot'(V), then the stress boundary conditions the PT
@@ -858,7 +862,9 @@ static int job_outer_lag_enqueue(fi_model_s* m, RefineJob& j)
         if (rh.relax != FI_RELAX_NONE) a.rh_rate = 1.0 - std::pow(1.0 - rh.rate, (double)j.since_eta);
         launch_eta_update(m->g, a, m->stream);
         m->eta_cur ^= 1;
-        for (int d = 0; d < nd; ++d) launch_bc_dim(m->g, neumann_batch(m->eta(), L_CCC, d), m->stream);
+        BcBatch b[3];
+        for (int d = 0; d < nd; ++d) b[d] = neumann_batch(m->eta(), L_CCC, d);
+        launch_bc_dims(m->g, b, m->stream);
     }
     j.since_eta = 0;
     for (int id = FI_F_P; id <= FI_F_VZ; ++id)

@@ -92,6 +92,46 @@ def test_mixed_bcs_linear_field_2d(pkg, orc):
     model.close()
 
 
+@pytest.mark.parametrize("case", ["mixed3d", "noslip3d", "freeslip3d", "mixed2d"])
+def test_bc_groups_bitwise(pkg, orc, case):
+    """Every boundary-condition group, applied to a random state (ghosts
+    included), equals the oracle's x-then-y-then-z passes bit for bit, edge
+    and corner ghosts included."""
+    _say("case: bc groups bitwise, %s" % case)
+    if case.endswith("3d"):
+        grid = pkg.UniformGrid(origin=(0.0, 0.0, 0.0), extent=(1.0, 0.8, 0.6), dims=(9, 7, 5))
+        bcs = {"mixed3d": dict(x=(pkg.SBC(0.0, 0.1, -0.2), pkg.TBC(0.3, -0.1, 0.05)),
+                               y=(pkg.VBC(0.02, 0.0, -0.01), pkg.SBC(0.0, 0.2, 0.1)),
+                               z=(pkg.VBC(0.1, -0.3, 0.0), pkg.TBC(0.0, 0.4, 0.0))),
+               "noslip3d": dict(x=(pkg.VBC(0.1, 0.2, 0.3), pkg.VBC(-0.1, 0.0, 0.5)),
+                                y=(pkg.VBC(0.0, 0.7, 0.0), pkg.VBC(0.2, 0.0, -0.4)),
+                                z=(pkg.VBC(0.3, 0.0, 0.0), pkg.VBC(0.0, -0.6, 0.1))),
+               "freeslip3d": dict(x=(pkg.SBC(0.0, 0.0, 0.0), pkg.SBC(0.0, 0.0, 0.0)),
+                                  y=(pkg.SBC(0.0, 0.0, 0.0), pkg.SBC(0.0, 0.0, 0.0)),
+                                  z=(pkg.SBC(0.0, 0.0, 0.0), pkg.TBC(0.0, 0.0, 0.0)))}[case]
+        nd = 3
+    else:
+        grid = pkg.UniformGrid(origin=(0.0, 0.0), extent=(1.0, 0.5), dims=(11, 6))
+        bcs = dict(x=(pkg.VBC(0.3, -0.2), pkg.SBC(0.0, 0.1)), y=(pkg.TBC(0.2, 0.0), pkg.VBC(0.0, 0.4)))
+        nd = 2
+    kw = dict(grid=grid, boundary_conditions=bcs, gravity=dict(x=0.0, y=1.0), rheology=pkg.LinearViscousRheology(1.0),
+              solver_params=pkg.pt_params(grid, r=0.7, re_mech=2 * math.pi))
+    model, ref = make_pair(pkg, orc, kw, seed=11, amp=1.0)
+    rng = np.random.default_rng(5)
+    for name in ("ETA", "ETA_NEXT"):
+        a = np.asfortranarray(1.0 + rng.random(grid.parent_shape()))
+        model.field(name).set(a)
+        ref.set(name, a)
+    names = state_names(nd) + ["ETA", "ETA_NEXT"]
+    for group in ("stress", "velocity", "eta"):
+        model.apply_bc(group)
+        ref.apply_bc(group)
+        for name in names:
+            a, b = model.field(name).parent(), ref.get(name)
+            assert np.array_equal(a, b), (group, name, float(np.max(np.abs(a - b))))
+    model.close()
+
+
 def test_nonfinite_is_reported(pkg, orc):
     _say("case: NaN injected into one pressure cell")
     kw, cad = pkg.manufactured_3d(12)
