@@ -28,6 +28,8 @@ Workloads
       2000 PT iterations timed, then a converged solve to residual 1e-8
       (ncheck 255) with its error norms against the closed form; value =
       T_eff of the iteration; the CPU oracle timed on a sample of iterations.
+  refined_solve_255x255x127, fp64_solve_255x255x127: one of the two solves
+      of the default line's converged-solve comparison, alone; value = seconds.
   serving_poisson40: Poisson arrivals at 40/s, half 2D glacier sections
       (n x n/2, n in 64..511), half 3D glaciers (n x n x n/2, n in 32..127),
       FP64 to 1e-8 or niter = 50n, continuously batched; value = completed
@@ -64,6 +66,8 @@ WORKLOADS = {
     "h5_glacier_255x255x127": None,
     "heat_diffusion": None,
     "manufactured3d_255": None,
+    "refined_solve_255x255x127": None,
+    "fp64_solve_255x255x127": None,
 }
 
 
@@ -361,6 +365,34 @@ def bench_h5(pkg, args):
     print(json.dumps(line), flush=True)
 
 
+def bench_glacier_solve(pkg, args, precision):
+    """One solve of the 255x255x127 glacier to FP64 residual 1e-8, alone:
+    FP32-refined (lagged PT) or plain FP64, the two solves the default
+    line's converged-solve comparison times one after the other (the
+    workloads the solve profiles cover)."""
+    dims = (255, 255, 127)
+    model, cad = pkg.glacier_3d(dims, seed=1, glen=True)
+    model.synchronize()
+    t0 = time.perf_counter()
+    if precision == "refined":
+        outer, sweeps, res = model.refined_solve(1e-8)
+        counts = dict(model.refined_counts)
+        info = {"outer_steps": outer, "fp32_sweeps": sweeps, "fp32_nonlinear_phase_sweeps": counts["nonlinear_phase"]}
+        dtype = "f32 sweeps, f64 outer steps"
+    else:
+        it, res = model.solve(400 * max(dims), cad["ncheck"], 1e-8)
+        info = {"fp64_iterations": it}
+        dtype = "f64"
+    model.synchronize()
+    el = time.perf_counter() - t0
+    model.close()
+    log("%s solve: %s, %.3f s, residual %.3e" % (precision, info, el, max(res)))
+    cfg = {"workload": "%s_solve_255x255x127" % precision, "grid": list(dims), "tol": 1e-8,
+           "residual": float(max(res))}
+    cfg.update(info)
+    print(json.dumps(_line("converged_solve_time", el, "s", el * 1e3, 1, 0, dtype, cfg, higher=False)), flush=True)
+
+
 def bench_manufactured(pkg, args):
     """The 3D manufactured free-slip solution at 255^3 in FP64: 2000 PT
     iterations (T_eff), then a solve to residual 1e-8 checked every 255
@@ -581,6 +613,8 @@ def main():
         return bench_heat(pkg, args)
     if args.workload == "manufactured3d_255":
         return bench_manufactured(pkg, args)
+    if args.workload in ("refined_solve_255x255x127", "fp64_solve_255x255x127"):
+        return bench_glacier_solve(pkg, args, args.workload.split("_")[0])
     dims = WORKLOADS[args.workload]
     cells = dims[0] * dims[1] * dims[2]
     glacier = args.workload.startswith("glacier")

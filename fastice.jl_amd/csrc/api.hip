@@ -455,7 +455,7 @@ size_t fi_model_bytes(const fi_model_desc* desc)
     size_t bytes = fields * g.elems * sizeof(double) + 4 * sizeof(unsigned long long);
     if (desc->immersed_boundary) bytes += g.elems;
     if (desc->immersed_boundary && g.nd == 3)
-        bytes += (2 * col_ranges_count(g) + tile_march_count(g) + 1) * sizeof(int2);
+        bytes += (2 * col_ranges_count(g) + tile_march_count(g) + 3) * sizeof(int2);
     return bytes;
 }
 
@@ -510,7 +510,7 @@ int fi_model_create(const fi_model_desc* desc, fi_model_t* out)
             const size_t nc = col_ranges_count(g);
             if (hipMalloc(&m->crange, nc * sizeof(int2)) != hipSuccess ||
                 hipMalloc(&m->erange, nc * sizeof(int2)) != hipSuccess ||
-                hipMalloc(&m->tmarch, (tile_march_count(g) + 1) * sizeof(int2)) != hipSuccess) {
+                hipMalloc(&m->tmarch, (tile_march_count(g) + 3) * sizeof(int2)) != hipSuccess) {
                 free_model(m);
                 return fail(FI_ERR_OOM, "device allocation of the column ranges failed");
             }

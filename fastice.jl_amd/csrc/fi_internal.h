@@ -187,7 +187,7 @@ void launch_col_ranges(const Geo& g, const uint8_t* flags, int2* own, int2* eta,
 inline size_t col_ranges_count(const Geo& g) { return (size_t)g.sy * (size_t)(g.n[1] + 3); }
 // tiles of any fused launch over g (64- or 66-lane tiles, at least 2 rows
 // and 1 plane per tile): the capacity of StokesArgs::tmarch (whose
-// allocation holds one more 8-byte slot, for k_eta_target's scalar)
+// allocation holds three more 8-byte slots, for k_eta_target's scalars)
 inline size_t tile_march_count(const Geo& g)
 {
     return (size_t)((g.n[0] + 61) / 62 + 2) * (size_t)((g.n[1] + 1) / 2 + 1) * (size_t)(g.n[2] + 1);

@@ -161,6 +161,9 @@ __device__ __forceinline__ T rheo_relax(const StokesArgs<T>& a, T st, T old)
     return pow(T(10), log10(old) * w + st * a.rh_rate);
 }
 
+__device__ __forceinline__ bool same_bits(double x, double y) { return __double_as_longlong(x) == __double_as_longlong(y); }
+__device__ __forceinline__ bool same_bits(float x, float y) { return __float_as_uint(x) == __float_as_uint(y); }
+
 template <typename T>
 __device__ __forceinline__ T rheology_o(const StokesArgs<T>& a, uint32_t bo, T tII, T old)
 {
@@ -1002,17 +1005,34 @@ __global__ __launch_bounds__(256) void k_eta_rest(StokesArgs<T> a, FusedGeom fg,
     const int2 r = a.tmarch[bx + fg.nbx * (by + nby * bz)];
     if (k >= r.x && k <= r.y) return;
     const uint32_t bo = (uint32_t)((i + a.sy * j + a.sz * k) * (long)sizeof(T));
-    // a constant-coefficient rheology has one target at tau_II = 0: its scaled
-    // value comes from k_eta_target (the same operations, once)
-    const T st = a.rh ? rheo_scaled(a, rheo_target(a, bo, T(0))) : *st0;
-    sto(a.eta_next, bo, rheo_relax(a, st, ldo(a.eta, bo)));
+    const T e = ldo(a.eta, bo);
+    T en;
+    if (a.rh) {
+        en = rheo_relax(a, rheo_scaled(a, rheo_target(a, bo, T(0))), e);
+    } else {
+        // a constant-coefficient rheology has one target at tau_II = 0, and
+        // cells that have only ever been relaxed towards it from the same
+        // start hold the same viscosity: k_eta_target relaxed one such value
+        // (st0[1] -> st0[2]) with the same operations, so a cell holding
+        // exactly its bits takes the result and skips the log and exp
+        en = same_bits(e, st0[1]) ? st0[2] : rheo_relax(a, st0[0], e);
+    }
+    sto(a.eta_next, bo, en);
 }
 
-// the scaled target viscosity at tau_II = 0 of a constant-coefficient rheology
+// st0[0]: the scaled target viscosity at tau_II = 0 of a constant-coefficient
+// rheology; st0[1], st0[2]: the viscosity of the cell at byte offset probe
+// and its relaxation towards that target
 template <typename T>
-__global__ void k_eta_target(StokesArgs<T> a, T* st0)
+__global__ void k_eta_target(StokesArgs<T> a, T* st0, uint32_t probe)
 {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *st0 = rheo_scaled(a, rheo_target(a, 0u, T(0)));
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        const T st = rheo_scaled(a, rheo_target(a, 0u, T(0)));
+        const T e = ldo(a.eta, probe);
+        st0[0] = st;
+        st0[1] = e;
+        st0[2] = rheo_relax(a, st, e);
+    }
 }
 
 // Tile order.  Workgroups are dispatched round-robin over the 8 XCDs
@@ -1784,9 +1804,12 @@ void launch_fused(const Geo& g, const StokesArgs<T>& a, const FusedGeom& fg0, bo
     // a narrowed immersed-boundary sweep with a viscosity update leaves the
     // viscosity of the planes it did not march to k_eta_rest
     if (ib && !a.frozen_eta && a.z_wsig && a.z_wvel && a.tmarch) {
-        // one scalar after the tile entries of tmarch holds the scaled target
+        // three scalars after the tile entries of tmarch: the scaled target
+        // and the relaxation of the viscosity of a cell on the region's top
+        // plane (in a glacier box, air: empty space)
         T* st0 = reinterpret_cast<T*>(a.tmarch + tile_march_count(g));
-        if (!a.rh) hipLaunchKernelGGL((k_eta_target<T>), dim3(1), dim3(64), 0, s, a, st0);
+        const uint32_t probe = (uint32_t)((fg.lo[0] + g.sy * fg.lo[1] + g.sz * fg.hi[2]) * (long)sizeof(T));
+        if (!a.rh) hipLaunchKernelGGL((k_eta_target<T>), dim3(1), dim3(64), 0, s, a, st0, probe);
         dim3 grd((fg.hi[0] - fg.lo[0] + 64) / 64, (fg.hi[1] - fg.lo[1] + 4) / 4, fg.hi[2] - fg.lo[2] + 1);
         hipLaunchKernelGGL((k_eta_rest<T>), grd, dim3(64, 4, 1), 0, s, a, fg, tx, ty, st0);
     }

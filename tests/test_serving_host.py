@@ -46,8 +46,8 @@ def _expected_bytes(dims, nfields_single, immersed):
         b += elems
         if len(n) == 3:
             b += 2 * sy * (n[1] + 3) * 8  # per-column plane ranges (two int2 per column)
-            # the z range each tile of an FP64 fused sweep marched (int2 per tile), and one scalar
-            b += (((n[0] + 61) // 62 + 2) * ((n[1] + 1) // 2 + 1) * (n[2] + 1) + 1) * 8
+            # the z range each tile of an FP64 fused sweep marched (int2 per tile), and three scalars
+            b += (((n[0] + 61) // 62 + 2) * ((n[1] + 1) // 2 + 1) * (n[2] + 1) + 3) * 8
     return b
 
 
