@@ -1,10 +1,12 @@
 // bc_norm_kernels.hip -- boundary-condition face kernels, Inf-norm reductions
 // and fills.
 //
-// Boundary conditions follow Chmy's bc! as this library fixes it: one launch
+// Boundary conditions follow Chmy's bc! as this library fixes it: one pass
 // per dimension (x, then y, then z), covering both sides and every field of
 // the batch, over the field's full padded extent in the other dimensions
 // (so edge and corner ghosts end up as functions of interior values only).
+// k_bc_dim runs one pass per launch; k_bc_all gives the result of all of
+// them in one launch.
 //   Dirichlet(v), Vertex in the normal dim: boundary node = v
 //   Dirichlet(v), Center in the normal dim: ghost = 2v - first interior
 //   Neumann(),   Center: ghost = first interior
