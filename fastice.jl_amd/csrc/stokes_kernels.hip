@@ -721,6 +721,14 @@ __device__ __forceinline__ void fused3_ib_tile(const StokesArgs<T>& a, const Fus
     if constexpr (VISC && ETA_HERE) {
         if (narrow && need_raw) {
             if (inner) {
+                // With a constant-coefficient rheology the result depends on
+                // the old viscosity alone, and the unmarched planes of a
+                // column (empty space) mostly hold the same one: the last
+                // input and result are kept, and a cell holding the same bits
+                // takes the result without a log and an exp.
+                const bool memo = a.rh == nullptr;
+                T m_in = T(0), m_out = T(0);
+                bool m_set = false;
                 auto eta_only = [&](int k0, int k1) {
                     for (int k = k0; k <= k1; k += 4) {
                         T o[4];
@@ -731,7 +739,16 @@ __device__ __forceinline__ void fused3_ib_tile(const StokesArgs<T>& a, const Fus
                         for (int u = 0; u < 4; ++u)
                             if (k + u <= k1) {
                                 const uint32_t cu = col + bsz * (uint32_t)(k + u);
-                                sto(a.eta_next, cu, rheology_o(a, cu, T(0), o[u]));
+                                T en;
+                                if (memo && m_set && same_bits(o[u], m_in)) {
+                                    en = m_out;
+                                } else {
+                                    en = rheology_o(a, cu, T(0), o[u]);
+                                    m_in = o[u];
+                                    m_out = en;
+                                    m_set = true;
+                                }
+                                sto(a.eta_next, cu, en);
                             }
                     }
                 };
