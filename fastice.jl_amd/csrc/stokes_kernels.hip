@@ -1009,32 +1009,52 @@ __device__ __forceinline__ void fused3_ib_tile(const StokesArgs<T>& a, const Fus
 // fg.kch) recorded its march [ks, ke] in a.tmarch; a cell of its chunk
 // outside it gets eta_next = rheology(tau_II = 0, eta), as the sweep would
 // have given it (every stress its tau_II averages is at an invalid location).
+// One thread per column and z chunk (blockIdx.z) walks the chunk's planes
+// outside the march of its tile.
+template <typename T>
+__device__ __forceinline__ T eta_rest_cell(const StokesArgs<T>& a, const T* st0, uint32_t bo, T e)
+{
+    if (a.rh) return rheo_relax(a, rheo_scaled(a, rheo_target(a, bo, T(0))), e);
+    // a constant-coefficient rheology has one target at tau_II = 0, and cells
+    // that have only ever been relaxed towards it from the same start hold
+    // the same viscosity: k_eta_target relaxed one such value (st0[1] ->
+    // st0[2]) with the same operations, so a cell holding exactly its bits
+    // takes the result and skips the log and exp
+    return same_bits(e, st0[1]) ? st0[2] : rheo_relax(a, st0[0], e);
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_eta_rest(StokesArgs<T> a, FusedGeom fg, int tx, int ty, const T* st0)
 {
     const int i = fg.lo[0] + (int)(blockIdx.x * blockDim.x + threadIdx.x);
     const int j = fg.lo[1] + (int)(blockIdx.y * blockDim.y + threadIdx.y);
-    const int k = fg.lo[2] + (int)blockIdx.z;
+    const int bz = (int)blockIdx.z;
     if (i > fg.hi[0] || j > fg.hi[1]) return;
     const int bx = (fg.nx0 && i < fg.xa) ? 0 : fg.nx0 + (i - fg.xa) / (tx - 2);
-    const int by = (j - fg.lo[1]) / (ty - 2), bz = (k - fg.lo[2]) / fg.kch;
+    const int by = (j - fg.lo[1]) / (ty - 2);
     const int nby = (fg.hi[1] - fg.lo[1] + 1 + ty - 3) / (ty - 2);
     const int2 r = a.tmarch[bx + fg.nbx * (by + nby * bz)];
-    if (k >= r.x && k <= r.y) return;
-    const uint32_t bo = (uint32_t)((i + a.sy * j + a.sz * k) * (long)sizeof(T));
-    const T e = ldo(a.eta, bo);
-    T en;
-    if (a.rh) {
-        en = rheo_relax(a, rheo_scaled(a, rheo_target(a, bo, T(0))), e);
+    const int k0 = fg.lo[2] + bz * fg.kch, k1 = min(k0 + fg.kch - 1, fg.hi[2]);
+    const uint32_t col = (uint32_t)((i + a.sy * j) * (long)sizeof(T)), bsz = (uint32_t)(a.sz * (long)sizeof(T));
+    auto run = [&](int ka, int kb) {
+        for (int k = ka; k <= kb; k += 4) {
+            T e[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) e[u] = k + u <= kb ? ldo(a.eta, col + bsz * (uint32_t)(k + u)) : T(1);
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (k + u <= kb) {
+                    const uint32_t bo = col + bsz * (uint32_t)(k + u);
+                    sto(a.eta_next, bo, eta_rest_cell(a, st0, bo, e[u]));
+                }
+        }
+    };
+    if (r.x > r.y) {
+        run(k0, k1);
     } else {
-        // a constant-coefficient rheology has one target at tau_II = 0, and
-        // cells that have only ever been relaxed towards it from the same
-        // start hold the same viscosity: k_eta_target relaxed one such value
-        // (st0[1] -> st0[2]) with the same operations, so a cell holding
-        // exactly its bits takes the result and skips the log and exp
-        en = same_bits(e, st0[1]) ? st0[2] : rheo_relax(a, st0[0], e);
+        run(k0, min(k1, r.x - 1));
+        run(max(k0, r.y + 1), k1);
     }
-    sto(a.eta_next, bo, en);
 }
 
 // st0[0]: the scaled target viscosity at tau_II = 0 of a constant-coefficient
@@ -1827,7 +1847,8 @@ void launch_fused(const Geo& g, const StokesArgs<T>& a, const FusedGeom& fg0, bo
         T* st0 = reinterpret_cast<T*>(a.tmarch + tile_march_count(g));
         const uint32_t probe = (uint32_t)((fg.lo[0] + g.sy * fg.lo[1] + g.sz * fg.hi[2]) * (long)sizeof(T));
         if (!a.rh) hipLaunchKernelGGL((k_eta_target<T>), dim3(1), dim3(64), 0, s, a, st0, probe);
-        dim3 grd((fg.hi[0] - fg.lo[0] + 64) / 64, (fg.hi[1] - fg.lo[1] + 4) / 4, fg.hi[2] - fg.lo[2] + 1);
+        const int nbz = (fg.hi[2] - fg.lo[2] + fg.kch) / fg.kch;
+        dim3 grd((fg.hi[0] - fg.lo[0] + 64) / 64, (fg.hi[1] - fg.lo[1] + 4) / 4, nbz);
         hipLaunchKernelGGL((k_eta_rest<T>), grd, dim3(64, 4, 1), 0, s, a, fg, tx, ty, st0);
     }
 }
