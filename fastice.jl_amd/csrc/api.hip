@@ -170,7 +170,7 @@ BcBatch neumann_batch(double* p, int L, int d)
     return b;
 }
 
-void run_bc(fi_model_s* m, int group)
+void run_bc(fi_model_s* m, int group, const int2* skip_ranges)
 {
     BcBatch b[3];
     for (int d = 0; d < m->g.nd; ++d) {
@@ -179,7 +179,7 @@ void run_bc(fi_model_s* m, int group)
         else if (group == FI_BCGROUP_ETA) b[d] = neumann_batch(m->eta(), L_CCC, d);
         else b[d] = residual_batch(m, d);
     }
-    launch_bc_dims(m->g, b, m->stream);
+    launch_bc_dims(m->g, b, m->stream, skip_ranges);
 }
 
 int prepare(fi_model_s* m, bool sync_shadow)
@@ -193,6 +193,7 @@ int prepare(fi_model_s* m, bool sync_shadow)
                                                 hipMemcpyDeviceToDevice, m->stream));
         m->zs[1 - m->sig_cur] = m->zs[m->sig_cur];
         m->zv[1 - m->v_cur] = m->zv[m->v_cur];
+        m->gs[0] = m->gs[1] = false;
         m->shadow_dirty = false;
     }
     if (m->desc.immersed_boundary && m->flags_dirty) {
@@ -207,6 +208,7 @@ int prepare(fi_model_s* m, bool sync_shadow)
         // locations may have changed validity: no buffer is known to be clean
         m->zs[0] = m->zs[1] = m->zv[0] = m->zv[1] = false;
         m->zs32[0] = m->zs32[1] = m->zv32[0] = m->zv32[1] = false;
+        m->gs[0] = m->gs[1] = m->gs32[0] = m->gs32[1] = false;
         m->flags_dirty = false;
     }
     return FI_OK;
@@ -236,7 +238,11 @@ void iterate(fi_model_s* m, hipEvent_t* ev)
     if (!(fused && m->fused_shell)) launch_sigma_boxes(m->g, a, boxes, ib, m->stream);
     m->sig_cur ^= 1;
     if (ev) (void)hipEventRecord(ev[2], m->stream);
-    run_bc(m, FI_BCGROUP_STRESS);
+    // the sweep stored nothing at invalid locations (z_wsig): the ghosts the
+    // last application on this buffer settled still hold their value
+    const bool skip = ib && m->g.nd == 3 && m->crange && a.z_wsig && m->gs[so];
+    run_bc(m, FI_BCGROUP_STRESS, skip ? m->crange : nullptr);
+    if (ib && m->g.nd == 3) m->gs[so] = a.z_wsig != 0;
     if (ev) (void)hipEventRecord(ev[3], m->stream);
     launch_velocity_boxes(m->g, a, boxes, ib, m->stream);
     m->v_cur ^= 1;

@@ -201,6 +201,7 @@ struct SweepOf<double> {
     {
         if (n >= 1) m->zs[1 - p0.sig] = m->zv[1 - p0.v] = true;
         if (n >= 2) m->zs[p0.sig] = m->zv[p0.v] = true;
+        m->gs[0] = m->gs[1] = false;
     }
     static BcBatch bc(fi_model_s* m, int group, int d)
     {
@@ -233,6 +234,7 @@ struct SweepOf<float> {
     {
         if (n >= 1) m->zs32[1 - p0.sig] = m->zv32[1 - p0.v] = true;
         if (n >= 2) m->zs32[p0.sig] = m->zv32[p0.v] = true;
+        m->gs32[0] = m->gs32[1] = false;
     }
     static BcBatch bc(fi_model_s* m, int group, int d)
     {

@@ -100,6 +100,7 @@ struct BcField {
 struct BcAll {
     BcField f[8];
     int nf, nd;
+    const int2* skip_ranges; // see launch_bc_dims; null: nothing is skipped
 };
 
 __device__ __forceinline__ int bc_target(int kind, int vert, int s, int nn)
@@ -177,15 +178,29 @@ __global__ __launch_bounds__(256) void k_bc_all(BcAll b, int n0, int n1, int n2,
             const int ke = F.kind[e][t];
             if (ke >= 0 && p[e] == bc_target(ke, (F.L >> e) & 1, t, n[e])) return;
         }
+    // an x-face ghost whose point and source lie on planes of their columns
+    // without a valid location: the sweep stored nothing at either since the
+    // last application, which wrote the value a +0 source gives (a
+    // Dirichlet vertex has no source)
+    if (b.skip_ranges && d == 0) {
+        const int2 rt = b.skip_ranges[p[0] + sy * p[1]];
+        bool inv = p[2] < rt.x || p[2] > rt.y;
+        if (inv && !(k == 0 && (F.L & 1))) {
+            const int2 rs = b.skip_ranges[bc_source(F.L & 1, s, n0) + sy * p[1]];
+            inv = p[2] < rs.x || p[2] > rs.y;
+        }
+        if (inv) return;
+    }
     if (F.f32) bc_chain<float>(F, b.nd, d, n, st, p);
     else bc_chain<double>(F, b.nd, d, n, st, p);
 }
 
-void launch_bc_dims(const Geo& g, const BcBatch* b, hipStream_t s)
+void launch_bc_dims(const Geo& g, const BcBatch* b, hipStream_t s, const int2* skip_ranges)
 {
     BcAll a;
     a.nf = 0;
     a.nd = g.nd;
+    a.skip_ranges = g.nd == 3 ? skip_ranges : nullptr;
     bool ok = true;
     for (int d = 0; d < g.nd && ok; ++d)
         for (int o = 0; o < b[d].nop && ok; ++o) {

@@ -205,8 +205,11 @@ void launch_eta_update(const Geo& g, const StokesArgs<T>& a, hipStream_t s);
 
 // boundary conditions, norms, fills (bc_norm_kernels.hip)
 void launch_bc_dim(const Geo& g, const BcBatch& b, hipStream_t s);
-// the batches of every dimension (b[0..nd-1]) in one launch, same result
-void launch_bc_dims(const Geo& g, const BcBatch* b, hipStream_t s);
+// the batches of every dimension (b[0..nd-1]) in one launch, same result;
+// skip_ranges (3D, a stress group): the per-column plane ranges of the
+// validity bits (StokesArgs::crange); the x-face ghosts whose point and
+// source lie outside them already hold their value and are skipped
+void launch_bc_dims(const Geo& g, const BcBatch* b, hipStream_t s, const int2* skip_ranges = nullptr);
 
 // packed sweeps over many instances (batch.hip builds the tables)
 // visc false: every listed instance sweeps with frozen viscosity (FP32 only)

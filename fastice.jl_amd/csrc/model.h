@@ -49,6 +49,14 @@ struct fi_model_s {
     // the same for the two FP32 state buffers of the refined solver
     bool zs32[2] = {false, false};
     bool zv32[2] = {false, false};
+    // gs[b]: the stress boundary conditions last applied to physical stress
+    // buffer b left every x-face ghost whose point and source are invalid
+    // locations at the value its rule gives a +0 source, and nothing has
+    // written there since, so the next application may skip those ghosts
+    // (set after an application that follows a sweep; anything that clears
+    // zs clears it); gs32: the same for the FP32 stress buffers
+    bool gs[2] = {false, false};
+    bool gs32[2] = {false, false};
     int mask_gen = 0; // bumped whenever a mask component changes
     int use_fused = 1;
     int refine_full_phase = 1;      // nonlinear refined solves start with a full FP32 PT phase
@@ -79,6 +87,7 @@ struct fi_model_s {
     {
         shadow_dirty = true;
         zs[0] = zs[1] = zv[0] = zv[1] = false;
+        gs[0] = gs[1] = gs32[0] = gs32[1] = false;
     }
     static bool is_sig(int id) { return id >= FI_F_P && id <= FI_F_TYZ; }
     static bool is_vel(int id) { return id >= FI_F_VX && id <= FI_F_VZ; }
@@ -107,7 +116,8 @@ namespace fi {
 StokesArgs<double> make_args(fi_model_s* m, bool in_place);
 BcBatch stress_batch(fi_model_s* m, int d);
 BcBatch velocity_batch(fi_model_s* m, int d, bool with_eta);
-void run_bc(fi_model_s* m, int group);
+// skip_ranges: the column plane ranges of a stress group whose settled ghosts may be skipped (gs)
+void run_bc(fi_model_s* m, int group, const int2* skip_ranges = nullptr);
 // before kernels run: the immersed-boundary validity bits and, with
 // sync_shadow, the ping-pong copies the FP64 iteration needs
 int prepare(fi_model_s* m, bool sync_shadow = true);

@@ -333,6 +333,7 @@ static int ensure_refine(fi_model_s* m)
     if (nd == 3 && m->g.elems * 8 < (1ull << 32)) r->has_interior = fused_split(m->g, 16, m->kch, r->fg, r->shell);
     // the allocations are cleared on the model's stream
     m->zs32[0] = m->zs32[1] = m->zv32[0] = m->zv32[1] = true;
+    m->gs32[0] = m->gs32[1] = false;
     return FI_OK;
 }
 
@@ -419,11 +420,11 @@ BcBatch refine_bc_batch(fi_model_s* m, int group, int d)
     return b;
 }
 
-static void bc32(fi_model_s* m, int group)
+static void bc32(fi_model_s* m, int group, const int2* skip_ranges = nullptr)
 {
     BcBatch b[3];
     for (int d = 0; d < m->g.nd; ++d) b[d] = refine_bc_batch(m, group, d);
-    launch_bc_dims(m->g, b, m->stream);
+    launch_bc_dims(m->g, b, m->stream, skip_ranges);
 }
 
 StokesArgs<float> refine_args(fi_model_s* m, bool in_place) { return make_args32(m, in_place); }
@@ -479,7 +480,11 @@ static void iterate32(fi_model_s* m, hipEvent_t* ev)
     if (!(fused && m->fused_shell)) launch_sigma_boxes(m->g, a, boxes, ib, m->stream);
     r->sig_cur ^= 1;
     if (ev) (void)hipEventRecord(ev[2], m->stream);
-    bc32(m, FI_BCGROUP_STRESS);
+    // the sweep stored nothing at invalid locations (z_wsig): the ghosts the
+    // last application on this buffer settled still hold their value
+    const bool skip = ib && m->g.nd == 3 && m->crange && a.z_wsig && m->gs32[so];
+    bc32(m, FI_BCGROUP_STRESS, skip ? m->crange : nullptr);
+    if (ib && m->g.nd == 3) m->gs32[so] = a.z_wsig != 0;
     if (ev) (void)hipEventRecord(ev[3], m->stream);
     launch_velocity_boxes(m->g, a, boxes, ib, m->stream);
     r->v_cur ^= 1;
@@ -730,6 +735,7 @@ static void phase1_begin(fi_model_s* m)
         }
     r->sig_cur = r->v_cur = 0;
     m->zs32[0] = m->zs32[1] = m->zv32[0] = m->zv32[1] = false;
+    m->gs32[0] = m->gs32[1] = false;
     cast(m, r->eta, m->eta(), 1.0);
     cast(m, r->eta2, m->eta_next(), 1.0);
     r->eta_cur = 0;
@@ -837,6 +843,7 @@ static int job_outer(fi_model_s* m, RefineJob& j)
         }
     r->sig_cur = r->v_cur = 0;
     m->zs32[0] = m->zs32[1] = m->zv32[0] = m->zv32[1] = true;
+    m->gs32[0] = m->gs32[1] = false;
     j.state = JOB_INNER;
     j.done = 0;
     next_chunk(j);
@@ -875,6 +882,7 @@ static int job_outer_lag_enqueue(fi_model_s* m, RefineJob& j)
     r->sig_cur = r->v_cur = 0;
     // the lag written below is +0 at every invalid location, like the fill
     m->zs32[0] = m->zs32[1] = m->zv32[0] = m->zv32[1] = true;
+    m->gs32[0] = m->gs32[1] = false;
     {
         StokesArgs<double> a = make_args(m, true);
         const int tid[6] = {FI_F_TXX, FI_F_TYY, FI_F_TZZ, FI_F_TXY, FI_F_TXZ, FI_F_TYZ};
