@@ -452,6 +452,57 @@ def test_ib_sweep_skips_empty_space_bitwise(pkg, orc):
     b.close()
 
 
+def test_ib_skipping_nonzero_traction_bitwise(pkg, orc):
+    """Empty-space skipping with prescribed non-zero tractions and slip on
+    the box faces, whose ghosts in empty space hold non-zero values: once a
+    stress buffer's x-face ghosts are settled, the boundary conditions skip
+    those whose point and source lie in empty space.  From a random state,
+    fused and pointwise sweeps agree bit for bit, and with the oracle (ghosts
+    included), over the iterations that skip nothing, skip reads, and skip
+    both stores and settled ghosts."""
+    _say("case: empty-space skipping, 40x32x16 glacier, non-zero traction and slip faces")
+    from fastice_jl_amd.problems import glacier_3d_spec
+    from fastice_jl_amd.level_sets import compute_levelset_from_dem, invert_levelset, omega_from_psi
+    kw, cad = glacier_3d_spec((40, 32, 16), seed=6, glen=False)
+    kw["boundary_conditions"] = dict(x=(pkg.TBC(0.1, -0.2, 0.3), pkg.TBC(-0.4, 0.05, 0.2)),
+                                     y=(pkg.SBC(0.0, 0.3, -0.1), pkg.TBC(0.2, 0.1, -0.3)),
+                                     z=(pkg.TBC(0.0, 0.25, 0.15), pkg.TBC(0.3, -0.2, 0.1)))
+    gl, dgrid, grid = cad["glacier"], cad["dem_grid"], kw["grid"]
+    ms = []
+    for fused in (1, 0):
+        m = pkg.IsothermalImmersedBoundaryFullStokesModel(**kw)
+        compute_levelset_from_dem(m.level_sets.na, gl.surface, dgrid, grid)
+        compute_levelset_from_dem(m.level_sets.ns, gl.bed, dgrid, grid)
+        invert_levelset(m.level_sets.ns)
+        omega_from_psi(m.field_masks.ns, m.level_sets.ns)
+        omega_from_psi(m.field_masks.na, m.level_sets.na)
+        m.set_option("fused", fused)
+        ms.append(m)
+    a, b = ms
+    ref = orc.OracleModel(a.desc)
+    for ph, base in (("NA", pkg._abi.FI_F_OM_NA0), ("NS", pkg._abi.FI_F_OM_NS0)):
+        for q in range(8):
+            ref.set("OM_%s%d" % (ph, q), pkg.full_stokes.DeviceField(a, base + q).parent())
+    rng = np.random.default_rng(23)
+    shape = grid.parent_shape()
+    for n in state_names(3) + ["ETA", "ETA_NEXT"]:
+        x = np.asfortranarray((1.0 + 0.1 * rng.random(shape)) if n.startswith("ETA") else 1e-2 * rng.standard_normal(shape))
+        for m in (a, b):
+            m.field(n).set(x)
+        ref.set(n, x)
+    for step, n_it in enumerate((1, 1, 1, 1, 4)):
+        for m in (a, b, ref):
+            m.advance(n_it)
+        for name in state_names(3) + ["ETA"]:
+            fa = a.field(name).parent()
+            assert np.all(np.isfinite(fa)), (step, name)
+            assert np.array_equal(fa, b.field(name).parent()), (step, name)
+        worst, bad = compare_state(a, ref, state_names(3) + ["ETA"], TOL)
+        assert not bad, (step, worst)
+    a.close()
+    b.close()
+
+
 @pytest.mark.parametrize("law", ["powerlaw_A_field", "linear_eta_field"])
 def test_ib_skipping_field_rheology_bitwise(pkg, orc, law):
     """Empty-space skipping with a rheology whose coefficient is a field: the
