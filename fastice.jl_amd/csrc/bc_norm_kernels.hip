@@ -13,8 +13,6 @@
 //   Neumann(),   Vertex: ghost = mirror of the node next to the boundary node
 #include "fi_internal.h"
 
-#include <algorithm>
-
 namespace fi {
 
 template <typename T>
@@ -97,10 +95,18 @@ struct BcField {
     int kind[3][2]; // -1: pass d does not write side s; else 0 Dirichlet, 1 Neumann
     double v[3][2];
 };
+// one face of one field: pass d, side s, ext1 x ext2 points, workgroups from start
+struct BcSlice {
+    int fi, d, s;
+    int ext1, ext2;
+    int start;
+};
 struct BcAll {
     BcField f[8];
     int nf, nd;
     const int2* skip_ranges; // see launch_bc_dims; null: nothing is skipped
+    BcSlice sl[48];
+    int nsl;
 };
 
 __device__ __forceinline__ int bc_target(int kind, int vert, int s, int nn)
@@ -115,84 +121,133 @@ __device__ __forceinline__ int bc_source(int vert, int s, int nn)
     return s == 0 ? 1 : nn;
 }
 
-template <typename T>
-__device__ __forceinline__ void bc_chain(const BcField& F, int nd, int d, const int n[3], const long st[3], int p[3])
+// Points as three scalars selected by dimension (no run-time indexed arrays,
+// which would live in scratch memory).
+struct BcPt {
+    int x, y, z;
+    __device__ __forceinline__ int at(int e) const { return e == 0 ? x : e == 1 ? y : z; }
+    __device__ __forceinline__ void set(int e, int v)
+    {
+        if (e == 0) x = v;
+        else if (e == 1) y = v;
+        else z = v;
+    }
+};
+
+// The chain of the point p of pass d (see above): the face rules met (bit
+// 2e: pass e applies a rule, bit 2e+1: its side), the point the value starts
+// from, or v at a Dirichlet vertex.
+struct BcChain {
+    unsigned rules;
+    bool from_v;
+    double v0;
+    long src, tgt;
+};
+
+__device__ __forceinline__ int bc_n(int e, int n0, int n1, int n2) { return e == 0 ? n0 : e == 1 ? n1 : n2; }
+
+__device__ __forceinline__ BcChain bc_chain(const BcField& F, int nd, int d, int n0, int n1, int n2, long sy, long sz,
+                                            BcPt p)
 {
-    // the face rules met along the chain, outermost pass first
-    int cd[3], cs[3], nc = 0;
-    bool from_v = false;
-    double v0 = 0.0;
-    int q[3] = {p[0], p[1], p[2]};
-    for (int e = d; e >= 0 && !from_v; --e) {
-        const int vert = (F.L >> e) & 1;
+    BcChain c;
+    c.rules = 0u;
+    c.from_v = false;
+    c.v0 = 0.0;
+    BcPt q = p;
+    for (int e = d; e >= 0 && !c.from_v; --e) {
+        const int vert = (F.L >> e) & 1, ne = bc_n(e, n0, n1, n2);
         for (int s = 0; s < 2; ++s) {
             const int k = F.kind[e][s];
-            if (k < 0 || q[e] != bc_target(k, vert, s, n[e])) continue;
+            if (k < 0 || q.at(e) != bc_target(k, vert, s, ne)) continue;
             if (k == 0 && vert) {
-                from_v = true;
-                v0 = F.v[e][s];
+                c.from_v = true;
+                c.v0 = F.v[e][s];
             } else {
-                cd[nc] = e;
-                cs[nc] = s;
-                ++nc;
-                q[e] = bc_source(vert, s, n[e]);
+                c.rules |= (1u | (unsigned)s << 1) << (2 * e);
+                q.set(e, bc_source(vert, s, ne));
             }
             break;
         }
     }
+    c.src = q.x + sy * q.y + (nd == 3 ? sz * q.z : 0);
+    c.tgt = p.x + sy * p.y + (nd == 3 ? sz * p.z : 0);
+    return c;
+}
+
+// the points of one face a thread handles: they are set up and their
+// sources loaded first, then their values stored (no thread of the launch
+// writes a source, so the loads may all go ahead of the stores)
+constexpr int BC_PTS = 4;
+
+template <typename T>
+__device__ __forceinline__ void bc_points(const BcAll& b, const BcSlice& S, const BcField& F, long w0, int n0, int n1,
+                                          int n2, long sy, long sz)
+{
+    const int d = S.d, s = S.s, k = F.kind[d][s];
+    const int e1 = d == 0 ? 1 : 0, e2 = b.nd == 3 ? (d == 2 ? 1 : 2) : -1;
     T* f = static_cast<T*>(F.ptr);
-    T x;
-    if (from_v) x = T(v0);
-    else x = f[q[0] * st[0] + q[1] * st[1] + (nd == 3 ? q[2] * st[2] : 0)];
-    for (int c = nc - 1; c >= 0; --c)
-        if (F.kind[cd[c]][cs[c]] == 0) x = T(2) * T(F.v[cd[c]][cs[c]]) - x;
-    f[p[0] * st[0] + p[1] * st[1] + (nd == 3 ? p[2] * st[2] : 0)] = x;
+    const long np = (long)S.ext1 * S.ext2;
+    BcChain c[BC_PTS];
+    bool live[BC_PTS];
+    T x[BC_PTS];
+#pragma unroll
+    for (int u = 0; u < BC_PTS; ++u) {
+        const long w = w0 + (long)u * blockDim.x;
+        live[u] = w < np;
+        x[u] = T(0);
+        c[u].rules = 0u;
+        c[u].tgt = 0;
+        if (!live[u]) continue;
+        BcPt p{0, 0, 0};
+        p.set(d, bc_target(k, (F.L >> d) & 1, s, bc_n(d, n0, n1, n2)));
+        p.set(e1, (int)(w % S.ext1));
+        if (e2 >= 0) p.set(e2, (int)(w / S.ext1));
+        // a later pass that writes p owns it
+        for (int e = d + 1; e < b.nd; ++e)
+            for (int t = 0; t < 2; ++t) {
+                const int ke = F.kind[e][t];
+                if (ke >= 0 && p.at(e) == bc_target(ke, (F.L >> e) & 1, t, bc_n(e, n0, n1, n2))) live[u] = false;
+            }
+        // an x-face ghost whose point and source lie on planes of their
+        // columns without a valid location: the sweep stored nothing at
+        // either since the last application, which wrote the value a +0
+        // source gives (a Dirichlet vertex has no source)
+        if (live[u] && b.skip_ranges && d == 0) {
+            const int2 rt = b.skip_ranges[p.x + sy * p.y];
+            bool inv = p.z < rt.x || p.z > rt.y;
+            if (inv && !(k == 0 && (F.L & 1))) {
+                const int2 rs = b.skip_ranges[bc_source(F.L & 1, s, n0) + sy * p.y];
+                inv = p.z < rs.x || p.z > rs.y;
+            }
+            if (inv) live[u] = false;
+        }
+        if (!live[u]) continue;
+        c[u] = bc_chain(F, b.nd, d, n0, n1, n2, sy, sz, p);
+        x[u] = c[u].from_v ? T(c[u].v0) : f[c[u].src];
+    }
+#pragma unroll
+    for (int u = 0; u < BC_PTS; ++u) {
+        if (!live[u]) continue;
+        T v = x[u];
+        for (int e = 0; e <= d; ++e) {
+            const unsigned r = c[u].rules >> (2 * e);
+            if ((r & 1u) && F.kind[e][(r >> 1) & 1u] == 0) v = T(2) * T(F.v[e][(r >> 1) & 1u]) - v;
+        }
+        f[c[u].tgt] = v;
+    }
 }
 
 __global__ __launch_bounds__(256) void k_bc_all(BcAll b, int n0, int n1, int n2, long sy, long sz)
 {
-    const int z = blockIdx.z;
-    const int fi = z / (2 * b.nd), d = (z >> 1) % b.nd, s = z & 1;
-    const BcField& F = b.f[fi];
-    const int k = F.kind[d][s];
-    if (k < 0) return;
-    const int n[3] = {n0, n1, n2};
-    const long st[3] = {1, sy, sz};
-    int e1 = -1, e2 = -1;
-    for (int e = 0; e < b.nd; ++e) {
-        if (e == d) continue;
-        if (e1 < 0) e1 = e; else e2 = e;
-    }
-    const int q1 = blockIdx.x * blockDim.x + threadIdx.x;
-    const int q2 = blockIdx.y * blockDim.y + threadIdx.y;
-    const int ext1 = n[e1] + ((F.L >> e1) & 1) + 2;
-    const int ext2 = (e2 >= 0) ? n[e2] + ((F.L >> e2) & 1) + 2 : 1;
-    if (q1 >= ext1 || q2 >= ext2) return;
-    int p[3] = {0, 0, 0};
-    p[d] = bc_target(k, (F.L >> d) & 1, s, n[d]);
-    p[e1] = q1;
-    if (e2 >= 0) p[e2] = q2;
-    // a later pass that writes p owns it
-    for (int e = d + 1; e < b.nd; ++e)
-        for (int t = 0; t < 2; ++t) {
-            const int ke = F.kind[e][t];
-            if (ke >= 0 && p[e] == bc_target(ke, (F.L >> e) & 1, t, n[e])) return;
-        }
-    // an x-face ghost whose point and source lie on planes of their columns
-    // without a valid location: the sweep stored nothing at either since the
-    // last application, which wrote the value a +0 source gives (a
-    // Dirichlet vertex has no source)
-    if (b.skip_ranges && d == 0) {
-        const int2 rt = b.skip_ranges[p[0] + sy * p[1]];
-        bool inv = p[2] < rt.x || p[2] > rt.y;
-        if (inv && !(k == 0 && (F.L & 1))) {
-            const int2 rs = b.skip_ranges[bc_source(F.L & 1, s, n0) + sy * p[1]];
-            inv = p[2] < rs.x || p[2] > rs.y;
-        }
-        if (inv) return;
-    }
-    if (F.f32) bc_chain<float>(F, b.nd, d, n, st, p);
-    else bc_chain<double>(F, b.nd, d, n, st, p);
+    // the slice of this workgroup (slices in launch order, starts ascending)
+    const int bid = blockIdx.x;
+    int q = 0;
+    while (q + 1 < b.nsl && bid >= b.sl[q + 1].start) ++q;
+    const BcSlice S = b.sl[q];
+    const BcField& F = b.f[S.fi];
+    const long w0 = (long)(bid - S.start) * blockDim.x * BC_PTS + threadIdx.x;
+    if (F.f32) bc_points<float>(b, S, F, w0, n0, n1, n2, sy, sz);
+    else bc_points<double>(b, S, F, w0, n0, n1, n2, sy, sz);
 }
 
 void launch_bc_dims(const Geo& g, const BcBatch* b, hipStream_t s, const int2* skip_ranges)
@@ -234,19 +289,33 @@ void launch_bc_dims(const Geo& g, const BcBatch* b, hipStream_t s, const int2* s
         return;
     }
     if (a.nf == 0) return;
-    int m1 = 1, m2 = 1;
-    for (int d = 0; d < g.nd; ++d) {
-        int e1 = -1, e2 = -1;
-        for (int e = 0; e < g.nd; ++e) {
-            if (e == d) continue;
-            if (e1 < 0) e1 = e; else e2 = e;
+    // one slice per face with a rule, each sized to its field's extents
+    a.nsl = 0;
+    int nblocks = 0;
+    for (int fi = 0; fi < a.nf; ++fi)
+        for (int d = 0; d < g.nd; ++d) {
+            int e1 = -1, e2 = -1;
+            for (int e = 0; e < g.nd; ++e) {
+                if (e == d) continue;
+                if (e1 < 0) e1 = e; else e2 = e;
+            }
+            const int L = a.f[fi].L;
+            const int ext1 = g.n[e1] + ((L >> e1) & 1) + 2;
+            const int ext2 = (e2 >= 0) ? g.n[e2] + ((L >> e2) & 1) + 2 : 1;
+            for (int t = 0; t < 2; ++t) {
+                if (a.f[fi].kind[d][t] < 0) continue;
+                BcSlice& S = a.sl[a.nsl++];
+                S.fi = fi;
+                S.d = d;
+                S.s = t;
+                S.ext1 = ext1;
+                S.ext2 = ext2;
+                S.start = nblocks;
+                nblocks += (int)(((long)ext1 * ext2 + 256 * BC_PTS - 1) / (256 * BC_PTS));
+            }
         }
-        m1 = std::max(m1, g.n[e1] + 3);
-        m2 = std::max(m2, (e2 >= 0) ? g.n[e2] + 3 : 1);
-    }
-    dim3 blk(64, 4, 1);
-    dim3 grd((m1 + 63) / 64, (m2 + 3) / 4, 2 * g.nd * a.nf);
-    hipLaunchKernelGGL(k_bc_all, grd, blk, 0, s, a, g.n[0], g.n[1], g.n[2], g.sy, g.sz);
+    if (a.nsl == 0) return;
+    hipLaunchKernelGGL(k_bc_all, dim3(nblocks), dim3(256), 0, s, a, g.n[0], g.n[1], g.n[2], g.sy, g.sz);
 }
 
 // Packed boundary conditions of many instances in one dimension: each op
