@@ -152,6 +152,7 @@ def _declare(lib):
         "fi_set_option": (C.c_int, [H, C.c_int, C.c_int]),
         "fi_fused_region": (C.c_int, [H, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "fi_fused_active_cells": (C.c_int, [H, C.POINTER(C.c_longlong)]),
+        "fi_ib_validity": (C.c_int, [H, C.POINTER(C.c_uint8), C.c_size_t]),
         "fi_levelset_from_dem": (C.c_int, [H, C.c_int, _DP, C.c_int, C.c_int, _DP, _DP, _DP]),
         "fi_invert_levelset": (C.c_int, [H, C.c_int]),
         "fi_levelset_from_polygon": (C.c_int, [H, C.c_int, _DP, C.c_int]),

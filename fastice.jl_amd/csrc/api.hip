@@ -747,6 +747,21 @@ int fi_fused_active_cells(fi_model_t m, long long out[2])
     return FI_OK;
 }
 
+int fi_ib_validity(fi_model_t m, uint8_t* host, size_t count)
+{
+    if (!m || !host) return fail(FI_ERR_ARG, "null argument");
+    if (!m->desc.immersed_boundary) return fail(FI_ERR_ARG, "model has no immersed boundary");
+    const Geo& g = m->g;
+    const size_t w = (size_t)(g.n[0] + 3);
+    const size_t rows = (g.nd == 3) ? (size_t)(g.n[1] + 3) * (g.n[2] + 3) : (size_t)(g.n[1] + 3);
+    if (count != w * rows) return fail(FI_ERR_ARG, "validity bits need %zu bytes, got %zu", w * rows, count);
+    int rc = prepare(m, false);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(m->stream));
+    HIPCHK(hipMemcpy2D(host, w, m->flags, (size_t)g.sy, w, rows, hipMemcpyDeviceToHost));
+    return FI_OK;
+}
+
 int fi_set_option(fi_model_t m, int key, int value)
 {
     if (!m) return fail(FI_ERR_ARG, "null model");

@@ -108,7 +108,7 @@ struct StokesArgs {
 };
 
 // validity bits: sigma locations use isvalid(na) && !isempty(ns), velocity
-// locations isvalid(ns) && !isempty(na) (IB kernels_3d.jl:144,164,170,176,188,194,200)
+// locations isvalid(ns) && !isempty(na) (IB kernels_3d.jl:6,26,32,38,50,56,62)
 enum { F_CCC = 1, F_VVC = 2, F_VCV = 4, F_CVV = 8, F_VCC = 16, F_CVC = 32, F_CCV = 64 };
 
 // inclusive index boxes over Chmy indices

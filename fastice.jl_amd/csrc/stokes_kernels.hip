@@ -35,31 +35,31 @@ __device__ __forceinline__ T lerp4(T a, T b, T c, T d)
 {
     return T(0.5) * (T(0.5) * (a + b) + T(0.5) * (c + d));
 }
-// P -= divV * eta * dtau.Pr                             (kernels_3d.jl:143)
+// P -= divV * eta * dtau.Pr                             (kernels_3d.jl:16)
 template <typename T>
 __device__ __forceinline__ T upd_P(T p, T divV, T eta, T dPr)
 {
     return p - divV * eta * dPr;
 }
-// tau_ii -= (tau_ii - 2 eta (e_ii - divV/3)) * dtau      (kernels_3d.jl:145-147)
+// tau_ii -= (tau_ii - 2 eta (e_ii - divV/3)) * dtau      (kernels_3d.jl:18-20)
 template <typename T>
 __device__ __forceinline__ T upd_td(T t, T eta, T e, T divV, T dt)
 {
     return t - (t - T(2) * eta * (e - divV / T(3))) * dt;
 }
-// tau_ij -= (tau_ij - 2 eta_ij e_ij) * dtau               (kernels_3d.jl:149-151)
+// tau_ij -= (tau_ij - 2 eta_ij e_ij) * dtau               (kernels_3d.jl:22-24)
 template <typename T>
 __device__ __forceinline__ T upd_to(T t, T eta, T e, T dt)
 {
     return t - (t - T(2) * eta * e) * dt;
 }
-// V += (-dP + dtau_a + dtau_b + dtau_c - g) / eta_V * dtau.V   (kernels_3d.jl:158-160)
+// V += (-dP + dtau_a + dtau_b + dtau_c - g) / eta_V * dtau.V   (kernels_3d.jl:31-33)
 template <typename T>
 __device__ __forceinline__ T upd_V(T v, T dP, T da, T db, T dc, T g, T ev, T dv)
 {
     return v + (-dP + da + db + dc - g) / ev * dv;
 }
-// IB variant: ... / eta_V / omega_ns * dtau.V             (IB kernels_3d.jl:189-190)
+// IB variant: ... / eta_V / omega_ns * dtau.V             (IB kernels_3d.jl:51-52)
 template <typename T>
 __device__ __forceinline__ T upd_V_ib(T v, T dP, T da, T db, T dc, T g, T ev, T w, T dv)
 {
@@ -88,7 +88,7 @@ __device__ __forceinline__ float upd_V_ib<float>(float v, float dP, float da, fl
     return v + (-dP + da + db + dc - g) * __builtin_amdgcn_rcpf(ev * w) * dv;
 }
 
-// tau_II (kernels_3d.jl:162-165)
+// tau_II (kernels_3d.jl:35-38)
 template <typename T>
 __device__ __forceinline__ T tauII(T xx, T yy, T zz, T lxy, T lxz, T lyz)
 {
@@ -187,7 +187,7 @@ __device__ __forceinline__ T rheology_o(const StokesArgs<T>& a, uint32_t bo, T t
 // ---------------------------------------------------------------------------
 // 3D pointwise sweeps (used on the boundary shell and for small grids)
 // ---------------------------------------------------------------------------
-// update_sigma!: FullStokes kernels_3d.jl:130-152, IB kernels_3d.jl:141-182
+// update_sigma!: FullStokes kernels_3d.jl:3-25, IB kernels_3d.jl:3-44
 // With an immersed boundary, a point whose stress locations are all invalid
 // is left alone once the output buffers are known to hold 0 there (z_wsig,
 // z_wvel: see fused3_ib_tile); so are the zero stores of invalid locations.
@@ -246,7 +246,7 @@ __device__ __forceinline__ void sigma_point3(const StokesArgs<T>& a, long c, lon
     }
 }
 
-// update_V!: FullStokes kernels_3d.jl:154-167, IB kernels_3d.jl:184-212 (reads the new stress)
+// update_V!: FullStokes kernels_3d.jl:27-40, IB kernels_3d.jl:46-74 (reads the new stress)
 template <typename T, bool IB>
 __device__ __forceinline__ void velocity_point3(const StokesArgs<T>& a, long c, long sy, long sz)
 {
@@ -863,7 +863,7 @@ __device__ __forceinline__ void fused3_ib_tile(const StokesArgs<T>& a, const Fus
         }
         __syncthreads();
 
-        // new stress at plane kk (IB kernels_3d.jl:144-181)
+        // new stress at plane kk (IB kernels_3d.jl:6-43)
         const int e_k = kk % 3, e_km = (kk + 2) % 3;
         T Pc = T(0), xxc = T(0), yyc = T(0), zzc = T(0), xyc = T(0), xzc = T(0), yzc = T(0);
         if (can_c && (fl0 & F_CCC)) {
@@ -919,7 +919,7 @@ __device__ __forceinline__ void fused3_ib_tile(const StokesArgs<T>& a, const Fus
         }
         __syncthreads();
 
-        // velocity (IB kernels_3d.jl:188-205) and viscosity at plane k = kk-1
+        // velocity (IB kernels_3d.jl:50-67) and viscosity at plane k = kk-1
         const int k = kk - 1;
         if (inner && k >= ks) {
             const uint32_t cv = c - bsz;
@@ -1179,7 +1179,7 @@ __global__ __launch_bounds__(TX * TY, MINW) void k_fused3_packed(const PackedIns
     const long sy = a.sy, sz = a.sz;                                              \
     const long c = (long)i + sy * j + sz * k;
 
-// compute_residuals!: FullStokes kernels_3d.jl:193-202, IB kernels_3d.jl:256-281
+// compute_residuals!: FullStokes kernels_3d.jl:66-75, IB kernels_3d.jl:118-143
 template <typename T, bool IB>
 __global__ __launch_bounds__(256) void k_residuals3(StokesArgs<T> a)
 {
@@ -1242,7 +1242,7 @@ __device__ __forceinline__ void put_tau0(T* dst, float* lag, long c)
     dst[c] = T(0);
 }
 
-// compute_tau!: kernels_3d.jl:171-191 (IB variant: kernels_3d.jl:216-254 with
+// compute_tau!: kernels_3d.jl:44-64 (IB variant: IB kernels_3d.jl:78-116 with
 // its undefined `Pr` location read as the Center location); writes the
 // current stress (txxn == txx)
 template <typename T, bool IB, bool LAG>
@@ -1300,7 +1300,7 @@ __global__ __launch_bounds__(256) void k_tau3(StokesArgs<T> a, TauLag lg)
 static inline dim3 block2();
 
 // eta_next <- rheology(tau_II(tau), I) at the current stress (the viscosity half of
-// update_V!, kernels_3d.jl:162-166 / kernels_2d.jl:229-230)
+// update_V!, kernels_3d.jl:35-39 / kernels_2d.jl:27-28)
 template <typename T>
 __global__ __launch_bounds__(256) void k_eta_update(StokesArgs<T> a, int nd)
 {
@@ -1477,7 +1477,7 @@ __device__ __forceinline__ void sigma_point2(const StokesArgs<T>& a, long c, lon
     const T ec = eta[c];
     const T exy_eta = lerp4(eta[c - 1 - sy], eta[c - sy], eta[c - 1], eta[c]);
     if (!IB) {
-        // kernels_2d.jl:205-220
+        // kernels_2d.jl:7-17
         T exx = dif(Vx[c + 1], Vx[c], ix);
         T eyy = dif(Vy[c + sy], Vy[c], iy);
         T exy = T(0.5) * (dif(Vy[c], Vy[c - 1], ix) + dif(Vx[c], Vx[c - sy], iy));
@@ -1513,7 +1513,7 @@ __device__ __forceinline__ void velocity_point2(const StokesArgs<T>& a, long c, 
     const T *P = a.Pn, *txx = a.txxn, *tyy = a.tyyn, *txy = a.txyn, *eta = a.eta;
     T gx = a.grav_field ? a.gx[c] : a.grav[0];
     T gy = a.grav_field ? a.gy[c] : a.grav[1];
-    // each component interpolates eta to its own location (kernels_2d.jl:226-227
+    // each component interpolates eta to its own location (kernels_2d.jl:24-25
     // reuses location(V.x) for both)
     T ex = T(0.5) * (eta[c - 1] + eta[c]);
     T ey = T(0.5) * (eta[c - sy] + eta[c]);
@@ -1532,7 +1532,7 @@ __device__ __forceinline__ void velocity_point2(const StokesArgs<T>& a, long c, 
                                 : T(0);
     }
     if (a.frozen_eta) return;
-    // kernels_2d.jl:229-230
+    // kernels_2d.jl:27-28
     T p = txx[c], q = tyy[c];
     T l = lerp4(txy[c], txy[c + 1], txy[c + sy], txy[c + 1 + sy]);
     a.eta_next[c] = rheology(a, c, sqrt(T(0.5) * (p * p + q * q) + l * l));
@@ -1589,7 +1589,7 @@ __global__ __launch_bounds__(256) void k_residuals2(StokesArgs<T> a)
     T gx = a.grav_field ? a.gx[c] : a.grav[0];
     T gy = a.grav_field ? a.gy[c] : a.grav[1];
     if (!IB) {
-        // kernels_2d.jl:248-256
+        // kernels_2d.jl:50-53
         a.rP[c] = (Vx[c + 1] - Vx[c]) * ix + (Vy[c + sy] - Vy[c]) * iy;
         if (a.fP) a.rP[c] = a.rP[c] + a.fP[c];
         a.rVx[c] = -((P[c] - P[c - 1]) * ix) + (txx[c] - txx[c - 1]) * ix + (txy[c + sy] - txy[c]) * iy - gx;

@@ -404,3 +404,12 @@ class IsothermalImmersedBoundaryFullStokesModel(IsothermalFullStokesModel):
                 v = _get(level_sets, k)
                 if v is not None:
                     getattr(self.level_sets, k).set(v)
+
+    def validity(self):
+        """Validity bits of every location (fi_ib_validity): a uint8 array of
+        the parent shape; bits 0-3 stress locations ccc vvc vcv cvv, bits 4-6
+        velocity locations vcc cvc ccv (null_spaces.jl:1-43 as the masked
+        kernels apply them)."""
+        a = np.zeros(self.grid.parent_shape(), dtype=np.uint8, order="F")
+        A.check(A.lib().fi_ib_validity(self._h, a.ctypes.data_as(C.POINTER(C.c_uint8)), a.size))
+        return a

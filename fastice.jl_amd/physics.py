@@ -1,6 +1,6 @@
 """Rheologies (src/Physics.jl:5-47) as descriptors for the HIP library.
 
-The reference evaluates `rheology(tauII, I)` inside update_V! (kernels_3d.jl:166).
+The reference evaluates `rheology(tauII, I)` inside update_V! (kernels_3d.jl:39).
 Here a rheology is lowered to the fi_rheology record of the C ABI: an inner
 law (linear constant / linear field / power law), an optional viscoplastic
 cap and an optional relaxation toward the law.

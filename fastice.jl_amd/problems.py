@@ -1,11 +1,11 @@
 """Problem set-ups of the reference drivers (the callers of the PT path).
 
 pt_params:        PT parameters, stokes_manufactured_solution_free_slip_3D.jl:92-101
-                  and test_vavilov.jl:266-274.
+                  and test_vavilov.jl:145-156.
 manufactured_3d:  confined free-slip flow, stokes_manufactured_solution_free_slip_3D.jl:25-48,70-89.
-manufactured_2d:  stokes_manufactured_solution_free_slip_2D.jl:24-32,264-281.
+manufactured_2d:  stokes_manufactured_solution_free_slip_2D.jl:24-32,49-77.
 manufactured_errors: relative L-inf/L1/L2 errors of |V| and tau_II against the
-                  closed form, ..._3D.jl:140-167 / ..._2D.jl:331-352.
+                  closed form, ..._3D.jl:140-167 / ..._2D.jl:116-139.
 """
 import math
 
@@ -185,7 +185,7 @@ def glacier_3d_spec(dims, seed=1, glen=True, eta_reg=30.0, relax_rate=0.05, lx=2
 def glacier_3d(dims, seed=1, glen=True, eta_reg=30.0, relax_rate=0.05, lx=2.0, ly=2.0, H=1.0):
     """Seeded synthetic glacier on [-lx/2, lx/2] x [-ly/2, ly/2] x [0, H].
 
-    Set-up order of scripts_future_API/test_vavilov.jl:244-295: level sets of
+    Set-up order of scripts_future_API/test_vavilov.jl:116-177: level sets of
     the ice surface (na) and of the bed (ns, inverted), volume fractions,
     Traction(0) on the box, gravity (0, 0, 1), PT parameters r = 0.9,
     re = 5 pi.  Rheology: Glen n = 3 (PowerLawRheology(1, 3)) capped by

@@ -25,7 +25,9 @@
  *   fi_compute_residuals <- compute_residuals!(model) + norm(r, Inf) per residual
  *                           (Isothermal.jl:114-125; tm_stokes_chmy.jl:99-106)
  *   fi_solve             <- the driver loop of tm_stokes_chmy.jl:96-108 with a tolerance exit
- *   fi_compute_tau       <- compute_tau! helper kernel (kernels_3d.jl:171-191)
+ *   fi_compute_tau       <- compute_tau! helper kernel
+ *                           (src/Models/FullStokes/Isothermal/kernels_3d.jl:44-64, kernels_2d.jl:33-44;
+ *                           IB: src/Models/ImmersedBoundaryFullStokes/Isothermal/kernels_3d.jl:78-116)
  *   fi_levelset_from_dem <- compute_levelset_from_dem!(arch, launch, psi, dem, grid2D, grid, R)
  *                           (src/LevelSets/compute_levelsets.jl:16-25)
  *   fi_invert_levelset   <- invert_levelset!(arch, launch, psi, grid) (src/LevelSets/compute_levelsets.jl:27-35)
@@ -166,6 +168,16 @@ int fi_fused_region(fi_model_t m, int lo[3], int hi[3]);
  * writes nothing on the other planes but the viscosity it updates; out[1] =
  * out[0] without an immersed boundary, both 0 without a fused interior). */
 int fi_fused_active_cells(fi_model_t m, long long out[2]);
+/* Validity bits of an immersed-boundary model, one byte per parent index
+ * ((nx+3)(ny+3)(nz+3) bytes, x fastest; 0 outside the work range [1, n+1]).
+ * A set bit marks a location the masked kernels update: bits 0-3 the stress
+ * locations ccc vvc vcv cvv, valid when isvalid(na) && !isempty(ns), bits 4-6
+ * the velocity locations vcc cvc ccv, valid when isvalid(ns) && !isempty(na)
+ * (src/Models/ImmersedBoundaryFullStokes/Isothermal/null_spaces.jl:1-43 as
+ * kernels_3d.jl:6,26,32,38,50,56,62 of that directory apply them; 2D uses
+ * bits 0, 1, 4, 5).  Inspection hook with no counterpart in the reference,
+ * where the predicates are evaluated inside every kernel. */
+int fi_ib_validity(fi_model_t m, uint8_t* host, size_t count);
 
 /* Options (no counterpart in the reference).  Kernel path, results do not
  * depend on them: FI_OPT_FUSED 1/0 selects the fused interior sweep or the

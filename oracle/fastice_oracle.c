@@ -287,6 +287,40 @@ static int valid2(double* const* w, double* const* w2, int loc, long c, long sy)
     return !(null2(w, loc, c, sy) || LT(w[k], c)) && !LT(w2[k], c);
 }
 
+/* Validity bits over the work range, the predicates the masked kernels test:
+ * stress locations isvalid(na) && !isempty(ns) (IB kernels_3d.jl:6,26,32,38),
+ * velocity locations isvalid(ns) && !isempty(na) (:50,56,62).  Bits 0-3:
+ * ccc vvc vcv cvv; bits 4-6: vcc cvc ccv (2D: bits 0, 1, 4, 5). */
+void or_ib_flags(const or_model* m, const or_fields* f, unsigned char* out)
+{
+    const int nd = m->ndims;
+    const long sy = m->sy, sz = m->sz;
+    const long total = (nd == 3) ? sz * (m->n[2] + 3) : sy * (m->n[1] + 3);
+    memset(out, 0, (size_t)total);
+    const int locs3[7] = {L_CCC, L_VVC, L_VCV, L_CVV, L_VCC, L_CVC, L_CCV};
+    const int locs2[4] = {L_CCC, L_VVC, L_VCC, L_CVC};
+    const int bit2[4] = {0, 1, 4, 5};
+    const int khi = (nd == 3) ? m->n[2] + 1 : 0, klo = (nd == 3) ? 1 : 0;
+    for (int k = klo; k <= khi; ++k)
+        for (int j = 1; j <= m->n[1] + 1; ++j)
+            for (int i = 1; i <= m->n[0] + 1; ++i) {
+                const long c = i + sy * j + sz * k;
+                unsigned v = 0;
+                if (nd == 3) {
+                    for (int q = 0; q < 7; ++q) {
+                        int ok = (q < 4) ? valid3(f->na, f->ns, locs3[q], c, sy, sz) : valid3(f->ns, f->na, locs3[q], c, sy, sz);
+                        if (ok) v |= 1u << q;
+                    }
+                } else {
+                    for (int q = 0; q < 4; ++q) {
+                        int ok = (q < 2) ? valid2(f->na, f->ns, locs2[q], c, sy) : valid2(f->ns, f->na, locs2[q], c, sy);
+                        if (ok) v |= 1u << bit2[q];
+                    }
+                }
+                out[c] = (unsigned char)v;
+            }
+}
+
 /* ------------------------------------------------------------------ */
 /* 3D kernels: kernels_3d.jl (FullStokes) and IB kernels_3d.jl          */
 /* ------------------------------------------------------------------ */
@@ -326,7 +360,7 @@ static void sigma3(const or_model* m, or_fields* f, int ib)
                     txz[c] = txz[c] - (txz[c] - 2.0 * exz_eta * exz) * m->dtau[4];
                     tyz[c] = tyz[c] - (tyz[c] - 2.0 * eyz_eta * eyz) * m->dtau[5];
                 } else {
-                    /* IB kernels_3d.jl:144-181 */
+                    /* IB kernels_3d.jl:6-43 */
                     if (valid3(na, ns, L_CCC, c, sy, sz)) {
                         double exx = (wx[c + 1] * Vx[c + 1] - wx[c] * Vx[c]) * ix;
                         double eyy = (wy[c + sy] * Vy[c + sy] - wy[c] * Vy[c]) * iy;
@@ -395,7 +429,7 @@ static void velocity3(const or_model* m, or_fields* f, int ib)
                 double ey = 0.5 * (eta[c - sy] + eta[c]);
                 double ez = 0.5 * (eta[c - sz] + eta[c]);
                 if (!ib) {
-                    /* kernels_3d.jl:158-160 */
+                    /* kernels_3d.jl:31-33 */
                     Vx[c] = Vx[c] + (-((P[c] - P[c - 1]) * ix) + (txx[c] - txx[c - 1]) * ix +
                                      (txy[c + sy] - txy[c]) * iy + (txz[c + sz] - txz[c]) * iz - gx) / ex * m->dV[0];
                     Vy[c] = Vy[c] + (-((P[c] - P[c - sy]) * iy) + (tyy[c] - tyy[c - sy]) * iy +
@@ -403,7 +437,7 @@ static void velocity3(const or_model* m, or_fields* f, int ib)
                     Vz[c] = Vz[c] + (-((P[c] - P[c - sz]) * iz) + (tzz[c] - tzz[c - sz]) * iz +
                                      (txz[c + 1] - txz[c]) * ix + (tyz[c + sy] - tyz[c]) * iy - gz) / ez * m->dV[2];
                 } else {
-                    /* IB kernels_3d.jl:188-205 */
+                    /* IB kernels_3d.jl:50-67 */
                     const double *wc = na[OM_CCC], *wxy = na[OM_VVC], *wxz = na[OM_VCV], *wyz = na[OM_CVV];
                     if (valid3(ns, na, L_VCC, c, sy, sz))
                         Vx[c] = Vx[c] + (-((wc[c] * P[c] - wc[c - 1] * P[c - 1]) * ix) +
@@ -427,7 +461,7 @@ static void velocity3(const or_model* m, or_fields* f, int ib)
                                             ez / ns[OM_CCV][c] * m->dV[2];
                     else Vz[c] = 0.0;
                 }
-                eta_next[c] = or_rheology(m, f, c, tauII3(f, c, sy, sz)); /* kernels_3d.jl:162-166 */
+                eta_next[c] = or_rheology(m, f, c, tauII3(f, c, sy, sz)); /* kernels_3d.jl:35-39, IB :69-73 */
             }
 }
 
@@ -450,7 +484,7 @@ static void residuals3(const or_model* m, or_fields* f, int ib)
                 double gy = m->grav_field ? f->gy[c] : m->grav[1];
                 double gz = m->grav_field ? f->gz[c] : m->grav[2];
                 if (!ib) {
-                    /* kernels_3d.jl:197-201 */
+                    /* kernels_3d.jl:70-74 */
                     rP[c] = (Vx[c + 1] - Vx[c]) * ix + (Vy[c + sy] - Vy[c]) * iy + (Vz[c + sz] - Vz[c]) * iz;
                     rVx[c] = -((P[c] - P[c - 1]) * ix) + (txx[c] - txx[c - 1]) * ix + (txy[c + sy] - txy[c]) * iy +
                              (txz[c + sz] - txz[c]) * iz - gx;
@@ -459,7 +493,7 @@ static void residuals3(const or_model* m, or_fields* f, int ib)
                     rVz[c] = -((P[c] - P[c - sz]) * iz) + (tzz[c] - tzz[c - sz]) * iz + (txz[c + 1] - txz[c]) * ix +
                              (tyz[c + sy] - tyz[c]) * iy - gz;
                 } else {
-                    /* IB kernels_3d.jl:260-279 */
+                    /* IB kernels_3d.jl:122-142 */
                     const double *wx = ns[OM_VCC], *wy = ns[OM_CVC], *wz = ns[OM_CCV];
                     const double *wc = na[OM_CCC], *wxy = na[OM_VVC], *wxz = na[OM_VCV], *wyz = na[OM_CVV];
                     if (valid3(na, ns, L_CCC, c, sy, sz))
@@ -489,7 +523,7 @@ static void residuals3(const or_model* m, or_fields* f, int ib)
 
 static void tau3(const or_model* m, or_fields* f, int ib)
 {
-    /* compute_tau!: kernels_3d.jl:171-191 (IB: kernels_3d.jl:216-254, with the
+    /* compute_tau!: kernels_3d.jl:44-64 (IB: IB kernels_3d.jl:78-116, with the
      * undefined `Pr` location read as the Center location of tau.xx) */
     const int nx = m->n[0], ny = m->n[1], nz = m->n[2];
     const long sy = m->sy, sz = m->sz;
@@ -565,7 +599,7 @@ static void sigma2(const or_model* m, or_fields* f, int ib)
             double ec = eta[c];
             double exy_eta = 0.5 * (0.5 * (eta[c - 1 - sy] + eta[c - sy]) + 0.5 * (eta[c - 1] + eta[c]));
             if (!ib) {
-                /* kernels_2d.jl:7-19 */
+                /* kernels_2d.jl:7-17 */
                 double exx = (Vx[c + 1] - Vx[c]) * ix;
                 double eyy = (Vy[c + sy] - Vy[c]) * iy;
                 double exy = 0.5 * ((Vy[c] - Vy[c - 1]) * ix + (Vx[c] - Vx[c - sy]) * iy);
@@ -595,7 +629,7 @@ static void sigma2(const or_model* m, or_fields* f, int ib)
 
 static double tauII2(const or_fields* f, long c, long sy)
 {
-    /* kernels_2d.jl:29 */
+    /* kernels_2d.jl:27 */
     const double* txy = f->txy;
     double a = f->txx[c], b = f->tyy[c];
     double l = 0.5 * (0.5 * (txy[c] + txy[c + 1]) + 0.5 * (txy[c + sy] + txy[c + 1 + sy]));
@@ -655,7 +689,7 @@ static void residuals2(const or_model* m, or_fields* f, int ib)
             double gx = m->grav_field ? f->gx[c] : m->grav[0];
             double gy = m->grav_field ? f->gy[c] : m->grav[1];
             if (!ib) {
-                /* kernels_2d.jl:252-255 */
+                /* kernels_2d.jl:50-53 */
                 rP[c] = (Vx[c + 1] - Vx[c]) * ix + (Vy[c + sy] - Vy[c]) * iy;
                 rVx[c] = -((P[c] - P[c - 1]) * ix) + (txx[c] - txx[c - 1]) * ix + (txy[c + sy] - txy[c]) * iy - gx;
                 rVy[c] = -((P[c] - P[c - sy]) * iy) + (tyy[c] - tyy[c - sy]) * iy + (txy[c + 1] - txy[c]) * ix - gy;
@@ -678,7 +712,7 @@ static void residuals2(const or_model* m, or_fields* f, int ib)
 
 static void tau2(const or_model* m, or_fields* f, int ib)
 {
-    /* compute_tau!: kernels_2d.jl:235-246 */
+    /* compute_tau!: kernels_2d.jl:33-44 */
     const int nx = m->n[0], ny = m->n[1];
     const long sy = m->sy;
     const double ix = m->idx[0], iy = m->idx[1];

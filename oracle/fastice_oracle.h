@@ -38,14 +38,24 @@
  *   or_heat_advance,        steady linear profile between two values, heat
  *   or_heat_bc,             conservation under zero flux, backward-Euler
  *   or_heat_residual_norms  decay of a discrete cosine mode
+ *   immersed-boundary       (tests/test_oracle_ib_pins.py) or_ib_flags and the
+ *   branches (ib != 0) of   masked update_sigma / update_V / compute_tau /
+ *   the kernels above,      compute_residuals, 3D and 2D, equal bit for bit an
+ *   or_ib_flags             independent numpy transcription that derives every
+ *                           neighbour from null_spaces.jl's generic
+ *                           flip/left/right rule, on random masks around the
+ *                           1e-6 threshold; all-one masks reduce them bit for
+ *                           bit to the plain kernels; a hydrostatic column on a
+ *                           cut bed is an exact rest state
  * Parity unpinned (choices this library specifies where the reference
  * delegates to unvendored Chmy.jl and no check can reach the reference's own
  * bits): the ghost values at edges and corners that the x-then-y-then-z
- * order of the boundary conditions produces, the masked derivative
- * (w_r f_r - w_l f_l) / dx of the immersed-boundary kernels and the validity
- * bits derived from it, zero halos of the volume-fraction masks, the Inf-norm
- * over interiors only, and the bitwise trajectory of the KernelAbstractions
- * CPU path (difference times inverse spacing, operation order).
+ * order of the boundary conditions produces, the definition of Chmy's masked
+ * derivative as (w_r f_r - w_l f_l) / dx (the transcription shares it; the
+ * indexing built on it is pinned), zero halos of the volume-fraction masks,
+ * the Inf-norm over interiors only, and the bitwise trajectory of the
+ * KernelAbstractions CPU path (difference times inverse spacing, operation
+ * order).
  */
 #ifndef FASTICE_ORACLE_H
 #define FASTICE_ORACLE_H
@@ -113,6 +123,9 @@ void or_bc_neumann_center(const or_model* m, double* a);
 void or_bc_neumann_vertex(const or_model* m, double* a);
 void or_compute_residuals(const or_model* m, or_fields* f, int ib);
 void or_compute_tau(const or_model* m, or_fields* f, int ib);
+/* validity bits of every location in the work range (parent layout, one
+ * byte per index; bits 0-3 stress ccc vvc vcv cvv, 4-6 velocity vcc cvc ccv) */
+void or_ib_flags(const or_model* m, const or_fields* f, unsigned char* out);
 /* Inf-norms over the interior of rP, rVx, rVy(, rVz); NaN propagates */
 void or_residual_norms(const or_model* m, const or_fields* f, double out[4]);
 /* fixed-cadence solve: every ncheck iterations compute residuals and stop

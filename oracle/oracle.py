@@ -95,6 +95,7 @@ def lib():
             "or_bc_neumann_vertex": (None, [M, _DP]),
             "or_compute_residuals": (None, [M, Fp, C.c_int]),
             "or_compute_tau": (None, [M, Fp, C.c_int]),
+            "or_ib_flags": (None, [M, Fp, C.POINTER(C.c_uint8)]),
             "or_residual_norms": (None, [M, Fp, _DP]),
             "or_solve": (C.c_int, [M, Fp, C.c_int, C.c_int, C.c_double, C.c_int, _DP]),
             "or_rheology": (C.c_double, [M, Fp, C.c_long, C.c_double]),
@@ -278,6 +279,28 @@ class OracleModel:
     def compute_tau(self):
         f = self._fields()
         lib().or_compute_tau(C.byref(self.m), C.byref(f), int(self.ib))
+
+    def update_sigma(self):
+        """update_σ! alone (no boundary conditions)."""
+        f = self._fields()
+        lib().or_update_sigma(C.byref(self.m), C.byref(f), int(self.ib))
+
+    def update_V(self):
+        """update_V! alone (no boundary conditions, no viscosity swap)."""
+        f = self._fields()
+        lib().or_update_V(C.byref(self.m), C.byref(f), int(self.ib))
+
+    def residuals_only(self):
+        """compute_residuals! (kernel and residual boundary conditions), no norms."""
+        f = self._fields()
+        lib().or_compute_residuals(C.byref(self.m), C.byref(f), int(self.ib))
+
+    def ib_flags(self):
+        """Validity bits (or_ib_flags) as a uint8 array of the parent shape."""
+        out = np.zeros(self.shape, dtype=np.uint8, order="F")
+        f = self._fields()
+        lib().or_ib_flags(C.byref(self.m), C.byref(f), out.ctypes.data_as(C.POINTER(C.c_uint8)))
+        return out
 
     def solve(self, max_iter, ncheck, tol):
         f = self._fields()

@@ -1,6 +1,6 @@
 """HDF5 + XDMF3 writers (src/Writers.jl) against the reference's writer test.
 
-test/test_writers.jl:36-86 writes two Center fields of a (4, 5, 6) grid on
+test/test_writers.jl:36-78 writes two Center fields of a (4, 5, 6) grid on
 origin (-0.4, -0.5, 0.0), extent (1.0, 1.1, 1.2), reads the datasets back
 with h5read and compares the XDMF3 document with a literal; that literal is
 the fixture golden/xdmf3_test_writers.xml (made by golden/make_xdmf_fixture.py).
