@@ -141,6 +141,7 @@ def _declare(lib):
         "fi_field_upload": (C.c_int, [H, C.c_int, _DP, C.c_size_t, C.c_int]),
         "fi_field_download": (C.c_int, [H, C.c_int, _DP, C.c_size_t, C.c_int]),
         "fi_field_fill": (C.c_int, [H, C.c_int, C.c_double]),
+        "fi_field_download_noalloc": (C.c_int, [H, C.c_int, _DP, C.c_size_t]),
         "fi_field_size": (C.c_size_t, [H, C.c_int, C.c_int]),
         "fi_apply_bc": (C.c_int, [H, C.c_int]),
         "fi_advance_iterations": (C.c_int, [H, C.c_int]),

@@ -9,6 +9,7 @@
 #include <new>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "model.h"
@@ -386,13 +387,15 @@ Geo geo_for(int nd, const int n[3], const double extent[3])
     return g;
 }
 
-// interior of a field <-> contiguous x-fastest array: stage[t], t = i + mx (j + my k),
-// is the field at Chmy index (i+1, j+1, k+1) (sz = 0 in 2D)
-__global__ __launch_bounds__(256) void k_interior_stage(double* stage, double* field, int mx, int my, long n, long sy,
-                                                        long sz, int unpack)
+// interior of a field <-> contiguous x-fastest array: stage[t] holds interior
+// element u = t0 + t, u = i + mx (j + my k), the field at Chmy index
+// (i+1, j+1, k+1) (sz = 0 in 2D); t in [0, n)
+__global__ __launch_bounds__(256) void k_interior_stage(double* stage, double* field, int mx, int my, long t0, long n,
+                                                        long sy, long sz, int unpack)
 {
     for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
-        const long i = t % mx, r = t / mx, j = r % my, k = r / my;
+        const long u = t0 + t;
+        const long i = u % mx, r = u / mx, j = r % my, k = r / my;
         const long c = 1 + i + sy * (1 + j) + sz * (1 + k);
         if (unpack) field[c] = stage[t];
         else stage[t] = field[c];
@@ -427,12 +430,69 @@ int field_transfer(const Geo& g, double* dev, int L, double* host, size_t count,
     if (blocks > 8192) blocks = 8192;
     if (up) {
         HIPCHK(hipMemcpyAsync(*stage, host, n * sizeof(double), hipMemcpyHostToDevice, stream));
-        hipLaunchKernelGGL(k_interior_stage, dim3((unsigned)blocks), dim3(256), 0, stream, *stage, dev, mx, my, (long)n,
-                           g.sy, sz, 1);
+        hipLaunchKernelGGL(k_interior_stage, dim3((unsigned)blocks), dim3(256), 0, stream, *stage, dev, mx, my, 0L,
+                           (long)n, g.sy, sz, 1);
     } else {
-        hipLaunchKernelGGL(k_interior_stage, dim3((unsigned)blocks), dim3(256), 0, stream, *stage, dev, mx, my, (long)n,
-                           g.sy, sz, 0);
+        hipLaunchKernelGGL(k_interior_stage, dim3((unsigned)blocks), dim3(256), 0, stream, *stage, dev, mx, my, 0L,
+                           (long)n, g.sy, sz, 0);
         HIPCHK(hipMemcpyAsync(host, *stage, n * sizeof(double), hipMemcpyDeviceToHost, stream));
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(stream));
+    return FI_OK;
+}
+
+// Staging area of fi_field_download_noalloc: 16 MiB of device memory per GPU,
+// held until the process ends.  Host code never uses this name as an
+// address: the device address of each GPU's copy comes from
+// hipGetSymbolAddress (scratch_address below).
+constexpr long kNoAllocStage = 2L << 20; // doubles
+static __device__ __align__(256) double scratch[kNoAllocStage];
+
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[64] = {};
+
+// device address of the current device's staging area, looked up once per device
+static int scratch_address(double** out)
+{
+    int dev = -1;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return fail(FI_ERR_UNSUPPORTED, "device index %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return fail(FI_ERR_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return FI_OK;
+}
+
+// interior download through the staging area, a run of whole x rows at a time
+static int download_noalloc(const Geo& g, double* dev, int L, double* host, size_t count, hipStream_t stream)
+{
+    const int mx = g.n[0] + (L & 1), my = g.n[1] + ((L >> 1) & 1);
+    const int mz = (g.nd == 3) ? g.n[2] + ((L >> 2) & 1) : 1;
+    const size_t n = (size_t)mx * my * mz;
+    if (n != count) return fail(FI_ERR_ARG, "interior transfer of %zu doubles, got %zu", n, count);
+    const long rows_per_chunk = kNoAllocStage / mx;
+    if (rows_per_chunk < 1)
+        return fail(FI_ERR_ARG, "an x row of %d doubles exceeds the %ld-double staging area", mx, kNoAllocStage);
+    double* stage = nullptr;
+    int rc = scratch_address(&stage);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(stream));
+    const long rows = (long)my * mz, sz = g.nd == 3 ? g.sz : 0;
+    for (long r0 = 0; r0 < rows; r0 += rows_per_chunk) {
+        const long nr = std::min(rows_per_chunk, rows - r0);
+        const long t0 = r0 * mx, nt = nr * mx;
+        long blocks = (nt + 255) / 256;
+        if (blocks > 8192) blocks = 8192;
+        // stream order: this chunk's packing follows the copy-out of the previous one
+        hipLaunchKernelGGL(k_interior_stage, dim3((unsigned)blocks), dim3(256), 0, stream, stage, dev, mx, my, t0, nt,
+                           g.sy, sz, 0);
+        HIPCHK(hipMemcpyAsync(host + t0, stage, (size_t)nt * sizeof(double), hipMemcpyDeviceToHost, stream));
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(stream));
@@ -478,7 +538,7 @@ int fi_model_create(const fi_model_desc* desc, fi_model_t* out)
     m->desc = *desc;
     m->g = make_geo(*desc);
     Geo& g = m->g;
-    if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess) {
+    if (hipGetDevice(&m->device) != hipSuccess || hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess) {
         free_model(m);
         return fail(FI_ERR_HIP, "hipStreamCreate failed");
     }
@@ -586,6 +646,22 @@ int fi_field_upload(fi_model_t m, int id, const double* host, size_t count, int 
 int fi_field_download(fi_model_t m, int id, double* host, size_t count, int layout)
 {
     return field_copy(m, id, host, count, layout, false);
+}
+
+int fi_field_download_noalloc(fi_model_t m, int id, double* host, size_t count)
+{
+    if (!m) return fail(FI_ERR_ARG, "null model");
+    if (id < 0 || id >= FI_F_COUNT || !m->f[id]) return fail(FI_ERR_ARG, "field %d not present in this model", id);
+    if (!host) return fail(FI_ERR_ARG, "null host pointer");
+    int cur = -1;
+    HIPCHK(hipGetDevice(&cur));
+    if (cur != m->device) HIPCHK(hipSetDevice(m->device));
+    int rc = download_noalloc(m->g, m->buf(id), field_loc(m->g.nd, id), host, count, m->stream);
+    if (cur != m->device) {
+        hipError_t e = hipSetDevice(cur);
+        if (!rc && e != hipSuccess) rc = fail(FI_ERR_HIP, "hipSetDevice: %s", hipGetErrorString(e));
+    }
+    return rc;
 }
 
 int fi_field_fill(fi_model_t m, int id, double value)

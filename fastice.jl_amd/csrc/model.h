@@ -23,6 +23,7 @@ using fi::Geo;
 struct fi_model_s {
     fi_model_desc desc;
     Geo g;
+    int device = 0; // the HIP device current when the model was created
     hipStream_t stream = nullptr;
     double* base[FI_F_COUNT] = {};
     double* f[FI_F_COUNT] = {};

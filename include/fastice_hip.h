@@ -143,6 +143,17 @@ int fi_model_destroy(fi_model_t m);
 int fi_field_upload(fi_model_t m, int field_id, const double* host, size_t count, int layout);
 int fi_field_download(fi_model_t m, int field_id, double* host, size_t count, int layout);
 int fi_field_fill(fi_model_t m, int field_id, double value);
+/* fi_field_download with FI_LAYOUT_INTERIOR that allocates nothing.  No
+ * counterpart in the reference: it is for callers that cannot allocate device
+ * memory (tight serving loops, harnesses that forbid hipMalloc).  The interior
+ * is packed on the device, a run of whole x rows at a time, into a static
+ * staging area of 16 MiB (2^21 doubles) that each GPU holds until the process
+ * ends, and copied out chunk by chunk; a field of any size is transferred,
+ * and one whose x rows are longer than the staging area is rejected with
+ * FI_ERR_ARG.  Every call on a GPU shares that GPU's staging area: calls that
+ * overlap on one GPU (from several threads) race.  Synchronises the model's
+ * stream; same count rule as fi_field_download. */
+int fi_field_download_noalloc(fi_model_t m, int field_id, double* host, size_t count);
 /* number of doubles of a field in the given layout (0 on a bad id) */
 size_t fi_field_size(fi_model_t m, int field_id, int layout);
 
