@@ -6,10 +6,11 @@ Workloads
   glacier_511x511x255 (default): 3D synthetic glacier, 511x511x255 cells,
       seeded 8-Gaussian-bump bed under a parabolic surface, level sets ->
       volume fractions (immersed boundary), Glen n = 3 regularised power law,
-      gravity-driven.  A step is one FP32 PT iteration of the defect-correction
-      solver (the production sweep); the FP64 iteration of the same model is
-      timed beside it, and a converged-solve comparison (FP32-refined vs plain
-      FP64, both to FP64 residual 1e-8) runs on the same glacier at 255x255x127.
+      gravity-driven.  A step is one cycle of the FP32-refined solver (32 FP32
+      PT sweeps, then its FP64 outer step); the FP64 iteration of the same
+      model is timed beside it, and a converged-solve comparison (FP32-refined
+      vs plain FP64, both to FP64 residual 1e-8) runs on the same glacier at
+      255x255x127.
   stokes3d_fp64_511x511x255: isoviscous free-slip box with manufactured
       forcing, plain FP64 iteration.
   batch96_glacier: 96 seeded glaciers, each dimension uniform in 48..192
@@ -35,15 +36,21 @@ Workloads
       FP64 to 1e-8 or niter = 50n, continuously batched; value = completed
       solves/s with p50/p95/p99 latency.
 
-value = T_eff = A_eff * K / t with A_eff = 23 * sizeof(T) * nx*ny*nz bytes per
-iteration (scripts_future_API/tm_stokes_mpi_wip.jl:180), inputs resident in
-HBM: the reference's formula counts every cell, also those the immersed-
-boundary sweep skips (planes of a column with no valid location, whose state
-stays 0).  roofline: the dominant kernel's algorithmic bytes per launch (each
-field it must read or write, once, over the cells it processes; masks not
-credited) over its HIP-event duration on the model's stream; traffic: its measured HBM bytes
-per launch from the rocprofv3 summary in profiles/.  cpu_baseline: the CPU oracle
-(OpenMP) on a bounded number of FP64 iterations of the same grid.
+value = T_eff over the cells a step processes: 23 * sizeof(T) bytes per
+processed cell and PT sweep (the reference's per-cell count,
+scripts_future_API/tm_stokes_mpi_wip.jl:180) over the step's wall time,
+inputs resident in HBM.  The glacier step is one cycle of the FP32-refined
+solver (32 FP32 sweeps plus the FP64 outer step); processed cells are the
+boundary shell plus the interior cells on planes of their column that hold a
+valid location (the immersed-boundary sweep reads and writes nothing on the
+other planes, whose state stays 0).  The reference's own formula, which
+counts every cell of the box, is reported beside it as
+teff_reference_formula_gbs.  roofline: the dominant kernel's algorithmic
+bytes per launch (23 * sizeof(T) per processed interior cell; masks and the
+shell points it also updates not credited) over its HIP-event duration on the
+model's stream; traffic: its measured HBM bytes per launch from the rocprofv3
+summary in profiles/.  cpu_baseline: the CPU oracle (OpenMP) on a bounded
+number of FP64 iterations of the same grid, which it sweeps cell by cell.
 """
 import argparse
 import ctypes as C
@@ -57,6 +64,12 @@ import numpy as np
 import __graft_entry__ as ge
 
 HBM_PEAK_GBS = 8000.0
+CYCLE = 32  # FP32 sweeps between FP64 outer steps of the refined solver (its default)
+RELAX_511 = 0.01  # log-space viscosity relaxation of the 511x511x255 glacier's timed model
+# The converged comparison's glacier: on 511x511x255 neither plain FP64 PT nor
+# the refined solver reaches residual 1e-8 (some ice-surface velocity faces
+# valid by the reference's null-space rule keep a residual of about 0.5)
+SOLVE_DIMS = (383, 383, 191)
 DEFAULT = "glacier_511x511x255"
 WORKLOADS = {
     "glacier_511x511x255": (511, 511, 255),
@@ -78,9 +91,9 @@ def parse():
     p.add_argument("--warmup", type=int, default=20)
     p.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS))
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU-oracle baseline")
-    p.add_argument("--no-solve", action="store_true", help="skip the converged-solve comparison")
+    p.add_argument("--no-solve", action="store_true",
+                   help="skip the plain FP64 converged solve (the refined solve runs: the timed cycles continue it)")
     p.add_argument("--cpu-seconds", type=float, default=15.0)
-    p.add_argument("--solve-dims", default="255,255,127")
     p.add_argument("--alone-sample", type=int, default=16,
                    help="batch96_glacier: models solved one at a time for the comparison")
     p.add_argument("--batch-precision", default="refined", choices=("refined", "fp64"),
@@ -159,38 +172,49 @@ def cpu_baseline(pkg, model, dims, seconds, ib):
                       % (n, dims[0], dims[1], dims[2], "immersed-boundary glacier" if ib else "grid", el)}
 
 
-def solve_comparison(pkg, dims):
-    """Converged glacier solves to FP64 residual 1e-8: plain FP64 vs FP32-refined."""
+def solve_comparison(pkg, dims, b=None, fp64=True):
+    """Converged glacier solves to FP64 residual 1e-8: plain FP64 (on model b,
+    reset to the initial state, when given) vs FP32-refined (a new model,
+    returned open with the result)."""
     a, cad = pkg.glacier_3d(dims, seed=1, glen=True)
-    b, _ = pkg.glacier_3d(dims, seed=1, glen=True)
     n = max(dims)
-    b.synchronize()
-    t0 = time.perf_counter()
-    it, nb = b.solve(400 * n, cad["ncheck"], 1e-8)
-    t64 = time.perf_counter() - t0
+    out = {"grid": list(dims), "tol": 1e-8}
+    if fp64:
+        own = b is None
+        if own:
+            b, _ = pkg.glacier_3d(dims, seed=1, glen=True)
+        else:
+            _reset(b)
+        b.synchronize()
+        t0 = time.perf_counter()
+        it, nb = b.solve(100 * n, cad["ncheck"], 1e-8)
+        t64 = time.perf_counter() - t0
+        out["fp64"] = {"iterations": it, "seconds": round(t64, 3), "residual": float(max(nb)),
+                       "cap": 100 * n, "ncheck": cad["ncheck"]}
+        log("fp64 solve: %d iterations, %.2f s, residual %.3e" % (it, t64, max(nb)))
     a.synchronize()
     t0 = time.perf_counter()
     outer, inner, na = a.refined_solve(1e-8)
     t32 = time.perf_counter() - t0
-    # per-field difference relative to the FP64 solution's stress / velocity scale
-    names = ("P", "TXX", "TYY", "TZZ", "TXY", "TXZ", "TYZ", "VX", "VY", "VZ")
-    vals = {k: (a.field(k).interior(), b.field(k).interior()) for k in names}
-    s_scale = max(np.max(np.abs(vals[k][1])) for k in names[:7])
-    v_scale = max(np.max(np.abs(vals[k][1])) for k in names[7:])
-    errs = {k: float(np.max(np.abs(x - y)) / (s_scale if k[0] in "PT" else v_scale)) for k, (x, y) in vals.items()}
     counts = dict(a.refined_counts)
-    a.close()
-    b.close()
-    return {"grid": list(dims), "tol": 1e-8,
-            "fp64": {"iterations": it, "seconds": round(t64, 3), "residual": float(max(nb))},
-            "fp32_refined": {"outer_loop": "lagged PT: an FP64 outer step (residuals, viscosity, stress lag) "
-                                           "every 32 FP32 sweeps, after a full-FP32 nonlinear start",
-                             "outer_steps": outer, "fp32_iterations": inner,
-                             "fp32_nonlinear_phase_iterations": counts["nonlinear_phase"],
-                             "seconds": round(t32, 3),
-                             "residual": float(max(na))},
-            "speedup": round(t64 / t32, 3),
-            "rel_error_vs_fp64": {k: float("%.3e" % v) for k, v in errs.items()}}
+    out["fp32_refined"] = {"outer_loop": "lagged PT: an FP64 outer step (residuals, viscosity, stress lag) "
+                                         "every 32 FP32 sweeps, after a full-FP32 nonlinear start",
+                           "outer_steps": outer, "fp32_iterations": inner,
+                           "fp32_nonlinear_phase_iterations": counts["nonlinear_phase"],
+                           "seconds": round(t32, 3), "residual": float(max(na))}
+    log("refined solve: %d sweeps, %d outer steps, %.2f s, residual %.3e" % (inner, outer, t32, max(na)))
+    if fp64:
+        # per-field difference relative to the FP64 solution's stress / velocity scale
+        names = ("P", "TXX", "TYY", "TZZ", "TXY", "TXZ", "TYZ", "VX", "VY", "VZ")
+        vals = {k: (a.field(k).interior(), b.field(k).interior()) for k in names}
+        s_scale = max(np.max(np.abs(vals[k][1])) for k in names[:7])
+        v_scale = max(np.max(np.abs(vals[k][1])) for k in names[7:])
+        errs = {k: float(np.max(np.abs(x - y)) / (s_scale if k[0] in "PT" else v_scale)) for k, (x, y) in vals.items()}
+        out["speedup"] = round(t64 / t32, 3)
+        out["rel_error_vs_fp64"] = {k: float("%.3e" % v) for k, v in errs.items()}
+        if own:
+            b.close()
+    return out, a
 
 
 def _line(metric, value, unit, ms_per_step, steps, warmup, dtype, config, higher=True, scaling="weak"):
@@ -622,7 +646,7 @@ def main():
     log("setting up %s" % args.workload)
     t0 = time.perf_counter()
     if glacier:
-        model, cad = pkg.glacier_3d(dims, seed=1, glen=True)
+        model, cad = pkg.glacier_3d(dims, seed=1, glen=True, relax_rate=RELAX_511)
         model.advance(50)  # leave the zero initial state
     else:
         kw, _ = pkg.manufactured_3d(dims)
@@ -650,45 +674,68 @@ def main():
     gfields = 3 if model.desc.gravity_is_field else 0
     log("fp64 %.3f ms/iter, T_eff %.1f GB/s" % (ms64, teff64))
 
+    # cells a step processes: the fused interior's cells on planes of their
+    # column that hold a valid location (the immersed-boundary sweep reads and
+    # writes nothing else there), plus every cell of the boundary shell
+    processed = acells + (cells - icells)
+    solve = None
     if glacier:
-        # FP32 correction iteration of a genuine correction problem: one
-        # refined outer cycle (32 full-FP32 sweeps, then the lagged-PT outer
-        # step that casts the forcing, freezes eta and writes the stress lag)
-        # leaves the FP32 state the solver's sweeps start from
-        model.refined_solve(1e-30, max_outer=1, max_inner=32, ncheck=32)
-        A.check(A.lib().fi_refined_advance_inner(model._h, args.warmup))
-        model.synchronize()
+        # The step is one cycle of the FP32-refined solver: 32 FP32 PT
+        # sweeps, then the FP64 outer step (residuals, viscosity update,
+        # stress lag, corrections added to the FP64 state).  The solve starts
+        # as a solve does (full-FP32 nonlinear sweeps, then outer cycles);
+        # the timed cycles continue it without a second full-FP32 start.
+        # Viscosity relaxation 0.01: at 0.05 the outer loop of this grid
+        # diverges within 13 cycles (the reference scheme does not reach
+        # 1e-8 on it either; the converged comparison runs at 383x383x191).
+        model32, _ = pkg.glacier_3d(dims, seed=1, glen=True, relax_rate=RELAX_511)
+        model32.refined_solve(1e-30, max_outer=2, ncheck=CYCLE)
+        model32.set_option("refine_full_phase", 0)
+        model32.refined_solve(1e-30, max_outer=max(1, args.warmup), ncheck=CYCLE)
+        model32.synchronize()
         t0 = time.perf_counter()
-        A.check(A.lib().fi_refined_advance_inner(model._h, args.steps))
-        model.synchronize()
-        ms32 = (time.perf_counter() - t0) / args.steps * 1e3
-        teff32 = 23 * 4 * cells / (ms32 * 1e-3) / 1e9
-        ev32 = event_times(A, A.lib().fi_refined_time_inner, model, max(10, min(50, args.steps // 4)))
-        log("fp32 %.3f ms/iter, T_eff %.1f GB/s" % (ms32, teff32))
-        value, ms_step, dtype = teff32, ms32, "f32 sweeps in f64 defect correction"
-        # fused FP32 correction sweep: R P tau(6) V(3) eta fP g(3), W P tau(6) V(3) per interior cell
-        # the sweep processes (on planes of its column with a valid location; it reads and writes
-        # nothing elsewhere); shell stress: R P tau(6) V(3) eta fP, W P tau(6) per shell point
-        kname, kms, kbytes = "k_fused3<float, IB>", ev32[1], 25 * 4 * acells + 19 * 4 * shell_pts
+        outer, sweeps, _ = model32.refined_solve(1e-30, max_outer=args.steps, ncheck=CYCLE)
+        model32.synchronize()
+        t_total = time.perf_counter() - t0
+        cycles = sweeps / CYCLE
+        ms_step = t_total / cycles * 1e3
+        ev32 = event_times(A, A.lib().fi_refined_time_inner, model32, max(10, min(50, args.steps)))
+        model32.close()
+        # converged solves to FP64 residual 1e-8, plain FP64 (unless --no-solve)
+        # and FP32-refined, at the largest glacier size on which they converge
+        log("converged-solve comparison at %s" % (SOLVE_DIMS,))
+        solve, a = solve_comparison(pkg, SOLVE_DIMS, fp64=not args.no_solve)
+        a.close()
+        log("solve: %s" % json.dumps(solve))
+        sweep_ms = ev32[0]
+        outer_ms = (t_total * 1e3 - sweeps * sweep_ms) / outer
+        value = CYCLE * 23 * 4 * processed / (ms_step * 1e-3) / 1e9
+        log("refined cycle %.3f ms (%d sweeps, %d outer steps in %.3f s), %.1f GB/s over processed cells"
+            % (ms_step, sweeps, outer, t_total, value))
+        dtype = "f32 sweeps in f64 defect correction"
+        # dominant kernel, the fused FP32 correction sweep: 23 arrays of 4 B per
+        # processed cell (the reference's per-cell count; masks, forcing and the
+        # shell's stress points it also updates are not credited)
+        kname, kms, kbytes = "k_fused3<float, IB>", ev32[1], 23 * 4 * acells
         role = "fp32_fused"
         events = {"fp32_iteration": ev32[0], "fp32_fused": ev32[1], "fp32_shell": ev32[2], "fp32_bc": ev32[3],
                   "fp64_iteration": ev64[0], "fp64_fused": ev64[1], "fp64_shell": ev64[2], "fp64_bc": ev64[3]}
+        step_info = {"step": "one cycle of the FP32-refined solver: %d FP32 PT sweeps of the glacier, then the "
+                             "FP64 outer step" % CYCLE,
+                     "fp32_sweeps": int(sweeps), "outer_steps": int(outer), "timed_seconds": round(t_total, 4),
+                     "fp32_sweep_ms": round(float(sweep_ms), 4), "fp64_outer_step_ms": round(float(outer_ms), 4),
+                     "fp32_sweeps_per_s": round(sweeps / t_total, 2),
+                     "teff_reference_formula_gbs": round(CYCLE * 23 * 4 * cells / (ms_step * 1e-3) / 1e9, 2)}
     else:
-        value, ms_step, dtype = teff64, ms64, "f64"
-        # fused FP64 sweep: R P tau(6) V(3) eta [g(3)], W P tau(6) V(3) eta_next per interior cell;
-        # shell stress: R P tau(6) V(3) eta, W P tau(6) per shell point
-        kname, kms, kbytes = "k_fused3<double>", ev64[1], (22 + gfields) * 8 * icells + 18 * 8 * shell_pts
+        value, ms_step, dtype = 23 * 8 * processed / (ms64 * 1e-3) / 1e9, ms64, "f64"
+        # fused FP64 sweep: 23 arrays of 8 B per interior cell (the shell's
+        # stress points it also updates are not credited)
+        kname, kms, kbytes = "k_fused3<double>", ev64[1], 23 * 8 * icells
         role = "fp64_fused"
         events = {"fp64_iteration": ev64[0], "fp64_fused": ev64[1], "fp64_shell": ev64[2], "fp64_bc": ev64[3]}
+        step_info = {"step": "one FP64 PT iteration of the whole grid"}
     achieved = kbytes / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
     traffic, tsrc = profile_traffic(args.workload, role)
-
-    solve = None
-    if glacier and not args.no_solve:
-        sd = tuple(int(x) for x in args.solve_dims.split(","))
-        log("converged-solve comparison at %s" % (sd,))
-        solve = solve_comparison(pkg, sd)
-        log("solve: %s" % json.dumps(solve))
 
     cpu = None
     if not args.no_cpu:
@@ -696,22 +743,26 @@ def main():
         cpu = cpu_baseline(pkg, model, dims, args.cpu_seconds, glacier)
 
     config = {"workload": args.workload, "grid": list(dims),
-              "step": "one PT iteration of the whole grid" + (" (FP32 correction sweep)" if glacier else ""),
+              "value_is": "23 * sizeof(T) bytes (tm_stokes_mpi_wip.jl:180) per processed cell and PT sweep, over the "
+                          "step's wall time",
               "teff_pct_hbm_peak": round(100.0 * value / HBM_PEAK_GBS, 2),
-              "iterations_per_s": round(1e3 / ms_step, 2),
-              "fp64_iteration": {"ms": round(ms64, 4), "teff_gbs": round(teff64, 2),
-                                 "teff_pct_hbm_peak": round(100.0 * teff64 / HBM_PEAK_GBS, 2)},
+              "processed_cells": int(processed), "cells": int(cells),
+              "fp64_iteration": {"ms": round(ms64, 4), "iterations_per_s": round(1e3 / ms64, 2),
+                                 "teff_processed_gbs": round(23 * 8 * processed / (ms64 * 1e-3) / 1e9, 2),
+                                 "teff_reference_formula_gbs": round(teff64, 2)},
               "setup_s": round(setup_s, 2)}
+    config.update(step_info)
     if glacier:
         config["description"] = ("synthetic glacier, 8 Gaussian bumps (seed 1), parabolic surface, volume-fraction "
-                                 "immersed boundary, Glen n=3 (A=1, eta_reg=30, log relaxation 0.05), traction-free box")
+                                 "immersed boundary, Glen n=3 (A=1, eta_reg=30, log relaxation %g; 0.05 in the converged "
+                                 "comparison), traction-free box" % RELAX_511)
         config["converged_solve"] = solve
-        # the reference's T_eff counts every cell of the box; the sweep reads
-        # and writes only the cells on planes of their column that hold a valid
-        # location (the rest holds no valid location and stays 0)
-        config["empty_space"] = {"interior_cells": int(icells), "processed_cells": int(acells),
-                                 "processed_fraction": round(acells / icells, 4) if icells else None,
-                                 "teff_counts_skipped_cells": True}
+        # the reference's T_eff (teff_reference_formula_gbs) counts every cell
+        # of the box; the sweep reads and writes only the cells on planes of
+        # their column that hold a valid location (the rest holds no valid
+        # location and stays 0), and value counts only the processed cells
+        config["empty_space"] = {"interior_cells": int(icells), "processed_interior_cells": int(acells),
+                                 "processed_fraction": round(processed / cells, 4)}
     line = {
         "metric": "PT-Stokes T_eff GB/s (% HBM peak) at 511x511x255",
         "value": round(value, 3),
