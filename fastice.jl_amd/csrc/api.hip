@@ -207,6 +207,7 @@ int prepare(fi_model_s* m, bool sync_shadow)
         launch_ib_flags(m->g, na, ns, m->flags, m->stream);
         if (m->crange) launch_col_ranges(m->g, m->flags, m->crange, m->erange, m->stream);
         // locations may have changed validity: no buffer is known to be clean
+        m->resid_clean = false;
         m->zs[0] = m->zs[1] = m->zv[0] = m->zv[1] = false;
         m->zs32[0] = m->zs32[1] = m->zv32[0] = m->zv32[1] = false;
         m->gs[0] = m->gs[1] = m->gs32[0] = m->gs32[1] = false;
@@ -281,12 +282,18 @@ int residual_norms_enqueue(fi_model_s* m, bool sync_shadow)
         if (rc) return rc;
     }
     StokesArgs<double> a = make_args(m, true);
+    // positions outside the column plane ranges hold only invalid locations,
+    // whose residual is 0: once the fields hold it there, neither the
+    // residual kernel nor the norms visit them
+    const bool cols = ib && m->g.nd == 3 && m->crange;
+    a.skip_empty = cols && m->resid_clean;
     launch_residuals(m->g, a, ib, m->stream);
+    m->resid_clean = cols;
     run_bc(m, FI_BCGROUP_RESIDUAL);
     const double* f[4] = {m->f[FI_F_RP], m->f[FI_F_RVX], m->f[FI_F_RVY], m->f[FI_F_RVZ]};
     const int L[4] = {L_CCC, L_VCC, L_CVC, L_CCV};
     const int nf = m->g.nd == 3 ? 4 : 3;
-    launch_norms(m->g, f, L, nf, m->d_norm, m->stream);
+    launch_norms(m->g, f, L, nf, m->d_norm, m->stream, cols ? m->crange : nullptr);
     HIPCHK(hipMemcpyAsync(m->h_norm, m->d_norm, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, m->stream));
     return FI_OK;
 }
@@ -618,6 +625,7 @@ size_t fi_field_size(fi_model_t m, int id, int layout)
 static void mark_dirty(fi_model_t m, int id)
 {
     if (id >= FI_F_P && id <= FI_F_VZ) m->state_written();
+    if (id >= FI_F_RP && id <= FI_F_RVZ) m->resid_clean = false;
     if (id >= FI_F_OM_NA0 && id < FI_F_COUNT) {
         m->flags_dirty = true;
         ++m->mask_gen;

@@ -351,6 +351,10 @@ void launch_bc_packed(const PackedBcOp* ops, const int* blk2op, int nblocks, hip
 struct NormArgs {
     const double* f[4];
     int hi[4][3];
+    // 3D immersed boundary, or null: per column, the planes holding a valid
+    // location; the fields are known to be 0 at every other position, which
+    // cannot raise the maximum, so those are not read
+    const int2* crange;
 };
 
 __global__ __launch_bounds__(256) void k_norms(NormArgs a, long sy, long sz, int nd, unsigned long long* out)
@@ -365,6 +369,10 @@ __global__ __launch_bounds__(256) void k_norms(NormArgs a, long sy, long sz, int
         long r = t / hx;
         long j = r % hy + 1;
         long k = (nd == 3) ? r / hy + 1 : 0;
+        if (a.crange) {
+            const int2 rg = a.crange[i + sy * j];
+            if (k < rg.x || k > rg.y) continue;
+        }
         unsigned long long b = (unsigned long long)__double_as_longlong(f[i + sy * j + sz * k]) & 0x7fffffffffffffffull;
         m = b > m ? b : m;
     }
@@ -389,9 +397,10 @@ __global__ void k_zero_u64(unsigned long long* p, int n)
 }
 
 void launch_norms(const Geo& g, const double* const f[4], const int L[4], int nf, unsigned long long* out,
-                  hipStream_t s)
+                  hipStream_t s, const int2* crange)
 {
     NormArgs a;
+    a.crange = g.nd == 3 ? crange : nullptr;
     for (int q = 0; q < 4; ++q) {
         a.f[q] = f[q < nf ? q : 0];
         for (int d = 0; d < 3; ++d) a.hi[q][d] = (d < g.nd) ? g.n[d] + ((L[q < nf ? q : 0] >> d) & 1) : 1;

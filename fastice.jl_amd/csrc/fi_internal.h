@@ -97,6 +97,11 @@ struct StokesArgs {
     int z_wsig, z_wvel;
     const T* fP;          // forcing of the continuity equation (defect correction), or null
     int frozen_eta;       // 1: eta is fixed, eta_next is not written
+    // 3D immersed boundary: 1 when the outputs of compute_residuals! /
+    // compute_tau! (and the stress lag) already hold 0 at every position
+    // outside their column's crange; those kernels then skip such positions
+    // (every location there is invalid, so they would store 0 again)
+    int skip_empty;
     int nx, ny, nz;
     long sy, sz;
     T ix, iy, iz;
@@ -232,9 +237,10 @@ struct PackedBcOp {
     int start;
 };
 void launch_bc_packed(const PackedBcOp* ops, const int* blk2op, int nblocks, hipStream_t s);
-// Inf-norm of up to 4 fields over their interiors; out = 4 uint64 (|x| bit patterns, NaN largest)
+// Inf-norm of up to 4 fields over their interiors; out = 4 uint64 (|x| bit patterns, NaN largest);
+// crange (3D, or null): per-column plane ranges outside which every field is known to be 0
 void launch_norms(const Geo& g, const double* const f[4], const int L[4], int nf, unsigned long long* out,
-                  hipStream_t s);
+                  hipStream_t s, const int2* crange = nullptr);
 void launch_fill(double* p, size_t n, double v, hipStream_t s);
 
 // level sets (levelset_kernels.hip)

@@ -59,6 +59,10 @@ struct fi_model_s {
     bool gs[2] = {false, false};
     bool gs32[2] = {false, false};
     int mask_gen = 0; // bumped whenever a mask component changes
+    // 3D immersed boundary: the residual fields hold 0 at every position
+    // outside their column's plane range (set by a residual computation over
+    // the whole grid; cleared by a mask change or an upload into them)
+    bool resid_clean = false;
     int use_fused = 1;
     int refine_full_phase = 1;      // nonlinear refined solves start with a full FP32 PT phase
     double refine_eta_rate = 1.0;   // log-space relaxation of the outer viscosity update (initial)

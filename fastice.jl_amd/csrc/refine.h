@@ -38,6 +38,11 @@ struct RefineJob {
     double norms[4] = {0, 0, 0, 0};
     bool converged = false;
     bool pending_outer = false; // the enqueued half was a lagged-PT outer step
+    // lagged PT, 3D immersed boundary with traction on every face: an outer
+    // step over the whole grid has run in this job, so the FP32 corrections,
+    // the stress lag and the FP64 stress are 0 outside the column ranges and
+    // later outer steps visit the ranges only
+    bool cols_clean = false;
 };
 
 // job control: begin (may finish at once), sweeps of the pending chunk, and

@@ -132,6 +132,67 @@ __global__ __launch_bounds__(256) void k_add_correction(double* x, const float* 
         x[t] = x[t] + (double)d[t];
 }
 
+// ---------------------------------------------------------------------------
+// column-restricted kernels of the lagged-PT outer step (3D immersed
+// boundary, traction on every face): a thread owns column (i, j), i in
+// [0, nx+2], j in [0, ny+2], and the planes of one z chunk that lie in the
+// column's range (crange: planes holding a valid location; erange: planes
+// whose viscosity a valid location reads).  Elsewhere every location is
+// invalid: the FP32 corrections, the stress lag and the residuals are 0 there
+// and stay 0, so the outer step need not visit it.
+// ---------------------------------------------------------------------------
+constexpr int COL_KCH = 16; // planes per thread and chunk
+
+struct ColGeom {
+    int nx3, ny3, nz3; // n + 3 per dimension
+    long sy, sz;
+};
+
+__device__ __forceinline__ bool col_span(const ColGeom& g, const int2* rng, int& i, int& j, int& klo, int& khi)
+{
+    i = blockIdx.x * blockDim.x + threadIdx.x;
+    j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= g.nx3 || j >= g.ny3) return false;
+    const int2 rg = rng[(long)i + g.sy * j];
+    const int k0 = blockIdx.z * COL_KCH;
+    klo = max(k0, rg.x);
+    khi = min(min(k0 + COL_KCH - 1, g.nz3 - 1), rg.y);
+    return klo <= khi;
+}
+
+struct LagApply {
+    double* x[10];     // FP64 state (current buffers)
+    const float* d[10]; // the correction of the chunk (current FP32 buffers)
+    float* z0[10];      // both FP32 buffers, zeroed for the next chunk
+    float* z1[10];
+    int nf;
+};
+
+// x += d for every state field, and both correction buffers back to 0
+__global__ __launch_bounds__(256) void k_lag_apply(LagApply a, const int2* crange, ColGeom g)
+{
+    int i, j, klo, khi;
+    if (!col_span(g, crange, i, j, klo, khi)) return;
+    for (int k = klo; k <= khi; ++k) {
+        const long c = (long)i + g.sy * j + g.sz * k;
+        for (int f = 0; f < a.nf; ++f) {
+            a.x[f][c] = a.x[f][c] + (double)a.d[f][c];
+            a.z0[f][c] = 0.0f;
+            a.z1[f][c] = 0.0f;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_cast_cols(float* dst, const double* src, double scale, const int2* rng, ColGeom g)
+{
+    int i, j, klo, khi;
+    if (!col_span(g, rng, i, j, klo, khi)) return;
+    for (int k = klo; k <= khi; ++k) {
+        const long c = (long)i + g.sy * j + g.sz * k;
+        dst[c] = (float)(scale * src[c]);
+    }
+}
+
 // Anderson: per-block partial sums of dF_a . dF_b (a <= b < c) and dF_a . f
 // for one field, in a fixed block order (deterministic after the host sum)
 constexpr int AA_DOT_BLOCKS = 512;
@@ -556,6 +617,35 @@ static void cast(fi_model_s* m, float* dst, const double* src, double scale)
                        m->g.elems);
 }
 
+static ColGeom col_geom(const fi_model_s* m)
+{
+    return ColGeom{m->g.n[0] + 3, m->g.n[1] + 3, m->g.n[2] + 3, m->g.sy, m->g.sz};
+}
+
+static dim3 col_grid(const fi_model_s* m)
+{
+    return dim3((unsigned)((m->g.n[0] + 3 + 63) / 64), (unsigned)((m->g.n[1] + 3 + 3) / 4),
+                (unsigned)((m->g.n[2] + 3 + COL_KCH - 1) / COL_KCH));
+}
+
+static void cast_cols(fi_model_s* m, float* dst, const double* src, double scale, const int2* rng)
+{
+    hipLaunchKernelGGL(k_cast_cols, col_grid(m), dim3(64, 4), 0, m->stream, dst, src, scale, rng, col_geom(m));
+}
+
+// The lagged-PT outer step may visit only the column ranges: a 3D immersed
+// boundary with traction on every face.  Then the boundary conditions set no
+// velocity ghost and set every stress ghost from interior values after each
+// update, so ghost positions outside the ranges never need zeroing.
+static bool lag_cols(const fi_model_s* m)
+{
+    if (!(m->desc.immersed_boundary && m->g.nd == 3 && m->crange)) return false;
+    for (int d = 0; d < 3; ++d)
+        for (int s = 0; s < 2; ++s)
+            if (m->desc.bc_kind[d][s] != FI_BC_TRACTION) return false;
+    return true;
+}
+
 static void quantise_mask(fi_model_s* m, uint8_t* dst, const double* src)
 {
     hipLaunchKernelGGL(k_quantise_mask, ew_grid(m->g.elems), dim3(256), 0, m->stream, dst - m->g.off, src - m->g.off,
@@ -874,17 +964,22 @@ static int job_outer_lag_enqueue(fi_model_s* m, RefineJob& j)
         launch_bc_dims(m->g, b, m->stream);
     }
     j.since_eta = 0;
-    for (int id = FI_F_P; id <= FI_F_VZ; ++id)
-        if (m->f[id]) {
-            fill32(m, r->d[id][0], 0.0f);
-            fill32(m, r->d[id][1], 0.0f);
-        }
+    // column ranges only: the last chunk's k_lag_apply has zeroed the FP32
+    // buffers there, and they hold 0 elsewhere
+    const bool cols = j.cols_clean;
+    if (!cols)
+        for (int id = FI_F_P; id <= FI_F_VZ; ++id)
+            if (m->f[id]) {
+                fill32(m, r->d[id][0], 0.0f);
+                fill32(m, r->d[id][1], 0.0f);
+            }
     r->sig_cur = r->v_cur = 0;
     // the lag written below is +0 at every invalid location, like the fill
     m->zs32[0] = m->zs32[1] = m->zv32[0] = m->zv32[1] = true;
     m->gs32[0] = m->gs32[1] = false;
     {
         StokesArgs<double> a = make_args(m, true);
+        a.skip_empty = cols;
         const int tid[6] = {FI_F_TXX, FI_F_TYY, FI_F_TZZ, FI_F_TXY, FI_F_TXZ, FI_F_TYZ};
         TauLag lg;
         for (int q = 0; q < 6; ++q) lg.t[q] = m->f[tid[q]] ? r->d[tid[q]][0] : nullptr;
@@ -895,11 +990,20 @@ static int job_outer_lag_enqueue(fi_model_s* m, RefineJob& j)
     bc32(m, FI_BCGROUP_STRESS); // homogeneous boundary values of the lag
     int rc = residual_norms_enqueue(m, false);
     if (rc) return rc;
-    // forcing and frozen viscosity of the next chunk (unused if the solve ends here)
-    cast(m, r->eta, m->eta(), 1.0);
-    cast(m, r->fP, m->f[FI_F_RP], 1.0);
+    // forcing and frozen viscosity of the next chunk (unused if the solve ends
+    // here); with column ranges, the forcing where a valid location reads it
+    // and the viscosity on the planes a valid location's averages reach
     const int idR[3] = {FI_F_RVX, FI_F_RVY, FI_F_RVZ};
-    for (int q = 0; q < nd; ++q) cast(m, r->g[q], m->f[idR[q]], -1.0);
+    if (cols) {
+        cast_cols(m, r->eta, m->eta(), 1.0, m->erange);
+        cast_cols(m, r->fP, m->f[FI_F_RP], 1.0, m->crange);
+        for (int q = 0; q < nd; ++q) cast_cols(m, r->g[q], m->f[idR[q]], -1.0, m->crange);
+    } else {
+        cast(m, r->eta, m->eta(), 1.0);
+        cast(m, r->fP, m->f[FI_F_RP], 1.0);
+        for (int q = 0; q < nd; ++q) cast(m, r->g[q], m->f[idR[q]], -1.0);
+    }
+    j.cols_clean = lag_cols(m);
     return FI_OK;
 }
 
@@ -1005,10 +1109,26 @@ int job_chunk_enqueue(fi_model_s* m, RefineJob& j)
         // lagged PT: the end of every ncheck sweeps is an outer step
         j.inner_total += j.done;
         j.since_eta += j.done;
-        for (int id = FI_F_P; id <= FI_F_VZ; ++id)
-            if (m->f[id])
-                hipLaunchKernelGGL(k_add_correction, ew_grid(m->g.elems), dim3(256), 0, m->stream, m->buf(id) - m->g.off,
-                                   m->rf->cur(id) - m->g.off, m->g.elems);
+        if (j.cols_clean) {
+            // corrections are non-zero only on the column ranges: add them
+            // there and zero both FP32 buffers for the next chunk in one pass
+            LagApply la;
+            std::memset(&la, 0, sizeof(la));
+            for (int id = FI_F_P; id <= FI_F_VZ; ++id)
+                if (m->f[id]) {
+                    la.x[la.nf] = m->buf(id);
+                    la.d[la.nf] = m->rf->cur(id);
+                    la.z0[la.nf] = m->rf->d[id][0];
+                    la.z1[la.nf] = m->rf->d[id][1];
+                    ++la.nf;
+                }
+            hipLaunchKernelGGL(k_lag_apply, col_grid(m), dim3(64, 4), 0, m->stream, la, m->crange, col_geom(m));
+        } else {
+            for (int id = FI_F_P; id <= FI_F_VZ; ++id)
+                if (m->f[id])
+                    hipLaunchKernelGGL(k_add_correction, ew_grid(m->g.elems), dim3(256), 0, m->stream,
+                                       m->buf(id) - m->g.off, m->rf->cur(id) - m->g.off, m->g.elems);
+        }
         m->state_written();
         ++j.outer;
         j.pending_outer = true;

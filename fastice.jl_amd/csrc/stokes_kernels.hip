@@ -1180,10 +1180,18 @@ __global__ __launch_bounds__(TX * TY, MINW) void k_fused3_packed(const PackedIns
     const long c = (long)i + sy * j + sz * k;
 
 // compute_residuals!: FullStokes kernels_3d.jl:66-75, IB kernels_3d.jl:118-143
+// positions of a column outside its plane range hold only invalid locations
+__device__ __forceinline__ bool outside_range(const int2* crange, int i, int j, int k, long sy)
+{
+    const int2 rg = crange[(long)i + sy * j];
+    return k < rg.x || k > rg.y;
+}
+
 template <typename T, bool IB>
 __global__ __launch_bounds__(256) void k_residuals3(StokesArgs<T> a)
 {
     IDX3
+    if (IB && a.skip_empty && outside_range(a.crange, i, j, k, sy)) return;
     const T ix = a.ix, iy = a.iy, iz = a.iz;
     const T *P = a.P, *txx = a.txx, *tyy = a.tyy, *tzz = a.tzz, *txy = a.txy, *txz = a.txz, *tyz = a.tyz;
     const T *Vx = a.Vx, *Vy = a.Vy, *Vz = a.Vz;
@@ -1249,6 +1257,7 @@ template <typename T, bool IB, bool LAG>
 __global__ __launch_bounds__(256) void k_tau3(StokesArgs<T> a, TauLag lg)
 {
     IDX3
+    if (IB && a.skip_empty && outside_range(a.crange, i, j, k, sy)) return;
     const T ix = a.ix, iy = a.iy, iz = a.iz;
     const T *Vx = a.Vx, *Vy = a.Vy, *Vz = a.Vz, *eta = a.eta;
     const auto* ns = a.ns;
