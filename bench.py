@@ -242,15 +242,14 @@ def bench_batch(pkg, args):
     outer steps fall on the same batch steps; fp64: plain FP64 PT to 1e-8 or
     the driver's niter = 50n with ncheck = 5n."""
     refined = args.batch_precision == "refined"
-    rng = np.random.default_rng(2024)
-    dims = [tuple(int(x) for x in rng.integers(48, 193, 3)) for _ in range(96)]
-    h = 2.0 / 192
+    specs = pkg.batch_glaciers(96, 48, 192)
+    dims = [d for d, _ in specs]
     cells = sum(d[0] * d[1] * d[2] for d in dims)
     log("setting up 96 glaciers, %.1f M cells in total (%s)" % (cells / 1e6, args.batch_precision))
     t0 = time.perf_counter()
     inst = []
-    for i, d in enumerate(dims):
-        m, cad = pkg.glacier_3d(d, seed=100 + i, glen=True, lx=d[0] * h, ly=d[1] * h, H=d[2] * h)
+    for d, geo in specs:
+        m, cad = pkg.glacier_3d(d, glen=True, **geo)
         inst.append((m, cad["ncheck"], cad["niter"]))
     for m, _, _ in inst:
         m.synchronize()

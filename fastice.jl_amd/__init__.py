@@ -21,8 +21,8 @@ from .geometries import (AABB, DataElevation, GaussianGlacier, SyntheticElevatio
 from .grids import Center, UniformGrid, Vertex, flip
 from .level_sets import compute_levelset_from_dem, compute_levelset_from_polygon, invert_levelset, omega_from_psi
 from .physics import LinearViscousRheology, PowerLawRheology, Relaxation, ViscoplasticRegularisation
-from .problems import (glacier_2d, glacier_2d_spec, glacier_3d, glacier_3d_spec, manufactured_2d, manufactured_3d,
-                       pt_params)
+from .problems import (batch_glaciers, glacier_2d, glacier_2d_spec, glacier_3d, glacier_3d_spec, manufactured_2d,
+                       manufactured_3d, pt_params)
 from .serving import Request, Server, percentiles, poisson_requests
 from . import _hdf5, heat, writers
 from .heat import HeatDiffusionModel

@@ -49,6 +49,7 @@ struct BatchTables {
     PackedBcOp* bc[2][2][3] = {}; // parity, group (0 stress, 1 velocity), dim
     int* blk2bc[2][2][3] = {};
     int nblk_bc[2][2][3] = {};
+    unsigned gen = 0; // bumped by every build: graphs record the generation they were captured against
 };
 
 } // namespace fi
@@ -63,6 +64,10 @@ struct fi_batch_s {
     int phase = 0; // iterations since the tables were built, mod 2
     BatchTables t;
     hipGraphExec_t graph[2] = {nullptr, nullptr};
+    // what each graph was captured against: the table generation and the
+    // running models (their device buffers are baked into the graph's launches)
+    unsigned graph_gen[2] = {0, 0};
+    std::vector<fi_model_s*> graph_models;
     int tx = 64, ty = 8, minw = 1, kch = 32; // fused variant and z chunk
     int refined = -1; // -1: no model yet, 0: FP64 solves, 1: refined solves
     int order = 2;                  // workgroup-to-tile order of the packed fused sweep
@@ -80,6 +85,8 @@ namespace fi {
 
 static void destroy_graphs(fi_batch_s* b)
 {
+    // a replay still in flight must end before its executable goes
+    if (b->graph[0] || b->graph[1]) (void)hipStreamSynchronize(b->stream);
     for (int p = 0; p < 2; ++p)
         if (b->graph[p]) {
             (void)hipGraphExecDestroy(b->graph[p]);
@@ -375,6 +382,7 @@ static int build_tables_t(fi_batch_s* b, int ty, int kch)
     b->kch = kch;
     b->phase = 0;
     b->rebuilds++;
+    ++t.gen;
     return FI_OK;
 }
 
@@ -434,9 +442,17 @@ static void launch_iteration(fi_batch_s* b, int p)
     launch_bcs(b, p, 1);
 }
 
+static std::vector<fi_model_s*> running_models(const fi_batch_s* b)
+{
+    std::vector<fi_model_s*> v;
+    for (int r : b->run) v.push_back(b->inst[r].m);
+    return v;
+}
+
 static int capture_graphs(fi_batch_s* b)
 {
     destroy_graphs(b);
+    b->graph_models.clear();
     if (b->graph_iters <= 0) return FI_OK;
     for (int p = 0; p < 2; ++p) {
         hipGraph_t g = nullptr;
@@ -446,7 +462,28 @@ static int capture_graphs(fi_batch_s* b)
         hipError_t e = hipGraphInstantiate(&b->graph[p], g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
         if (e != hipSuccess) return fail(FI_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+        b->graph_gen[p] = b->t.gen;
     }
+    b->graph_models = running_models(b);
+    return FI_OK;
+}
+
+// Replay the graph of parity p, capturing both graphs first when there are
+// none for the current tables (capture is lazy: a running set that changes
+// before its graphs would be replayed never pays for them).  A graph's
+// launches point into the table buffer and the member models' fields: one
+// captured against other tables or another running set is never replayed
+// (the tables are rewritten in place, and removed models may be destroyed).
+static int graph_launch(fi_batch_s* b, int p)
+{
+    if (!b->graph[0] || !b->graph[1]) {
+        int rc = capture_graphs(b);
+        if (rc) return rc;
+    }
+    if (!b->graph[p] || b->graph_gen[p] != b->t.gen || b->graph_models != running_models(b))
+        return fail(FI_ERR_STATE, "batch graph %d is stale (captured at table generation %u, tables at %u)", p,
+                    b->graph_gen[p], b->t.gen);
+    HIPCHK(hipGraphLaunch(b->graph[p], b->stream));
     return FI_OK;
 }
 
@@ -463,6 +500,8 @@ static long running_cells(fi_batch_s* b)
 // Time the fused sweep of every tile candidate on genuine iterations of the
 // running set (the tile shape does not change any result) and keep the
 // fastest.  Uses at most `budget` iterations; returns how many it ran.
+static std::vector<unsigned> fp64_zbits(fi_batch_s* b);
+
 static int autotune(fi_batch_s* b, int budget, int* used)
 {
     *used = 0;
@@ -526,7 +565,11 @@ static int autotune(fi_batch_s* b, int budget, int* used)
     b->order = cands[best].order;
     int rc = build_tables(b, cands[best].ty, cands[best].kch);
     if (rc) return rc;
-    return capture_graphs(b);
+    destroy_graphs(b); // captured when first replayed
+    // the tuning iterations changed the models' clean-buffer bits, and the
+    // tables just built hold the new ones
+    b->zbuilt = fp64_zbits(b);
+    return FI_OK;
 }
 
 static std::vector<unsigned> fp64_zbits(fi_batch_s* b)
@@ -547,8 +590,7 @@ static int rebuild(fi_batch_s* b)
     }
     int rc = build_tables(b, b->ty, b->kch);
     if (rc) return rc;
-    rc = capture_graphs(b);
-    if (rc) return rc;
+    destroy_graphs(b); // captured when first replayed
     b->zbuilt = fp64_zbits(b);
     b->dirty = false;
     return FI_OK;
@@ -612,14 +654,14 @@ static int step_refined(fi_batch_s* b, int* n_finished)
     if (b->dirty || sg != b->sig) {
         int rc = build_tables(b, b->ty, b->kch);
         if (rc) return rc;
-        rc = capture_graphs(b);
-        if (rc) return rc;
+        destroy_graphs(b); // captured when first replayed
         b->sig = sg;
         b->dirty = false;
     }
     int done = 0, p = 0;
-    while (b->graph_iters > 0 && b->graph[p] && delta - done >= b->graph_iters) {
-        HIPCHK(hipGraphLaunch(b->graph[p], b->stream));
+    while (b->graph_iters > 0 && delta - done >= b->graph_iters) {
+        int rc = graph_launch(b, p);
+        if (rc) return rc;
         done += b->graph_iters;
         p ^= (b->graph_iters & 1);
     }
@@ -684,7 +726,8 @@ static int step(fi_batch_s* b, int* n_finished)
     }
     int left = delta - done;
     while (b->graph_iters > 0 && left >= b->graph_iters) {
-        HIPCHK(hipGraphLaunch(b->graph[b->phase], b->stream));
+        int rc = graph_launch(b, b->phase);
+        if (rc) return rc;
         left -= b->graph_iters;
         b->phase ^= (b->graph_iters & 1);
     }

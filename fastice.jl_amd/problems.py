@@ -209,6 +209,20 @@ def glacier_3d(dims, seed=1, glen=True, eta_reg=30.0, relax_rate=0.05, lx=2.0, l
     return model, cad
 
 
+def batch_glaciers(count=96, lo=48, hi=192, seed=2024):
+    """The uneven glacier set of the batched workload: count instances whose
+    cell counts are drawn uniformly in lo..hi per axis (seeded), cubic cells
+    of side 2 / hi (so the domain's aspect ratio is the grid's), glacier
+    seeds 100, 101, ...  Returns [(dims, dict(seed, lx, ly, H))]."""
+    rng = np.random.default_rng(seed)
+    h = 2.0 / hi
+    out = []
+    for i in range(count):
+        d = tuple(int(x) for x in rng.integers(lo, hi + 1, 3))
+        out.append((d, dict(seed=100 + i, lx=d[0] * h, ly=d[1] * h, H=d[2] * h)))
+    return out
+
+
 def glacier_2d_spec(dims, seed=1, glen=True, eta_reg=30.0, relax_rate=0.05, lx=2.0, H=1.0, y0=0.0):
     """Model arguments, cadence and the two polygons of a 2D glacier: the
     vertical section y = y0 of the seeded synthetic glacier on

@@ -49,7 +49,8 @@ def test_refined_manufactured_3d_matches_fp64(pkg):
     assert max(na) < 1e-8, na
     errs = _field_errors(a, b, state_names(3))
     _say("outer %d inner %d fp64 iters %d errors %s" % (outer, inner, it, errs))
-    assert max(errs.values()) < 1e-5
+    # both states satisfy the discrete equations to 1e-8: measured differences ~1e-9
+    assert max(errs.values()) < 1e-7, errs
     a.close()
     b.close()
 
@@ -66,7 +67,7 @@ def test_refined_manufactured_2d(pkg):
     outer, inner, na = a.refined_solve(1e-9)
     assert max(na) < 1e-9 and max(nb) < 1e-9
     errs = _field_errors(a, b, state_names(2))
-    assert max(errs.values()) < 1e-6, errs
+    assert max(errs.values()) < 1e-7, errs
 
 
 @pytest.mark.parametrize("mode", [1, 0])
@@ -83,7 +84,10 @@ def test_refined_glacier_matches_fp64(pkg, glen, mode):
     assert max(na) < 1e-8, na
     errs = _field_errors(a, b, ["P", "TXX", "TXY", "VX", "VY", "VZ"])
     _say("errors %s" % errs)
-    assert max(errs.values()) < 1e-4
+    # both states satisfy the discrete equations to residual 1e-8; measured
+    # differences up to 1.2e-7 (pressure of the linear glacier, whose Stokes
+    # operator on cut cells is the worst conditioned of these cases)
+    assert max(errs.values()) < 1e-6, errs
     a.close()
     b.close()
 
