@@ -10,7 +10,7 @@ Workloads
       PT sweeps, then its FP64 outer step); the FP64 iteration of the same
       model is timed beside it, and a converged-solve comparison (FP32-refined
       vs plain FP64, both to FP64 residual 1e-8) runs on the same glacier at
-      255x255x127.
+      383x383x191.
   stokes3d_fp64_511x511x255: isoviscous free-slip box with manufactured
       forcing, plain FP64 iteration.
   batch96_glacier: 96 seeded glaciers, each dimension uniform in 48..192
@@ -32,9 +32,10 @@ Workloads
   refined_solve_255x255x127, fp64_solve_255x255x127: one of the two solves
       of the default line's converged-solve comparison, alone; value = seconds.
   serving_poisson40: Poisson arrivals at 40/s, half 2D glacier sections
-      (n x n/2, n in 64..511), half 3D glaciers (n x n x n/2, n in 32..127),
-      FP64 to 1e-8 or niter = 50n, continuously batched; value = completed
-      solves/s with p50/p95/p99 latency.
+      (n x n/2, n in 64..511), half 3D glaciers (n x n x n/2, n in 64..511,
+      --serving-3d), FP64 to 1e-8 or 100n iterations, continuously batched;
+      value = solves/s of the requests that reached 1e-8, with p50/p95/p99
+      latency and the counts that hit the cap or failed.
 
 value = T_eff over the cells a step processes: 23 * sizeof(T) bytes per
 processed cell and PT sweep (the reference's per-cell count,
@@ -104,6 +105,8 @@ def parse():
     p.add_argument("--serving-precision", default="fp64", choices=("fp64", "refined"),
                    help="serving_poisson40: plain FP64 with the driver's cadence (default) or FP32-refined solves")
     p.add_argument("--requests", type=int, default=200, help="serving_poisson40: number of requests")
+    p.add_argument("--serving-3d", type=int, nargs=2, default=(64, 511), metavar=("LO", "HI"),
+                   help="serving_poisson40: range of n of the 3D requests (n x n x ceil(n/2))")
     p.add_argument("--copy-calib", action="store_true",
                    help="run a 1 GiB device copy first (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     return p.parse_args()
@@ -569,7 +572,7 @@ def heat_cpu_baseline(seconds):
 
 def bench_serving(pkg, args):
     """Continuous-batching service under Poisson arrivals at 40/s."""
-    reqs = pkg.poisson_requests(args.requests, 40.0, seed=7)
+    reqs = pkg.poisson_requests(args.requests, 40.0, seed=7, range3d=tuple(args.serving_3d))
     log("serving %d requests (%d 3D), arrivals over %.1f s" % (len(reqs), sum(r.ndims == 3 for r in reqs),
                                                                 reqs[-1].t_arrival))
     srv = pkg.Server(pkg, max_slots=96, precision=args.serving_precision)
@@ -589,11 +592,11 @@ def bench_serving(pkg, args):
     lat2 = [r["latency_s"] for r in recs if r["ndims"] == 2]
     config = {"workload": "serving_poisson40", "requests": len(reqs), "arrival_rate_per_s": 40.0,
               "offered_window_s": round(reqs[-1].t_arrival, 3),
-              "mix": "p=0.5 3D glacier (n log-uniform 32..127, n x n x ceil(n/2)) / 2D glacier section "
-                     "(n log-uniform 64..511, n x ceil(n/2)); Glen n=3, "
+              "mix": "p=0.5 3D glacier (n log-uniform %d..%d, n x n x ceil(n/2)) / 2D glacier section "
+                     "(n log-uniform 64..511, n x ceil(n/2)); Glen n=3, " % tuple(args.serving_3d)
                      + ("FP32 sweeps in FP64 defect correction (lagged PT, 32 sweeps per outer step) to residual "
                         "1e-8 or 400n sweeps" if args.serving_precision == "refined" else
-                        "FP64 to residual 1e-8 or niter 50n, ncheck 5n"),
+                        "FP64 to residual 1e-8 or 100n iterations, ncheck 5n"),
               "precision": args.serving_precision,
               "latency_s": {k: (round(v, 4) if v is not None else None) for k, v in s["latency_s"].items()},
               "latency_s_3d": {k: round(v, 4) for k, v in pkg.percentiles(lat3).items()} if lat3 else None,
@@ -602,6 +605,11 @@ def bench_serving(pkg, args):
                                 "makespan_s": round(off["makespan_s"], 3)},
               "serving_vs_offline": round(s["solves_per_s"] / off["solves_per_s"], 3),
               "makespan_s": round(s["makespan_s"], 3), "converged": s["converged"],
+              "max_iter": s["max_iter"], "failed": s["failed"],
+              "requests_per_s": round(s["requests_per_s"], 4),
+              "latency_s_converged": {k: (round(v, 4) if v is not None else None)
+                                      for k, v in s["latency_s_converged"].items()},
+              "value_counts": "requests that reached residual 1e-8",
               "batch_steps": s["batch_steps"], "setup_s_mean": round(s["setup_s_mean"], 4)}
     line = _line("serving solves/s (p99 latency in config), Poisson 40/s", s["solves_per_s"], "solves/s",
                  s["makespan_s"] * 1e3, 1, 0, "f32+f64" if args.serving_precision == "refined" else "f64", config)

@@ -21,7 +21,7 @@ def test_poisson_requests_deterministic_and_in_range(pkg):
     for r in a:
         if r.ndims == 3:
             s = r.dims[0]
-            assert 32 <= s <= 127 and r.dims == (s, s, (s + 1) // 2)
+            assert 64 <= s <= 511 and r.dims == (s, s, (s + 1) // 2)
         else:
             s = r.dims[0]
             assert 64 <= s <= 511 and r.dims == (s, (s + 1) // 2)
