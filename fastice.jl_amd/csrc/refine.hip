@@ -333,6 +333,25 @@ static int alloc32(fi_model_s* m, float** out)
     return FI_OK;
 }
 
+// The two buffers of a correction field are one allocation, the second
+// pair_stride elements after the first (the same for every field): the fused
+// FP32 immersed-boundary sweep addresses both through one base pointer.
+static size_t pair_stride(const fi_model_s* m) { return (m->g.elems + 63) / 64 * 64; }
+
+static int alloc32_pair(fi_model_s* m, float** b0, float** b1)
+{
+    Refine* r = m->rf;
+    float* base = nullptr;
+    const size_t s = pair_stride(m);
+    if (hipMalloc(&base, 2 * s * sizeof(float)) != hipSuccess)
+        return fail(FI_ERR_OOM, "FP32 workspace allocation failed");
+    r->allocs.push_back(base);
+    HIPCHK(hipMemsetAsync(base, 0, 2 * s * sizeof(float), m->stream));
+    *b0 = base + m->g.off;
+    *b1 = base + s + m->g.off;
+    return FI_OK;
+}
+
 static int alloc_mask8(fi_model_s* m, uint8_t** out)
 {
     Refine* r = m->rf;
@@ -353,7 +372,7 @@ static int ensure_refine(fi_model_s* m)
     int rc = FI_OK;
     for (int id = FI_F_P; id <= FI_F_VZ && !rc; ++id)
         if (m->f[id])
-            for (int b = 0; b < 2 && !rc; ++b) rc = alloc32(m, &r->d[id][b]);
+            rc = alloc32_pair(m, &r->d[id][0], &r->d[id][1]);
     if (!rc) rc = alloc32(m, &r->eta);
     if (!rc) rc = alloc32(m, &r->eta2);
     if (!rc && m->f[FI_F_RHEO]) rc = alloc32(m, &r->rh);
@@ -404,6 +423,9 @@ static StokesArgs<float> make_args32(fi_model_s* m, bool in_place)
     const fi_model_desc& d = m->desc;
     StokesArgs<float> a;
     std::memset(&a, 0, sizeof(a));
+    // every stress field reads buffer sig_cur and every velocity field buffer
+    // v_cur of its pair (alloc32_pair): the fused immersed-boundary sweep
+    // relies on one input and one output offset per group
     auto in = [&](int id) { return r->cur(id); };
     auto out = [&](int id) { return in_place ? r->cur(id) : r->alt(id); };
     a.P = in(FI_F_P); a.txx = in(FI_F_TXX); a.tyy = in(FI_F_TYY); a.tzz = in(FI_F_TZZ);

@@ -422,6 +422,7 @@ __device__ __forceinline__ void fused3_tile(const StokesArgs<T>& a, const FusedG
     T oP = (n0 & 1u) ? ldo(a.P, c1) : T(0), oxx = T(0), oyy = T(0), ozz = (n0 & 8u) ? ldo(a.tzz, c1) : T(0),
       oxy = T(0), oxz = T(0), oyz = T(0);
     T gx = T(0), gy = T(0), gz = T(0);
+    T ofP = a.fP ? ldo(a.fP, c1) : T(0);
     T Pk = T(0), xxk = T(0), yyk = T(0), zzk = T(0), Pkm = T(0), zzkm = T(0);
     sE[(ks - 2) % 3][ty][tx] = em;
 
@@ -434,9 +435,10 @@ __device__ __forceinline__ void fused3_tile(const StokesArgs<T>& a, const FusedG
         sE[kk % 3][ty][tx] = e0;
         // prefetch plane kk+1 (velocity z at kk+2, gravity at kk)
         T nvx = T(0), nvy = T(0), nvz2 = T(0), ne = T(0), nP = T(0), nxx = T(0), nyy = T(0), nzz = T(0), nxy = T(0),
-          nxz = T(0), nyz = T(0), ngx = T(0), ngy = T(0), ngz = T(0);
+          nxz = T(0), nyz = T(0), ngx = T(0), ngy = T(0), ngz = T(0), nfP = T(0);
         if (kk <= ke) {
             const uint32_t cn = c + bsz;
+            if (a.fP) nfP = ldo(a.fP, cn);
             nvx = ldo(a.Vx, cn);
             nvy = ldo(a.Vy, cn);
             nvz2 = ldo(a.Vz, cn + bsz);
@@ -469,7 +471,7 @@ __device__ __forceinline__ void fused3_tile(const StokesArgs<T>& a, const FusedG
                 T eyy = dif(sVy[ty + 1][tx], vy0, iy);
                 T ezz = dif(vzp, vz0, iz);
                 T divV = exx + eyy + ezz;
-                Pc = upd_P(oP, a.fP ? divV + ldo(a.fP, c) : divV, e0, a.dPr);
+                Pc = upd_P(oP, a.fP ? divV + ofP : divV, e0, a.dPr);
                 xxc = upd_td(oxx, e0, exx, divV, a.dtau[0]);
                 yyc = upd_td(oyy, e0, eyy, divV, a.dtau[1]);
                 zzc = upd_td(ozz, e0, ezz, divV, a.dtau[2]);
@@ -498,7 +500,7 @@ __device__ __forceinline__ void fused3_tile(const StokesArgs<T>& a, const FusedG
                 T eyy = dif(ldo(wy, c + bsy) * sVy[ty + 1][tx], wy0 * vy0, iy);
                 T ezz = dif(ldo(wz, c + bsz) * vzp, wz0 * vz0, iz);
                 T divV = exx + eyy + ezz;
-                Pc = upd_P(oP, a.fP ? divV + ldo(a.fP, c) : divV, e0, a.dPr);
+                Pc = upd_P(oP, a.fP ? divV + ofP : divV, e0, a.dPr);
                 xxc = upd_td(oxx, e0, exx, divV, a.dtau[0]);
                 yyc = upd_td(oyy, e0, eyy, divV, a.dtau[1]);
                 zzc = upd_td(ozz, e0, ezz, divV, a.dtau[2]);
@@ -608,6 +610,7 @@ __device__ __forceinline__ void fused3_tile(const StokesArgs<T>& a, const FusedG
         oxy = nxy;
         oxz = nxz;
         oyz = nyz;
+        ofP = nfP;
         gx = ngx;
         gy = ngy;
         gz = ngz;
@@ -652,7 +655,7 @@ __device__ __forceinline__ void fused3_ib_tile(const StokesArgs<T>& a, const Fus
 {
     // FP32 sweeps read masks and flags from the packed words (a.mw)
     constexpr bool PACKED = sizeof(T) == 4;
-    __shared__ T sVx[TY][TX], sVy[TY][TX], sVz[TY][TX];
+    __shared__ T sVx[2][TY][TX], sVy[2][TY][TX], sVz[2][TY][TX]; // by plane parity
     __shared__ T sE[3][TY][TX];
     __shared__ T sS[2][VISC ? 9 : 6][TY][TX]; // na-weighted P txx tyy txy txz tyz; raw txy txz tyz
 
@@ -676,10 +679,10 @@ __device__ __forceinline__ void fused3_ib_tile(const StokesArgs<T>& a, const Fus
     const T ix = a.ix, iy = a.iy, iz = a.iz;
     const auto wvx = a.ns[OM_VCC], wvy = a.ns[OM_CVC], wvz = a.ns[OM_CCV];
     const auto wc = a.na[OM_CCC], wxy = a.na[OM_VVC], wxz = a.na[OM_VCV], wyz = a.na[OM_CVV];
-    // old-stress components this thread's row needs (as in fused3_tile), and
-    // those it computes at all
-    const unsigned rneed = ty == 0 ? 0x05u : ty == TY - 1 ? 0x50u : 0x7fu;
-    const unsigned rcan = (can_c ? 0x0fu : 0u) | (can_xy ? 0x10u : 0u) | (can_xz ? 0x20u : 0u) | (can_yz ? 0x40u : 0u);
+    // old-stress components this thread's row needs (as in fused3_tile); with
+    // 64 lanes a row is one wavefront, and the mask is wavefront-uniform
+    const int tyw = TX == 64 ? __builtin_amdgcn_readfirstlane(ty) : ty;
+    const unsigned rneed = tyw == 0 ? 0x05u : tyw == TY - 1 ? 0x50u : 0x7fu;
     // planes of this column with a valid location, and with a needed viscosity
     const int2 rg = a.crange[(long)ic + a.sy * (long)jc];
     const int2 re = a.erange[(long)ic + a.sy * (long)jc];
@@ -764,11 +767,34 @@ __device__ __forceinline__ void fused3_ib_tile(const StokesArgs<T>& a, const Fus
     if (ks > ke) return;
 
     const uint32_t c2 = col + bsz * (uint32_t)(ks - 2), c1 = c2 + bsz, c0 = c1 + bsz, c3 = c2 - bsz;
+    // FP32: the two buffers of a stress or velocity field are one allocation,
+    // the second a fixed number of bytes after the first (the same for every
+    // field), so the sweep reads and writes a field through one base pointer,
+    // with one input and one output byte offset for the stress fields and
+    // for the velocity fields: fewer pointers held across the march.  FP64:
+    // the input and output buffers themselves, offsets 0.
+    const T *bP = a.P, *bxx = a.txx, *byy = a.tyy, *bzz = a.tzz, *bxy = a.txy, *bxz = a.txz, *byz = a.tyz;
+    const T *bvx = a.Vx, *bvy = a.Vy, *bvz = a.Vz;
+    T *uP = a.Pn, *uxx = a.txxn, *uyy = a.tyyn, *uzz = a.tzzn, *uxy = a.txyn, *uxz = a.txzn, *uyz = a.tyzn;
+    T *uvx = a.Vxn, *uvy = a.Vyn, *uvz = a.Vzn;
+    uint32_t osi = 0, oso = 0, ovi = 0, ovo = 0;
+    if constexpr (PACKED) {
+        auto lo = [](const T* x, const T* y) { return x < y ? x : y; };
+        bP = lo(a.P, a.Pn), bxx = lo(a.txx, a.txxn), byy = lo(a.tyy, a.tyyn), bzz = lo(a.tzz, a.tzzn);
+        bxy = lo(a.txy, a.txyn), bxz = lo(a.txz, a.txzn), byz = lo(a.tyz, a.tyzn);
+        bvx = lo(a.Vx, a.Vxn), bvy = lo(a.Vy, a.Vyn), bvz = lo(a.Vz, a.Vzn);
+        uP = const_cast<T*>(bP), uxx = const_cast<T*>(bxx), uyy = const_cast<T*>(byy), uzz = const_cast<T*>(bzz);
+        uxy = const_cast<T*>(bxy), uxz = const_cast<T*>(bxz), uyz = const_cast<T*>(byz);
+        uvx = const_cast<T*>(bvx), uvy = const_cast<T*>(bvy), uvz = const_cast<T*>(bvz);
+        auto db = [](const T* x, const T* b) { return (uint32_t)((const char*)x - (const char*)b); };
+        osi = db(a.P, bP), oso = db(a.Pn, bP), ovi = db(a.Vx, bvx), ovo = db(a.Vxn, bvx);
+    }
     // raw velocity and ns weights of the own column
-    T vxm = ldif(inv(ks - 2), a.Vx, c2), vym = ldif(inv(ks - 2), a.Vy, c2), vzm = ldif(inv(ks - 2), a.Vz, c2);
-    T vx0 = ldif(inv(ks - 1), a.Vx, c1), vy0 = ldif(inv(ks - 1), a.Vy, c1), vz0 = ldif(inv(ks - 1), a.Vz, c1);
-    T vzp = ldif(inv(ks), a.Vz, c0);
+    T vxm = ldif(inv(ks - 2), bvx, c2 + ovi), vym = ldif(inv(ks - 2), bvy, c2 + ovi), vzm = ldif(inv(ks - 2), bvz, c2 + ovi);
+    T vx0 = ldif(inv(ks - 1), bvx, c1 + ovi), vy0 = ldif(inv(ks - 1), bvy, c1 + ovi), vz0 = ldif(inv(ks - 1), bvz, c1 + ovi);
+    T vzp = ldif(inv(ks), bvz, c0 + ovi);
     T mxm, mym, mzm, mx0, my0, mz0, mzp, mc0, mxy0, mxz0, myz0;
+    // flags of planes k-1 and k
     unsigned flm, fl0;
     // FP32: the mask word of the plane after the current one (read a plane ahead)
     uint2 wn1 = make_uint2(0u, 0u);
@@ -793,6 +819,31 @@ __device__ __forceinline__ void fused3_ib_tile(const StokesArgs<T>& a, const Fus
         myz0 = ldif(inr(ks - 1), wyz, c1);
         fl0 = ldif(inr(ks - 1), a.flags, c1 / E);
     }
+    // FP32: the old stress and the continuity forcing of a plane are read one
+    // plane ahead of the plane's update (FP64, whose registers are full: in
+    // the plane's own iteration), and the gravity of a velocity location one
+    // plane ahead of the velocity update, so that their latency overlaps the
+    // plane in between.  Stress components are read where the tile needs them
+    // at a valid location (the halo planes ks-1 and ke+1 only feed P, tzz and
+    // txz, tyz): one branch per location class (the needed components are
+    // uniform over the row).
+    const bool gf = a.grav_field != 0 && inner;
+    T oP = T(0), oxx = T(0), oyy = T(0), ozz = T(0), oxy = T(0), oxz = T(0), oyz = T(0), ofP = T(0);
+    auto ld_stress = [&](unsigned need, unsigned f, uint32_t cq, T& P_, T& xx_, T& yy_, T& zz_, T& xy_, T& xz_, T& yz_,
+                         T& fP_) {
+        if (can_c && (f & F_CCC)) {
+            if (need & 1u) P_ = ldo(bP, cq + osi);
+            if (need & 2u) xx_ = ldo(bxx, cq + osi);
+            if (need & 4u) yy_ = ldo(byy, cq + osi);
+            if (need & 8u) zz_ = ldo(bzz, cq + osi);
+            if (a.fP) fP_ = ldo(a.fP, cq);
+        }
+        if ((need & 16u) && can_xy && (f & F_VVC)) xy_ = ldo(bxy, cq + osi);
+        if ((need & 32u) && can_xz && (f & F_VCV)) xz_ = ldo(bxz, cq + osi);
+        if ((need & 64u) && can_yz && (f & F_CVV)) yz_ = ldo(byz, cq + osi);
+    };
+    if constexpr (PACKED) ld_stress(rneed & 0x09u, fl0, c1, oP, oxx, oyy, ozz, oxy, oxz, oyz, ofP);
+    T gvx = T(0), gvy = T(0), gvz = T(0);
     T qxm = mxm * vxm, qym = mym * vym;
     T qx0 = mx0 * vx0, qy0 = my0 * vy0, qz0 = mz0 * vz0;
     T qzp = mzp * vzp;
@@ -803,18 +854,23 @@ __device__ __forceinline__ void fused3_ib_tile(const StokesArgs<T>& a, const Fus
 
     for (int kk = ks - 1; kk <= ke + 1; ++kk) {
         const uint32_t c = col + bsz * (uint32_t)kk;
-        // previous-iteration stress at plane kk (used after the barrier): the
-        // components this thread computes at a valid location that the tile
-        // needs (the halo planes ks-1 and ke+1 only feed P, tzz and txz, tyz)
-        const unsigned valid = (fl0 & F_CCC ? 0x0fu : 0u) | (fl0 & F_VVC ? 0x10u : 0u) | (fl0 & F_VCV ? 0x20u : 0u) |
-                               (fl0 & F_CVV ? 0x40u : 0u);
-        const unsigned nd = rneed & rcan & valid & (kk == ks - 1 ? 0x09u : kk == ke + 1 ? 0x60u : 0x7fu);
-        const T oP = ldif(nd & 1u, a.P, c), oxx = ldif(nd & 2u, a.txx, c), oyy = ldif(nd & 4u, a.tyy, c),
-                ozz = ldif(nd & 8u, a.tzz, c), oxy = ldif(nd & 16u, a.txy, c), oxz = ldif(nd & 32u, a.txz, c),
-                oyz = ldif(nd & 64u, a.tyz, c);
-        sVx[ty][tx] = qx0;
-        sVy[ty][tx] = qy0;
-        sVz[ty][tx] = qz0;
+        // plane kk+1: its old stress and forcing; plane kk: the gravity of
+        // its velocity locations (updated in the next iteration)
+        T nP = T(0), nxx = T(0), nyy = T(0), nzz = T(0), nxy = T(0), nxz = T(0), nyz = T(0), nfP = T(0);
+        if constexpr (PACKED) {
+            if (kk <= ke)
+                ld_stress(rneed & (kk == ke ? 0x60u : 0x7fu), mw_flags(wn1), c + bsz, nP, nxx, nyy, nzz, nxy, nxz, nyz,
+                          nfP);
+        } else {
+            ld_stress(rneed & (kk == ks - 1 ? 0x09u : kk == ke + 1 ? 0x60u : 0x7fu), fl0, c, oP, oxx, oyy, ozz, oxy, oxz,
+                      oyz, ofP);
+        }
+        const bool gk = gf && kk >= ks && kk <= ke;
+        const T ngx = ldif(gk && (fl0 & F_VCC), a.gx, c), ngy = ldif(gk && (fl0 & F_CVC), a.gy, c),
+                ngz = ldif(gk && (fl0 & F_CCV), a.gz, c);
+        sVx[kk & 1][ty][tx] = qx0;
+        sVy[kk & 1][ty][tx] = qy0;
+        sVz[kk & 1][ty][tx] = qz0;
         sE[kk % 3][ty][tx] = e0;
         T nvx = T(0), nvy = T(0), nvz2 = T(0), nmx = T(0), nmy = T(0), nmz2 = T(0), nmc = T(0), nmxy = T(0),
           nmxz = T(0), nmyz = T(0), ne = T(0);
@@ -823,13 +879,13 @@ __device__ __forceinline__ void fused3_ib_tile(const StokesArgs<T>& a, const Fus
             const uint32_t cn = c + bsz;
             const bool v1 = inv(kk + 1), v2 = inv(kk + 2);
             if (v1) {
-                nvx = ldo(a.Vx, cn);
-                nvy = ldo(a.Vy, cn);
+                nvx = ldo(bvx, cn + ovi);
+                nvy = ldo(bvy, cn + ovi);
             }
             if constexpr (PACKED) {
                 uint2 wn2 = make_uint2(0u, 0u);
                 if (v2) {
-                    nvz2 = ldo(a.Vz, cn + bsz);
+                    nvz2 = ldo(bvz, cn + bsz + ovi);
                     wn2 = ldo(a.mw, 2 * (cn + bsz));
                 }
                 nmx = mw_mask(wn1, MW_NS_VCC);
@@ -848,7 +904,7 @@ __device__ __forceinline__ void fused3_ib_tile(const StokesArgs<T>& a, const Fus
                     nmy = ldo(wvy, cn);
                 }
                 if (v2) {
-                    nvz2 = ldo(a.Vz, cn + bsz);
+                    nvz2 = ldo(bvz, cn + bsz + ovi);
                     nmz2 = ldo(wvz, cn + bsz);
                 }
                 if (r1) {
@@ -867,27 +923,27 @@ __device__ __forceinline__ void fused3_ib_tile(const StokesArgs<T>& a, const Fus
         const int e_k = kk % 3, e_km = (kk + 2) % 3;
         T Pc = T(0), xxc = T(0), yyc = T(0), zzc = T(0), xyc = T(0), xzc = T(0), yzc = T(0);
         if (can_c && (fl0 & F_CCC)) {
-            T exx = dif(sVx[ty][tx + 1], qx0, ix);
-            T eyy = dif(sVy[ty + 1][tx], qy0, iy);
+            T exx = dif(sVx[kk & 1][ty][tx + 1], qx0, ix);
+            T eyy = dif(sVy[kk & 1][ty + 1][tx], qy0, iy);
             T ezz = dif(qzp, qz0, iz);
             T divV = exx + eyy + ezz;
-            Pc = upd_P(oP, a.fP ? divV + ldo(a.fP, c) : divV, e0, a.dPr);
+            Pc = upd_P(oP, a.fP ? divV + ofP : divV, e0, a.dPr);
             xxc = upd_td(oxx, e0, exx, divV, a.dtau[0]);
             yyc = upd_td(oyy, e0, eyy, divV, a.dtau[1]);
             zzc = upd_td(ozz, e0, ezz, divV, a.dtau[2]);
         }
         if (can_xy && (fl0 & F_VVC)) {
-            T exy = T(0.5) * (dif(qy0, sVy[ty][tx - 1], ix) + dif(qx0, sVx[ty - 1][tx], iy));
+            T exy = T(0.5) * (dif(qy0, sVy[kk & 1][ty][tx - 1], ix) + dif(qx0, sVx[kk & 1][ty - 1][tx], iy));
             T el = lerp4(sE[e_k][ty - 1][tx - 1], sE[e_k][ty - 1][tx], sE[e_k][ty][tx - 1], e0);
             xyc = upd_to(oxy, el, exy, a.dtau[3]);
         }
         if (can_xz && (fl0 & F_VCV)) {
-            T exz = T(0.5) * (dif(qz0, sVz[ty][tx - 1], ix) + dif(qx0, qxm, iz));
+            T exz = T(0.5) * (dif(qz0, sVz[kk & 1][ty][tx - 1], ix) + dif(qx0, qxm, iz));
             T el = lerp4(sE[e_km][ty][tx - 1], em, sE[e_k][ty][tx - 1], e0);
             xzc = upd_to(oxz, el, exz, a.dtau[4]);
         }
         if (can_yz && (fl0 & F_CVV)) {
-            T eyz = T(0.5) * (dif(qz0, sVz[ty - 1][tx], iy) + dif(qy0, qym, iz));
+            T eyz = T(0.5) * (dif(qz0, sVz[kk & 1][ty - 1][tx], iy) + dif(qy0, qym, iz));
             T el = lerp4(sE[e_km][ty - 1][tx], em, sE[e_k][ty - 1][tx], e0);
             yzc = upd_to(oyz, el, eyz, a.dtau[5]);
         }
@@ -908,16 +964,23 @@ __device__ __forceinline__ void fused3_ib_tile(const StokesArgs<T>& a, const Fus
         }
         if (inner && kk >= ks && kk <= ke) {
             if (ws || (fl0 & F_CCC)) {
-                sto(a.Pn, c, Pc);
-                sto(a.txxn, c, xxc);
-                sto(a.tyyn, c, yyc);
-                sto(a.tzzn, c, zzc);
+                sto(uP, c + oso, Pc);
+                sto(uxx, c + oso, xxc);
+                sto(uyy, c + oso, yyc);
+                sto(uzz, c + oso, zzc);
             }
-            if (ws || (fl0 & F_VVC)) sto(a.txyn, c, xyc);
-            if (ws || (fl0 & F_VCV)) sto(a.txzn, c, xzc);
-            if (ws || (fl0 & F_CVV)) sto(a.tyzn, c, yzc);
+            if (ws || (fl0 & F_VVC)) sto(uxy, c + oso, xyc);
+            if (ws || (fl0 & F_VCV)) sto(uxz, c + oso, xzc);
+            if (ws || (fl0 & F_CVV)) sto(uyz, c + oso, yzc);
         }
-        __syncthreads();
+        // The velocity at plane k = kk-1 reads the neighbours' stress of plane
+        // k, published before this iteration's barrier, and its own stress of
+        // plane kk; only tau_II (viscosity update) reads neighbours' stress of
+        // plane kk.  Without it the iteration needs one barrier: the stress
+        // buffer of a parity is written after the barrier that follows its
+        // last read (by the velocity of the plane before), and the velocity
+        // planes are double buffered the same way.
+        if constexpr (VISC) __syncthreads();
 
         // velocity (IB kernels_3d.jl:50-67) and viscosity at plane k = kk-1
         const int k = kk - 1;
@@ -929,27 +992,27 @@ __device__ __forceinline__ void fused3_ib_tile(const StokesArgs<T>& a, const Fus
             const T ez = T(0.5) * (emm + em);
             const T xyw = sS[bo][3][ty][tx], xzw = sS[bo][4][ty][tx], yzw = sS[bo][5][ty][tx];
             if (flm & F_VCC) {
-                const T g0 = a.grav_field ? ldo(a.gx, cv) : a.grav[0];
-                sto(a.Vxn, cv, upd_V_ib(vxm, dif(Pw, sS[bo][0][ty][tx - 1], ix), dif(xxw, sS[bo][1][ty][tx - 1], ix),
+                const T g0 = a.grav_field ? gvx : a.grav[0];
+                sto(uvx, cv + ovo, upd_V_ib(vxm, dif(Pw, sS[bo][0][ty][tx - 1], ix), dif(xxw, sS[bo][1][ty][tx - 1], ix),
                                         dif(sS[bo][3][ty + 1][tx], xyw, iy), dif(sS[bn][4][ty][tx], xzw, iz), g0, ex,
                                         mxm, a.dV[0]));
             } else if (wv) {
-                sto(a.Vxn, cv, T(0));
+                sto(uvx, cv + ovo, T(0));
             }
             if (flm & F_CVC) {
-                const T g1 = a.grav_field ? ldo(a.gy, cv) : a.grav[1];
-                sto(a.Vyn, cv, upd_V_ib(vym, dif(Pw, sS[bo][0][ty - 1][tx], iy), dif(yyw, sS[bo][2][ty - 1][tx], iy),
+                const T g1 = a.grav_field ? gvy : a.grav[1];
+                sto(uvy, cv + ovo, upd_V_ib(vym, dif(Pw, sS[bo][0][ty - 1][tx], iy), dif(yyw, sS[bo][2][ty - 1][tx], iy),
                                         dif(sS[bo][3][ty][tx + 1], xyw, ix), dif(sS[bn][5][ty][tx], yzw, iz), g1, ey,
                                         mym, a.dV[1]));
             } else if (wv) {
-                sto(a.Vyn, cv, T(0));
+                sto(uvy, cv + ovo, T(0));
             }
             if (flm & F_CCV) {
-                const T g2 = a.grav_field ? ldo(a.gz, cv) : a.grav[2];
-                sto(a.Vzn, cv, upd_V_ib(vzm, dif(Pw, Pwm, iz), dif(zzw, zzwm, iz), dif(sS[bo][4][ty][tx + 1], xzw, ix),
+                const T g2 = a.grav_field ? gvz : a.grav[2];
+                sto(uvz, cv + ovo, upd_V_ib(vzm, dif(Pw, Pwm, iz), dif(zzw, zzwm, iz), dif(sS[bo][4][ty][tx + 1], xzw, ix),
                                         dif(sS[bo][5][ty + 1][tx], yzw, iy), g2, ez, mzm, a.dV[2]));
             } else if (wv) {
-                sto(a.Vzn, cv, T(0));
+                sto(uvz, cv + ovo, T(0));
             }
             if constexpr (VISC) {
                 if (need_raw) {
@@ -1001,6 +1064,17 @@ __device__ __forceinline__ void fused3_ib_tile(const StokesArgs<T>& a, const Fus
         emm = em;
         em = e0;
         e0 = ne;
+        oP = nP;
+        oxx = nxx;
+        oyy = nyy;
+        ozz = nzz;
+        oxy = nxy;
+        oxz = nxz;
+        oyz = nyz;
+        ofP = nfP;
+        gvx = ngx;
+        gvy = ngy;
+        gvz = ngz;
     }
 }
 
