@@ -28,7 +28,7 @@ FI_BCGROUP_STRESS, FI_BCGROUP_VELOCITY, FI_BCGROUP_ETA, FI_BCGROUP_RESIDUAL = 0,
 FI_PHASE_NA, FI_PHASE_NS = 0, 1
 FI_OPT_FUSED, FI_OPT_ZCHUNK, FI_OPT_REFINE_FULL_PHASE, FI_OPT_REFINE_ETA_RATE, FI_OPT_REFINE_PHASE1_DIGITS = 1, 2, 3, 4, 5
 FI_OPT_FUSED_VARIANT, FI_OPT_FUSED_VARIANT_FP32, FI_OPT_TILE_ORDER, FI_OPT_REFINE_ANDERSON = 6, 7, 8, 9
-FI_OPT_REFINE_MODE, FI_OPT_FUSED_SHELL = 10, 11
+FI_OPT_REFINE_MODE, FI_OPT_FUSED_SHELL, FI_OPT_REFINE_COLS = 10, 11, 12
 FI_BATCH_RUNNING, FI_BATCH_CONVERGED, FI_BATCH_MAX_ITER, FI_BATCH_FAILED = 0, 1, 2, 3
 FI_BOPT_VARIANT, FI_BOPT_ZCHUNK, FI_BOPT_AUTOTUNE, FI_BOPT_GRAPH_ITERS, FI_BOPT_TILE_ORDER = 1, 2, 3, 4, 5
 

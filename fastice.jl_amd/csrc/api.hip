@@ -877,6 +877,8 @@ int fi_set_option(fi_model_t m, int key, int value)
         }
     } else if (key == FI_OPT_FUSED_SHELL) {
         m->fused_shell = value != 0;
+    } else if (key == FI_OPT_REFINE_COLS) {
+        m->refine_cols = value != 0;
     } else if (key == FI_OPT_REFINE_MODE) {
         if (value < 0 || value > 1) return fail(FI_ERR_ARG, "refinement mode must be 0 or 1");
         m->refine_mode = value;

@@ -391,6 +391,42 @@ __global__ __launch_bounds__(256) void k_norms(NormArgs a, long sy, long sz, int
     }
 }
 
+// The same maximum over the column ranges: one thread per column (i, j) of
+// a field's interior and chunk of NORM_KCH planes (blockIdx.z = chunk * nf +
+// field) reads the planes of its chunk inside the column's range only.
+constexpr int NORM_KCH = 64;
+
+__global__ __launch_bounds__(256) void k_norms_cols(NormArgs a, long sy, long sz, int nf, unsigned long long* out)
+{
+    const int q = (int)blockIdx.z % nf, chunk = (int)blockIdx.z / nf;
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) + 1;
+    const int j = (int)(blockIdx.y * blockDim.y + threadIdx.y) + 1;
+    unsigned long long m = 0ull;
+    if (i <= a.hi[q][0] && j <= a.hi[q][1]) {
+        const int2 rg = a.crange[i + sy * j];
+        const int k0 = max(1 + chunk * NORM_KCH, rg.x), k1 = min(min(chunk * NORM_KCH + NORM_KCH, a.hi[q][2]), rg.y);
+        const double* f = a.f[q] + i + sy * j;
+        for (int k = k0; k <= k1; ++k) {
+            const unsigned long long b = (unsigned long long)__double_as_longlong(f[sz * k]) & 0x7fffffffffffffffull;
+            m = b > m ? b : m;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        unsigned long long v = __shfl_xor(m, o);
+        m = v > m ? v : m;
+    }
+    __shared__ unsigned long long red[4];
+    const int t = threadIdx.x + blockDim.x * threadIdx.y;
+    const int lane = t & 63, wv = t >> 6;
+    if (lane == 0) red[wv] = m;
+    __syncthreads();
+    if (t == 0) {
+        unsigned long long r = red[0];
+        for (int w = 1; w < 4; ++w) r = red[w] > r ? red[w] : r;
+        if (r) atomicMax(out + q, r);
+    }
+}
+
 __global__ void k_zero_u64(unsigned long long* p, int n)
 {
     if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
@@ -406,6 +442,17 @@ void launch_norms(const Geo& g, const double* const f[4], const int L[4], int nf
         for (int d = 0; d < 3; ++d) a.hi[q][d] = (d < g.nd) ? g.n[d] + ((L[q < nf ? q : 0] >> d) & 1) : 1;
     }
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, out, 4);
+    if (a.crange) {
+        int hx = 1, hy = 1, hz = 1;
+        for (int q = 0; q < nf; ++q) {
+            hx = std::max(hx, a.hi[q][0]);
+            hy = std::max(hy, a.hi[q][1]);
+            hz = std::max(hz, a.hi[q][2]);
+        }
+        const dim3 grd((unsigned)((hx + 63) / 64), (unsigned)((hy + 3) / 4), (unsigned)(((hz + NORM_KCH - 1) / NORM_KCH) * nf));
+        hipLaunchKernelGGL(k_norms_cols, grd, dim3(64, 4), 0, s, a, g.sy, g.sz, nf, out);
+        return;
+    }
     long total = (long)a.hi[0][0] * a.hi[0][1] * a.hi[0][2];
     int blocks = (int)((total + 255) / 256);
     if (blocks > 1024) blocks = 1024;

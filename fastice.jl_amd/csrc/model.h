@@ -70,6 +70,7 @@ struct fi_model_s {
     int refine_anderson = 2;        // Anderson depth of the nonlinear outer loop (0: plain defect correction)
     int refine_mode = 1;            // outer loop: 1 lagged PT, 0 correction solves
     int fused_shell = 1;            // the shell's stress update rides in the fused launch
+    int refine_cols = 1;            // lagged-PT outer steps visit the column ranges only (where exact)
     int kch = 0;                             // z planes per fused workgroup (0: sized to the grid)
     int ftx = 64, fty = 8, fminw = 1;        // fused sweep variant of the FP64 iteration (lanes, rows, min waves)
     int ftx32 = 64, fty32 = 8, fminw32 = 1;  // and of the FP32 sweeps of the refined solver

@@ -134,62 +134,91 @@ __global__ __launch_bounds__(256) void k_add_correction(double* x, const float* 
 
 // ---------------------------------------------------------------------------
 // column-restricted kernels of the lagged-PT outer step (3D immersed
-// boundary, traction on every face): a thread owns column (i, j), i in
-// [0, nx+2], j in [0, ny+2], and the planes of one z chunk that lie in the
+// boundary, traction on every face): a thread owns position (i, j, k) of the
+// padded box, i in [0, nx+2], j in [0, ny+2], and works only if k lies in its
 // column's range (crange: planes holding a valid location; erange: planes
 // whose viscosity a valid location reads).  Elsewhere every location is
 // invalid: the FP32 corrections, the stress lag and the residuals are 0 there
 // and stay 0, so the outer step need not visit it.
 // ---------------------------------------------------------------------------
-constexpr int COL_KCH = 16; // planes per thread and chunk
-
 struct ColGeom {
     int nx3, ny3, nz3; // n + 3 per dimension
     long sy, sz;
 };
 
-__device__ __forceinline__ bool col_span(const ColGeom& g, const int2* rng, int& i, int& j, int& klo, int& khi)
-{
-    i = blockIdx.x * blockDim.x + threadIdx.x;
-    j = blockIdx.y * blockDim.y + threadIdx.y;
-    if (i >= g.nx3 || j >= g.ny3) return false;
-    const int2 rg = rng[(long)i + g.sy * j];
-    const int k0 = blockIdx.z * COL_KCH;
-    klo = max(k0, rg.x);
-    khi = min(min(k0 + COL_KCH - 1, g.nz3 - 1), rg.y);
-    return klo <= khi;
-}
-
 struct LagApply {
-    double* x[10];     // FP64 state (current buffers)
+    double* x[10];      // FP64 state (current buffers): P, the six stresses, V
     const float* d[10]; // the correction of the chunk (current FP32 buffers)
-    float* z0[10];      // both FP32 buffers, zeroed for the next chunk
-    float* z1[10];
-    int nf;
+    float* z[4];        // buffer 0 of the FP32 pressure and velocity, zeroed for the next chunk
 };
 
-// x += d for every state field, and both correction buffers back to 0
+// x += d for the ten state fields of a 3D model, and buffer 0 of the FP32
+// pressure and velocity back to 0, at the positions inside their column's
+// range: one thread per position, the workgroups of a launch sweeping whole
+// planes (blockIdx.z = plane), and every load issued before the stores.
+// Neither the other FP32 buffers nor buffer 0 of the stress need clearing:
+// compute_tau writes the stress lag into buffer 0 at every position of the
+// ranges (the stress conditions its ghosts), and the next chunk's first
+// sweep writes every valid location of buffer 1, which holds +0 at every
+// invalid one (no sweep stores anything else there), the conditions after
+// it its ghosts.
 __global__ __launch_bounds__(256) void k_lag_apply(LagApply a, const int2* crange, ColGeom g)
 {
-    int i, j, klo, khi;
-    if (!col_span(g, crange, i, j, klo, khi)) return;
-    for (int k = klo; k <= khi; ++k) {
-        const long c = (long)i + g.sy * j + g.sz * k;
-        for (int f = 0; f < a.nf; ++f) {
-            a.x[f][c] = a.x[f][c] + (double)a.d[f][c];
-            a.z0[f][c] = 0.0f;
-            a.z1[f][c] = 0.0f;
-        }
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= g.nx3 || j >= g.ny3) return;
+    const int2 rg = crange[(long)i + g.sy * j];
+    if (k < rg.x || k > rg.y) return;
+    const long c = (long)i + g.sy * j + g.sz * k;
+    double xv[10];
+    float dv[10];
+#pragma unroll
+    for (int f = 0; f < 10; ++f) {
+        xv[f] = a.x[f][c];
+        dv[f] = a.d[f][c];
     }
+#pragma unroll
+    for (int f = 0; f < 10; ++f) a.x[f][c] = xv[f] + (double)dv[f];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) a.z[q][c] = 0.0f;
 }
 
-__global__ __launch_bounds__(256) void k_cast_cols(float* dst, const double* src, double scale, const int2* rng, ColGeom g)
+// The forcing and frozen viscosity of the next chunk, cast to FP32 where a
+// valid location reads them: the viscosity on the column's erange, the
+// forcing (r_P and -r_V) on its crange (a subset).  One thread per column
+// and chunk of CAST_KCH planes (blockIdx.z), each plane's five loads issued
+// together.
+constexpr int CAST_KCH = 16;
+
+struct CastOuter {
+    float* eta;
+    const double* eta64;
+    float* f[4];        // fP, g[0..2]
+    const double* r[4]; // rP, rVx, rVy, rVz
+    double scale[4];    // 1, -1, -1, -1
+};
+
+__global__ __launch_bounds__(256) void k_cast_outer(CastOuter a, const int2* crange, const int2* erange, ColGeom g)
 {
-    int i, j, klo, khi;
-    if (!col_span(g, rng, i, j, klo, khi)) return;
-    for (int k = klo; k <= khi; ++k) {
-        const long c = (long)i + g.sy * j + g.sz * k;
-        dst[c] = (float)(scale * src[c]);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= g.nx3 || j >= g.ny3) return;
+    const long col = (long)i + g.sy * j;
+    const int2 re = erange[col], rc = crange[col];
+    const int k0 = max((int)blockIdx.z * CAST_KCH, re.x);
+    const int k1 = min(min((int)blockIdx.z * CAST_KCH + CAST_KCH - 1, g.nz3 - 1), re.y);
+    for (int k = k0; k <= k1; ++k) {
+        const long c = col + g.sz * k;
+        const bool in = k >= rc.x && k <= rc.y;
+        double v[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v[q] = in ? a.r[q][c] : 0.0;
+        a.eta[c] = (float)(1.0 * a.eta64[c]);
+        if (in) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) a.f[q][c] = (float)(a.scale[q] * v[q]);
+        }
     }
 }
 
@@ -644,15 +673,9 @@ static ColGeom col_geom(const fi_model_s* m)
     return ColGeom{m->g.n[0] + 3, m->g.n[1] + 3, m->g.n[2] + 3, m->g.sy, m->g.sz};
 }
 
-static dim3 col_grid(const fi_model_s* m)
+static dim3 plane_grid(const fi_model_s* m)
 {
-    return dim3((unsigned)((m->g.n[0] + 3 + 63) / 64), (unsigned)((m->g.n[1] + 3 + 3) / 4),
-                (unsigned)((m->g.n[2] + 3 + COL_KCH - 1) / COL_KCH));
-}
-
-static void cast_cols(fi_model_s* m, float* dst, const double* src, double scale, const int2* rng)
-{
-    hipLaunchKernelGGL(k_cast_cols, col_grid(m), dim3(64, 4), 0, m->stream, dst, src, scale, rng, col_geom(m));
+    return dim3((unsigned)((m->g.n[0] + 3 + 63) / 64), (unsigned)((m->g.n[1] + 3 + 3) / 4), (unsigned)(m->g.n[2] + 3));
 }
 
 // The lagged-PT outer step may visit only the column ranges: a 3D immersed
@@ -661,7 +684,7 @@ static void cast_cols(fi_model_s* m, float* dst, const double* src, double scale
 // update, so ghost positions outside the ranges never need zeroing.
 static bool lag_cols(const fi_model_s* m)
 {
-    if (!(m->desc.immersed_boundary && m->g.nd == 3 && m->crange)) return false;
+    if (!(m->refine_cols && m->desc.immersed_boundary && m->g.nd == 3 && m->crange)) return false;
     for (int d = 0; d < 3; ++d)
         for (int s = 0; s < 2; ++s)
             if (m->desc.bc_kind[d][s] != FI_BC_TRACTION) return false;
@@ -977,6 +1000,9 @@ static int job_outer_lag_enqueue(fi_model_s* m, RefineJob& j)
     const int nd = m->g.nd;
     if (j.nonlinear && j.since_eta > 0) {
         StokesArgs<double> a = make_args(m, true);
+        // after a column-range outer step, the stress is 0 at every invalid
+        // location: outside erange the update needs no stress
+        a.skip_empty = j.cols_clean;
         const fi_rheology& rh = m->desc.rheology;
         if (rh.relax != FI_RELAX_NONE) a.rh_rate = 1.0 - std::pow(1.0 - rh.rate, (double)j.since_eta);
         launch_eta_update(m->g, a, m->stream);
@@ -1017,9 +1043,20 @@ static int job_outer_lag_enqueue(fi_model_s* m, RefineJob& j)
     // and the viscosity on the planes a valid location's averages reach
     const int idR[3] = {FI_F_RVX, FI_F_RVY, FI_F_RVZ};
     if (cols) {
-        cast_cols(m, r->eta, m->eta(), 1.0, m->erange);
-        cast_cols(m, r->fP, m->f[FI_F_RP], 1.0, m->crange);
-        for (int q = 0; q < nd; ++q) cast_cols(m, r->g[q], m->f[idR[q]], -1.0, m->crange);
+        CastOuter ca;
+        ca.eta = r->eta;
+        ca.eta64 = m->eta();
+        ca.f[0] = r->fP;
+        ca.r[0] = m->f[FI_F_RP];
+        ca.scale[0] = 1.0;
+        for (int q = 0; q < 3; ++q) {
+            ca.f[1 + q] = r->g[q];
+            ca.r[1 + q] = m->f[idR[q]];
+            ca.scale[1 + q] = -1.0;
+        }
+        const dim3 grd((unsigned)((m->g.n[0] + 3 + 63) / 64), (unsigned)((m->g.n[1] + 3 + 3) / 4),
+                       (unsigned)((m->g.n[2] + 3 + CAST_KCH - 1) / CAST_KCH));
+        hipLaunchKernelGGL(k_cast_outer, grd, dim3(64, 4), 0, m->stream, ca, m->crange, m->erange, col_geom(m));
     } else {
         cast(m, r->eta, m->eta(), 1.0);
         cast(m, r->fP, m->f[FI_F_RP], 1.0);
@@ -1133,18 +1170,17 @@ int job_chunk_enqueue(fi_model_s* m, RefineJob& j)
         j.since_eta += j.done;
         if (j.cols_clean) {
             // corrections are non-zero only on the column ranges: add them
-            // there and zero both FP32 buffers for the next chunk in one pass
+            // there and clear the FP32 buffers the next chunk starts from in
+            // the same pass (lag_cols: 3D, all ten fields)
             LagApply la;
             std::memset(&la, 0, sizeof(la));
-            for (int id = FI_F_P; id <= FI_F_VZ; ++id)
-                if (m->f[id]) {
-                    la.x[la.nf] = m->buf(id);
-                    la.d[la.nf] = m->rf->cur(id);
-                    la.z0[la.nf] = m->rf->d[id][0];
-                    la.z1[la.nf] = m->rf->d[id][1];
-                    ++la.nf;
-                }
-            hipLaunchKernelGGL(k_lag_apply, col_grid(m), dim3(64, 4), 0, m->stream, la, m->crange, col_geom(m));
+            for (int id = FI_F_P; id <= FI_F_VZ; ++id) {
+                la.x[id - FI_F_P] = m->buf(id);
+                la.d[id - FI_F_P] = m->rf->cur(id);
+            }
+            la.z[0] = m->rf->d[FI_F_P][0];
+            for (int q = 0; q < 3; ++q) la.z[1 + q] = m->rf->d[FI_F_VX + q][0];
+            hipLaunchKernelGGL(k_lag_apply, plane_grid(m), dim3(64, 4), 0, m->stream, la, m->crange, col_geom(m));
         } else {
             for (int id = FI_F_P; id <= FI_F_VZ; ++id)
                 if (m->f[id])

@@ -1262,10 +1262,10 @@ __device__ __forceinline__ bool outside_range(const int2* crange, int i, int j, 
 }
 
 template <typename T, bool IB>
-__global__ __launch_bounds__(256) void k_residuals3(StokesArgs<T> a)
+__device__ __forceinline__ void residual3_at(const StokesArgs<T>& a, int i, int j, int k)
 {
-    IDX3
-    if (IB && a.skip_empty && outside_range(a.crange, i, j, k, sy)) return;
+    const long sy = a.sy, sz = a.sz;
+    const long c = (long)i + sy * j + sz * k;
     const T ix = a.ix, iy = a.iy, iz = a.iz;
     const T *P = a.P, *txx = a.txx, *tyy = a.tyy, *tzz = a.tzz, *txy = a.txy, *txz = a.txz, *tyz = a.tyz;
     const T *Vx = a.Vx, *Vy = a.Vy, *Vz = a.Vz;
@@ -1306,6 +1306,15 @@ __global__ __launch_bounds__(256) void k_residuals3(StokesArgs<T> a)
     }
 }
 
+template <typename T, bool IB>
+__global__ __launch_bounds__(256) void k_residuals3(StokesArgs<T> a)
+{
+    IDX3
+    (void)c;
+    if (IB && a.skip_empty && outside_range(a.crange, i, j, k, sy)) return;
+    residual3_at<T, IB>(a, i, j, k);
+}
+
 // compute_tau! store; LAG: also the FP32 difference old - new of the stress
 // it replaces (written in place, each location by one thread)
 template <bool LAG, typename T>
@@ -1328,10 +1337,10 @@ __device__ __forceinline__ void put_tau0(T* dst, float* lag, long c)
 // its undefined `Pr` location read as the Center location); writes the
 // current stress (txxn == txx)
 template <typename T, bool IB, bool LAG>
-__global__ __launch_bounds__(256) void k_tau3(StokesArgs<T> a, TauLag lg)
+__device__ __forceinline__ void tau3_at(const StokesArgs<T>& a, const TauLag& lg, int i, int j, int k)
 {
-    IDX3
-    if (IB && a.skip_empty && outside_range(a.crange, i, j, k, sy)) return;
+    const long sy = a.sy, sz = a.sz;
+    const long c = (long)i + sy * j + sz * k;
     const T ix = a.ix, iy = a.iy, iz = a.iz;
     const T *Vx = a.Vx, *Vy = a.Vy, *Vz = a.Vz, *eta = a.eta;
     const auto* ns = a.ns;
@@ -1380,19 +1389,24 @@ __global__ __launch_bounds__(256) void k_tau3(StokesArgs<T> a, TauLag lg)
     } else put_tau0<LAG>(a.tyzn, lg.t[5], c);
 }
 
+template <typename T, bool IB, bool LAG>
+__global__ __launch_bounds__(256) void k_tau3(StokesArgs<T> a, TauLag lg)
+{
+    IDX3
+    (void)c;
+    if (IB && a.skip_empty && outside_range(a.crange, i, j, k, sy)) return;
+    tau3_at<T, IB, LAG>(a, lg, i, j, k);
+}
+
 static inline dim3 block2();
+static inline dim3 grid3(const Geo& g);
 
 // eta_next <- rheology(tau_II(tau), I) at the current stress (the viscosity half of
 // update_V!, kernels_3d.jl:35-39 / kernels_2d.jl:27-28)
 template <typename T>
-__global__ __launch_bounds__(256) void k_eta_update(StokesArgs<T> a, int nd)
+__device__ __forceinline__ void eta_update_at(const StokesArgs<T>& a, long c, int nd)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    const int k = (nd == 3) ? (int)blockIdx.z + 1 : 0;
-    if (i > a.nx + 1 || j > a.ny + 1) return;
     const long sy = a.sy, sz = a.sz;
-    const long c = (long)i + sy * j + sz * k;
     const T *txx = a.txx, *tyy = a.tyy, *txy = a.txy;
     T t;
     if (nd == 3) {
@@ -1408,8 +1422,63 @@ __global__ __launch_bounds__(256) void k_eta_update(StokesArgs<T> a, int nd)
 }
 
 template <typename T>
+__global__ __launch_bounds__(256) void k_eta_update(StokesArgs<T> a, int nd)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = (nd == 3) ? (int)blockIdx.z + 1 : 0;
+    if (i > a.nx + 1 || j > a.ny + 1) return;
+    const long sy = a.sy, sz = a.sz;
+    const long c = (long)i + sy * j + sz * k;
+    eta_update_at(a, c, nd);
+}
+
+// The same over [1, n+1]^3 of a 3D immersed-boundary model whose stress is
+// 0 at every invalid location, one thread per column and chunk of ETA_KCH
+// planes: on the planes of the column's erange as above; elsewhere every
+// stress tau_II averages sits at an invalid location, so tau_II = +0 and
+// only the old viscosity is read.  With a constant-coefficient rheology
+// eta_next then depends on the old viscosity alone: a thread keeps its last
+// input and result, and a cell holding the same bits takes the result
+// without a log and an exp.
+constexpr int ETA_KCH = 16;
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_eta_update_cols(StokesArgs<T> a)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    if (i > a.nx + 1 || j > a.ny + 1) return;
+    const long sy = a.sy, sz = a.sz;
+    const int2 re = a.erange[(long)i + sy * j];
+    const int k0 = 1 + (int)blockIdx.z * ETA_KCH, k1 = min(k0 + ETA_KCH - 1, a.nz + 1);
+    const bool memo = a.rh == nullptr;
+    T m_in = T(0), m_out = T(0);
+    bool m_set = false;
+    for (int k = k0; k <= k1; ++k) {
+        const long c = (long)i + sy * j + sz * k;
+        if (k >= re.x && k <= re.y) {
+            eta_update_at(a, c, 3);
+        } else {
+            const T e = a.eta[c];
+            if (!(memo && m_set && same_bits(e, m_in))) {
+                m_in = e;
+                m_out = rheology(a, c, T(0));
+                m_set = true;
+            }
+            a.eta_next[c] = m_out;
+        }
+    }
+}
+
+template <typename T>
 void launch_eta_update(const Geo& g, const StokesArgs<T>& a, hipStream_t s)
 {
+    if (g.nd == 3 && a.skip_empty && a.erange) {
+        const dim3 grd((g.n[0] + 1 + 63) / 64, (g.n[1] + 1 + 3) / 4, (g.n[2] + 1 + ETA_KCH - 1) / ETA_KCH);
+        hipLaunchKernelGGL((k_eta_update_cols<T>), grd, block2(), 0, s, a);
+        return;
+    }
     dim3 grd((g.n[0] + 1 + 63) / 64, (g.n[1] + 1 + 3) / 4, g.nd == 3 ? g.n[2] + 1 : 1);
     hipLaunchKernelGGL((k_eta_update<T>), grd, block2(), 0, s, a, g.nd);
 }

@@ -344,7 +344,7 @@ class IsothermalFullStokesModel:
              "refine_phase1_digits": A.FI_OPT_REFINE_PHASE1_DIGITS, "fused_variant": A.FI_OPT_FUSED_VARIANT,
              "fused_variant_fp32": A.FI_OPT_FUSED_VARIANT_FP32, "tile_order": A.FI_OPT_TILE_ORDER,
              "refine_anderson": A.FI_OPT_REFINE_ANDERSON, "refine_mode": A.FI_OPT_REFINE_MODE,
-             "fused_shell": A.FI_OPT_FUSED_SHELL}.get(key, key)
+             "fused_shell": A.FI_OPT_FUSED_SHELL, "refine_cols": A.FI_OPT_REFINE_COLS}.get(key, key)
         A.check(A.lib().fi_set_option(self._h, int(k), int(value)))
         if k == A.FI_OPT_REFINE_MODE:
             self._refine_mode = int(value)

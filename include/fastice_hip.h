@@ -215,7 +215,11 @@ int fi_ib_validity(fi_model_t m, uint8_t* host, size_t count);
  * (max_inner caps the full-FP32 nonlinear start; inner_rtol, the
  * Anderson depth and the rate halving are not used); 0 correction solves:
  * each outer step solves the correction problem to inner_rtol with the
- * stress lag dropped, the viscosity following a relaxed Picard iteration. */
+ * stress lag dropped, the viscosity following a relaxed Picard iteration.
+ * FI_OPT_REFINE_COLS 1/0: on a 3D immersed-boundary model with traction on
+ * every face, the lagged-PT outer steps after the first visit only the
+ * planes of each column that hold (or neighbour) a valid location (default
+ * 1), or every cell (0); the results are the same bit for bit. */
 enum {
     FI_OPT_FUSED = 1,
     FI_OPT_ZCHUNK = 2,
@@ -227,7 +231,8 @@ enum {
     FI_OPT_TILE_ORDER = 8,
     FI_OPT_REFINE_ANDERSON = 9,
     FI_OPT_REFINE_MODE = 10,
-    FI_OPT_FUSED_SHELL = 11
+    FI_OPT_FUSED_SHELL = 11,
+    FI_OPT_REFINE_COLS = 12
 };
 int fi_set_option(fi_model_t m, int key, int value);
 /* Compiled variants of the fused sweep (FI_OPT_FUSED_VARIANT* take an index

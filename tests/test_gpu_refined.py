@@ -133,3 +133,26 @@ def test_every_fp32_variant_equals_pointwise(pkg):
         assert o[0] == o_ref[0] and o[1] == o_ref[1], cfg
         for n in want:
             assert np.array_equal(got[n], want[n]), (cfg, n)
+
+
+def test_refined_column_outer_steps_bitwise(pkg):
+    """Lagged-PT outer steps restricted to the column ranges (the default on a
+    3D immersed-boundary glacier with traction on every face) give the same
+    bits as outer steps over every cell: state, viscosity and residuals."""
+    _say("case: refined glacier 70x40x20, column-range outer steps vs whole grid, 6 outer steps")
+
+    def run(cols):
+        m, _ = pkg.glacier_3d((70, 40, 20), seed=3, glen=True)
+        m.set_option("refine_cols", cols)
+        out = m.refined_solve(1e-12, max_outer=6, max_inner=140, ncheck=32)
+        fields = {n: m.field(n).parent() for n in state_names(3) + ["ETA", "RP", "RVX", "RVY", "RVZ"]}
+        m.close()
+        return out, fields
+
+    (oa, fa), (ob, fb) = run(1), run(0)
+    _say("outer %d inner %d" % (oa[0], oa[1]))
+    assert oa[0] == ob[0] and oa[1] == ob[1]
+    assert np.array_equal(np.asarray(oa[2]), np.asarray(ob[2]))
+    for n in fa:
+        assert np.all(np.isfinite(fa[n])), n
+        assert np.array_equal(fa[n], fb[n]), n
