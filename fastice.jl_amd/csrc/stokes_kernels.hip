@@ -655,7 +655,11 @@ __device__ __forceinline__ void fused3_ib_tile(const StokesArgs<T>& a, const Fus
 {
     // FP32 sweeps read masks and flags from the packed words (a.mw)
     constexpr bool PACKED = sizeof(T) == 4;
-    __shared__ T sVx[2][TY][TX], sVy[2][TY][TX], sVz[2][TY][TX]; // by plane parity
+    // the FP32 correction sweep (no viscosity update) reads the old stress
+    // and forcing a plane ahead and needs one barrier per plane
+    constexpr bool AHEAD = PACKED && !VISC;
+    constexpr int NV = AHEAD ? 2 : 1;
+    __shared__ T sVx[NV][TY][TX], sVy[NV][TY][TX], sVz[NV][TY][TX]; // by plane parity (AHEAD)
     __shared__ T sE[3][TY][TX];
     __shared__ T sS[2][VISC ? 9 : 6][TY][TX]; // na-weighted P txx tyy txy txz tyz; raw txy txz tyz
 
@@ -819,8 +823,8 @@ __device__ __forceinline__ void fused3_ib_tile(const StokesArgs<T>& a, const Fus
         myz0 = ldif(inr(ks - 1), wyz, c1);
         fl0 = ldif(inr(ks - 1), a.flags, c1 / E);
     }
-    // FP32: the old stress and the continuity forcing of a plane are read one
-    // plane ahead of the plane's update (FP64, whose registers are full: in
+    // FP32 correction sweep (AHEAD): the old stress and the continuity forcing
+    // of a plane are read one plane ahead of the plane's update (otherwise in
     // the plane's own iteration), and the gravity of a velocity location one
     // plane ahead of the velocity update, so that their latency overlaps the
     // plane in between.  Stress components are read where the tile needs them
@@ -842,7 +846,7 @@ __device__ __forceinline__ void fused3_ib_tile(const StokesArgs<T>& a, const Fus
         if ((need & 32u) && can_xz && (f & F_VCV)) xz_ = ldo(bxz, cq + osi);
         if ((need & 64u) && can_yz && (f & F_CVV)) yz_ = ldo(byz, cq + osi);
     };
-    if constexpr (PACKED) ld_stress(rneed & 0x09u, fl0, c1, oP, oxx, oyy, ozz, oxy, oxz, oyz, ofP);
+    if constexpr (AHEAD) ld_stress(rneed & 0x09u, fl0, c1, oP, oxx, oyy, ozz, oxy, oxz, oyz, ofP);
     T gvx = T(0), gvy = T(0), gvz = T(0);
     T qxm = mxm * vxm, qym = mym * vym;
     T qx0 = mx0 * vx0, qy0 = my0 * vy0, qz0 = mz0 * vz0;
@@ -857,20 +861,31 @@ __device__ __forceinline__ void fused3_ib_tile(const StokesArgs<T>& a, const Fus
         // plane kk+1: its old stress and forcing; plane kk: the gravity of
         // its velocity locations (updated in the next iteration)
         T nP = T(0), nxx = T(0), nyy = T(0), nzz = T(0), nxy = T(0), nxz = T(0), nyz = T(0), nfP = T(0);
-        if constexpr (PACKED) {
+        if constexpr (AHEAD) {
             if (kk <= ke)
                 ld_stress(rneed & (kk == ke ? 0x60u : 0x7fu), mw_flags(wn1), c + bsz, nP, nxx, nyy, nzz, nxy, nxz, nyz,
                           nfP);
         } else {
-            ld_stress(rneed & (kk == ks - 1 ? 0x09u : kk == ke + 1 ? 0x60u : 0x7fu), fl0, c, oP, oxx, oyy, ozz, oxy, oxz,
-                      oyz, ofP);
+            // the components this thread computes at a valid location that the
+            // tile needs, one predicated load each
+            const unsigned rcan =
+                (can_c ? 0x0fu : 0u) | (can_xy ? 0x10u : 0u) | (can_xz ? 0x20u : 0u) | (can_yz ? 0x40u : 0u);
+            const unsigned valid = (fl0 & F_CCC ? 0x0fu : 0u) | (fl0 & F_VVC ? 0x10u : 0u) |
+                                   (fl0 & F_VCV ? 0x20u : 0u) | (fl0 & F_CVV ? 0x40u : 0u);
+            const unsigned nd = rneed & rcan & valid & (kk == ks - 1 ? 0x09u : kk == ke + 1 ? 0x60u : 0x7fu);
+            const uint32_t cs = c + osi;
+            oP = ldif(nd & 1u, bP, cs), oxx = ldif(nd & 2u, bxx, cs), oyy = ldif(nd & 4u, byy, cs);
+            ozz = ldif(nd & 8u, bzz, cs), oxy = ldif(nd & 16u, bxy, cs), oxz = ldif(nd & 32u, bxz, cs);
+            oyz = ldif(nd & 64u, byz, cs);
+            ofP = ldif(a.fP != nullptr && (nd & 1u), a.fP, c);
         }
         const bool gk = gf && kk >= ks && kk <= ke;
         const T ngx = ldif(gk && (fl0 & F_VCC), a.gx, c), ngy = ldif(gk && (fl0 & F_CVC), a.gy, c),
                 ngz = ldif(gk && (fl0 & F_CCV), a.gz, c);
-        sVx[kk & 1][ty][tx] = qx0;
-        sVy[kk & 1][ty][tx] = qy0;
-        sVz[kk & 1][ty][tx] = qz0;
+        const int pv = AHEAD ? (kk & 1) : 0; // velocity plane buffer
+        sVx[pv][ty][tx] = qx0;
+        sVy[pv][ty][tx] = qy0;
+        sVz[pv][ty][tx] = qz0;
         sE[kk % 3][ty][tx] = e0;
         T nvx = T(0), nvy = T(0), nvz2 = T(0), nmx = T(0), nmy = T(0), nmz2 = T(0), nmc = T(0), nmxy = T(0),
           nmxz = T(0), nmyz = T(0), ne = T(0);
@@ -923,8 +938,8 @@ __device__ __forceinline__ void fused3_ib_tile(const StokesArgs<T>& a, const Fus
         const int e_k = kk % 3, e_km = (kk + 2) % 3;
         T Pc = T(0), xxc = T(0), yyc = T(0), zzc = T(0), xyc = T(0), xzc = T(0), yzc = T(0);
         if (can_c && (fl0 & F_CCC)) {
-            T exx = dif(sVx[kk & 1][ty][tx + 1], qx0, ix);
-            T eyy = dif(sVy[kk & 1][ty + 1][tx], qy0, iy);
+            T exx = dif(sVx[pv][ty][tx + 1], qx0, ix);
+            T eyy = dif(sVy[pv][ty + 1][tx], qy0, iy);
             T ezz = dif(qzp, qz0, iz);
             T divV = exx + eyy + ezz;
             Pc = upd_P(oP, a.fP ? divV + ofP : divV, e0, a.dPr);
@@ -933,17 +948,17 @@ __device__ __forceinline__ void fused3_ib_tile(const StokesArgs<T>& a, const Fus
             zzc = upd_td(ozz, e0, ezz, divV, a.dtau[2]);
         }
         if (can_xy && (fl0 & F_VVC)) {
-            T exy = T(0.5) * (dif(qy0, sVy[kk & 1][ty][tx - 1], ix) + dif(qx0, sVx[kk & 1][ty - 1][tx], iy));
+            T exy = T(0.5) * (dif(qy0, sVy[pv][ty][tx - 1], ix) + dif(qx0, sVx[pv][ty - 1][tx], iy));
             T el = lerp4(sE[e_k][ty - 1][tx - 1], sE[e_k][ty - 1][tx], sE[e_k][ty][tx - 1], e0);
             xyc = upd_to(oxy, el, exy, a.dtau[3]);
         }
         if (can_xz && (fl0 & F_VCV)) {
-            T exz = T(0.5) * (dif(qz0, sVz[kk & 1][ty][tx - 1], ix) + dif(qx0, qxm, iz));
+            T exz = T(0.5) * (dif(qz0, sVz[pv][ty][tx - 1], ix) + dif(qx0, qxm, iz));
             T el = lerp4(sE[e_km][ty][tx - 1], em, sE[e_k][ty][tx - 1], e0);
             xzc = upd_to(oxz, el, exz, a.dtau[4]);
         }
         if (can_yz && (fl0 & F_CVV)) {
-            T eyz = T(0.5) * (dif(qz0, sVz[kk & 1][ty - 1][tx], iy) + dif(qy0, qym, iz));
+            T eyz = T(0.5) * (dif(qz0, sVz[pv][ty - 1][tx], iy) + dif(qy0, qym, iz));
             T el = lerp4(sE[e_km][ty - 1][tx], em, sE[e_k][ty - 1][tx], e0);
             yzc = upd_to(oyz, el, eyz, a.dtau[5]);
         }
@@ -976,11 +991,11 @@ __device__ __forceinline__ void fused3_ib_tile(const StokesArgs<T>& a, const Fus
         // The velocity at plane k = kk-1 reads the neighbours' stress of plane
         // k, published before this iteration's barrier, and its own stress of
         // plane kk; only tau_II (viscosity update) reads neighbours' stress of
-        // plane kk.  Without it the iteration needs one barrier: the stress
-        // buffer of a parity is written after the barrier that follows its
-        // last read (by the velocity of the plane before), and the velocity
-        // planes are double buffered the same way.
-        if constexpr (VISC) __syncthreads();
+        // plane kk.  Without it (AHEAD) the iteration needs one barrier: the
+        // stress buffer of a parity is written after the barrier that follows
+        // its last read (by the velocity of the plane before), and the
+        // velocity planes are double buffered the same way.
+        if constexpr (!AHEAD) __syncthreads();
 
         // velocity (IB kernels_3d.jl:50-67) and viscosity at plane k = kk-1
         const int k = kk - 1;
