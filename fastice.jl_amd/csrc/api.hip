@@ -340,8 +340,6 @@ void free_model(fi_model_s* m)
     free_refine(m);
     for (int q = 0; q < FI_F_COUNT; ++q)
         if (m->base[q]) (void)hipFree(m->base[q]);
-    for (int q = 0; q < 10; ++q)
-        if (m->sbase[q]) (void)hipFree(m->sbase[q]);
     if (m->flags_base) (void)hipFree(m->flags_base);
     if (m->crange) (void)hipFree(m->crange);
     if (m->erange) (void)hipFree(m->erange);
@@ -522,10 +520,10 @@ size_t fi_model_bytes(const fi_model_desc* desc)
 {
     if (check_desc(desc) != FI_OK) return 0;
     const Geo g = make_geo(*desc);
-    size_t fields = 0;
+    size_t bytes = 4 * sizeof(unsigned long long);
     for (int id = 0; id < FI_F_COUNT; ++id)
-        if (field_used(*desc, id)) fields += (id >= FI_F_P && id <= FI_F_VZ) ? 2 : 1;
-    size_t bytes = fields * g.elems * sizeof(double) + 4 * sizeof(unsigned long long);
+        if (field_used(*desc, id))
+            bytes += ((id >= FI_F_P && id <= FI_F_VZ) ? 2 * pair_stride(g) : g.elems) * sizeof(double);
     if (desc->immersed_boundary) bytes += g.elems;
     if (desc->immersed_boundary && g.nd == 3)
         bytes += (2 * col_ranges_count(g) + tile_march_count(g) + 3) * sizeof(int2);
@@ -549,25 +547,24 @@ int fi_model_create(const fi_model_desc* desc, fi_model_t* out)
         free_model(m);
         return fail(FI_ERR_HIP, "hipStreamCreate failed");
     }
+    // the stress and velocity fields and their second buffers are one
+    // allocation each, the second buffer pair_stride(g) elements after the
+    // first: the fused immersed-boundary sweep addresses both through one
+    // base pointer
     for (int id = 0; id < FI_F_COUNT; ++id) {
         if (!field_used(*desc, id)) continue;
-        if (hipMalloc(&m->base[id], g.elems * sizeof(double)) != hipSuccess) {
+        const bool pair = id >= FI_F_P && id <= FI_F_VZ;
+        const size_t n = pair ? 2 * pair_stride(g) : g.elems;
+        if (hipMalloc(&m->base[id], n * sizeof(double)) != hipSuccess) {
             free_model(m);
-            return fail(FI_ERR_OOM, "device allocation of field %d (%zu bytes) failed", id, g.elems * sizeof(double));
+            return fail(FI_ERR_OOM, "device allocation of field %d (%zu bytes) failed", id, n * sizeof(double));
         }
         m->f[id] = m->base[id] + g.off;
-        if (hipMemsetAsync(m->base[id], 0, g.elems * sizeof(double), m->stream) != hipSuccess) {
+        if (pair) m->sf[id] = m->base[id] + pair_stride(g) + g.off;
+        if (hipMemsetAsync(m->base[id], 0, n * sizeof(double), m->stream) != hipSuccess) {
             free_model(m);
             return fail(FI_ERR_HIP, "hipMemsetAsync failed");
         }
-    }
-    for (int id = FI_F_P; id <= FI_F_VZ; ++id) {
-        if (!m->f[id]) continue;
-        if (hipMalloc(&m->sbase[id], g.elems * sizeof(double)) != hipSuccess) {
-            free_model(m);
-            return fail(FI_ERR_OOM, "device allocation of the second buffer of field %d failed", id);
-        }
-        m->sf[id] = m->sbase[id] + g.off;
     }
     if (desc->immersed_boundary) {
         if (hipMalloc(&m->flags_base, g.elems) != hipSuccess) {
@@ -589,9 +586,7 @@ int fi_model_create(const fi_model_desc* desc, fi_model_t* out)
             }
         }
     }
-    // the fused sweep addresses fields with 32-bit byte offsets
-    if (g.nd == 3 && g.elems * sizeof(double) < (1ull << 32))
-        m->has_interior = fused_split(g, 8, m->kch, m->fg, m->shell);
+    if (fused_fits(g)) m->has_interior = fused_split(g, 8, m->kch, m->fg, m->shell);
     if (hipMalloc(&m->d_norm, 4 * sizeof(unsigned long long)) != hipSuccess ||
         hipHostMalloc(&m->h_norm, 4 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess) {
         free_model(m);
@@ -854,8 +849,7 @@ int fi_set_option(fi_model_t m, int key, int value)
     } else if (key == FI_OPT_ZCHUNK) {
         if (value < 0) return fail(FI_ERR_ARG, "z chunk must be >= 1 (0: sized to the grid)");
         m->kch = value;
-        if (m->g.nd == 3 && m->g.elems * sizeof(double) < (1ull << 32))
-            m->has_interior = fused_split(m->g, 8, m->kch, m->fg, m->shell);
+        if (fused_fits(m->g)) m->has_interior = fused_split(m->g, 8, m->kch, m->fg, m->shell);
     } else if (key == FI_OPT_REFINE_FULL_PHASE) {
         m->refine_full_phase = value ? 1 : 0;
     } else if (key == FI_OPT_REFINE_ETA_RATE) {

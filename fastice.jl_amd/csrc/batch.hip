@@ -127,7 +127,7 @@ void batch_detach(fi_model_s* m)
 
 static bool fused_ok(const fi_model_s* m)
 {
-    return m->g.nd == 3 && m->g.elems * sizeof(double) < (1ull << 32);
+    return fused_fits(m->g);
 }
 
 static int kind_of(const fi_model_s* m)

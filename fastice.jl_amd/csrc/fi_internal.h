@@ -31,6 +31,15 @@ struct Geo {
     size_t elems; // elements per allocation
 };
 
+// Elements from a double-buffered field's first buffer to its second, both
+// in one allocation (a multiple of 64, so both buffers keep the alignment of
+// their rows)
+inline size_t pair_stride(const Geo& g) { return (g.elems + 63) / 64 * 64; }
+
+// The FP64 fused sweep addresses fields with 32-bit byte offsets (the
+// immersed-boundary sweep both buffers of a pair from the first)
+inline bool fused_fits(const Geo& g) { return g.nd == 3 && 2 * pair_stride(g) * sizeof(double) < (1ull << 32); }
+
 // Volume fractions as the sweeps read them: FP64 sweeps read the FP64 masks;
 // FP32 sweeps (the corrections of the refined solver, whose FP64 residuals
 // use the FP64 masks) read them quantised to q / 255, q in 0..255, which is

@@ -27,8 +27,8 @@ struct fi_model_s {
     hipStream_t stream = nullptr;
     double* base[FI_F_COUNT] = {};
     double* f[FI_F_COUNT] = {};
-    // second buffer of the stress (P..TYZ) and velocity (VX..VZ) state
-    double* sbase[10] = {};
+    // second buffer of the stress (P..TYZ) and velocity (VX..VZ) state, in the
+    // allocation of the first (base[id] + pair_stride + off)
     double* sf[10] = {};
     int sig_cur = 0; // 1: current stress lives in sf
     int v_cur = 0;   // 1: current velocity lives in sf

@@ -365,13 +365,12 @@ static int alloc32(fi_model_s* m, float** out)
 // The two buffers of a correction field are one allocation, the second
 // pair_stride elements after the first (the same for every field): the fused
 // FP32 immersed-boundary sweep addresses both through one base pointer.
-static size_t pair_stride(const fi_model_s* m) { return (m->g.elems + 63) / 64 * 64; }
 
 static int alloc32_pair(fi_model_s* m, float** b0, float** b1)
 {
     Refine* r = m->rf;
     float* base = nullptr;
-    const size_t s = pair_stride(m);
+    const size_t s = pair_stride(m->g);
     if (hipMalloc(&base, 2 * s * sizeof(float)) != hipSuccess)
         return fail(FI_ERR_OOM, "FP32 workspace allocation failed");
     r->allocs.push_back(base);
@@ -438,8 +437,9 @@ static int ensure_refine(fi_model_s* m)
         free_refine(m);
         return rc;
     }
-    // the fused sweep addresses FP32 fields and the 8-byte mask words with 32-bit byte offsets
-    if (nd == 3 && m->g.elems * 8 < (1ull << 32)) r->has_interior = fused_split(m->g, 16, m->kch, r->fg, r->shell);
+    // the fused sweep addresses FP32 fields (both buffers of a pair from the
+    // first) and the 8-byte mask words with 32-bit byte offsets
+    if (nd == 3 && 8 * pair_stride(m->g) < (1ull << 32)) r->has_interior = fused_split(m->g, 16, m->kch, r->fg, r->shell);
     // the allocations are cleared on the model's stream
     m->zs32[0] = m->zs32[1] = m->zv32[0] = m->zv32[1] = true;
     m->gs32[0] = m->gs32[1] = false;

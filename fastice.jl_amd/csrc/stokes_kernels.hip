@@ -771,28 +771,20 @@ __device__ __forceinline__ void fused3_ib_tile(const StokesArgs<T>& a, const Fus
     if (ks > ke) return;
 
     const uint32_t c2 = col + bsz * (uint32_t)(ks - 2), c1 = c2 + bsz, c0 = c1 + bsz, c3 = c2 - bsz;
-    // FP32: the two buffers of a stress or velocity field are one allocation,
-    // the second a fixed number of bytes after the first (the same for every
+    // The two buffers of a stress or velocity field are one allocation, the
+    // second a fixed number of bytes after the first (the same for every
     // field), so the sweep reads and writes a field through one base pointer,
     // with one input and one output byte offset for the stress fields and
-    // for the velocity fields: fewer pointers held across the march.  FP64:
-    // the input and output buffers themselves, offsets 0.
-    const T *bP = a.P, *bxx = a.txx, *byy = a.tyy, *bzz = a.tzz, *bxy = a.txy, *bxz = a.txz, *byz = a.tyz;
-    const T *bvx = a.Vx, *bvy = a.Vy, *bvz = a.Vz;
-    T *uP = a.Pn, *uxx = a.txxn, *uyy = a.tyyn, *uzz = a.tzzn, *uxy = a.txyn, *uxz = a.txzn, *uyz = a.tyzn;
-    T *uvx = a.Vxn, *uvy = a.Vyn, *uvz = a.Vzn;
-    uint32_t osi = 0, oso = 0, ovi = 0, ovo = 0;
-    if constexpr (PACKED) {
-        auto lo = [](const T* x, const T* y) { return x < y ? x : y; };
-        bP = lo(a.P, a.Pn), bxx = lo(a.txx, a.txxn), byy = lo(a.tyy, a.tyyn), bzz = lo(a.tzz, a.tzzn);
-        bxy = lo(a.txy, a.txyn), bxz = lo(a.txz, a.txzn), byz = lo(a.tyz, a.tyzn);
-        bvx = lo(a.Vx, a.Vxn), bvy = lo(a.Vy, a.Vyn), bvz = lo(a.Vz, a.Vzn);
-        uP = const_cast<T*>(bP), uxx = const_cast<T*>(bxx), uyy = const_cast<T*>(byy), uzz = const_cast<T*>(bzz);
-        uxy = const_cast<T*>(bxy), uxz = const_cast<T*>(bxz), uyz = const_cast<T*>(byz);
-        uvx = const_cast<T*>(bvx), uvy = const_cast<T*>(bvy), uvz = const_cast<T*>(bvz);
-        auto db = [](const T* x, const T* b) { return (uint32_t)((const char*)x - (const char*)b); };
-        osi = db(a.P, bP), oso = db(a.Pn, bP), ovi = db(a.Vx, bvx), ovo = db(a.Vxn, bvx);
-    }
+    // for the velocity fields: fewer pointers held across the march.
+    auto lo = [](const T* x, const T* y) { return x < y ? x : y; };
+    const T *bP = lo(a.P, a.Pn), *bxx = lo(a.txx, a.txxn), *byy = lo(a.tyy, a.tyyn), *bzz = lo(a.tzz, a.tzzn);
+    const T *bxy = lo(a.txy, a.txyn), *bxz = lo(a.txz, a.txzn), *byz = lo(a.tyz, a.tyzn);
+    const T *bvx = lo(a.Vx, a.Vxn), *bvy = lo(a.Vy, a.Vyn), *bvz = lo(a.Vz, a.Vzn);
+    T *uP = const_cast<T*>(bP), *uxx = const_cast<T*>(bxx), *uyy = const_cast<T*>(byy), *uzz = const_cast<T*>(bzz);
+    T *uxy = const_cast<T*>(bxy), *uxz = const_cast<T*>(bxz), *uyz = const_cast<T*>(byz);
+    T *uvx = const_cast<T*>(bvx), *uvy = const_cast<T*>(bvy), *uvz = const_cast<T*>(bvz);
+    auto db = [](const T* x, const T* b) { return (uint32_t)((const char*)x - (const char*)b); };
+    const uint32_t osi = db(a.P, bP), oso = db(a.Pn, bP), ovi = db(a.Vx, bvx), ovo = db(a.Vxn, bvx);
     // raw velocity and ns weights of the own column
     T vxm = ldif(inv(ks - 2), bvx, c2 + ovi), vym = ldif(inv(ks - 2), bvy, c2 + ovi), vzm = ldif(inv(ks - 2), bvz, c2 + ovi);
     T vx0 = ldif(inv(ks - 1), bvx, c1 + ovi), vy0 = ldif(inv(ks - 1), bvy, c1 + ovi), vz0 = ldif(inv(ks - 1), bvz, c1 + ovi);

@@ -34,14 +34,26 @@ def test_percentiles(pkg):
     assert pkg.percentiles([]) == {"p50": None, "p95": None, "p99": None}
 
 
-def _expected_bytes(dims, nfields_single, immersed):
+def _elems(dims):
     # padded layout: rows of n0+3 elements rounded up to 32, 31 leading
-    # elements, 32 trailing; the ten stress/velocity fields are double buffered
+    # elements, 32 trailing
     n = list(dims)
     sy = (n[0] + 3 + 31) // 32 * 32
     rows = (n[1] + 3) * (n[2] + 3) if len(n) == 3 else n[1] + 3
-    elems = 31 + sy * rows + 32
-    b = (nfields_single + 10) * elems * 8 + 32
+    return sy, 31 + sy * rows + 32
+
+
+def _pair(elems):
+    # a double-buffered field: both buffers in one allocation, the second
+    # starting a multiple of 64 elements after the first
+    return 2 * ((elems + 63) // 64 * 64)
+
+
+def _expected_bytes(dims, nfields_single, immersed):
+    # the ten stress/velocity fields are double buffered
+    n = list(dims)
+    sy, elems = _elems(dims)
+    b = nfields_single * elems * 8 + 10 * _pair(elems) * 8 + 32
     if immersed:
         b += elems
         if len(n) == 3:
@@ -53,13 +65,12 @@ def _expected_bytes(dims, nfields_single, immersed):
 
 def test_model_bytes_matches_layout(pkg):
     kw, _ = pkg.manufactured_3d((30, 20, 10))
-    # P tau(6) V(3) eta eta_next rP rV(3) gravity fields(3)
-    assert pkg.model_bytes(**kw) == _expected_bytes((30, 20, 10), 10 + 2 + 4 + 3, False)
+    # P tau(6) V(3) double buffered; eta eta_next rP rV(3) gravity fields(3)
+    assert pkg.model_bytes(**kw) == _expected_bytes((30, 20, 10), 2 + 4 + 3, False)
     kw, _ = pkg.manufactured_2d((50, 40))
-    # P tau(3) V(2) eta eta_next rP rV(2) gravity fields(2), second buffers of 6 fields
-    sy = (50 + 3 + 31) // 32 * 32
-    elems = 31 + sy * 43 + 32
-    assert pkg.model_bytes(**kw) == (6 + 2 + 3 + 2 + 6) * elems * 8 + 32
+    # P tau(3) V(2) double buffered; eta eta_next rP rV(2) gravity fields(2)
+    _, elems = _elems((50, 40))
+    assert pkg.model_bytes(**kw) == 6 * _pair(elems) * 8 + (2 + 3 + 2) * elems * 8 + 32
     kw, _ = pkg.glacier_3d_spec((40, 30, 20), seed=3)
     # + psi(2) + omega na/ns (8 each), constant gravity, no rheology field
-    assert pkg.model_bytes(immersed_boundary=True, **kw) == _expected_bytes((40, 30, 20), 10 + 2 + 4 + 2 + 16, True)
+    assert pkg.model_bytes(immersed_boundary=True, **kw) == _expected_bytes((40, 30, 20), 2 + 4 + 2 + 16, True)
