@@ -42,33 +42,43 @@ inline bool fused_fits(const Geo& g) { return g.nd == 3 && 2 * pair_stride(g) * 
 
 // Volume fractions as the sweeps read them: FP64 sweeps read the FP64 masks;
 // FP32 sweeps (the corrections of the refined solver, whose FP64 residuals
-// use the FP64 masks) read them quantised to q / 255, q in 0..255, which is
-// exact for empty and full locations (255 * (1/255) rounds to 1 in FP32)
-// and within 1/510 elsewhere, except that a fraction in (0, 1/510) becomes
-// 1/255 rather than 0 (the velocity update divides by it).
-struct U8Mask {
-    const uint8_t* p;
-    int s; // bytes from one element to the next: 1, or 8 in the packed words below
-    __device__ __forceinline__ float operator[](long i) const { return (float)p[i * s] * (1.0f / 255.0f); }
-};
-// 3D FP32 sweeps with an immersed boundary keep the seven u8 masks they read
-// and the validity byte of a cell in one 8-byte word (one load per cell and
-// plane): bytes 0-3 na at CCC VVC VCV CVV, bytes 4-6 ns at VCC CVC CCV, byte
-// 7 the validity bits
-enum { MW_NA_CCC = 0, MW_NA_VVC, MW_NA_VCV, MW_NA_CVV, MW_NS_VCC, MW_NS_CVC, MW_NS_CCV, MW_FLAGS };
-__device__ __forceinline__ float mw_mask(uint2 w, int byte)
+// use the FP64 masks) read them in IEEE half precision, scaled by 2^15 so
+// that every fraction down to 2^-29 is a normal half: 0 and 1 exactly, every
+// fraction the reference can call non-empty (>= 1e-6) to within 2^-11 of
+// itself.  The PT step at a cut location is the velocity residual divided by
+// the ns fraction there, and near the glacier bed small fractions meet PT
+// steps that are close to the stability limit: an 8-bit code (q/255, or an
+// 8-bit float) perturbs such weights by several per cent, enough to make the
+// FP32 correction sweeps unstable at those cells on 383x383x191 and larger
+// glaciers although the FP64 iteration is stable there.
+enum : uint32_t { H16_SCALE_EXP = 15 };
+__device__ __forceinline__ float h16_decode(uint32_t h)
 {
-    const uint32_t d = byte < 4 ? w.x : w.y;
-    return (float)((d >> (8 * (byte & 3))) & 0xffu) * (1.0f / 255.0f);
+    return (float)__builtin_bit_cast(_Float16, (uint16_t)h) * (1.0f / (float)(1u << H16_SCALE_EXP));
 }
-__device__ __forceinline__ unsigned mw_flags(uint2 w) { return w.y >> 24; }
+struct H16Mask {
+    const uint16_t* p;
+    int s; // halves from one element to the next: 1, or 8 in the packed words below
+    __device__ __forceinline__ float operator[](long i) const { return h16_decode(p[i * s]); }
+};
+// 3D FP32 sweeps with an immersed boundary keep the seven half-precision
+// masks they read and the validity byte of a cell in one 16-byte word (one
+// load per cell and plane): halves 0-3 na at CCC VVC VCV CVV, halves 4-6 ns
+// at VCC CVC CCV, byte 14 the validity bits, byte 15 zero
+enum { MW_NA_CCC = 0, MW_NA_VVC, MW_NA_VCV, MW_NA_CVV, MW_NS_VCC, MW_NS_CVC, MW_NS_CCV };
+__device__ __forceinline__ float mw_mask(uint4 w, int half)
+{
+    const uint32_t d = half < 2 ? w.x : half < 4 ? w.y : half < 6 ? w.z : w.w;
+    return h16_decode((d >> (16 * (half & 1))) & 0xffffu);
+}
+__device__ __forceinline__ unsigned mw_flags(uint4 w) { return (w.w >> 16) & 0xffu; }
 template <typename T>
 struct MaskOf {
     using ptr = const T*;
 };
 template <>
 struct MaskOf<float> {
-    using ptr = U8Mask;
+    using ptr = H16Mask;
 };
 
 // Arguments of the PT sweep / residual kernels (passed by value).  The
@@ -89,7 +99,7 @@ struct StokesArgs {
     typename MaskOf<T>::ptr na[8];
     typename MaskOf<T>::ptr ns[8];
     const uint8_t* flags; // immersed-boundary validity bits (F_* below)
-    const uint2* mw;      // 3D FP32 immersed boundary: the packed mask words (MW_*), else null
+    const uint4* mw;      // 3D FP32 immersed boundary: the packed mask words (MW_*), else null
     // 3D immersed boundary, per column i + sy j: the planes k holding a valid
     // location (crange) and the planes whose viscosity a valid location may
     // read (erange); empty columns have x > y

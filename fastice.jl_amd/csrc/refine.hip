@@ -46,9 +46,9 @@ struct Refine {
     bool full = false;      // sweeps solve the full state (true) or the correction (false)
     float* fP = nullptr;    // continuity forcing r_P
     float* g[3] = {};       // momentum forcing -r_V
-    uint8_t* wna[8] = {};   // 2D masks of the FP32 sweeps (IB), quantised to q / 255
-    uint8_t* wns[8] = {};
-    uint8_t* wpk = nullptr; // 3D: the packed mask words (MW_*), 8 bytes per element
+    uint16_t* wna[8] = {};  // 2D masks of the FP32 sweeps (IB), half precision
+    uint16_t* wns[8] = {};
+    uint16_t* wpk = nullptr; // 3D: the packed mask words (MW_*), 16 bytes per element
     float* r[4] = {};       // inner residuals
     std::vector<void*> allocs;
     int sig_cur = 0, v_cur = 0;
@@ -80,38 +80,35 @@ __global__ __launch_bounds__(256) void k_cast(float* dst, const double* src, dou
         dst[t] = (float)(scale * src[t]);
 }
 
-// volume fraction in [0, 1] -> q = round(255 w)
-// q = rint(255 w) of the fraction clamped to [0, 1], and at least 1 for any
-// w > 0: a location that is not empty keeps a non-zero weight (the velocity
-// update divides by it)
-__device__ __forceinline__ uint8_t quantise_u8(double v)
+// volume fraction -> half precision bits of 2^15 w (round to nearest even),
+// the fraction clamped to [0, 1] (h16_decode)
+__device__ __forceinline__ uint16_t quantise_h16(double v)
 {
-    const double w = v < 0.0 ? 0.0 : (v > 1.0 ? 1.0 : v);
-    const double q = rint(255.0 * w);
-    return (uint8_t)(w > 0.0 && q < 1.0 ? 1.0 : q);
+    const float w = v > 0.0 ? (v < 1.0 ? (float)v : 1.0f) : 0.0f;
+    return __builtin_bit_cast(uint16_t, (_Float16)(w * (float)(1u << H16_SCALE_EXP)));
 }
 
-__global__ __launch_bounds__(256) void k_quantise_mask(uint8_t* dst, const double* src, size_t n)
+__global__ __launch_bounds__(256) void k_quantise_mask(uint16_t* dst, const double* src, size_t n)
 {
     for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
-        dst[t] = quantise_u8(src[t]);
+        dst[t] = quantise_h16(src[t]);
     }
 }
 
-// 3D: the seven masks the FP32 sweeps read, quantised as k_quantise_mask
-// does, and the validity byte, one 8-byte word per element (MW_*)
+// 3D: the seven masks the FP32 sweeps read, in half precision as
+// k_quantise_mask writes them, and the validity byte, one 16-byte word per
+// element (MW_*)
 struct PackSrc {
     const double* w[7];
 };
-__global__ __launch_bounds__(256) void k_pack_masks(uint2* dst, PackSrc src, const uint8_t* flags, size_t n)
+__global__ __launch_bounds__(256) void k_pack_masks(uint4* dst, PackSrc src, const uint8_t* flags, size_t n)
 {
     for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
-        uint32_t b[8];
+        uint32_t h[7];
         for (int q = 0; q < 7; ++q) {
-            b[q] = quantise_u8(src.w[q][t]);
+            h[q] = quantise_h16(src.w[q][t]);
         }
-        b[7] = flags[t];
-        dst[t] = make_uint2(b[0] | b[1] << 8 | b[2] << 16 | b[3] << 24, b[4] | b[5] << 8 | b[6] << 16 | b[7] << 24);
+        dst[t] = make_uint4(h[0] | h[1] << 16, h[2] | h[3] << 16, h[4] | h[5] << 16, h[6] | (uint32_t)flags[t] << 16);
     }
 }
 
@@ -380,13 +377,13 @@ static int alloc32_pair(fi_model_s* m, float** b0, float** b1)
     return FI_OK;
 }
 
-static int alloc_mask8(fi_model_s* m, uint8_t** out)
+static int alloc_mask16(fi_model_s* m, uint16_t** out)
 {
     Refine* r = m->rf;
-    uint8_t* base = nullptr;
-    if (hipMalloc(&base, m->g.elems) != hipSuccess) return fail(FI_ERR_OOM, "FP32 mask allocation failed");
+    uint16_t* base = nullptr;
+    if (hipMalloc(&base, m->g.elems * sizeof(uint16_t)) != hipSuccess) return fail(FI_ERR_OOM, "FP32 mask allocation failed");
     r->allocs.push_back(base);
-    HIPCHK(hipMemsetAsync(base, 0, m->g.elems, m->stream));
+    HIPCHK(hipMemsetAsync(base, 0, m->g.elems * sizeof(uint16_t), m->stream));
     *out = base + m->g.off;
     return FI_OK;
 }
@@ -416,17 +413,17 @@ static int ensure_refine(fi_model_s* m)
         (void)na3;
         (void)ns3;
         if (nd == 3) {
-            uint8_t* base = nullptr;
-            if (hipMalloc(&base, m->g.elems * 8) != hipSuccess) {
+            uint16_t* base = nullptr;
+            if (hipMalloc(&base, m->g.elems * 16) != hipSuccess) {
                 rc = fail(FI_ERR_OOM, "FP32 mask allocation failed");
             } else {
                 r->allocs.push_back(base);
-                if (hipMemsetAsync(base, 0, m->g.elems * 8, m->stream) != hipSuccess) rc = fail(FI_ERR_HIP, "hipMemsetAsync failed");
+                if (hipMemsetAsync(base, 0, m->g.elems * 16, m->stream) != hipSuccess) rc = fail(FI_ERR_HIP, "hipMemsetAsync failed");
                 r->wpk = base + 8 * m->g.off;
             }
         } else {
-            for (int q = 0; q < 2 && !rc; ++q) rc = alloc_mask8(m, &r->wna[na2[q]]);
-            for (int q = 0; q < 2 && !rc; ++q) rc = alloc_mask8(m, &r->wns[ns2[q]]);
+            for (int q = 0; q < 2 && !rc; ++q) rc = alloc_mask16(m, &r->wna[na2[q]]);
+            for (int q = 0; q < 2 && !rc; ++q) rc = alloc_mask16(m, &r->wns[ns2[q]]);
         }
     }
     if (!rc && (hipMalloc(&r->d_norm, 4 * sizeof(unsigned long long)) != hipSuccess ||
@@ -438,8 +435,8 @@ static int ensure_refine(fi_model_s* m)
         return rc;
     }
     // the fused sweep addresses FP32 fields (both buffers of a pair from the
-    // first) and the 8-byte mask words with 32-bit byte offsets
-    if (nd == 3 && 8 * pair_stride(m->g) < (1ull << 32)) r->has_interior = fused_split(m->g, 16, m->kch, r->fg, r->shell);
+    // first) and the 16-byte mask words with 32-bit byte offsets
+    if (nd == 3 && 16 * pair_stride(m->g) < (1ull << 32)) r->has_interior = fused_split(m->g, 16, m->kch, r->fg, r->shell);
     // the allocations are cleared on the model's stream
     m->zs32[0] = m->zs32[1] = m->zv32[0] = m->zv32[1] = true;
     m->gs32[0] = m->gs32[1] = false;
@@ -482,18 +479,18 @@ static StokesArgs<float> make_args32(fi_model_s* m, bool in_place)
     }
     a.rP = r->r[0]; a.rVx = r->r[1]; a.rVy = r->r[2]; a.rVz = r->r[3];
     for (int q = 0; q < 8; ++q) {
-        a.na[q] = U8Mask{r->wna[q], 1};
-        a.ns[q] = U8Mask{r->wns[q], 1};
+        a.na[q] = H16Mask{r->wna[q], 1};
+        a.ns[q] = H16Mask{r->wns[q], 1};
     }
     if (r->wpk) {
-        a.na[OM_CCC] = U8Mask{r->wpk + MW_NA_CCC, 8};
-        a.na[OM_VVC] = U8Mask{r->wpk + MW_NA_VVC, 8};
-        a.na[OM_VCV] = U8Mask{r->wpk + MW_NA_VCV, 8};
-        a.na[OM_CVV] = U8Mask{r->wpk + MW_NA_CVV, 8};
-        a.ns[OM_VCC] = U8Mask{r->wpk + MW_NS_VCC, 8};
-        a.ns[OM_CVC] = U8Mask{r->wpk + MW_NS_CVC, 8};
-        a.ns[OM_CCV] = U8Mask{r->wpk + MW_NS_CCV, 8};
-        a.mw = reinterpret_cast<const uint2*>(r->wpk);
+        a.na[OM_CCC] = H16Mask{r->wpk + MW_NA_CCC, 8};
+        a.na[OM_VVC] = H16Mask{r->wpk + MW_NA_VVC, 8};
+        a.na[OM_VCV] = H16Mask{r->wpk + MW_NA_VCV, 8};
+        a.na[OM_CVV] = H16Mask{r->wpk + MW_NA_CVV, 8};
+        a.ns[OM_VCC] = H16Mask{r->wpk + MW_NS_VCC, 8};
+        a.ns[OM_CVC] = H16Mask{r->wpk + MW_NS_CVC, 8};
+        a.ns[OM_CCV] = H16Mask{r->wpk + MW_NS_CCV, 8};
+        a.mw = reinterpret_cast<const uint4*>(r->wpk);
     }
     a.flags = m->flags;
     a.crange = m->crange;
@@ -691,7 +688,7 @@ static bool lag_cols(const fi_model_s* m)
     return true;
 }
 
-static void quantise_mask(fi_model_s* m, uint8_t* dst, const double* src)
+static void quantise_mask(fi_model_s* m, uint16_t* dst, const double* src)
 {
     hipLaunchKernelGGL(k_quantise_mask, ew_grid(m->g.elems), dim3(256), 0, m->stream, dst - m->g.off, src - m->g.off,
                        m->g.elems);
@@ -709,7 +706,7 @@ static void sync_masks(fi_model_s* m)
         for (int q = 0; q < 4; ++q) src.w[q] = m->f[FI_F_OM_NA0 + na[q]] - m->g.off;
         for (int q = 0; q < 3; ++q) src.w[4 + q] = m->f[FI_F_OM_NS0 + ns[q]] - m->g.off;
         hipLaunchKernelGGL(k_pack_masks, ew_grid(m->g.elems), dim3(256), 0, m->stream,
-                           reinterpret_cast<uint2*>(r->wpk - 8 * m->g.off), src, m->flags - m->g.off, m->g.elems);
+                           reinterpret_cast<uint4*>(r->wpk - 8 * m->g.off), src, m->flags - m->g.off, m->g.elems);
     }
     for (int q = 0; q < 8; ++q) {
         if (r->wna[q]) quantise_mask(m, r->wna[q], m->f[FI_F_OM_NA0 + q]);
@@ -1086,6 +1083,14 @@ static int job_outer_lag_finish(fi_model_s* m, RefineJob& j)
         j.converged = r0 < j.tol;
         j.state = JOB_DONE;
         j.chunk = 0;
+        // the outer step wrote T = 2 eta edot'(V) over tau and kept the PT
+        // stress memory tau - T in the FP32 lag: give tau back its PT value,
+        // so that a later solve or iteration continues the same iterate
+        for (int id = FI_F_TXX; id <= FI_F_TYZ; ++id)
+            if (m->f[id])
+                hipLaunchKernelGGL(k_add_correction, ew_grid(m->g.elems), dim3(256), 0, m->stream,
+                                   m->buf(id) - m->g.off, m->rf->cur(id) - m->g.off, m->g.elems);
+        m->state_written();
         return FI_OK;
     }
     // a viscosity lag of ncheck sweeps can hold a stiff problem in a limit

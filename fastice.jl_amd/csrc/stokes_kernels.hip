@@ -353,10 +353,10 @@ __device__ __forceinline__ T ldo(const T* base, uint32_t bo)
     return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + bo);
 }
 // a mask read at the byte offset of the FP32 fields it weights
-__device__ __forceinline__ float ldo(U8Mask base, uint32_t bo) { return base[bo / sizeof(float)]; }
-__device__ __forceinline__ uint2 ldo(const uint2* base, uint32_t bo)
+__device__ __forceinline__ float ldo(H16Mask base, uint32_t bo) { return base[bo / sizeof(float)]; }
+__device__ __forceinline__ uint4 ldo(const uint4* base, uint32_t bo)
 {
-    return *reinterpret_cast<const uint2*>(reinterpret_cast<const char*>(base) + bo);
+    return *reinterpret_cast<const uint4*>(reinterpret_cast<const char*>(base) + bo);
 }
 template <typename T>
 __device__ __forceinline__ void sto(T* base, uint32_t bo, T v)
@@ -643,9 +643,9 @@ __device__ __forceinline__ auto ldif(bool p, P base, uint32_t bo) -> decltype(ld
     return p ? ldo(base, bo) : R(0);
 }
 
-__device__ __forceinline__ uint2 ldw(bool p, const uint2* base, uint32_t bo)
+__device__ __forceinline__ uint4 ldw(bool p, const uint4* base, uint32_t bo)
 {
-    return p ? *reinterpret_cast<const uint2*>(reinterpret_cast<const char*>(base) + bo) : make_uint2(0u, 0u);
+    return p ? *reinterpret_cast<const uint4*>(reinterpret_cast<const char*>(base) + bo) : make_uint4(0u, 0u, 0u, 0u);
 }
 
 // ETA_HERE: packed launches, which keep no march table, update the viscosity
@@ -793,12 +793,12 @@ __device__ __forceinline__ void fused3_ib_tile(const StokesArgs<T>& a, const Fus
     // flags of planes k-1 and k
     unsigned flm, fl0;
     // FP32: the mask word of the plane after the current one (read a plane ahead)
-    uint2 wn1 = make_uint2(0u, 0u);
+    uint4 wn1 = make_uint4(0u, 0u, 0u, 0u);
     if constexpr (PACKED) {
         // a word read where the velocity is (a superset of crange): outside
         // crange its flags are 0 and its na weights multiply 0, as above
-        const uint2 w2 = ldw(inv(ks - 2), a.mw, 2 * c2), w1 = ldw(inv(ks - 1), a.mw, 2 * c1);
-        wn1 = ldw(inv(ks), a.mw, 2 * c0);
+        const uint4 w2 = ldw(inv(ks - 2), a.mw, 4 * c2), w1 = ldw(inv(ks - 1), a.mw, 4 * c1);
+        wn1 = ldw(inv(ks), a.mw, 4 * c0);
         mxm = mw_mask(w2, MW_NS_VCC), mym = mw_mask(w2, MW_NS_CVC), mzm = mw_mask(w2, MW_NS_CCV);
         flm = mw_flags(w2);
         mx0 = mw_mask(w1, MW_NS_VCC), my0 = mw_mask(w1, MW_NS_CVC), mz0 = mw_mask(w1, MW_NS_CCV);
@@ -890,10 +890,10 @@ __device__ __forceinline__ void fused3_ib_tile(const StokesArgs<T>& a, const Fus
                 nvy = ldo(bvy, cn + ovi);
             }
             if constexpr (PACKED) {
-                uint2 wn2 = make_uint2(0u, 0u);
+                uint4 wn2 = make_uint4(0u, 0u, 0u, 0u);
                 if (v2) {
                     nvz2 = ldo(bvz, cn + bsz + ovi);
-                    wn2 = ldo(a.mw, 2 * (cn + bsz));
+                    wn2 = ldo(a.mw, 4 * (cn + bsz));
                 }
                 nmx = mw_mask(wn1, MW_NS_VCC);
                 nmy = mw_mask(wn1, MW_NS_CVC);
