@@ -158,9 +158,32 @@ def manufactured_errors(grid, V, tau):
             n(vm - v, 2) / n(vm, 2), n(tm - t, 2) / n(tm, 2))
 
 
-def glacier_3d_spec(dims, seed=1, glen=True, eta_reg=30.0, relax_rate=0.05, lx=2.0, ly=2.0, H=1.0):
+# PT parameters (r, re_mech, vdtau_div) of the glacier driver,
+# scripts_future_API/test_vavilov.jl:145-156, which runs a linear viscosity
+GLACIER_PT = (0.9, 5 * PI, 1.1)
+# With Glen's law regularised at a strain rate of 1e-6 (a viscosity contrast
+# of about 1e4) those parameters diverge; r = 0.5 and vdtau_div = 4 (a PT
+# velocity scale vdtau 1.9 times smaller) converge.  Measured at
+# 255x255x127, FP64: r 0.9, vdtau_div 1.1 and relaxation 0.01 or 0.05
+# overflow within 4000 iterations; r 0.5, vdtau_div 2 grows 1e7-fold per
+# 4000 iterations; r 0.9, re_mech 3 pi, vdtau_div 4 grows 10-fold; r 0.5,
+# re_mech 5 pi, vdtau_div 4 with relaxation 0.01 falls about 5-fold per 4000
+# iterations and reaches 1e-8 after 44-48 thousand.
+GLACIER_PT_REG = (0.5, 5 * PI, 4.0)
+
+
+def eta_reg_from_rate(eps0, A=1.0, n=3):
+    """Viscosity of Glen's law eta = 1/2 A^(-1/n) edot^((1-n)/n) at the
+    strain rate eps0: the cap that regularises it at that rate."""
+    return 0.5 * A ** (-1.0 / n) * eps0 ** ((1.0 - n) / n)
+
+
+def glacier_3d_spec(dims, seed=1, glen=True, eta_reg=30.0, relax_rate=0.05, lx=2.0, ly=2.0, H=1.0, eps0=None):
     """Model arguments, driver cadence and geometry of the seeded synthetic
-    glacier (see glacier_3d), without allocating anything on the device."""
+    glacier (see glacier_3d), without allocating anything on the device.
+    eps0 (with glen): regularise Glen's law at strain rate eps0 instead of
+    eta_reg, with the PT parameters and log relaxation 0.01 that converge at
+    that viscosity contrast (GLACIER_PT_REG)."""
     from .full_stokes import TBC
     from .geometries import GaussianGlacier
     from .physics import PowerLawRheology, Relaxation, ViscoplasticRegularisation
@@ -170,6 +193,9 @@ def glacier_3d_spec(dims, seed=1, glen=True, eta_reg=30.0, relax_rate=0.05, lx=2
     grid = UniformGrid(origin=(-lx / 2, -ly / 2, 0.0), extent=(lx, ly, H), dims=dims)
     dgrid = UniformGrid(origin=(-lx / 2, -ly / 2), extent=(lx, ly), dims=dims[:2])
     tr = TBC(0.0, 0.0, 0.0)
+    pt = GLACIER_PT
+    if glen and eps0 is not None:
+        eta_reg, relax_rate, pt = eta_reg_from_rate(eps0), 0.01, GLACIER_PT_REG
     if glen:
         rheology = Relaxation("eta", ViscoplasticRegularisation(PowerLawRheology(1.0, 3), eta_reg), relax_rate,
                               scale="log")
@@ -177,12 +203,12 @@ def glacier_3d_spec(dims, seed=1, glen=True, eta_reg=30.0, relax_rate=0.05, lx=2
         rheology = LinearViscousRheology(1.0)
     kw = dict(grid=grid, boundary_conditions=dict(x=(tr, tr), y=(tr, tr), z=(tr, tr)),
               gravity=dict(x=0.0, y=0.0, z=1.0), rheology=rheology,
-              solver_params=pt_params(grid, r=0.9, re_mech=5 * PI, vdtau_div=1.1))
+              solver_params=pt_params(grid, r=pt[0], re_mech=pt[1], vdtau_div=pt[2]))
     n = max(dims)
     return kw, dict(niter=50 * n, ncheck=5 * n, dem_grid=dgrid, glacier=gl)
 
 
-def glacier_3d(dims, seed=1, glen=True, eta_reg=30.0, relax_rate=0.05, lx=2.0, ly=2.0, H=1.0):
+def glacier_3d(dims, seed=1, glen=True, eta_reg=30.0, relax_rate=0.05, lx=2.0, ly=2.0, H=1.0, eps0=None):
     """Seeded synthetic glacier on [-lx/2, lx/2] x [-ly/2, ly/2] x [0, H].
 
     Set-up order of scripts_future_API/test_vavilov.jl:116-177: level sets of
@@ -195,7 +221,7 @@ def glacier_3d(dims, seed=1, glen=True, eta_reg=30.0, relax_rate=0.05, lx=2.0, l
     from .full_stokes import IsothermalImmersedBoundaryFullStokesModel
     from .level_sets import compute_levelset_from_dem, invert_levelset, omega_from_psi
 
-    kw, cad = glacier_3d_spec(dims, seed, glen, eta_reg, relax_rate, lx, ly, H)
+    kw, cad = glacier_3d_spec(dims, seed, glen, eta_reg, relax_rate, lx, ly, H, eps0)
     gl, dgrid, grid = cad["glacier"], cad["dem_grid"], kw["grid"]
     model = IsothermalImmersedBoundaryFullStokesModel(**kw)
     compute_levelset_from_dem(model.level_sets.na, gl.surface, dgrid, grid)

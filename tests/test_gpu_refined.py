@@ -156,3 +156,30 @@ def test_refined_column_outer_steps_bitwise(pkg):
     for n in fa:
         assert np.all(np.isfinite(fa[n])), n
         assert np.array_equal(fa[n], fb[n]), n
+
+
+def test_refined_glacier_configured_regularisation_255(pkg):
+    """The precision workload as configured: the seeded glacier at
+    255x255x127 with Glen's law regularised at strain rate 1e-6 (eta_reg =
+    5000, a viscosity contrast of about 1e4; PT parameters r 0.5, re 5 pi,
+    vdtau_div 4 and log relaxation 0.01, with which plain FP64 PT converges
+    there).  FP64 and FP32-refined solves both reach FP64 residual 1e-8; the
+    per-field difference of the converged states, relative to the FP64
+    stress and velocity scales, is stated and bounded.  About a minute."""
+    dims = (255, 255, 127)
+    _say("case: refined vs FP64, glacier %s regularised at strain rate 1e-6, tol 1e-8" % (dims,))
+    b, cad = pkg.glacier_3d(dims, seed=1, glen=True, eps0=1e-6)
+    it, nb = b.solve(400 * max(dims), cad["ncheck"], 1e-8)
+    _say("fp64: %d iterations, norms %s" % (it, nb))
+    assert max(nb) < 1e-8, nb
+    a, _ = pkg.glacier_3d(dims, seed=1, glen=True, eps0=1e-6)
+    outer, inner, na = a.refined_solve(1e-8)
+    _say("refined: %d outer steps, %d FP32 sweeps, norms %s" % (outer, inner, na))
+    assert max(na) < 1e-8, na
+    errs = _field_errors(a, b, state_names(3))
+    _say("per-field difference (relative to the FP64 stress / velocity scale): %s" % errs)
+    assert max(errs.values()) < 1e-6, errs
+    eta = a.field("ETA").interior()
+    assert eta.max() <= 5000.0 * (1 + 1e-12) and eta.max() > 1000.0, (eta.min(), eta.max())
+    a.close()
+    b.close()
