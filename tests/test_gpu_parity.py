@@ -53,14 +53,16 @@ def test_manufactured_2d_converged_iteration_count(pkg, orc):
     model.close()
 
 
-@pytest.mark.parametrize("dims", [(20, 17, 12), (9, 13, 6)])
-def test_mixed_bcs_powerlaw_3d(pkg, orc, dims):
-    _say("case: 3D mixed BCs + regularised relaxed power law, dims %s" % (dims,))
+@pytest.mark.parametrize("dims,scale", [((20, 17, 12), None), ((9, 13, 6), None), ((20, 17, 12), "log"),
+                                        ((20, 17, 12), "log10")])
+def test_mixed_bcs_powerlaw_3d(pkg, orc, dims, scale):
+    _say("case: 3D mixed BCs + regularised power law relaxed in scale %s, dims %s" % (scale, dims))
     grid = pkg.UniformGrid(origin=(-0.5, -0.3, 0.0), extent=(1.0, 0.8, 0.6), dims=dims)
     bcs = dict(x=(pkg.SBC(0.0, 0.1, -0.2), pkg.TBC(0.3, -0.1, 0.05)),
                y=(pkg.VBC(0.02, 0.0, -0.01), pkg.SBC(0.0, 0.2, 0.1)),
                z=(pkg.VBC(0.0, 0.0, 0.0), pkg.TBC(0.0, 0.0, 0.0)))
-    rheo = pkg.Relaxation("eta", pkg.ViscoplasticRegularisation(pkg.PowerLawRheology(0.7, 3), 50.0), 0.3)
+    rheo = pkg.Relaxation("eta", pkg.ViscoplasticRegularisation(pkg.PowerLawRheology(0.7, 3), 50.0), 0.3,
+                          scale=scale)
     kw = dict(grid=grid, boundary_conditions=bcs, gravity=dict(x=0.1, y=-0.2, z=1.0), rheology=rheo,
               solver_params=pkg.pt_params(grid, r=0.9, re_mech=5 * math.pi))
     model, ref = make_pair(pkg, orc, kw, seed=7)
