@@ -77,6 +77,11 @@ struct fi_batch_s {
     int tune_runs = 0;
     int rebuilds = 0;
     hipEvent_t ev[2] = {nullptr, nullptr};
+    // refined batches: the FP64 outer steps of the models due after a chunk
+    // are independent of one another and run spread over these streams
+    static constexpr int NAUX = 4;
+    hipStream_t aux[NAUX] = {};
+    hipEvent_t ev_fork = nullptr;
     std::vector<long> sig; // refined batches: what the current tables were built from
     std::vector<unsigned> zbuilt; // FP64 batches: the running models' clean-buffer bits the tables hold
 };
@@ -676,11 +681,25 @@ static int step_refined(fi_batch_s* b, int* n_finished)
         bi.left -= delta;
         if (bi.left <= 0) due.push_back(r);
     }
-    for (int r : due) {
-        int rc = job_chunk_enqueue(b->inst[r].m, b->inst[r].job);
-        if (rc) return rc;
+    if (!due.empty()) {
+        // each due model's outer step (a dozen small launches and a copy of
+        // its norms) on one of the auxiliary streams, after the sweeps: the
+        // steps of different models overlap instead of queueing one launch
+        // at a time behind each other; every model computes what it
+        // computes alone, on its own buffers
+        HIPCHK(hipEventRecord(b->ev_fork, b->stream));
+        for (int q = 0; q < fi_batch_s::NAUX; ++q) HIPCHK(hipStreamWaitEvent(b->aux[q], b->ev_fork, 0));
+        int qi = 0;
+        for (int r : due) {
+            fi_model_s* m = b->inst[r].m;
+            m->stream = b->aux[qi];
+            qi = (qi + 1) % fi_batch_s::NAUX;
+            const int rc = job_chunk_enqueue(m, b->inst[r].job);
+            m->stream = b->stream;
+            if (rc) return rc;
+        }
+        for (int q = 0; q < fi_batch_s::NAUX; ++q) HIPCHK(hipStreamSynchronize(b->aux[q]));
     }
-    if (!due.empty()) HIPCHK(hipStreamSynchronize(b->stream));
     for (int r : due) {
         BatchInst& bi = b->inst[r];
         int rc = job_chunk_finish(bi.m, bi.job);
@@ -780,8 +799,18 @@ int fi_batch_create(fi_batch_t* out)
     *out = nullptr;
     fi_batch_s* b = new (std::nothrow) fi_batch_s();
     if (!b) return fail(FI_ERR_OOM, "host allocation failed");
-    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&b->ev[0]) != hipSuccess || hipEventCreate(&b->ev[1]) != hipSuccess) {
+    bool ok = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreate(&b->ev[0]) == hipSuccess && hipEventCreate(&b->ev[1]) == hipSuccess &&
+              hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) == hipSuccess;
+    for (int q = 0; q < fi_batch_s::NAUX && ok; ++q)
+        ok = hipStreamCreateWithFlags(&b->aux[q], hipStreamNonBlocking) == hipSuccess;
+    if (!ok) {
+        for (int q = 0; q < fi_batch_s::NAUX; ++q)
+            if (b->aux[q]) (void)hipStreamDestroy(b->aux[q]);
+        if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
+        for (int k = 0; k < 2; ++k)
+            if (b->ev[k]) (void)hipEventDestroy(b->ev[k]);
+        if (b->stream) (void)hipStreamDestroy(b->stream);
         delete b;
         return fail(FI_ERR_HIP, "stream/event creation failed");
     }
@@ -798,6 +827,11 @@ int fi_batch_destroy(fi_batch_t b)
     if (b->t.dev) (void)hipFree(b->t.dev);
     for (int k = 0; k < 2; ++k)
         if (b->ev[k]) (void)hipEventDestroy(b->ev[k]);
+    for (int q = 0; q < fi_batch_s::NAUX; ++q) {
+        (void)hipStreamSynchronize(b->aux[q]);
+        (void)hipStreamDestroy(b->aux[q]);
+    }
+    (void)hipEventDestroy(b->ev_fork);
     (void)hipStreamDestroy(b->stream);
     delete b;
     return FI_OK;
