@@ -87,3 +87,36 @@ def test_serving_refined_mix_equals_alone(pkg):
             alone.close()
     finally:
         srv.close()
+
+
+def test_serving_configured_sizes_mix_equals_alone(pkg):
+    """Requests across the served size range (64 to 511 per 2D section, 64 to
+    383 per 3D glacier: a 511x511x256 glacier does not reach 1e-8 and would
+    run to its 100 n cap twice), solved in one mix to FP64 residual 1e-8:
+    the large ones converge (a small glacier may stop at its 100 n cap, as it
+    does alone), and every request ends bitwise equal to its solve alone."""
+    R = pkg.serving.Request
+    reqs = [R(0, 0.0, 2, (64, 32), 101), R(1, 0.0, 3, (383, 383, 192), 102), R(2, 0.01, 2, (511, 256), 103),
+            R(3, 0.02, 3, (64, 64, 32), 104), R(4, 0.03, 3, (160, 160, 80), 105), R(5, 0.04, 2, (230, 115), 106)]
+    _say("case: serving %d requests across the configured sizes %s" % (len(reqs), reqs))
+    srv = pkg.Server(pkg, max_slots=6, keep_models=True)
+    try:
+        summary = srv.run(reqs, realtime=False)
+        _say("summary %s" % summary)
+        states = {rid: rec["state"] for rid, rec in srv.results.items()}
+        assert all(v in ("converged", "max_iter") for v in states.values()), states
+        assert states[1] == "converged" and states[2] == "converged", states
+        for rid, rec in srv.results.items():
+            req = reqs[rid]
+            m = rec["model"]
+            alone = pkg.serving.build_model(pkg, req)
+            _, _, ncheck, max_iter = pkg.serving.request_spec(req)
+            it, norms = alone.solve(max_iter, ncheck, srv.tol)
+            _say("request %s: %d iterations in the mix, %d alone" % (req, rec["iterations"], it))
+            assert it == rec["iterations"], (req, it, rec["iterations"])
+            assert float(np.max(norms)) == rec["residual"]
+            for n in state_names(req.ndims) + ["ETA"]:
+                assert np.array_equal(m.field(n).parent(), alone.field(n).parent()), (req, n)
+            alone.close()
+    finally:
+        srv.close()
