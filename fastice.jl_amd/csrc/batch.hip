@@ -503,24 +503,36 @@ static long running_cells(fi_batch_s* b)
 }
 
 // Time the fused sweep of every tile candidate on genuine iterations of the
-// running set (the tile shape does not change any result) and keep the
-// fastest.  Uses at most `budget` iterations; returns how many it ran.
+// running set (the tile shape does not change any result: every compiled
+// variant equals the pointwise sweep bit for bit) and keep the fastest.
+// FP64 batches try every FP64 variant with z chunks of 32 and 64 and tile
+// orders 0 and 2; refined batches every FP32 variant (their sweeps run in
+// chunks of about 32 between outer steps).  Each candidate runs one warm-up
+// iteration and three timed ones.  Uses at most `budget` iterations;
+// returns how many it ran.
 static std::vector<unsigned> fp64_zbits(fi_batch_s* b);
+static std::vector<long> refined_signature(fi_batch_s* b);
 
 static int autotune(fi_batch_s* b, int budget, int* used)
 {
     *used = 0;
+    const bool f32 = b->refined == 1;
     struct Cand {
         int tx, ty, minw, kch, order;
     };
     std::vector<Cand> cands;
     int cfg[16][3];
-    const int nv = fused_configs(false, cfg, 16);
-    for (int v = 0; v < nv; ++v)
+    const int nv = fused_configs(f32, cfg, 16);
+    for (int v = 0; v < nv; ++v) {
+        if (f32) {
+            cands.push_back(Cand{cfg[v][0], cfg[v][1], cfg[v][2], 32, 2});
+            continue;
+        }
         for (int kch : {32, 64})
             for (int order : {0, 2}) cands.push_back(Cand{cfg[v][0], cfg[v][1], cfg[v][2], kch, order});
+    }
     const int nc = (int)cands.size();
-    const int reps = 2; // one warm-up iteration, one timed
+    const int reps = 4;
     if (budget < nc * reps) return FI_OK;
     int best = -1;
     float best_ms = 0.0f;
@@ -532,9 +544,10 @@ static int autotune(fi_batch_s* b, int budget, int* used)
         if (rc) return rc;
         float ms = 0.0f;
         for (int k = 0; k < reps; ++k) {
-            // full iterations; only the fused launches of the last one are timed
+            // full iterations; iterations 1 to reps-1 are timed up to the
+            // last one's fused launches (the rest costs every candidate the same)
             const int p = b->phase;
-            if (k == reps - 1) HIPCHK(hipEventRecord(b->ev[0], b->stream));
+            if (k == 1) HIPCHK(hipEventRecord(b->ev[0], b->stream));
             launch_fused_all(b, p);
             if (k == reps - 1) HIPCHK(hipEventRecord(b->ev[1], b->stream));
             launch_points(b, p, true);
@@ -545,11 +558,18 @@ static int autotune(fi_batch_s* b, int budget, int* used)
             // in step with every iteration run here
             for (int r : b->run) {
                 fi_model_s* m = b->inst[r].m;
-                SweepOf<double>::swept(m, SweepOf<double>::get(m), 1);
-                m->sig_cur ^= 1;
-                m->v_cur ^= 1;
-                m->eta_cur ^= 1;
-                b->inst[r].iters++;
+                if (f32) {
+                    using S = SweepOf<float>;
+                    S::swept(m, S::get(m), 1);
+                    S::set(m, S::flipped(m, S::get(m), 1));
+                    b->inst[r].left--;
+                } else {
+                    SweepOf<double>::swept(m, SweepOf<double>::get(m), 1);
+                    m->sig_cur ^= 1;
+                    m->v_cur ^= 1;
+                    m->eta_cur ^= 1;
+                    b->inst[r].iters++;
+                }
             }
             b->phase ^= 1;
             (*used)++;
@@ -571,9 +591,11 @@ static int autotune(fi_batch_s* b, int budget, int* used)
     int rc = build_tables(b, cands[best].ty, cands[best].kch);
     if (rc) return rc;
     destroy_graphs(b); // captured when first replayed
-    // the tuning iterations changed the models' clean-buffer bits, and the
-    // tables just built hold the new ones
-    b->zbuilt = fp64_zbits(b);
+    // the tuning iterations changed the models' parities and clean-buffer
+    // bits, and the tables just built hold the new ones
+    if (f32) b->sig = refined_signature(b);
+    else b->zbuilt = fp64_zbits(b);
+    b->dirty = false;
     return FI_OK;
 }
 
@@ -663,22 +685,33 @@ static int step_refined(fi_batch_s* b, int* n_finished)
         b->sig = sg;
         b->dirty = false;
     }
+    // tile candidates timed on the first sweeps of this chunk whenever the
+    // running set's size has moved by more than a factor of two; the tuning
+    // sweeps are sweeps of the chunk (accounted for by autotune)
+    int tuned = 0;
+    const long cells = running_cells(b);
+    if (b->autotune && (b->tuned_cells == 0 || cells > 2 * b->tuned_cells || 2 * cells < b->tuned_cells)) {
+        b->phase = 0;
+        int rc = autotune(b, delta, &tuned);
+        if (rc) return rc;
+    }
+    const int rest = delta - tuned;
     int done = 0, p = 0;
-    while (b->graph_iters > 0 && delta - done >= b->graph_iters) {
+    while (b->graph_iters > 0 && rest - done >= b->graph_iters) {
         int rc = graph_launch(b, p);
         if (rc) return rc;
         done += b->graph_iters;
         p ^= (b->graph_iters & 1);
     }
-    for (; done < delta; ++done, p ^= 1) launch_iteration(b, p);
+    for (; done < rest; ++done, p ^= 1) launch_iteration(b, p);
     HIPCHK(hipGetLastError());
     using S = SweepOf<float>;
     std::vector<int> due;
     for (int r : b->run) {
         BatchInst& bi = b->inst[r];
-        S::swept(bi.m, S::get(bi.m), delta);
-        S::set(bi.m, S::flipped(bi.m, S::get(bi.m), delta));
-        bi.left -= delta;
+        S::swept(bi.m, S::get(bi.m), rest);
+        S::set(bi.m, S::flipped(bi.m, S::get(bi.m), rest));
+        bi.left -= rest;
         if (bi.left <= 0) due.push_back(r);
     }
     if (!due.empty()) {
@@ -873,7 +906,6 @@ int fi_batch_add_refined(fi_batch_t b, fi_model_t m, int max_outer, int max_inne
         b->ty = 8;
         b->minw = 1;
         b->kch = 32;
-        b->autotune = 0;
     }
     HIPCHK(hipStreamSynchronize(m->stream));
     BatchInst bi;
