@@ -562,3 +562,25 @@ def test_ib_skipping_field_rheology_bitwise(pkg, orc, law):
         assert not bad, (step, worst)
     a.close()
     b.close()
+
+
+def test_outer_width_changes_nothing(pkg):
+    """outer_width (the reference's split of the domain into an interior
+    launched first and boundary slabs overlapped with the halo exchange,
+    Isothermal.jl:60-89) has nothing to overlap on one device: a model built
+    with it iterates bit for bit as one built without."""
+    _say("case: outer_width (16, 8, 4) vs none, 3D manufactured 40x36x24, 200 iterations")
+    kw, _ = pkg.manufactured_3d((40, 36, 24))
+    a = pkg.IsothermalFullStokesModel(**kw)
+    b = pkg.IsothermalFullStokesModel(outer_width=(16, 8, 4), **kw)
+    assert b.outer_width == (16, 8, 4)
+    for m in (a, b):
+        m.viscosity.eta.set(1.0)
+        m.viscosity.eta_next.set(1.0)
+        m.apply_bc("eta")
+        m.advance(200)
+    for name in state_names(3):
+        assert np.array_equal(a.field(name).parent(), b.field(name).parent()), name
+    assert np.array_equal(a.compute_residuals(), b.compute_residuals())
+    a.close()
+    b.close()
