@@ -108,6 +108,9 @@ def parse():
                    help="batch96_glacier: models solved one at a time for the comparison")
     p.add_argument("--batch-precision", default="refined", choices=("refined", "fp64"),
                    help="batch96_glacier: FP32-refined solves to 1e-8 (default) or plain FP64")
+    p.add_argument("--batch-graph-iters", type=int, default=None,
+                   help="batch96_glacier: iterations per HIP graph replay (0: direct launches, for kernel-trace "
+                        "profiling; default: the batch's own)")
     p.add_argument("--batch-chunk", type=int, default=32,
                    help="batch96_glacier refined: FP32 sweeps between FP64 outer steps, all models")
     p.add_argument("--heat-n", type=int, default=1023, help="heat_diffusion: cells per axis")
@@ -271,7 +274,7 @@ def bench_batch(pkg, args):
     def refined_args(d):
         return dict(max_outer=-(-400 * max(d) // args.batch_chunk), ncheck=args.batch_chunk)
 
-    batch = pkg.Batch()
+    batch = pkg.Batch(graph_iters=args.batch_graph_iters)
     for (m, nc, mi), d in zip(inst, dims):
         if refined:
             batch.add_refined(m, 1e-8, **refined_args(d))
