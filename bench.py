@@ -31,6 +31,10 @@ Workloads
       T_eff of the iteration; the CPU oracle timed on a sample of iterations.
   refined_solve_255x255x127, fp64_solve_255x255x127: one of the two solves
       of the default line's converged-solve comparison, alone; value = seconds.
+  converged_511x511x255: the 511 glacier solved by plain FP64 and by the
+      FP32-refined solver until r_P and r_V outside its 16 margin faces
+      (whose residual no iteration changes) are below 1e-8; value = the
+      refined speedup, with the per-field difference.
   serving_poisson40: Poisson arrivals at 40/s, half 2D glacier sections
       (n x n/2, n in 64..511), half 3D glaciers (n x n x n/2, n in 64..511,
       --serving-3d), FP64 to 1e-8 or 100n iterations, continuously batched;
@@ -88,6 +92,7 @@ WORKLOADS = {
     "heat_diffusion": None,
     "manufactured3d_255": None,
     "refined_solve_255x255x127": None,
+    "converged_511x511x255": None,
     "fp64_solve_255x255x127": None,
 }
 
@@ -403,6 +408,123 @@ def bench_h5(pkg, args):
     print(json.dumps(line), flush=True)
 
 
+def margin_residual(model, margin=None):
+    """Inf-norms of a glacier's residuals (r_P, and r_V outside its margin
+    faces) and the margin faces: velocity faces whose residual exceeds 1e-2
+    (on the 511x511x255 glacier, 16 faces at the ice margin that the
+    reference's null-space rule calls valid and non-empty with an na fraction
+    of 1.2e-6; their residual is about -rho g and no PT iteration changes it).
+    Pass the margin of a converged solve to use the same faces."""
+    model.compute_residuals()
+    r = {k: model.field(k).interior() for k in ("RP", "RVX", "RVY", "RVZ")}
+    if margin is None:
+        margin = {k: np.abs(r[k]) > 1e-2 for k in ("RVX", "RVY", "RVZ")}
+    rv = max(float(np.max(np.abs(np.where(margin[k], 0.0, r[k])))) for k in ("RVX", "RVY", "RVZ"))
+    return float(np.max(np.abs(r["RP"]))), rv, margin
+
+
+# FP32 sweeps per outer step of the 511 comparison: the margin faces' velocity
+# drifts, and with it the viscosity around them; a viscosity lagged over 32
+# sweeps leaves r_V near them at about 1e-7, over 16 sweeps at about 3e-8
+CHUNK_511 = 16
+
+
+def bench_converged_511(pkg, args):
+    """The precision workload at its configured size: the 511x511x255 glacier
+    solved by plain FP64 PT and by the FP32-refined solver (16 sweeps per
+    outer step), each checked every 1000 iterations / 1600 sweeps for r_P and
+    r_V outside the margin faces.  The margin faces keep r_V = 0.5 and their
+    velocity drifts, so the problem has no fixed point there: FP64 reaches
+    1e-8 elsewhere, the refined solver's lagged viscosity holds it near 3e-8.
+    value = the refined speedup to 1e-7 (a level both reach); the time and
+    count to 1e-8 (FP64) and the refined solver's floor are in the config,
+    with the per-field difference of the final states (velocity outside the
+    margin faces) relative to the FP64 stress and velocity scales."""
+    dims = (511, 511, 255)
+    levels = (1e-7, 1e-8)
+    b, _ = pkg.glacier_3d(dims, seed=1, glen=True, relax_rate=RELAX_511)
+    t64, it64, margin, reach64 = 0.0, 0, None, {}
+    while True:
+        b.synchronize()
+        t0 = time.perf_counter()
+        it, _ = b.solve(1000, 1000, 1e-30)
+        b.synchronize()
+        t64 += time.perf_counter() - t0
+        it64 += it
+        if it64 < 8000:
+            continue
+        rp, rv, mg = margin_residual(b)
+        r = max(rp, rv)
+        for lv in levels:
+            if r < lv and lv not in reach64:
+                reach64[lv] = (it64, t64)
+        log("fp64 %d iterations %.1f s: r_P %.3e, r_V outside %d margin faces %.3e"
+            % (it64, t64, rp, sum(int(v.sum()) for v in mg.values()), rv))
+        if r < levels[-1] or it64 >= 60000:
+            margin = mg
+            break
+    a, _ = pkg.glacier_3d(dims, seed=1, glen=True, relax_rate=RELAX_511)
+    t32, sweeps, outer, first, reach32, best, since = 0.0, 0, 0, True, {}, None, 0
+    while True:
+        a.synchronize()
+        t0 = time.perf_counter()
+        o, s, _ = a.refined_solve(1e-30, max_outer=1600 // CHUNK_511, ncheck=CHUNK_511)
+        a.synchronize()
+        t32 += time.perf_counter() - t0
+        if first:
+            a.set_option("refine_full_phase", 0)
+            first = False
+        sweeps += s
+        outer += o
+        rp32, rv32, _ = margin_residual(a, margin)
+        r = max(rp32, rv32)
+        for lv in levels:
+            if r < lv and lv not in reach32:
+                reach32[lv] = (sweeps, t32)
+        log("refined %d sweeps %d outer steps %.1f s: r_P %.3e, r_V outside the margin faces %.3e"
+            % (sweeps, outer, t32, rp32, rv32))
+        # stop at the tolerance, or at the floor: no halving over 5 checks
+        if best is None or r < 0.5 * best:
+            best, since = r, 0
+        else:
+            since += 1
+        if r < levels[-1] or since >= 5 or sweeps >= 60000:
+            break
+    names = ("P", "TXX", "TYY", "TZZ", "TXY", "TXZ", "TYZ", "VX", "VY", "VZ")
+    vm = {"VX": margin["RVX"], "VY": margin["RVY"], "VZ": margin["RVZ"]}
+    vals = {k: (a.field(k).interior(), b.field(k).interior()) for k in names}
+    s_scale = max(float(np.max(np.abs(vals[k][1]))) for k in names[:7])
+    v_scale = max(float(np.max(np.abs(np.where(vm[k], 0.0, vals[k][1])))) for k in names[7:])
+    errs = {}
+    for k in names:
+        d = np.abs(vals[k][0] - vals[k][1])
+        if k in vm:
+            d = np.where(vm[k], 0.0, d)
+        errs[k] = float("%.3e" % (float(np.max(d)) / (s_scale if k[0] in "PT" else v_scale)))
+    a.close()
+    b.close()
+    to = lambda d: {("%g" % lv): {"count": d[lv][0], "seconds": round(d[lv][1], 3)} for lv in levels if lv in d}
+    speedup = reach64[1e-7][1] / reach32[1e-7][1] if 1e-7 in reach64 and 1e-7 in reach32 else 0.0
+    config = {"workload": "converged_511x511x255", "grid": list(dims),
+              "criterion": "r_P and r_V outside the margin faces (which keep |r_V| = 0.5 and whose velocity drifts)",
+              "margin_faces": {k: int(v.sum()) for k, v in margin.items()},
+              "fp64": {"iterations": it64, "seconds": round(t64, 3), "r_P": rp, "r_V_outside_margin": rv,
+                       "reached": to(reach64)},
+              "fp32_refined": {"fp32_sweeps": sweeps, "outer_steps": outer, "sweeps_per_outer_step": CHUNK_511,
+                               "seconds": round(t32, 3), "r_P": rp32, "r_V_outside_margin": rv32,
+                               "reached": to(reach32)},
+              "value_is": "FP64 time / refined time to r < 1e-7 outside the margin faces",
+              "rel_error_vs_fp64": errs,
+              "description": "synthetic glacier (seed 1), Glen n=3, eta_reg=30, log relaxation %g, PT r=0.9 re=5pi "
+                             "vdtau_div=1.1, traction-free box; solver time only (residual checks not timed)"
+                             % RELAX_511}
+    line = _line("refined/FP64 converged-solve speedup at 511x511x255", speedup, "x", t32 * 1e3, 1, 0,
+                 "f32 sweeps in f64 defect correction", config)
+    line["roofline"] = None
+    line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+
+
 def bench_glacier_solve(pkg, args, precision):
     """One solve of the 255x255x127 glacier to FP64 residual 1e-8, alone:
     FP32-refined (lagged PT) or plain FP64, the two solves the default
@@ -656,6 +778,8 @@ def main():
         return bench_heat(pkg, args)
     if args.workload == "manufactured3d_255":
         return bench_manufactured(pkg, args)
+    if args.workload == "converged_511x511x255":
+        return bench_converged_511(pkg, args)
     if args.workload in ("refined_solve_255x255x127", "fp64_solve_255x255x127"):
         return bench_glacier_solve(pkg, args, args.workload.split("_")[0])
     dims = WORKLOADS[args.workload]
