@@ -163,7 +163,11 @@ def event_times(A, fn, model, n):
     return out  # iteration, fused, shell, bc (ms)
 
 
-def cpu_baseline(pkg, model, dims, seconds, ib):
+def cpu_baseline(pkg, model, dims, seconds, ib, processed=None):
+    """The CPU oracle (OpenMP, FP64) on a bounded number of PT iterations of
+    the same model, in the headline's unit: 23 * 8 bytes per processed cell
+    (processed: the cells the GPU step credits; the oracle sweeps every cell)
+    and iteration; the reference formula over every cell beside it."""
     orc = ge.load_oracle()
     ref = orc.OracleModel(model.desc)
     names = ["P", "TXX", "TYY", "TZZ", "TXY", "TXZ", "TYZ", "VX", "VY", "VZ", "ETA", "ETA_NEXT"]
@@ -185,11 +189,16 @@ def cpu_baseline(pkg, model, dims, seconds, ib):
         if el >= seconds or n >= 200:
             break
     cells = dims[0] * dims[1] * dims[2]
-    teff = 23 * 8 * cells * n / el / 1e9
+    credited = cells if processed is None else processed
+    teff = 23 * 8 * credited * n / el / 1e9
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {"value": round(teff, 3), "unit": "GB/s", "cores": threads, "kind": "port",
-            "sample": "%d FP64 PT iterations of the same %dx%dx%d %s (CPU oracle, OpenMP, %.1f s)"
-                      % (n, dims[0], dims[1], dims[2], "immersed-boundary glacier" if ib else "grid", el)}
+            "teff_reference_formula_gbs": round(23 * 8 * cells * n / el / 1e9, 3),
+            "iterations_per_s": round(n / el, 3),
+            "sample": "%d FP64 PT iterations of the same %dx%dx%d %s (CPU oracle, OpenMP, %.1f s); value credits "
+                      "the %d cells the GPU step processes" % (n, dims[0], dims[1], dims[2],
+                                                              "immersed-boundary glacier" if ib else "grid", el,
+                                                              credited)}
 
 
 def solve_comparison(pkg, dims, b=None, fp64=True, eps0=EPS0):
@@ -883,7 +892,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         log("cpu baseline")
-        cpu = cpu_baseline(pkg, model, dims, args.cpu_seconds, glacier)
+        cpu = cpu_baseline(pkg, model, dims, args.cpu_seconds, glacier, processed)
 
     config = {"workload": args.workload, "grid": list(dims),
               "value_is": "23 * sizeof(T) bytes (tm_stokes_mpi_wip.jl:180) per processed cell and PT sweep, over the "
