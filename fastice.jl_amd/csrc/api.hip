@@ -838,6 +838,7 @@ int fi_ib_validity(fi_model_t m, uint8_t* host, size_t count)
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(m->stream));
     HIPCHK(hipMemcpy2D(host, w, m->flags, (size_t)g.sy, w, rows, hipMemcpyDeviceToHost));
+    for (size_t q = 0; q < count; ++q) host[q] &= (uint8_t)~F_FULL; // validity bits only
     return FI_OK;
 }
 

@@ -134,6 +134,12 @@ struct StokesArgs {
 // validity bits: sigma locations use isvalid(na) && !isempty(ns), velocity
 // locations isvalid(ns) && !isempty(na) (IB kernels_3d.jl:6,26,32,38,50,56,62)
 enum { F_CCC = 1, F_VVC = 2, F_VCV = 4, F_CVV = 8, F_VCC = 16, F_CVC = 32, F_CCV = 64 };
+// 3D: every FP64 volume fraction (both phases, all eight locations) of the
+// cell and of its six face neighbours is exactly 1.  The masked stencils of
+// compute_tau! and compute_residuals! then equal the plain ones bit for bit
+// (1 * x is x), so those kernels skip the mask loads there.  Not a validity
+// bit: fi_ib_validity clears it.
+enum { F_FULL = 128 };
 
 // inclusive index boxes over Chmy indices
 struct Box {

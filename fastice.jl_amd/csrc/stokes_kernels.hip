@@ -1288,6 +1288,17 @@ __device__ __forceinline__ void residual3_at(const StokesArgs<T>& a, int i, int 
                    (tyz[c + sz] - tyz[c]) * iz - gy;
         a.rVz[c] = -((P[c] - P[c - sz]) * iz) + (tzz[c] - tzz[c - sz]) * iz + (txz[c + 1] - txz[c]) * ix +
                    (tyz[c + sy] - tyz[c]) * iy - gz;
+    } else if (a.flags[c] & F_FULL) {
+        // every weight of the stencils below is exactly 1: the masked
+        // expressions without their (exact) multiplications by 1
+        a.rP[c] = (Vx[c + 1] - Vx[c]) * ix + (Vy[c + sy] - Vy[c]) * iy + (Vz[c + sz] - Vz[c]) * iz +
+                  (a.fP ? a.fP[c] : T(0));
+        a.rVx[c] = -((P[c] - P[c - 1]) * ix) + (txx[c] - txx[c - 1]) * ix + (txy[c + sy] - txy[c]) * iy +
+                   (txz[c + sz] - txz[c]) * iz - gx;
+        a.rVy[c] = -((P[c] - P[c - sy]) * iy) + (tyy[c] - tyy[c - sy]) * iy + (txy[c + 1] - txy[c]) * ix +
+                   (tyz[c + sz] - tyz[c]) * iz - gy;
+        a.rVz[c] = -((P[c] - P[c - sz]) * iz) + (tzz[c] - tzz[c - sz]) * iz + (txz[c + 1] - txz[c]) * ix +
+                   (tyz[c + sy] - tyz[c]) * iy - gz;
     } else {
         const auto* na = a.na;
         const auto* ns = a.ns;
@@ -1352,10 +1363,13 @@ __device__ __forceinline__ void tau3_at(const StokesArgs<T>& a, const TauLag& lg
     const T *Vx = a.Vx, *Vy = a.Vy, *Vz = a.Vz, *eta = a.eta;
     const auto* ns = a.ns;
     const unsigned fl = IB ? a.flags[c] : 0xffu;
+    // masked derivatives only where a weight may differ from 1 (F_FULL: the
+    // plain expressions are the masked ones bit for bit)
+    const bool wtd = IB && !(fl & F_FULL);
     T ec = eta[c];
     if (fl & F_CCC) {
         T exx, eyy, ezz;
-        if (IB) {
+        if (wtd) {
             exx = (ns[OM_VCC][c + 1] * Vx[c + 1] - ns[OM_VCC][c] * Vx[c]) * ix;
             eyy = (ns[OM_CVC][c + sy] * Vy[c + sy] - ns[OM_CVC][c] * Vy[c]) * iy;
             ezz = (ns[OM_CCV][c + sz] * Vz[c + sz] - ns[OM_CCV][c] * Vz[c]) * iz;
@@ -1375,21 +1389,21 @@ __device__ __forceinline__ void tau3_at(const StokesArgs<T>& a, const TauLag& lg
     }
     if (fl & F_VVC) {
         T exy;
-        if (IB) exy = T(0.5) * ((ns[OM_CVC][c] * Vy[c] - ns[OM_CVC][c - 1] * Vy[c - 1]) * ix +
+        if (wtd) exy = T(0.5) * ((ns[OM_CVC][c] * Vy[c] - ns[OM_CVC][c - 1] * Vy[c - 1]) * ix +
                                 (ns[OM_VCC][c] * Vx[c] - ns[OM_VCC][c - sy] * Vx[c - sy]) * iy);
         else exy = T(0.5) * ((Vy[c] - Vy[c - 1]) * ix + (Vx[c] - Vx[c - sy]) * iy);
         put_tau<LAG>(a.txyn, a.txy, lg.t[3], c, T(2) * lerp4(eta[c - 1 - sy], eta[c - sy], eta[c - 1], eta[c]) * exy);
     } else put_tau0<LAG>(a.txyn, lg.t[3], c);
     if (fl & F_VCV) {
         T exz;
-        if (IB) exz = T(0.5) * ((ns[OM_CCV][c] * Vz[c] - ns[OM_CCV][c - 1] * Vz[c - 1]) * ix +
+        if (wtd) exz = T(0.5) * ((ns[OM_CCV][c] * Vz[c] - ns[OM_CCV][c - 1] * Vz[c - 1]) * ix +
                                 (ns[OM_VCC][c] * Vx[c] - ns[OM_VCC][c - sz] * Vx[c - sz]) * iz);
         else exz = T(0.5) * ((Vz[c] - Vz[c - 1]) * ix + (Vx[c] - Vx[c - sz]) * iz);
         put_tau<LAG>(a.txzn, a.txz, lg.t[4], c, T(2) * lerp4(eta[c - 1 - sz], eta[c - sz], eta[c - 1], eta[c]) * exz);
     } else put_tau0<LAG>(a.txzn, lg.t[4], c);
     if (fl & F_CVV) {
         T eyz;
-        if (IB) eyz = T(0.5) * ((ns[OM_CCV][c] * Vz[c] - ns[OM_CCV][c - sy] * Vz[c - sy]) * iy +
+        if (wtd) eyz = T(0.5) * ((ns[OM_CCV][c] * Vz[c] - ns[OM_CCV][c - sy] * Vz[c - sy]) * iy +
                                 (ns[OM_CVC][c] * Vy[c] - ns[OM_CVC][c - sz] * Vy[c - sz]) * iz);
         else eyz = T(0.5) * ((Vz[c] - Vz[c - sy]) * iy + (Vy[c] - Vy[c - sz]) * iz);
         put_tau<LAG>(a.tyzn, a.tyz, lg.t[5], c, T(2) * lerp4(eta[c - sy - sz], eta[c - sz], eta[c - sy], eta[c]) * eyz);
@@ -1545,6 +1559,11 @@ __global__ __launch_bounds__(256) void k_flags(MaskPtrs m, uint8_t* flags, int n
             bool v = !(null3(w, locs[q], c, sy, sz) || lt(w[comp[q]], c)) && !lt(w2[comp[q]], c);
             if (v) f |= 1u << q;
         }
+        bool full = true;
+        const long nb[7] = {0, 1, -1, sy, -sy, sz, -sz};
+        for (int q = 0; q < 8 && full; ++q)
+            for (int o = 0; o < 7; ++o) full = full && m.na[q][c + nb[o]] == 1.0 && m.ns[q][c + nb[o]] == 1.0;
+        if (full) f |= F_FULL;
     } else {
         const int locs[4] = {L_CCC, L_VVC, L_VCC, L_CVC};
         const int comp[4] = {OM2_CC, OM2_VV, OM2_VC, OM2_CV};
