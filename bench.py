@@ -738,7 +738,7 @@ def bench_serving(pkg, args):
               "mix": "p=0.5 3D glacier (n log-uniform %d..%d, n x n x ceil(n/2)) / 2D glacier section "
                      "(n log-uniform 64..511, n x ceil(n/2)); Glen n=3, " % tuple(args.serving_3d)
                      + ("FP32 sweeps in FP64 defect correction (lagged PT, 32 sweeps per outer step) to residual "
-                        "1e-8 or 100n sweeps" if args.serving_precision == "refined" else
+                        "1e-8 or 400n sweeps" if args.serving_precision == "refined" else
                         "FP64 to residual 1e-8 or 100n iterations, ncheck 5n"),
               "precision": args.serving_precision,
               "latency_s": {k: (round(v, 4) if v is not None else None) for k, v in s["latency_s"].items()},

@@ -7,7 +7,7 @@ a 2D section of one (polygon level sets), solved to a residual tolerance:
 in FP64 with the reference's glacier check cadence (ncheck = 5n) up to 100n
 iterations, or
 (precision="refined") by FP32 sweeps inside FP64 defect correction (lagged
-PT, an outer step every 32 sweeps, at most 100n sweeps).  The server
+PT, an outer step every 32 sweeps, at most 400n sweeps).  The server
 
   * admits a waiting request when a batch slot is free and the device has the
     model's bytes (fi_model_bytes) plus a reserve free (hipMemGetInfo),
@@ -118,9 +118,7 @@ class Server:
         _, _, ncheck, max_iter = request_spec(req)
         m = build_model(self.pkg, req)
         if self.precision == "refined":
-            # the same cap as FP64 requests: max_iter FP32 sweeps in all, in
-            # outer steps of 32 sweeps
-            self.batch.add_refined(m, self.tol, max_outer=-(-max_iter // 32), ncheck=32)
+            self.batch.add_refined(m, self.tol)
         else:
             self.batch.add(m, max_iter, ncheck, self.tol)
         self.models[req.rid] = (req, m)

@@ -79,8 +79,7 @@ def test_serving_refined_mix_equals_alone(pkg):
             req = reqs[rid]
             m = rec["model"]
             alone = pkg.serving.build_model(pkg, req)
-            _, _, _, max_iter = pkg.serving.request_spec(req)
-            outer, sweeps, norms = alone.refined_solve(srv.tol, max_outer=-(-max_iter // 32), ncheck=32)
+            outer, sweeps, norms = alone.refined_solve(srv.tol)
             assert sweeps == rec["iterations"], (req, sweeps, rec["iterations"])
             assert float(np.max(norms)) == rec["residual"]
             for n in state_names(req.ndims) + ["ETA"]:
