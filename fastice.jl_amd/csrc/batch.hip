@@ -49,6 +49,12 @@ struct BatchTables {
     PackedBcOp* bc[2][2][3] = {}; // parity, group (0 stress, 1 velocity), dim
     int* blk2bc[2][2][3] = {};
     int nblk_bc[2][2][3] = {};
+    // the same passes as one k_bc_all launch per parity and group, where
+    // every instance's group has the form it takes (bc_all_build)
+    bool bc_all_ok[2][2] = {};
+    PackedBcAll* bc_all[2][2] = {};
+    int* blk2bcall[2][2] = {};
+    int nblk_bcall[2][2] = {};
     unsigned gen = 0; // bumped by every build: graphs record the generation they were captured against
 };
 
@@ -181,6 +187,28 @@ static void add_bc_ops(std::vector<PackedBcOp>& ops, std::vector<int>& blk, cons
     }
 }
 
+// one instance's group as a k_bc_all launch inside the packed one; ok turns
+// false when the group needs its passes launched one dimension at a time
+static void add_bc_all(std::vector<PackedBcAll>& all, std::vector<int>& blk, bool& ok, const fi_model_s* m,
+                       const BcBatch* bb)
+{
+    PackedBcAll p;
+    std::memset(&p, 0, sizeof(p));
+    int nb = 0;
+    if (!bc_all_build(m->g, bb, nullptr, p.a, nb)) {
+        ok = false;
+        return;
+    }
+    if (p.a.nsl == 0) return;
+    for (int d = 0; d < 3; ++d) p.n[d] = m->g.n[d];
+    p.sy = m->g.sy;
+    p.sz = m->g.sz;
+    p.start = (int)blk.size();
+    const int idx = (int)all.size();
+    all.push_back(p);
+    for (int k = 0; k < nb; ++k) blk.push_back(idx);
+}
+
 // the FP64 sweep of a model (its PT iteration) and the FP32 sweep of a
 // refined model (phase 1 or correction), as the table builder sees them
 template <typename T>
@@ -273,6 +301,9 @@ static int build_tables_t(fi_batch_s* b, int ty, int kch)
     std::vector<int> blk2box[4];
     std::vector<PackedBcOp> bcops[2][2][3];
     std::vector<int> bcblk[2][2][3];
+    std::vector<PackedBcAll> bcall[2][2];
+    std::vector<int> bcallblk[2][2];
+    bool bcall_ok[2][2] = {{true, true}, {true, true}};
     for (int r = 0; r < nrun; ++r) {
         fi_model_s* m = b->inst[b->run[r]].m;
         const int kind = kind_of(m);
@@ -302,12 +333,17 @@ static int build_tables_t(fi_batch_s* b, int ty, int kch)
             S::zflags(m, pp, pi.a);
             pinst[p].push_back(pi);
             // stress BCs see the new stress buffers, velocity BCs the new velocity
-            pp.sig ^= 1;
-            S::set(m, pp);
-            for (int d = 0; d < m->g.nd; ++d) add_bc_ops(bcops[p][0][d], bcblk[p][0][d], m, S::bc(m, 0, d));
-            pp.v ^= 1;
-            S::set(m, pp);
-            for (int d = 0; d < m->g.nd; ++d) add_bc_ops(bcops[p][1][d], bcblk[p][1][d], m, S::bc(m, 1, d));
+            for (int gr = 0; gr < 2; ++gr) {
+                if (gr == 0) pp.sig ^= 1;
+                else pp.v ^= 1;
+                S::set(m, pp);
+                BcBatch bb[3];
+                for (int d = 0; d < m->g.nd; ++d) {
+                    bb[d] = S::bc(m, gr, d);
+                    add_bc_ops(bcops[p][gr][d], bcblk[p][gr][d], m, bb[d]);
+                }
+                if (bcall_ok[p][gr]) add_bc_all(bcall[p][gr], bcallblk[p][gr], bcall_ok[p][gr], m, bb);
+            }
             S::set(m, p0);
         }
         if (interior) {
@@ -352,6 +388,16 @@ static int build_tables_t(fi_batch_s* b, int ty, int kch)
                 o_bc[p][gr][d] = put(h, bcops[p][gr][d]);
                 o_bcblk[p][gr][d] = put(h, bcblk[p][gr][d]);
             }
+    size_t o_bcall[2][2], o_bcallblk[2][2];
+    for (int p = 0; p < 2; ++p)
+        for (int gr = 0; gr < 2; ++gr) {
+            if (!bcall_ok[p][gr]) {
+                bcall[p][gr].clear();
+                bcallblk[p][gr].clear();
+            }
+            o_bcall[p][gr] = put(h, bcall[p][gr]);
+            o_bcallblk[p][gr] = put(h, bcallblk[p][gr]);
+        }
     BatchTables& t = b->t;
     if (h.size() > t.cap) {
         HIPCHK(hipStreamSynchronize(b->stream)); // graphs in flight may still read the old tables
@@ -383,6 +429,13 @@ static int build_tables_t(fi_batch_s* b, int ty, int kch)
                 t.blk2bc[p][gr][d] = reinterpret_cast<int*>(t.dev + o_bcblk[p][gr][d]);
                 t.nblk_bc[p][gr][d] = (int)bcblk[p][gr][d].size();
             }
+    for (int p = 0; p < 2; ++p)
+        for (int gr = 0; gr < 2; ++gr) {
+            t.bc_all_ok[p][gr] = bcall_ok[p][gr];
+            t.bc_all[p][gr] = reinterpret_cast<PackedBcAll*>(t.dev + o_bcall[p][gr]);
+            t.blk2bcall[p][gr] = reinterpret_cast<int*>(t.dev + o_bcallblk[p][gr]);
+            t.nblk_bcall[p][gr] = (int)bcallblk[p][gr].size();
+        }
     b->ty = ty;
     b->kch = kch;
     b->phase = 0;
@@ -433,6 +486,10 @@ static void launch_points(fi_batch_s* b, int p, bool sigma)
 static void launch_bcs(fi_batch_s* b, int p, int group)
 {
     const BatchTables& t = b->t;
+    if (t.bc_all_ok[p][group]) {
+        launch_bc_all_packed(t.bc_all[p][group], t.blk2bcall[p][group], t.nblk_bcall[p][group], b->stream);
+        return;
+    }
     for (int d = 0; d < 3; ++d)
         launch_bc_packed(t.bc[p][group][d], t.blk2bc[p][group][d], t.nblk_bc[p][group][d], b->stream);
 }

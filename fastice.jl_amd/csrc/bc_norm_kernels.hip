@@ -89,26 +89,6 @@ void launch_bc_dim(const Geo& g, const BcBatch& b, hipStream_t s)
 // chain starts from (a point that no pass writes, so no thread of this
 // launch changes it), or starts from v at a Dirichlet vertex, and applies
 // the face rules of the chain in pass order, in the same arithmetic.
-struct BcField {
-    void* ptr;
-    int L, f32;
-    int kind[3][2]; // -1: pass d does not write side s; else 0 Dirichlet, 1 Neumann
-    double v[3][2];
-};
-// one face of one field: pass d, side s, ext1 x ext2 points, workgroups from start
-struct BcSlice {
-    int fi, d, s;
-    int ext1, ext2;
-    int start;
-};
-struct BcAll {
-    BcField f[8];
-    int nf, nd;
-    const int2* skip_ranges; // see launch_bc_dims; null: nothing is skipped
-    BcSlice sl[48];
-    int nsl;
-};
-
 __device__ __forceinline__ int bc_target(int kind, int vert, int s, int nn)
 {
     if (kind == 0 && vert) return s == 0 ? 1 : nn + 1;
@@ -250,20 +230,24 @@ __global__ __launch_bounds__(256) void k_bc_all(BcAll b, int n0, int n1, int n2,
     else bc_points<double>(b, S, F, w0, n0, n1, n2, sy, sz);
 }
 
-void launch_bc_dims(const Geo& g, const BcBatch* b, hipStream_t s, const int2* skip_ranges)
+// The k_bc_all launch of one model's group (b: one batch per dimension):
+// fills a and its slice starts, nblocks = workgroups.  false: a field has
+// two rules on one face or there are more than 8 fields, and the passes
+// must run one launch per dimension (launch_bc_dim) to keep their order.
+bool bc_all_build(const Geo& g, const BcBatch* b, const int2* skip_ranges, BcAll& a, int& nblocks)
 {
-    BcAll a;
     a.nf = 0;
     a.nd = g.nd;
+    a.nsl = 0;
     a.skip_ranges = g.nd == 3 ? skip_ranges : nullptr;
-    bool ok = true;
-    for (int d = 0; d < g.nd && ok; ++d)
-        for (int o = 0; o < b[d].nop && ok; ++o) {
+    nblocks = 0;
+    for (int d = 0; d < g.nd; ++d)
+        for (int o = 0; o < b[d].nop; ++o) {
             const BcOp& op = b[d].op[o];
             int fi = 0;
             while (fi < a.nf && a.f[fi].ptr != op.ptr) ++fi;
             if (fi == a.nf) {
-                if (a.nf == 8) { ok = false; break; }
+                if (a.nf == 8) return false;
                 BcField& F = a.f[a.nf++];
                 F.ptr = op.ptr;
                 F.L = op.L;
@@ -275,23 +259,16 @@ void launch_bc_dims(const Geo& g, const BcBatch* b, hipStream_t s, const int2* s
                     }
             }
             BcField& F = a.f[fi];
-            if (F.L != op.L || F.f32 != op.f32) { ok = false; break; }
+            if (F.L != op.L || F.f32 != op.f32) return false;
             const int sides = op.kind >> 1;
             for (int t = 0; t < 2; ++t) {
                 if ((sides == 1 && t != 0) || (sides == 2 && t != 1)) continue;
-                if (F.kind[d][t] >= 0) { ok = false; break; } // two rules on one face: keep the launch order
+                if (F.kind[d][t] >= 0) return false; // two rules on one face: keep the launch order
                 F.kind[d][t] = op.kind & 1;
                 F.v[d][t] = op.v;
             }
         }
-    if (!ok) {
-        for (int d = 0; d < g.nd; ++d) launch_bc_dim(g, b[d], s);
-        return;
-    }
-    if (a.nf == 0) return;
     // one slice per face with a rule, each sized to its field's extents
-    a.nsl = 0;
-    int nblocks = 0;
     for (int fi = 0; fi < a.nf; ++fi)
         for (int d = 0; d < g.nd; ++d) {
             int e1 = -1, e2 = -1;
@@ -314,8 +291,42 @@ void launch_bc_dims(const Geo& g, const BcBatch* b, hipStream_t s, const int2* s
                 nblocks += (int)(((long)ext1 * ext2 + 256 * BC_PTS - 1) / (256 * BC_PTS));
             }
         }
+    return true;
+}
+
+void launch_bc_dims(const Geo& g, const BcBatch* b, hipStream_t s, const int2* skip_ranges)
+{
+    BcAll a;
+    int nblocks = 0;
+    if (!bc_all_build(g, b, skip_ranges, a, nblocks)) {
+        for (int d = 0; d < g.nd; ++d) launch_bc_dim(g, b[d], s);
+        return;
+    }
     if (a.nsl == 0) return;
     hipLaunchKernelGGL(k_bc_all, dim3(nblocks), dim3(256), 0, s, a, g.n[0], g.n[1], g.n[2], g.sy, g.sz);
+}
+
+// k_bc_all of many models in one launch: workgroup w works for model
+// blk2inst[w], as workgroup w - start of that model's own launch would
+__global__ __launch_bounds__(256) void k_bc_all_packed(const PackedBcAll* __restrict__ all,
+                                                       const int* __restrict__ blk2inst)
+{
+    const PackedBcAll& P = all[blk2inst[blockIdx.x]];
+    const BcAll& b = P.a;
+    const int bid = (int)blockIdx.x - P.start;
+    int q = 0;
+    while (q + 1 < b.nsl && bid >= b.sl[q + 1].start) ++q;
+    const BcSlice S = b.sl[q];
+    const BcField& F = b.f[S.fi];
+    const long w0 = (long)(bid - S.start) * blockDim.x * BC_PTS + threadIdx.x;
+    if (F.f32) bc_points<float>(b, S, F, w0, P.n[0], P.n[1], P.n[2], P.sy, P.sz);
+    else bc_points<double>(b, S, F, w0, P.n[0], P.n[1], P.n[2], P.sy, P.sz);
+}
+
+void launch_bc_all_packed(const PackedBcAll* all, const int* blk2inst, int nblocks, hipStream_t s)
+{
+    if (nblocks <= 0) return;
+    hipLaunchKernelGGL(k_bc_all_packed, dim3(nblocks), dim3(256), 0, s, all, blk2inst);
 }
 
 // Packed boundary conditions of many instances in one dimension: each op

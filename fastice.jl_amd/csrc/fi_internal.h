@@ -240,6 +240,36 @@ void launch_bc_dim(const Geo& g, const BcBatch& b, hipStream_t s);
 // validity bits (StokesArgs::crange); the x-face ghosts whose point and
 // source lie outside them already hold their value and are skipped
 void launch_bc_dims(const Geo& g, const BcBatch* b, hipStream_t s, const int2* skip_ranges = nullptr);
+// the k_bc_all launch of a group: its fields, with per pass and side the
+// face rule, and one slice of workgroups per written face
+struct BcField {
+    void* ptr;
+    int L, f32;
+    int kind[3][2]; // -1: pass d does not write side s; else 0 Dirichlet, 1 Neumann
+    double v[3][2];
+};
+// one face of one field: pass d, side s, ext1 x ext2 points, workgroups from start
+struct BcSlice {
+    int fi, d, s;
+    int ext1, ext2;
+    int start;
+};
+struct BcAll {
+    BcField f[8];
+    int nf, nd;
+    const int2* skip_ranges; // see launch_bc_dims; null: nothing is skipped
+    BcSlice sl[48];
+    int nsl;
+};
+bool bc_all_build(const Geo& g, const BcBatch* b, const int2* skip_ranges, BcAll& a, int& nblocks);
+// one model's k_bc_all launch inside a packed launch over many models
+struct PackedBcAll {
+    BcAll a;
+    int n[3];
+    long sy, sz;
+    int start; // its first workgroup in the packed launch
+};
+void launch_bc_all_packed(const PackedBcAll* all, const int* blk2inst, int nblocks, hipStream_t s);
 
 // packed sweeps over many instances (batch.hip builds the tables)
 // visc false: every listed instance sweeps with frozen viscosity (FP32 only)
