@@ -171,7 +171,7 @@ BcBatch neumann_batch(double* p, int L, int d)
     return b;
 }
 
-void run_bc(fi_model_s* m, int group, const int2* skip_ranges)
+void run_bc(fi_model_s* m, int group, bool skip)
 {
     BcBatch b[3];
     for (int d = 0; d < m->g.nd; ++d) {
@@ -180,7 +180,7 @@ void run_bc(fi_model_s* m, int group, const int2* skip_ranges)
         else if (group == FI_BCGROUP_ETA) b[d] = neumann_batch(m->eta(), L_CCC, d);
         else b[d] = residual_batch(m, d);
     }
-    launch_bc_dims(m->g, b, m->stream, skip_ranges);
+    launch_bc_dims(m->g, b, m->stream, skip ? m->crange : nullptr, skip ? m->empty_faces : 0u);
 }
 
 int prepare(fi_model_s* m, bool sync_shadow)
@@ -205,7 +205,13 @@ int prepare(fi_model_s* m, bool sync_shadow)
             ns[q] = m->f[FI_F_OM_NS0 + q];
         }
         launch_ib_flags(m->g, na, ns, m->flags, m->stream);
-        if (m->crange) launch_col_ranges(m->g, m->flags, m->crange, m->erange, m->stream);
+        if (m->crange) {
+            launch_col_ranges(m->g, m->flags, m->crange, m->erange, m->stream);
+            std::vector<int2> r(col_ranges_count(m->g));
+            HIPCHK(hipMemcpyAsync(r.data(), m->crange, r.size() * sizeof(int2), hipMemcpyDeviceToHost, m->stream));
+            HIPCHK(hipStreamSynchronize(m->stream));
+            m->empty_faces = face_layers_empty(m->g, r.data());
+        }
         // locations may have changed validity: no buffer is known to be clean
         m->resid_clean = false;
         m->zs[0] = m->zs[1] = m->zv[0] = m->zv[1] = false;
@@ -230,23 +236,28 @@ void iterate(fi_model_s* m, hipEvent_t* ev)
     a.z_wvel = m->zv[vo];
     const bool fused = m->use_fused && m->has_interior;
     const BoxList boxes = fused ? m->shell : full_box(m->g);
+    // shell boxes in a face layer without a valid location: every point of
+    // theirs returns at once (sigma_point3, velocity_point3), so they are
+    // left out of the launches
+    const BoxList sboxes = ib && a.z_wsig ? drop_empty_boxes(m->g, boxes, m->empty_faces) : boxes;
+    const BoxList vboxes = ib && a.z_wvel && a.frozen_eta ? drop_empty_boxes(m->g, boxes, m->empty_faces) : boxes;
     if (ev) (void)hipEventRecord(ev[0], m->stream);
     if (fused) {
         FusedGeom fg = m->fg;
         fg.order = m->forder;
-        launch_fused(m->g, a, fg, ib, m->ftx, m->fty, m->fminw, m->stream, m->fused_shell ? &boxes : nullptr);
+        launch_fused(m->g, a, fg, ib, m->ftx, m->fty, m->fminw, m->stream, m->fused_shell ? &sboxes : nullptr);
     }
     if (ev) (void)hipEventRecord(ev[1], m->stream);
-    if (!(fused && m->fused_shell)) launch_sigma_boxes(m->g, a, boxes, ib, m->stream);
+    if (!(fused && m->fused_shell)) launch_sigma_boxes(m->g, a, sboxes, ib, m->stream);
     m->sig_cur ^= 1;
     if (ev) (void)hipEventRecord(ev[2], m->stream);
     // the sweep stored nothing at invalid locations (z_wsig): the ghosts the
     // last application on this buffer settled still hold their value
     const bool skip = ib && m->g.nd == 3 && m->crange && a.z_wsig && m->gs[so];
-    run_bc(m, FI_BCGROUP_STRESS, skip ? m->crange : nullptr);
+    run_bc(m, FI_BCGROUP_STRESS, skip);
     if (ib && m->g.nd == 3) m->gs[so] = a.z_wsig != 0;
     if (ev) (void)hipEventRecord(ev[3], m->stream);
-    launch_velocity_boxes(m->g, a, boxes, ib, m->stream);
+    launch_velocity_boxes(m->g, a, vboxes, ib, m->stream);
     m->v_cur ^= 1;
     if (ev) (void)hipEventRecord(ev[4], m->stream);
     run_bc(m, FI_BCGROUP_VELOCITY);

@@ -195,7 +195,7 @@ static void add_bc_all(std::vector<PackedBcAll>& all, std::vector<int>& blk, boo
     PackedBcAll p;
     std::memset(&p, 0, sizeof(p));
     int nb = 0;
-    if (!bc_all_build(m->g, bb, nullptr, p.a, nb)) {
+    if (!bc_all_build(m->g, bb, nullptr, 0u, p.a, nb)) {
         ok = false;
         return;
     }

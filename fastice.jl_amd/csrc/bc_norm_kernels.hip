@@ -122,6 +122,7 @@ struct BcChain {
     bool from_v;
     double v0;
     long src, tgt;
+    BcPt sp; // the point the value starts from (unless from_v)
 };
 
 __device__ __forceinline__ int bc_n(int e, int n0, int n1, int n2) { return e == 0 ? n0 : e == 1 ? n1 : n2; }
@@ -149,9 +150,17 @@ __device__ __forceinline__ BcChain bc_chain(const BcField& F, int nd, int d, int
             break;
         }
     }
+    c.sp = q;
     c.src = q.x + sy * q.y + (nd == 3 ? sz * q.z : 0);
     c.tgt = p.x + sy * p.y + (nd == 3 ? sz * p.z : 0);
     return c;
+}
+
+// p lies on a plane of its column (x, y) without a valid location
+__device__ __forceinline__ bool bc_outside(const int2* ranges, long sy, const BcPt& p)
+{
+    const int2 r = ranges[p.x + sy * p.y];
+    return p.z < r.x || p.z > r.y;
 }
 
 // the points of one face a thread handles: they are set up and their
@@ -160,19 +169,20 @@ __device__ __forceinline__ BcChain bc_chain(const BcField& F, int nd, int d, int
 constexpr int BC_PTS = 4;
 
 template <typename T>
-__device__ __forceinline__ void bc_points(const BcAll& b, const BcSlice& S, const BcField& F, long w0, int n0, int n1,
+// (a face holds fewer than 2^32 points: 32-bit index arithmetic)
+__device__ __forceinline__ void bc_points(const BcAll& b, const BcSlice& S, const BcField& F, unsigned w0, int n0, int n1,
                                           int n2, long sy, long sz)
 {
     const int d = S.d, s = S.s, k = F.kind[d][s];
     const int e1 = d == 0 ? 1 : 0, e2 = b.nd == 3 ? (d == 2 ? 1 : 2) : -1;
     T* f = static_cast<T*>(F.ptr);
-    const long np = (long)S.ext1 * S.ext2;
+    const unsigned np = (unsigned)S.ext1 * (unsigned)S.ext2, ext1 = (unsigned)S.ext1;
     BcChain c[BC_PTS];
     bool live[BC_PTS];
     T x[BC_PTS];
 #pragma unroll
     for (int u = 0; u < BC_PTS; ++u) {
-        const long w = w0 + (long)u * blockDim.x;
+        const unsigned w = w0 + (unsigned)u * blockDim.x;
         live[u] = w < np;
         x[u] = T(0);
         c[u].rules = 0u;
@@ -180,29 +190,26 @@ __device__ __forceinline__ void bc_points(const BcAll& b, const BcSlice& S, cons
         if (!live[u]) continue;
         BcPt p{0, 0, 0};
         p.set(d, bc_target(k, (F.L >> d) & 1, s, bc_n(d, n0, n1, n2)));
-        p.set(e1, (int)(w % S.ext1));
-        if (e2 >= 0) p.set(e2, (int)(w / S.ext1));
+        p.set(e1, (int)(w % ext1));
+        if (e2 >= 0) p.set(e2, (int)(w / ext1));
         // a later pass that writes p owns it
         for (int e = d + 1; e < b.nd; ++e)
             for (int t = 0; t < 2; ++t) {
                 const int ke = F.kind[e][t];
                 if (ke >= 0 && p.at(e) == bc_target(ke, (F.L >> e) & 1, t, bc_n(e, n0, n1, n2))) live[u] = false;
             }
-        // an x-face ghost whose point and source lie on planes of their
-        // columns without a valid location: the sweep stored nothing at
-        // either since the last application, which wrote the value a +0
-        // source gives (a Dirichlet vertex has no source)
-        if (live[u] && b.skip_ranges && d == 0) {
-            const int2 rt = b.skip_ranges[p.x + sy * p.y];
-            bool inv = p.z < rt.x || p.z > rt.y;
-            if (inv && !(k == 0 && (F.L & 1))) {
-                const int2 rs = b.skip_ranges[bc_source(F.L & 1, s, n0) + sy * p.y];
-                inv = p.z < rs.x || p.z > rs.y;
-            }
-            if (inv) live[u] = false;
-        }
         if (!live[u]) continue;
         c[u] = bc_chain(F, b.nd, d, n0, n1, n2, sy, sz, p);
+        // a ghost whose point and the point its chain starts from lie on
+        // planes of their columns without a valid location: the sweep stored
+        // nothing at either since the last application, which wrote the
+        // value the chain gives a +0 start (a chain from a Dirichlet vertex
+        // starts from v and reads nothing)
+        if (b.skip_ranges && bc_outside(b.skip_ranges, sy, p) &&
+            (c[u].from_v || bc_outside(b.skip_ranges, sy, c[u].sp))) {
+            live[u] = false;
+            continue;
+        }
         x[u] = c[u].from_v ? T(c[u].v0) : f[c[u].src];
     }
 #pragma unroll
@@ -225,7 +232,7 @@ __global__ __launch_bounds__(256) void k_bc_all(BcAll b, int n0, int n1, int n2,
     while (q + 1 < b.nsl && bid >= b.sl[q + 1].start) ++q;
     const BcSlice S = b.sl[q];
     const BcField& F = b.f[S.fi];
-    const long w0 = (long)(bid - S.start) * blockDim.x * BC_PTS + threadIdx.x;
+    const unsigned w0 = (unsigned)(bid - S.start) * blockDim.x * BC_PTS + threadIdx.x;
     if (F.f32) bc_points<float>(b, S, F, w0, n0, n1, n2, sy, sz);
     else bc_points<double>(b, S, F, w0, n0, n1, n2, sy, sz);
 }
@@ -234,8 +241,10 @@ __global__ __launch_bounds__(256) void k_bc_all(BcAll b, int n0, int n1, int n2,
 // fills a and its slice starts, nblocks = workgroups.  false: a field has
 // two rules on one face or there are more than 8 fields, and the passes
 // must run one launch per dimension (launch_bc_dim) to keep their order.
-bool bc_all_build(const Geo& g, const BcBatch* b, const int2* skip_ranges, BcAll& a, int& nblocks)
+bool bc_all_build(const Geo& g, const BcBatch* b, const int2* skip_ranges, unsigned empty_faces, BcAll& a,
+                  int& nblocks)
 {
+    if (!skip_ranges || g.nd != 3) empty_faces = 0;
     a.nf = 0;
     a.nd = g.nd;
     a.nsl = 0;
@@ -281,6 +290,9 @@ bool bc_all_build(const Geo& g, const BcBatch* b, const int2* skip_ranges, BcAll
             const int ext2 = (e2 >= 0) ? g.n[e2] + ((L >> e2) & 1) + 2 : 1;
             for (int t = 0; t < 2; ++t) {
                 if (a.f[fi].kind[d][t] < 0) continue;
+                // every point and chain start of the face lies in its layer:
+                // all of them would be skipped
+                if (empty_faces & (1u << (2 * d + t))) continue;
                 BcSlice& S = a.sl[a.nsl++];
                 S.fi = fi;
                 S.d = d;
@@ -294,11 +306,11 @@ bool bc_all_build(const Geo& g, const BcBatch* b, const int2* skip_ranges, BcAll
     return true;
 }
 
-void launch_bc_dims(const Geo& g, const BcBatch* b, hipStream_t s, const int2* skip_ranges)
+void launch_bc_dims(const Geo& g, const BcBatch* b, hipStream_t s, const int2* skip_ranges, unsigned empty_faces)
 {
     BcAll a;
     int nblocks = 0;
-    if (!bc_all_build(g, b, skip_ranges, a, nblocks)) {
+    if (!bc_all_build(g, b, skip_ranges, empty_faces, a, nblocks)) {
         for (int d = 0; d < g.nd; ++d) launch_bc_dim(g, b[d], s);
         return;
     }
@@ -318,7 +330,7 @@ __global__ __launch_bounds__(256) void k_bc_all_packed(const PackedBcAll* __rest
     while (q + 1 < b.nsl && bid >= b.sl[q + 1].start) ++q;
     const BcSlice S = b.sl[q];
     const BcField& F = b.f[S.fi];
-    const long w0 = (long)(bid - S.start) * blockDim.x * BC_PTS + threadIdx.x;
+    const unsigned w0 = (unsigned)(bid - S.start) * blockDim.x * BC_PTS + threadIdx.x;
     if (F.f32) bc_points<float>(b, S, F, w0, P.n[0], P.n[1], P.n[2], P.sy, P.sz);
     else bc_points<double>(b, S, F, w0, P.n[0], P.n[1], P.n[2], P.sy, P.sz);
 }

@@ -237,9 +237,12 @@ void launch_eta_update(const Geo& g, const StokesArgs<T>& a, hipStream_t s);
 void launch_bc_dim(const Geo& g, const BcBatch& b, hipStream_t s);
 // the batches of every dimension (b[0..nd-1]) in one launch, same result;
 // skip_ranges (3D, a stress group): the per-column plane ranges of the
-// validity bits (StokesArgs::crange); the x-face ghosts whose point and
-// source lie outside them already hold their value and are skipped
-void launch_bc_dims(const Geo& g, const BcBatch* b, hipStream_t s, const int2* skip_ranges = nullptr);
+// validity bits (StokesArgs::crange); the ghosts whose point and chain
+// start lie outside them already hold their value and are skipped; with
+// them, the faces in empty_faces (face_layers_empty) are left out of the
+// launch, every ghost of theirs being one of those
+void launch_bc_dims(const Geo& g, const BcBatch* b, hipStream_t s, const int2* skip_ranges = nullptr,
+                    unsigned empty_faces = 0);
 // the k_bc_all launch of a group: its fields, with per pass and side the
 // face rule, and one slice of workgroups per written face
 struct BcField {
@@ -261,7 +264,15 @@ struct BcAll {
     BcSlice sl[48];
     int nsl;
 };
-bool bc_all_build(const Geo& g, const BcBatch* b, const int2* skip_ranges, BcAll& a, int& nblocks);
+bool bc_all_build(const Geo& g, const BcBatch* b, const int2* skip_ranges, unsigned empty_faces, BcAll& a,
+                  int& nblocks);
+// Faces of a 3D grid whose layer holds no valid location (bit 2d+s): the
+// planes 0..2 (s = 0) or n..n+2 (s = 1) of dimension d, from the column
+// ranges (host copy, indexed i + sy * j).  The shell box and every boundary
+// condition point and source of such a face lie in its layer.
+unsigned face_layers_empty(const Geo& g, const int2* host_ranges);
+// bl without the boxes that lie in the layer of a face of empty_faces
+BoxList drop_empty_boxes(const Geo& g, const BoxList& bl, unsigned empty_faces);
 // one model's k_bc_all launch inside a packed launch over many models
 struct PackedBcAll {
     BcAll a;

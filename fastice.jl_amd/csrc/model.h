@@ -40,6 +40,8 @@ struct fi_model_s {
     // 3D immersed boundary: per-column plane ranges of the flags (launch_col_ranges)
     int2* crange = nullptr;
     int2* erange = nullptr;
+    // faces whose layer holds no valid location (face_layers_empty), with the ranges
+    unsigned empty_faces = 0;
     int2* tmarch = nullptr; // the z range each tile of the last FP64 fused sweep marched
     // zs[b] / zv[b]: physical buffer b (0: f, 1: sf) of the stress / velocity
     // is known to hold +0 at every invalid location of the fused footprint
@@ -51,8 +53,8 @@ struct fi_model_s {
     bool zs32[2] = {false, false};
     bool zv32[2] = {false, false};
     // gs[b]: the stress boundary conditions last applied to physical stress
-    // buffer b left every x-face ghost whose point and source are invalid
-    // locations at the value its rule gives a +0 source, and nothing has
+    // buffer b left every ghost whose point and chain start are invalid
+    // locations at the value its rules give a +0 start, and nothing has
     // written there since, so the next application may skip those ghosts
     // (set after an application that follows a sweep; anything that clears
     // zs clears it); gs32: the same for the FP32 stress buffers
@@ -123,7 +125,10 @@ StokesArgs<double> make_args(fi_model_s* m, bool in_place);
 BcBatch stress_batch(fi_model_s* m, int d);
 BcBatch velocity_batch(fi_model_s* m, int d, bool with_eta);
 // skip_ranges: the column plane ranges of a stress group whose settled ghosts may be skipped (gs)
-void run_bc(fi_model_s* m, int group, const int2* skip_ranges = nullptr);
+// skip (3D immersed boundary, stress group): the settled ghosts of the
+// written buffer are skipped (launch_bc_dims with the model's column ranges
+// and empty faces)
+void run_bc(fi_model_s* m, int group, bool skip = false);
 // before kernels run: the immersed-boundary validity bits and, with
 // sync_shadow, the ping-pong copies the FP64 iteration needs
 int prepare(fi_model_s* m, bool sync_shadow = true);

@@ -529,11 +529,12 @@ BcBatch refine_bc_batch(fi_model_s* m, int group, int d)
     return b;
 }
 
-static void bc32(fi_model_s* m, int group, const int2* skip_ranges = nullptr)
+// skip: as run_bc
+static void bc32(fi_model_s* m, int group, bool skip = false)
 {
     BcBatch b[3];
     for (int d = 0; d < m->g.nd; ++d) b[d] = refine_bc_batch(m, group, d);
-    launch_bc_dims(m->g, b, m->stream, skip_ranges);
+    launch_bc_dims(m->g, b, m->stream, skip ? m->crange : nullptr, skip ? m->empty_faces : 0u);
 }
 
 StokesArgs<float> refine_args(fi_model_s* m, bool in_place) { return make_args32(m, in_place); }
@@ -579,23 +580,26 @@ static void iterate32(fi_model_s* m, hipEvent_t* ev)
     a.z_wvel = m->zv32[vo];
     const bool fused = m->use_fused && r->has_interior;
     const BoxList boxes = fused ? r->shell : full_box(m->g);
+    // shell boxes in a face layer without a valid location are left out (as in iterate)
+    const BoxList sboxes = ib && a.z_wsig ? drop_empty_boxes(m->g, boxes, m->empty_faces) : boxes;
+    const BoxList vboxes = ib && a.z_wvel && a.frozen_eta ? drop_empty_boxes(m->g, boxes, m->empty_faces) : boxes;
     if (ev) (void)hipEventRecord(ev[0], m->stream);
     if (fused) {
         FusedGeom fg = r->fg;
         fg.order = m->forder;
-        launch_fused(m->g, a, fg, ib, m->ftx32, m->fty32, m->fminw32, m->stream, m->fused_shell ? &boxes : nullptr);
+        launch_fused(m->g, a, fg, ib, m->ftx32, m->fty32, m->fminw32, m->stream, m->fused_shell ? &sboxes : nullptr);
     }
     if (ev) (void)hipEventRecord(ev[1], m->stream);
-    if (!(fused && m->fused_shell)) launch_sigma_boxes(m->g, a, boxes, ib, m->stream);
+    if (!(fused && m->fused_shell)) launch_sigma_boxes(m->g, a, sboxes, ib, m->stream);
     r->sig_cur ^= 1;
     if (ev) (void)hipEventRecord(ev[2], m->stream);
     // the sweep stored nothing at invalid locations (z_wsig): the ghosts the
     // last application on this buffer settled still hold their value
     const bool skip = ib && m->g.nd == 3 && m->crange && a.z_wsig && m->gs32[so];
-    bc32(m, FI_BCGROUP_STRESS, skip ? m->crange : nullptr);
+    bc32(m, FI_BCGROUP_STRESS, skip);
     if (ib && m->g.nd == 3) m->gs32[so] = a.z_wsig != 0;
     if (ev) (void)hipEventRecord(ev[3], m->stream);
-    launch_velocity_boxes(m->g, a, boxes, ib, m->stream);
+    launch_velocity_boxes(m->g, a, vboxes, ib, m->stream);
     r->v_cur ^= 1;
     if (ev) (void)hipEventRecord(ev[4], m->stream);
     bc32(m, FI_BCGROUP_VELOCITY);

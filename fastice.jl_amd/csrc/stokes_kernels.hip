@@ -296,6 +296,29 @@ __device__ __forceinline__ void velocity_point3(const StokesArgs<T>& a, long c, 
     a.eta_next[c] = rheology(a, c, t);
 }
 
+// point t of box b in flat (i fastest) order; false past its end (32-bit
+// index arithmetic where the box holds fewer than 2^32 points)
+__device__ __forceinline__ bool box_point(const Box& b, long t, int& i, int& j, int& k)
+{
+    const long nx = b.i1 - b.i0 + 1, ny = b.j1 - b.j0 + 1, nz = b.k1 - b.k0 + 1;
+    const long n = nx * ny * nz;
+    if (t >= n) return false;
+    if (n <= 0xffffffffL) {
+        const unsigned tu = (unsigned)t, nxu = (unsigned)nx, nyu = (unsigned)ny;
+        const unsigned r = tu / nxu;
+        i = b.i0 + (int)(tu - r * nxu);
+        const unsigned s = r / nyu;
+        j = b.j0 + (int)(r - s * nyu);
+        k = b.k0 + (int)s;
+    } else {
+        i = b.i0 + (int)(t % nx);
+        const long r = t / nx;
+        j = b.j0 + (int)(r % ny);
+        k = b.k0 + (int)(r / ny);
+    }
+    return true;
+}
+
 // flat (i fastest) indexing over the boxes of a list: every lane of a block
 // is busy however thin the box, and wide boxes keep coalesced x rows
 __device__ __forceinline__ bool box_index(const BoxList& bl, int& i, int& j, int& k)
@@ -303,15 +326,8 @@ __device__ __forceinline__ bool box_index(const BoxList& bl, int& i, int& j, int
     const int bid = blockIdx.x;
     int q = 0;
     while (q + 1 < bl.nb && bid >= bl.start[q + 1]) ++q;
-    const Box b = bl.b[q];
-    const long nx = b.i1 - b.i0 + 1, ny = b.j1 - b.j0 + 1, nz = b.k1 - b.k0 + 1;
     const long t = (long)(bid - bl.start[q]) * blockDim.x + threadIdx.x;
-    if (t >= nx * ny * nz) return false;
-    i = b.i0 + (int)(t % nx);
-    const long r = t / nx;
-    j = b.j0 + (int)(r % ny);
-    k = b.k0 + (int)(r / ny);
-    return true;
+    return box_point(bl.b[q], t, i, j, k);
 }
 
 template <typename T, bool IB>
@@ -1203,13 +1219,9 @@ __global__ __launch_bounds__(TX * TY, MINW) void k_fused3(StokesArgs<T> a, Fused
         const int bid = (int)blockIdx.x - nfused;
         int q = 0;
         while (q + 1 < shell.nb && bid >= shell.start[q + 1]) ++q;
-        const Box b = shell.b[q];
-        const long nx = b.i1 - b.i0 + 1, ny = b.j1 - b.j0 + 1, nz = b.k1 - b.k0 + 1;
         const long t = (long)(bid - shell.start[q]) * (TX * TY) + threadIdx.x + TX * threadIdx.y;
-        if (t >= nx * ny * nz) return;
-        const int i = b.i0 + (int)(t % nx);
-        const long r = t / nx;
-        const int j = b.j0 + (int)(r % ny), k = b.k0 + (int)(r / ny);
+        int i, j, k;
+        if (!box_point(shell.b[q], t, i, j, k)) return;
         sigma_point3<T, IB>(a, (long)i + a.sy * j + a.sz * k, a.sy, a.sz);
         return;
     }
@@ -1739,13 +1751,10 @@ __global__ __launch_bounds__(256) void k_point_packed(const PackedInst<T>* __res
                                                       const int* __restrict__ blk2box)
 {
     const PackedBox b = boxes[blk2box[blockIdx.x]];
-    const long nx = b.i1 - b.i0 + 1, ny = b.j1 - b.j0 + 1, nz = b.k1 - b.k0 + 1;
     const long t = (long)(blockIdx.x - b.start) * 256 + threadIdx.x;
-    if (t >= nx * ny * nz) return;
-    const long i = b.i0 + t % nx;
-    const long r = t / nx;
-    const long j = b.j0 + r % ny;
-    const long k = b.k0 + r / ny;
+    int ii, jj, kk;
+    if (!box_point(Box{b.i0, b.i1, b.j0, b.j1, b.k0, b.k1}, t, ii, jj, kk)) return;
+    const long i = ii, j = jj, k = kk;
     const StokesArgs<T>& a = inst[b.inst].a;
     if constexpr (ND == 3) {
         const long sy = a.sy, sz = a.sz;
@@ -1885,6 +1894,44 @@ bool fused_split(const Geo& g, int ty, int kch, FusedGeom& fg, BoxList& shell)
     add(hi[0] + 1, N0, lo[1], hi[1], lo[2], hi[2]);        // x high
     finish_boxes(bl);
     return true;
+}
+
+unsigned face_layers_empty(const Geo& g, const int2* r)
+{
+    if (g.nd != 3) return 0u;
+    const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
+    unsigned full = 0u; // faces whose layer holds a valid location
+    for (int j = 0; j <= n1 + 2; ++j)
+        for (int i = 0; i <= n0 + 2; ++i) {
+            const int2 c = r[(size_t)i + (size_t)g.sy * j];
+            if (c.x > c.y) continue; // no valid location in the column
+            if (i <= 2) full |= 1u << 0;
+            if (i >= n0) full |= 1u << 1;
+            if (j <= 2) full |= 1u << 2;
+            if (j >= n1) full |= 1u << 3;
+            if (c.x <= 2) full |= 1u << 4;
+            if (c.y >= n2) full |= 1u << 5;
+        }
+    return ~full & 0x3fu;
+}
+
+BoxList drop_empty_boxes(const Geo& g, const BoxList& bl, unsigned empty)
+{
+    if (!empty || g.nd != 3) return bl;
+    BoxList out;
+    out.nb = 0;
+    for (int q = 0; q < bl.nb; ++q) {
+        const Box& b = bl.b[q];
+        const int lo[3] = {b.i0, b.j0, b.k0}, hi[3] = {b.i1, b.j1, b.k1};
+        bool drop = false;
+        for (int d = 0; d < 3; ++d) {
+            if ((empty & (1u << (2 * d))) && hi[d] <= 2) drop = true;
+            if ((empty & (1u << (2 * d + 1))) && lo[d] >= g.n[d]) drop = true;
+        }
+        if (!drop) out.b[out.nb++] = b;
+    }
+    finish_boxes(out);
+    return out;
 }
 
 template <typename T>
