@@ -178,7 +178,8 @@ def test_refined_glacier_configured_regularisation_255(pkg):
     assert max(na) < 1e-8, na
     errs = _field_errors(a, b, state_names(3))
     _say("per-field difference (relative to the FP64 stress / velocity scale): %s" % errs)
-    assert max(errs.values()) < 1e-6, errs
+    # measured at most 1.4e-9 (both states at residual 1e-8)
+    assert max(errs.values()) < 1e-7, errs
     eta = a.field("ETA").interior()
     assert eta.max() <= 5000.0 * (1 + 1e-12) and eta.max() > 1000.0, (eta.min(), eta.max())
     a.close()
