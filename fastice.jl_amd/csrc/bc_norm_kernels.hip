@@ -165,11 +165,11 @@ __device__ __forceinline__ bool bc_outside(const int2* ranges, long sy, const Bc
 
 // the points of one face a thread handles: they are set up and their
 // sources loaded first, then their values stored (no thread of the launch
-// writes a source, so the loads may all go ahead of the stores)
+// writes a source, so the loads may all go ahead of the stores); a face
+// holds fewer than 2^32 points (32-bit index arithmetic)
 constexpr int BC_PTS = 4;
 
 template <typename T>
-// (a face holds fewer than 2^32 points: 32-bit index arithmetic)
 __device__ __forceinline__ void bc_points(const BcAll& b, const BcSlice& S, const BcField& F, unsigned w0, int n0, int n1,
                                           int n2, long sy, long sz)
 {
