@@ -76,6 +76,50 @@ def test_mixed_bcs_powerlaw_3d(pkg, orc, dims, scale):
     model.close()
 
 
+@pytest.mark.parametrize("dims", [(1, 1, 1), (2, 3, 1), (1, 6, 2), (3, 4, 5), (4, 4, 4), (7, 2, 3)])
+def test_tiny_grids_3d(pkg, orc, dims):
+    """Grids too small for the fused interior (fewer than 4 cells along some
+    axis: the pointwise kernels sweep the whole box), and the smallest ones
+    that have it: 40 iterations with every boundary-condition kind, bitwise
+    to the oracle, ghosts and residuals included."""
+    _say("case: tiny 3D grid %s, mixed BCs, relaxed power law" % (dims,))
+    grid = pkg.UniformGrid(origin=(0.0, 0.0, 0.0), extent=(0.3, 0.5, 0.4), dims=dims)
+    bcs = dict(x=(pkg.SBC(0.0, 0.1, -0.2), pkg.TBC(0.3, -0.1, 0.05)),
+               y=(pkg.VBC(0.02, 0.0, -0.01), pkg.SBC(0.0, 0.2, 0.1)),
+               z=(pkg.VBC(0.0, 0.0, 0.0), pkg.TBC(0.0, 0.0, 0.0)))
+    rheo = pkg.Relaxation("eta", pkg.ViscoplasticRegularisation(pkg.PowerLawRheology(0.7, 3), 50.0), 0.3)
+    kw = dict(grid=grid, boundary_conditions=bcs, gravity=dict(x=0.1, y=-0.2, z=1.0), rheology=rheo,
+              solver_params=pkg.pt_params(grid, r=0.7, re_mech=2 * math.pi))
+    model, ref = make_pair(pkg, orc, kw, seed=11)
+    for m in (model, ref):
+        m.apply_bc("eta")
+    model.advance(40)
+    ref.advance(40)
+    worst, bad = compare_state(model, ref, state_names(3) + ["ETA", "ETA_NEXT"], TOL)
+    assert not bad, worst
+    np.testing.assert_allclose(model.compute_residuals(), ref.compute_residuals(), rtol=TOL, atol=0)
+    model.close()
+
+
+@pytest.mark.parametrize("dims", [(1, 1), (2, 5), (4, 1), (3, 3)])
+def test_tiny_grids_2d(pkg, orc, dims):
+    """2D sections of one to five cells per axis: 40 iterations bitwise to the
+    oracle."""
+    _say("case: tiny 2D grid %s, traction / slip / no-slip" % (dims,))
+    grid = pkg.UniformGrid(origin=(0.0, 0.0), extent=(0.4, 0.3), dims=dims)
+    bcs = dict(x=(pkg.SBC(0.0, 0.0), pkg.TBC(0.1, -0.05)), y=(pkg.VBC(0.0, 0.0), pkg.TBC(0.0, 0.0)))
+    kw = dict(grid=grid, boundary_conditions=bcs, gravity=dict(x=0.2, y=1.0),
+              rheology=pkg.LinearViscousRheology(1.0),
+              solver_params=pkg.pt_params(grid, r=0.7, re_mech=2 * math.pi))
+    model, ref = make_pair(pkg, orc, kw, seed=5)
+    model.advance(40)
+    ref.advance(40)
+    worst, bad = compare_state(model, ref, state_names(2), TOL)
+    assert not bad, worst
+    np.testing.assert_allclose(model.compute_residuals(), ref.compute_residuals(), rtol=TOL, atol=0)
+    model.close()
+
+
 def test_mixed_bcs_linear_field_2d(pkg, orc):
     _say("case: 2D traction/no-slip with a viscosity field")
     grid = pkg.UniformGrid(origin=(0.0, 0.0), extent=(1.0, 0.5), dims=(40, 23))
