@@ -245,9 +245,12 @@ def test_levelset_and_volume_fractions_3d(pkg, orc):
     model.close()
 
 
-def test_immersed_boundary_iterations_3d(pkg, orc):
-    _say("case: immersed-boundary PT iterations, 24x20x12 glacier, 400 iterations")
-    gl, grid, dgrid, model, ref = _glacier_ib_pair(pkg, orc, (24, 20, 12))
+@pytest.mark.parametrize("dims", [(24, 20, 12), (6, 5, 4), (3, 7, 3), (9, 2, 5)])
+def test_immersed_boundary_iterations_3d(pkg, orc, dims):
+    """400 immersed-boundary iterations of a glacier against the oracle, at a
+    size with a fused interior and at tiny sizes swept pointwise."""
+    _say("case: immersed-boundary PT iterations, %s glacier, 400 iterations" % (dims,))
+    gl, grid, dgrid, model, ref = _glacier_ib_pair(pkg, orc, dims)
     from fastice_jl_amd.level_sets import dem_parent
     pkg.compute_levelset_from_dem(model.level_sets.na, gl.surface, dgrid, grid)
     pkg.compute_levelset_from_dem(model.level_sets.ns, gl.bed, dgrid, grid)
