@@ -70,6 +70,44 @@ def test_refined_manufactured_2d(pkg):
     assert max(errs.values()) < 1e-7, errs
 
 
+@pytest.mark.parametrize("dims", [(6, 5, 4), (12, 10, 6)])
+def test_refined_tiny_glacier_matches_fp64(pkg, dims):
+    """Refined solves of glaciers too small for the fused interior (every
+    sweep pointwise, FP32 masks from the quantised fractions) reach the FP64
+    solve's state."""
+    _say("case: refined vs FP64, tiny immersed-boundary glacier %s, tol 1e-8" % (dims,))
+    gl = pkg.GaussianGlacier(seed=5)
+    grid = pkg.UniformGrid(origin=(-1.0, -1.0, 0.0), extent=(2.0, 2.0, 1.0), dims=dims)
+    dgrid = pkg.UniformGrid(origin=(-1.0, -1.0), extent=(2.0, 2.0), dims=(dims[0], dims[1]))
+    tr = pkg.TBC(0.0, 0.0, 0.0)
+    kw = dict(grid=grid, boundary_conditions=dict(x=(tr, tr), y=(tr, tr), z=(tr, tr)),
+              gravity=dict(x=0.0, y=0.0, z=1.0), rheology=pkg.LinearViscousRheology(1.0),
+              solver_params=pkg.pt_params(grid, r=0.9, re_mech=5 * math.pi))
+    models = []
+    for _ in range(2):
+        m = pkg.IsothermalImmersedBoundaryFullStokesModel(**kw)
+        m.viscosity.eta.set(1.0)
+        m.viscosity.eta_next.set(1.0)
+        pkg.compute_levelset_from_dem(m.level_sets.na, gl.surface, dgrid, grid)
+        pkg.compute_levelset_from_dem(m.level_sets.ns, gl.bed, dgrid, grid)
+        pkg.invert_levelset(m.level_sets.ns)
+        pkg.omega_from_psi(m.field_masks.ns, m.level_sets.ns)
+        pkg.omega_from_psi(m.field_masks.na, m.level_sets.na)
+        m.apply_bc("eta")
+        models.append(m)
+    a, b = models
+    it, nb = b.solve(20000, 100, 1e-8)
+    assert max(nb) < 1e-8, nb
+    outer, inner, na = a.refined_solve(1e-8)
+    _say("outer %d inner %d fp64 iters %d" % (outer, inner, it))
+    assert max(na) < 1e-8, na
+    errs = _field_errors(a, b, state_names(3))
+    _say("errors %s" % errs)
+    assert max(errs.values()) < 1e-6, errs
+    a.close()
+    b.close()
+
+
 @pytest.mark.parametrize("mode", [1, 0])
 @pytest.mark.parametrize("glen", [False, True])
 def test_refined_glacier_matches_fp64(pkg, glen, mode):
