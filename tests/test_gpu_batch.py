@@ -36,13 +36,16 @@ def _glacier(pkg, dims, seed, glen):
 
 
 # (kind, dims, seed): a mix of immersed-boundary Glen and linear glaciers,
-# plain 3D and 2D manufactured problems; sizes give partial fused tiles
+# plain 3D and 2D manufactured problems; sizes give partial fused tiles, and
+# two instances too small for a fused interior (swept pointwise)
 CASES = [
     ("glacier_glen", (40, 34, 18), 3),
     ("manufactured", (22, 19, 25), None),
     ("manufactured", (48, 40), None),
     ("glacier_linear", (30, 26, 14), 4),
     ("glacier_glen", (26, 30, 20), 5),
+    ("glacier_linear", (7, 6, 5), 6),
+    ("manufactured", (3, 5, 2), None),
 ]
 
 
@@ -149,6 +152,7 @@ REFINED_CASES = [
     ("manufactured", (22, 19, 25), None),
     ("glacier_2d", (64, 32), 7),
     ("glacier_linear", (56, 40, 24), 2),
+    ("glacier_linear", (7, 6, 5), 6),
 ]
 
 
