@@ -49,7 +49,7 @@ def _upload_masks(pkg, model, ref, na, ns):
 
 
 @pytest.mark.parametrize("ib", [False, True])
-@pytest.mark.parametrize("dims", [(20, 17, 12), (70, 36, 9), (40, 23)])
+@pytest.mark.parametrize("dims", [(20, 17, 12), (70, 36, 9), (40, 23), (2, 3, 1), (1, 3, 2), (4, 3)])
 def test_compute_tau_bitwise(pkg, orc, dims, ib):
     _say("case: compute_tau, dims %s, immersed boundary %s" % (dims, ib))
     nd = len(dims)
@@ -72,7 +72,7 @@ def test_compute_tau_bitwise(pkg, orc, dims, ib):
     model.close()
 
 
-@pytest.mark.parametrize("dims", [(5, 4, 6), (70, 36, 9), (9, 7)])
+@pytest.mark.parametrize("dims", [(5, 4, 6), (70, 36, 9), (9, 7), (1, 2, 1), (2, 1)])
 def test_validity_bits_bitwise(pkg, orc, dims):
     _say("case: validity bits on random masks, dims %s" % (dims,))
     nd = len(dims)
