@@ -22,6 +22,8 @@ CASES = [
     ((70, 6, 5), [[2, 2], [2, 2], [2, 2]]),
     ((17, 9, 70), [[0, 2], [1, 2], [1, 2]]),  # z chunks of 32 planes: seams at 32 and 64
     ((33, 27), [[1, 1], [2, 0]]),
+    ((1, 2, 3), [[1, 2], [2, 1], [1, 1]]),  # tiny grids: one or two cells along an axis
+    ((2, 1), [[2, 1], [1, 2]]),
 ]
 
 
