@@ -32,3 +32,12 @@ def test_line_carries_the_contract_keys():
     t = bench._line("t", 3.0, "s", 3000.0, 1, 0, "f64", {"workload": "w"}, higher=False, scaling="strong")
     assert t["higher_is_better"] is False and t["scaling"] == "strong"
 
+
+
+def test_headline_traffic_comes_from_a_committed_profile():
+    # roofline.traffic of the default line is the measured HBM bytes per
+    # launch recorded in profiles/ (FETCH_SIZE x 2 + WRITE_SIZE)
+    for role in ("fp32_fused", "fp64_fused"):
+        t, src = bench.profile_traffic(bench.DEFAULT, role)
+        assert isinstance(t, int) and t > 10 ** 9, (role, t)
+        assert src.startswith("profiles" + os.sep) and os.path.isfile(os.path.join(ROOT, src))
